@@ -1,0 +1,230 @@
+/*
+ * zindex_gpu.h -- C ABI of libzindex_amd.so, the MI355X (gfx950) query path
+ * for zorindex indexes.
+ *
+ * The reference (Rust crate `zorindex`) has no FFI; its drop-in boundary is the
+ * library API re-exported at src/power_merge/mod.rs:45-49 plus the CLI
+ * (src/bin/zorindex.rs). Each entry point below names the reference function
+ * it replaces. Signatures carry plain pointers and sizes only; every buffer is
+ * caller-owned unless stated otherwise, and only opaque handles cross the ABI.
+ *
+ * Errors: every int-returning call returns ZG_OK (0) or a negative ZG_ERR_*
+ * code and leaves a message in the calling thread's zg_last_error().
+ * No C++ exception crosses the boundary.
+ *
+ * Threading: one handle per host thread, or external serialization. Device
+ * state owned by an index is read-only during queries.
+ */
+#ifndef ZINDEX_GPU_H
+#define ZINDEX_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZG_OK 0
+#define ZG_ERR_INVALID (-1)     /* bad argument or failed validation */
+#define ZG_ERR_IO (-2)          /* file system or decoder failure */
+#define ZG_ERR_FORMAT (-3)      /* index/manifest/bank format violation */
+#define ZG_ERR_DEVICE (-4)      /* HIP runtime failure or no usable GPU */
+#define ZG_ERR_NOMEM (-5)       /* host or device allocation failed */
+#define ZG_ERR_UNSUPPORTED (-6) /* valid index, mode this library lacks */
+#define ZG_ERR_BUDGET (-7)      /* HBM admission budget exceeded */
+
+/* Query flags. */
+#define ZG_QUERY_RECORDS 0x1u     /* per-read counts (else aggregate only) */
+#define ZG_QUERY_NO_RECOVERY 0x2u /* ignore the abandoned-key recovery map */
+
+/* Thread-local message for the last failing call on this thread. */
+const char *zg_last_error(void);
+/* Library version string. */
+const char *zg_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Index handle                                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct zg_index zg_index;
+
+typedef struct zg_index_meta {
+  uint32_t version;
+  uint32_t k;
+  uint32_t hashes;
+  uint32_t fingerprint_bits;
+  uint32_t index_mode;        /* 0 kmers, 1 minimizers, 2 modimizers, 3 findere */
+  uint32_t minimizer_size;
+  uint32_t minimizer_scheme;  /* 0 simd, 1 open-closed-mod */
+  uint32_t n_colors;
+  uint32_t n_banks;
+  uint32_t n_classes;         /* distinct (component, source_slots) groups */
+  uint64_t seed;
+  uint64_t segment_length;    /* 0 when the manifest has none (legacy) */
+  int32_t abandoned_recovery;
+  uint32_t reserved;
+  uint64_t abandoned_hashes;
+  uint64_t abandoned_origins;
+  uint64_t payload_bytes;     /* sum of bank payloads */
+} zg_index_meta;
+
+/* Replaces Index::load_mode(dir, recover, None, metadata_only=true)
+ * (src/power_merge/storage.rs:2273-2637): parses manifest.json and applies the
+ * full validation set before any payload is touched. */
+int zg_index_open(const char *dir, zg_index **out);
+/* Releases host and device memory of the handle. */
+int zg_index_close(zg_index *idx);
+int zg_index_get_meta(const zg_index *idx, zg_index_meta *out);
+/* Colour name and unique_kmers (storage.rs:57-63); *name stays valid for the
+ * lifetime of idx. */
+int zg_index_color(const zg_index *idx, uint32_t color, const char **name,
+                   uint64_t *unique_kmers);
+
+/* Replaces load_modern_bank + bank_io::read_parallel (storage.rs:2232-2256,
+ * bank_io.rs:122-291) and QueryRecovery/load_abandoned (query_stream.rs:8-88,
+ * storage.rs:2136-2181): decodes every bank and (unless no_recovery) the
+ * recovery map, then makes them resident in HBM. Fails with ZG_ERR_BUDGET
+ * when the payload exceeds hbm_budget_bytes (0 = no budget). */
+int zg_index_load(zg_index *idx, int recover, uint64_t hbm_budget_bytes);
+/* Host copy of one decoded bank payload (valid after zg_index_load, or after
+ * an in-memory build). */
+int zg_index_bank_payload(const zg_index *idx, uint32_t bank,
+                          const uint8_t **data, uint64_t *bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Queries on host reads                                                     */
+/* ------------------------------------------------------------------------ */
+/* Replaces query_ordinary (src/power_merge/query_stream.rs:163-452) with
+ * record_query::Worker::query_prepared (record_query.rs:302-404):
+ * reads are seq[read_off[i] .. read_off[i+1]) (ASCII, any bytes; non-ACGT
+ * splits runs). Outputs (each may be NULL):
+ *   totals[n_reads]            seeds per read (queried_seeds / valid_kmers)
+ *   counts[n_reads * n_colors] per-read per-colour matches (ZG_QUERY_RECORDS)
+ *   aggregate[n_colors]        matches summed over all reads
+ * Matches are count[c] += [filter hit(c) OR c recovered] per seed occurrence,
+ * the exact semantics of Index::lookup_selected (storage.rs:2761-2925). */
+int zg_query_reads(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
+                   uint64_t n_reads, uint32_t flags, uint64_t *totals,
+                   uint32_t *counts, uint64_t *aggregate);
+
+/* Seeds the GPU selects for each read (scan_index_features, src/lib.rs:574;
+ * open_closed_minimizers::scan_positions, utils/ocm-minimizers/src/lib.rs:4):
+ * key_off[n_reads+1]; keys and positions (offset within the read) hold up to
+ * cap entries. Returns ZG_ERR_INVALID if cap is too small (key_off[n_reads]
+ * then holds the required size). */
+int zg_extract_seeds(zg_index *idx, const uint8_t *seq,
+                     const uint64_t *read_off, uint64_t n_reads,
+                     uint64_t *key_off, uint64_t *keys, uint64_t *positions,
+                     uint64_t cap);
+
+/* ------------------------------------------------------------------------ */
+/* CLI-equivalent commands (byte-identical stdout to the reference)          */
+/* ------------------------------------------------------------------------ */
+/* query_command_with_memory (storage.rs:3731-3767) via
+ * query_stream::query_aggregate (query_stream.rs:864-946). */
+int zg_query_command(const char *dir, const char *query, double min_ratio,
+                     int recover, int out_fd, int err_fd);
+/* query_records_command_with_memory (storage.rs:3626-3720); format 0 = tsv,
+ * 1 = compact (record_output.rs:5-63). */
+int zg_query_records_command(const char *dir, const char *query,
+                             double min_ratio, int recover,
+                             uint64_t batch_size, int format, int out_fd,
+                             int err_fd);
+/* info_command (storage.rs:3901-4092). */
+int zg_info_command(const char *dir, int out_fd);
+
+/* ------------------------------------------------------------------------ */
+/* Index construction in the reference on-disk format                        */
+/* ------------------------------------------------------------------------ */
+typedef struct zg_build_config {
+  uint32_t k;                  /* default 31 */
+  uint32_t hashes;             /* 2..32, default 4 */
+  uint32_t fingerprint_bits;   /* 1,2,4,8,16,32, default 8 */
+  uint32_t index_mode;         /* 0 kmers, 1 minimizers (open-closed mod) */
+  uint32_t minimizer_size;     /* m for minimizers */
+  uint32_t segment_length;     /* default 64 */
+  uint64_t seed;               /* default 69 */
+  double slot_scale;           /* default 1.0 */
+  uint64_t max_subindex_size;  /* power of two, default 8 GiB */
+  uint32_t merge_group_size;   /* 1 = singleton groups; n>1 merges n
+                                  consecutive datasets per group */
+  uint32_t threads;            /* 0 = all host cores */
+  int32_t drop_abandoned;      /* 1 = no recovery map */
+  uint32_t reserved;
+} zg_build_config;
+
+/* Fills cfg with the reference CLI defaults (src/bin/zorindex.rs:55-159). */
+void zg_build_config_default(zg_build_config *cfg);
+
+/* power_merge::build (storage.rs:500-882) with the naive planner and
+ * ColumnOrder::Width: reads a file-of-files ("name<TAB>path" or "path" lines,
+ * src/lib.rs read_fof) and writes manifest.json, bank files and
+ * abandoned.zormap.zst into out_dir (which must not exist). */
+int zg_build_command(const zg_build_config *cfg, const char *fof,
+                     const char *out_dir, int err_fd);
+
+typedef struct zg_builder zg_builder;
+int zg_builder_create(const zg_build_config *cfg, zg_builder **out);
+/* Adds one dataset: records are seq[rec_off[i] .. rec_off[i+1]). The dataset
+ * is extracted, deduplicated and peeled immediately; only its cells are kept. */
+int zg_builder_add_color(zg_builder *b, const char *name, const uint8_t *seq,
+                         const uint64_t *rec_off, uint64_t n_records);
+/* Packs banks and returns an index usable without touching disk
+ * (zg_index_load still uploads it). The builder is consumed. */
+int zg_builder_finish(zg_builder *b, zg_index **out);
+/* Writes the built index to out_dir in the reference format. */
+int zg_index_write(const zg_index *idx, const char *out_dir);
+int zg_builder_destroy(zg_builder *b);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident batches (persistent kernel, HIP graph, plain path)         */
+/* ------------------------------------------------------------------------ */
+typedef struct zg_batch zg_batch;
+
+typedef struct zg_tuning {
+  uint32_t span;           /* seed positions per work unit (multiple of 64) */
+  uint32_t tiles_per_wave; /* row tiles (64 lanes x field bytes) per wave */
+  uint32_t prefetch;       /* seeds whose rows are in flight per wave: 1,2,4 */
+  uint32_t waves_per_cu;   /* persistent waves per compute unit */
+} zg_tuning;
+
+typedef struct zg_batch_stats {
+  uint64_t n_reads;
+  uint64_t bases;
+  uint64_t seeds;            /* seed occurrences over all reads */
+  uint64_t lookups;          /* seed x bank probes executed */
+  uint64_t algorithmic_bytes;/* bytes the lookups must read, per full run */
+  double last_lookup_ms;     /* HIP-event time of the last zg_batch_run lookups */
+  double last_seed_ms;       /* HIP-event time of packing + seed extraction */
+  double last_total_ms;      /* HIP-event time of the whole last run */
+  zg_tuning tuning;
+} zg_batch_stats;
+
+/* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base
+ * mask and extracts seeds on the device. */
+int zg_batch_create(zg_index *idx, const uint8_t *seq,
+                    const uint64_t *read_off, uint64_t n_reads, uint32_t flags,
+                    zg_batch **out);
+/* Runs the batched persistent lookup over every resident bank (replayed from
+ * a captured HIP graph after the first call). */
+int zg_batch_run(zg_index *idx, zg_batch *batch);
+/* The plain baseline: one lookup launch per read and bank, reads
+ * [first, first + count). */
+int zg_batch_run_plain(zg_index *idx, zg_batch *batch, uint64_t first,
+                       uint64_t count);
+/* Re-extracts seeds (pack + seed kernels) on the device; timed separately. */
+int zg_batch_reseed(zg_index *idx, zg_batch *batch);
+int zg_batch_results(zg_batch *batch, uint64_t *totals, uint32_t *counts,
+                     uint64_t *aggregate);
+int zg_batch_get_stats(const zg_batch *batch, zg_batch_stats *out);
+int zg_batch_destroy(zg_batch *batch);
+
+int zg_index_set_tuning(zg_index *idx, const zg_tuning *tuning);
+int zg_index_get_tuning(const zg_index *idx, zg_tuning *out);
+/* Times candidate tunings on the batch and keeps the fastest. */
+int zg_autotune(zg_index *idx, zg_batch *batch, zg_tuning *chosen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

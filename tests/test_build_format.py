@@ -1,0 +1,249 @@
+"""CPU tests of the product's host side: the C ABI exports, index
+construction in the reference format, manifest validation and info. None of
+these launch a kernel."""
+import ctypes as C
+import json
+import os
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import zindex_ref as zr
+from zindex_ref import ROOT, GOLDEN, oracle, vectors
+
+zindex_amd = pytest.importorskip("zindex_amd")
+from zindex_amd import synth  # noqa: E402
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "zindex_gpu.h").read_text()
+    names = set(re.findall(r"^\s*(?:[\w\s\*]+?)\b(zg_\w+)\s*\(", header, re.M))
+    assert len(names) >= 25
+    lib = C.CDLL(str(ROOT / "zindex_amd" / "libzindex_amd.so"))
+    for name in sorted(names):
+        assert hasattr(lib, name), name
+    assert b"gfx950" in zindex_amd._lib.lib().zg_version()
+
+
+def _random_genome(rng, n):
+    return synth.random_genome(rng, n).tobytes()
+
+
+def _mutate(rng, seq, rate):
+    return synth.substitute(rng, np.frombuffer(seq, np.uint8), rate).tobytes()
+
+
+@pytest.fixture(scope="module")
+def small_colors():
+    rng = np.random.default_rng(7)
+    root = _random_genome(rng, 30000)
+    colors = [root] + [_mutate(rng, root, r) for r in (0.01, 0.05, 0.1, 0.2)]
+    colors.append(_random_genome(rng, 20000) + b"N" + _random_genome(rng, 5000))
+    return colors
+
+
+def _build(colors, **cfg):
+    b = zindex_amd.Builder(**cfg)
+    for i, seq in enumerate(colors):
+        b.add_color(f"c{i}", [seq])
+    return b.finish()
+
+
+@pytest.mark.parametrize("bits,hashes,group", [(8, 4, 1), (1, 3, 1), (2, 2, 1), (4, 4, 2),
+                                               (16, 3, 1), (32, 8, 3), (8, 4, 3)])
+def test_build_round_trip_no_false_negatives(tmp_path, small_colors, bits, hashes, group):
+    idx = _build(small_colors, fingerprint_bits=bits, hashes=hashes, merge_group_size=group,
+                 slot_scale=1.0)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["fingerprint_bits"] == bits and man["hashes"] == hashes
+    assert len(man["colors"]) == len(small_colors)
+    for i, payload in enumerate(disk["banks"]):
+        assert np.array_equal(payload, idx.bank_payload(i))
+    # manifest passes the product's own validation when reopened
+    with zindex_amd.Index.open(str(out)) as reopened:
+        assert reopened.meta["n_colors"] == len(small_colors)
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        keys = np.unique(zr.scan_kmers(seq, 31))
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:3000]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+
+
+def test_false_positive_rate_within_bound(small_colors):
+    bits = 4
+    idx = _build(small_colors, fingerprint_bits=bits, hashes=4)
+    man = json.loads(json.dumps(idx_manifest(idx)))
+    payloads = [idx.bank_payload(i) for i in range(len(man["banks"]))]
+    o = zr.OracleIndex(man, payloads, None)
+    present = set()
+    for seq in small_colors:
+        present.update(zr.scan_kmers(seq, 31).tolist())
+    rng = np.random.default_rng(11)
+    probes = [int(x) for x in rng.integers(0, 1 << 62, 20000, dtype=np.uint64)
+              if int(x) not in present]
+    scores = np.zeros(len(small_colors), np.uint64)
+    for key in probes:
+        o.lookup(key, scores)
+    n = len(probes) * len(small_colors)
+    p = 2.0 ** -bits
+    hits = int(scores.sum())
+    # option_coverage_cli.rs:247-290: within 8 sigma of the binomial mean
+    assert abs(hits - n * p) <= 8 * (n * p * (1 - p)) ** 0.5
+
+
+def idx_manifest(idx):
+    """Manifest of an in-memory index, via a temporary write."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        out = Path(d) / "i"
+        idx.write(str(out))
+        return json.loads((out / "manifest.json").read_text())
+
+
+def test_high_load_abandons_into_recovery_and_stays_exact(tmp_path):
+    rng = np.random.default_rng(5)
+    colors = [_random_genome(rng, 60000) for _ in range(3)]
+    # slot_scale 0.5 over-fills the filter: peeling must abandon keys
+    idx = _build(colors, fingerprint_bits=8, hashes=4, slot_scale=0.5)
+    out = tmp_path / "hl"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["abandoned_hashes"] > 0 and disk["recovery"] is not None
+    assert sum(g["abandoned_keys"] for b in man["banks"] for g in b["groups"]) > 0
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(colors):
+        scores = np.zeros(len(colors), np.uint64)
+        keys = np.unique(zr.scan_kmers(seq, 31))
+        for key in keys:
+            o.lookup(int(key), scores)
+        assert int(scores[c]) == len(keys)
+    # without recovery, stored keys go missing
+    o2 = zr.OracleIndex(man, disk["banks"], disk["recovery"], recover=False)
+    scores = np.zeros(len(colors), np.uint64)
+    keys = np.unique(zr.scan_kmers(colors[0], 31))
+    for key in keys:
+        o2.lookup(int(key), scores)
+    assert int(scores[0]) < len(keys)
+
+
+@pytest.fixture(scope="module")
+def readme_index(tmp_path_factory):
+    d = tmp_path_factory.mktemp("readme")
+    fof = d / "three.fof"
+    names = list(vectors()["readme_bacteria"]["unique_kmers_k31"])
+    fof.write_text("".join(f"{n}\t{GOLDEN / 'bacteria' / 'genomes' / (n + '.fna.gz')}\n"
+                           for n in names))
+    out = d / "idx"
+    zindex_amd.build(str(fof), str(out), k=31)
+    return out
+
+
+def test_readme_unique_kmers_known_answers(readme_index):
+    man = json.loads((readme_index / "manifest.json").read_text())
+    expected = vectors()["readme_bacteria"]["unique_kmers_k31"]
+    assert {c["name"]: c["unique_kmers"] for c in man["colors"]} == expected
+
+
+def test_readme_query_known_answer_on_oracle(readme_index):
+    v = vectors()["readme_bacteria"]
+    o = zr.OracleIndex.from_dir(readme_index)
+    recs = zr.read_fasta_records(GOLDEN / "bacteria" / "query.fa")
+    buf, off = zindex_amd.pack_reads([s for _, s in recs])
+    totals, counts = o.query_reads(buf, off)
+    names = [c["name"] for c in o.manifest["colors"]]
+    src = names.index(v["query_fa_source"])
+    assert int(totals.sum()) == v["query_fa_total"]
+    assert int(counts[:, src].sum()) == v["query_fa_matches"]
+
+
+def test_info_command_runs_on_cpu(readme_index, tmp_path):
+    out = tmp_path / "info.txt"
+    fd = os.open(out, os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+    try:
+        zindex_amd.info_command(str(readme_index), out_fd=fd)
+    finally:
+        os.close(fd)
+    text = out.read_text()
+    assert "indexed_documents\t3\n" in text and "fingerprint_bits\t8\n" in text
+    assert text.count("bank-") >= 1
+
+
+def _tamper(src: Path, dst: Path, edit):
+    shutil.copytree(src, dst)
+    man = json.loads((dst / "manifest.json").read_text())
+    edit(man)
+    (dst / "manifest.json").write_text(json.dumps(man))
+
+
+@pytest.mark.parametrize("name,edit", [
+    ("version", lambda m: m.update(version=11)),
+    ("hashes", lambda m: m.update(hashes=1)),
+    ("bits", lambda m: m.update(fingerprint_bits=3)),
+    ("segment", lambda m: m.update(segment_length=None)),
+    ("width", lambda m: m["banks"][0]["groups"][0].update(width=9)),
+    ("offset", lambda m: m["banks"][0]["groups"][1].update(offset=0)),
+    ("member", lambda m: m["banks"][0]["groups"][0].update(members=[99])),
+    ("dup", lambda m: m["banks"][0]["groups"][1].update(members=[0])),
+    ("file", lambda m: m["banks"][0].update(file="bank-3.zorbank.zst")),
+    ("slots", lambda m: m["banks"][0].update(slots=3)),
+    ("recovery", lambda m: m.update(abandoned_file=None)),
+    ("abandoned_keys", lambda m: m["banks"][0]["groups"][0].pop("abandoned_keys")),
+    ("missing", lambda m: m.pop("seed")),
+    ("backend", lambda m: m.update(backend="xor")),
+    ("minimizer", lambda m: m.update(index_mode=1, k=48)),
+])
+def test_manifest_validation_rejects(readme_index, tmp_path, name, edit):
+    bad = tmp_path / name
+    _tamper(readme_index, bad, edit)
+    with pytest.raises(zindex_amd.ZindexError) as e:
+        zindex_amd.Index.open(str(bad))
+    assert e.value.code == -3
+
+
+def test_truncated_bank_is_rejected(readme_index, tmp_path):
+    bad = tmp_path / "trunc"
+    shutil.copytree(readme_index, bad)
+    bank = next(bad.glob("bank-*.zorbank.zst"))
+    data = bank.read_bytes()
+    bank.write_bytes(data[: len(data) // 2])
+    fd = os.open(os.devnull, os.O_WRONLY)
+    try:
+        with pytest.raises(zindex_amd.ZindexError):
+            zindex_amd.info_command(str(bad), out_fd=fd)
+    finally:
+        os.close(fd)
+
+
+def test_cli_build_and_info(tmp_path):
+    cli = ROOT / "zindex_amd" / "zorindex"
+    fof = tmp_path / "f.fof"
+    rng = np.random.default_rng(2)
+    paths = []
+    for i in range(3):
+        p = tmp_path / f"g{i}.fa"
+        p.write_bytes(b">r\n" + _random_genome(rng, 5000) + b"\n")
+        paths.append(p)
+    fof.write_text("".join(f"{p}\n" for p in paths))
+    out = tmp_path / "cli_idx"
+    r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(out), "-b", "16", "-H", "3"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    man = json.loads((out / "manifest.json").read_text())
+    assert [c["name"] for c in man["colors"]] == ["g0", "g1", "g2"]
+    r = subprocess.run([str(cli), "info", "-i", str(out)], capture_output=True, text=True)
+    assert r.returncode == 0 and "hashes\t3" in r.stdout
+    r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "exists" in r.stderr

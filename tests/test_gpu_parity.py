@@ -1,0 +1,292 @@
+"""GPU parity: the HIP query path (through the C ABI) against the CPU oracle,
+bit-exact on per-read totals, per-read per-colour counts and aggregates."""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import zindex_ref as zr
+from zindex_ref import GOLDEN, ROOT
+
+zindex_amd = pytest.importorskip("zindex_amd")
+from zindex_amd import synth  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def make_colors(seed, n_colors, length, rates=(0.01, 0.2)):
+    rng = np.random.default_rng(seed)
+    root = synth.random_genome(rng, length)
+    colors = [root]
+    for i in range(1, n_colors):
+        colors.append(synth.substitute(rng, root, float(rng.uniform(*rates))))
+    return [c.tobytes() for c in colors]
+
+
+def make_reads(seed, colors, n, lo=20, hi=1500, error=0.02):
+    rng = np.random.default_rng(seed)
+    arrs = [np.frombuffer(c, np.uint8) for c in colors]
+    lengths = rng.integers(lo, hi, n)
+    buf, off, _ = synth.sample_reads(rng, arrs, n, lengths, error)
+    reads = [buf[off[i]:off[i + 1]].tobytes() for i in range(n)]
+    # edge cases: empty, shorter than k, N runs, lower case, all-N
+    reads[0] = b""
+    reads[1] = reads[1][:10]
+    if len(reads[2]) > 100:
+        r = bytearray(reads[2])
+        r[40:45] = b"NNNNN"
+        reads[2] = bytes(r)
+    reads[3] = reads[3].lower()
+    reads[4] = b"N" * 50
+    return reads
+
+
+def build(colors, **cfg):
+    b = zindex_amd.Builder(**cfg)
+    for i, seq in enumerate(colors):
+        b.add_color(f"color{i}", [seq])
+    return b.finish()
+
+
+def oracle_for(idx, recover=True):
+    with tempfile.TemporaryDirectory() as d:
+        out = Path(d) / "idx"
+        idx.write(str(out))
+        disk = zr.read_index_dir(out)
+    return zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"], recover)
+
+
+def check_parity(idx, orc, reads, recover=True, label=""):
+    buf, off = zindex_amd.pack_reads(reads)
+    say(f"parity {label}: {len(reads)} reads, {int(off[-1])} bases")
+    exp_totals, exp_counts = orc.query_reads(buf, off, threads=zr.cpu_count())
+    totals, counts, agg = idx.query_reads(buf, off, records=True, recover=recover)
+    assert np.array_equal(totals, exp_totals), label
+    bad = np.argwhere(counts.astype(np.uint64) != exp_counts)
+    assert len(bad) == 0, f"{label}: {len(bad)} mismatches, first {bad[:5].tolist()}"
+    assert np.array_equal(agg, exp_counts.sum(axis=0)), label
+    t2, _, agg2 = idx.query_reads(buf, off, records=False, recover=recover)
+    assert np.array_equal(t2, exp_totals) and np.array_equal(agg2, exp_counts.sum(axis=0)), label
+    return exp_counts
+
+
+@pytest.fixture(scope="module")
+def base_case():
+    colors = make_colors(1, 10, 40000)
+    reads = make_reads(2, colors, 300)
+    idx = build(colors, fingerprint_bits=8, hashes=4)
+    idx.load()
+    return colors, reads, idx, oracle_for(idx)
+
+
+def test_kmers_b8_h4_parity(base_case):
+    colors, reads, idx, orc = base_case
+    counts = check_parity(idx, orc, reads, label="b8h4")
+    assert counts.sum() > 0
+
+
+@pytest.mark.parametrize("bits,hashes", [(1, 3), (2, 4), (4, 2), (16, 3), (32, 8), (8, 2),
+                                         (4, 8), (16, 4)])
+def test_width_and_hash_sweep(bits, hashes):
+    colors = make_colors(10 + bits + hashes, 6, 25000)
+    reads = make_reads(20 + bits, colors, 120)
+    idx = build(colors, fingerprint_bits=bits, hashes=hashes)
+    idx.load()
+    check_parity(idx, oracle_for(idx), reads, label=f"b{bits}h{hashes}")
+
+
+@pytest.mark.parametrize("group,bits", [(3, 8), (5, 4), (2, 16), (7, 1)])
+def test_merged_groups(group, bits):
+    colors = make_colors(30 + group, 9, 20000, rates=(0.0, 0.05))
+    reads = make_reads(40 + group, colors, 150)
+    idx = build(colors, fingerprint_bits=bits, hashes=4, merge_group_size=group)
+    idx.load()
+    check_parity(idx, oracle_for(idx), reads, label=f"merged g{group} b{bits}")
+
+
+@pytest.mark.parametrize("bits,group", [(8, 1), (8, 3), (2, 1)])
+def test_recovery_high_load(bits, group):
+    colors = make_colors(50 + bits, 6, 30000)
+    reads = make_reads(60, colors, 200)
+    idx = build(colors, fingerprint_bits=bits, hashes=4, slot_scale=0.5, merge_group_size=group)
+    assert idx.meta["abandoned_hashes"] > 0
+    idx.load(recover=True)
+    check_parity(idx, oracle_for(idx, True), reads, True, label=f"recovery b{bits} g{group}")
+    idx2 = build(colors, fingerprint_bits=bits, hashes=4, slot_scale=0.5, merge_group_size=group)
+    idx2.load(recover=False)
+    check_parity(idx2, oracle_for(idx2, False), reads, False, label=f"no-recovery b{bits}")
+
+
+def test_many_tiles_and_tile_grouping():
+    # 1100 singleton colours at b=8: two 1024-column tiles per row
+    rng = np.random.default_rng(70)
+    root = synth.random_genome(rng, 3000)
+    colors = [synth.substitute(rng, root, 0.02).tobytes() for _ in range(1100)]
+    reads = make_reads(71, colors, 60, lo=40, hi=600)
+    idx = build(colors, fingerprint_bits=8, hashes=4)
+    idx.load()
+    orc = oracle_for(idx)
+    for tiles in (1, 2):
+        for prefetch in (1, 4):
+            idx.tuning = {"span": 1024, "tiles_per_wave": tiles, "prefetch": prefetch,
+                          "waves_per_cu": 0}
+            check_parity(idx, orc, reads, label=f"tiles{tiles} d{prefetch}")
+
+
+def test_minimizer_positions_and_counts():
+    colors = make_colors(80, 5, 30000)
+    reads = make_reads(81, colors, 120, lo=30, hi=3000, error=0.05)
+    idx = build(colors, fingerprint_bits=16, hashes=3, index_mode=1, k=49, minimizer_size=31)
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, pos = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp_pos, exp_keys = zr.scan_ocm(r, 49, 31, 69)
+        a, b = int(key_off[i]), int(key_off[i + 1])
+        assert pos[a:b].tolist() == exp_pos.tolist(), i
+        assert keys[a:b].tolist() == exp_keys.tolist(), i
+    check_parity(idx, oracle_for(idx), reads, label="ocm k49 m31 b16 h3")
+
+
+@pytest.mark.parametrize("k,m", [(1, 1), (7, 7), (19, 12), (63, 21), (255, 31), (31, 4)])
+def test_minimizer_parameter_edges(k, m):
+    colors = make_colors(90 + k, 3, 8000)
+    reads = make_reads(91 + m, colors, 40, lo=1, hi=800)
+    idx = build(colors, fingerprint_bits=8, hashes=3, index_mode=1, k=k, minimizer_size=m)
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, pos = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp_pos, exp_keys = zr.scan_ocm(r, k, m, 69)
+        a, b = int(key_off[i]), int(key_off[i + 1])
+        assert pos[a:b].tolist() == exp_pos.tolist(), (k, m, i)
+        assert keys[a:b].tolist() == exp_keys.tolist(), (k, m, i)
+    check_parity(idx, oracle_for(idx), reads, label=f"ocm k{k} m{m}")
+
+
+def test_kmer_seeds_match_oracle_scan(base_case):
+    colors, reads, idx, orc = base_case
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, pos = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.scan_kmers(r, 31)
+        a, b = int(key_off[i]), int(key_off[i + 1])
+        assert keys[a:b].tolist() == exp.tolist(), i
+
+
+def test_plain_path_graph_replays_and_autotune_agree(base_case):
+    colors, reads, idx, orc = base_case
+    buf, off = zindex_amd.pack_reads(reads)
+    exp_totals, exp_counts = orc.query_reads(buf, off)
+    with idx.batch(buf, off, records=True) as b:
+        for rep in range(3):  # graph captured on the first run, replayed after
+            b.run()
+            totals, counts, agg = b.results()
+            assert np.array_equal(counts.astype(np.uint64), exp_counts), rep
+            _ = np.zeros(10).sum()  # host work between replays
+        b.run_plain()
+        totals, counts, agg = b.results()
+        assert np.array_equal(counts.astype(np.uint64), exp_counts), "plain"
+        chosen = b.autotune()
+        b.run()
+        totals, counts, agg = b.results()
+        assert np.array_equal(counts.astype(np.uint64), exp_counts), f"autotuned {chosen}"
+        st = b.stats()
+        assert st["seeds"] == int(exp_totals.sum()) and st["last_lookup_ms"] > 0
+
+
+def test_aggregate_batch_and_stats(base_case):
+    colors, reads, idx, orc = base_case
+    buf, off = zindex_amd.pack_reads(reads)
+    _, exp_counts = orc.query_reads(buf, off)
+    with idx.batch(buf, off, records=False) as b:
+        b.run()
+        b.run()
+        totals, _, agg = b.results()
+        assert np.array_equal(agg, exp_counts.sum(axis=0))
+
+
+@pytest.fixture(scope="module")
+def bacteria_index(tmp_path_factory):
+    d = tmp_path_factory.mktemp("bacteria")
+    out = d / "idx"
+    say("building the 10-genome bacteria index on the host")
+    zindex_amd.build(str(GOLDEN / "bacteria" / "samples.fof"), str(out), k=31)
+    return out
+
+
+def _run_cli(args):
+    cli = ROOT / "zindex_amd" / "zorindex"
+    r = subprocess.run([str(cli)] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_bacteria_readme_and_demo_known_answers(bacteria_index):
+    v = zr.vectors()["readme_bacteria"]
+    orc = zr.OracleIndex.from_dir(bacteria_index)
+    colors = [(c["name"], c["unique_kmers"]) for c in orc.manifest["colors"]]
+    # aggregate query.fa: byte-identical to the oracle's restated output
+    out = _run_cli(["query", "-i", str(bacteria_index), "-q", str(GOLDEN / "bacteria" / "query.fa")])
+    recs = zr.read_fasta_records(GOLDEN / "bacteria" / "query.fa")
+    buf, off = zindex_amd.pack_reads([s for _, s in recs])
+    totals, counts = orc.query_reads(buf, off)
+    assert out == zr.aggregate_text(colors, counts.sum(axis=0), int(totals.sum()), 0.0)
+    src = [c[0] for c in colors].index(v["query_fa_source"])
+    assert f"{v['query_fa_source']}\t{v['query_fa_matches']}\t1.00000000\t" in out
+    # records with --min-ratio 0.8 (demo.sh:30-44): every positive matches its source fully
+    qpath = GOLDEN / "bacteria" / "queries.fa"
+    out = _run_cli(["query", "-i", str(bacteria_index), "-q", str(qpath), "--records",
+                    "--min-ratio", "0.8"])
+    recs = zr.read_fasta_records(qpath)
+    buf, off = zindex_amd.pack_reads([s for _, s in recs])
+    totals, counts = orc.query_reads(buf, off)
+    names = [n for n, _ in recs]
+    assert out == zr.records_text(names, totals, counts, colors, 0.8, "tsv")
+    truth = [l.split("\t") for l in (GOLDEN / "bacteria" / "source-truth.tsv").read_text().splitlines()[1:]]
+    for qid, doc, *_ in truth:
+        i = names.index(qid)
+        assert int(counts[i, int(doc)]) == int(totals[i]) > 0, qid
+    for first in v["records_first_two"]:
+        assert "\t".join([first[0], str(first[1]), colors[first[1]][0], str(first[2]),
+                          str(first[3]), "1.00000000"]) in out
+    # compact format and ratio 0
+    out = _run_cli(["query", "-i", str(bacteria_index), "-q", str(qpath), "--records",
+                    "-F", "compact"])
+    assert out == zr.records_text(names, totals, counts, colors, 0.0, "compact")
+
+
+def test_bacteria_minimizer_demo(tmp_path):
+    out = tmp_path / "min_idx"
+    zindex_amd.build(str(GOLDEN / "bacteria" / "samples.fof"), str(out), index_mode=1, k=49,
+                     minimizer_size=31)
+    orc = zr.OracleIndex.from_dir(out)
+    qpath = GOLDEN / "bacteria" / "queries.fa"
+    recs = zr.read_fasta_records(qpath)
+    names = [n for n, _ in recs]
+    buf, off = zindex_amd.pack_reads([s for _, s in recs])
+    totals, counts = orc.query_reads(buf, off)
+    text = _run_cli(["query", "-i", str(out), "-q", str(qpath), "--records", "--min-ratio", "0.8"])
+    colors = [(c["name"], c["unique_kmers"]) for c in orc.manifest["colors"]]
+    assert text == zr.records_text(names, totals, counts, colors, 0.8, "tsv")
+    truth = [l.split("\t") for l in (GOLDEN / "bacteria" / "source-truth.tsv").read_text().splitlines()[1:]]
+    for qid, doc, *_ in truth:
+        i = names.index(qid)
+        assert int(counts[i, int(doc)]) == int(totals[i]) > 0, qid
+
+
+def test_hbm_budget_rejects_oversized_index(base_case):
+    colors, reads, idx, orc = base_case
+    small = build(colors[:2], fingerprint_bits=8, hashes=4)
+    with pytest.raises(zindex_amd.ZindexError) as e:
+        small.load(hbm_budget=1024)
+    assert e.value.code == -7

@@ -1,0 +1,1199 @@
+// zg_api.hip -- the C ABI of libzindex_amd.so: index handles, HBM residency,
+// device-resident query batches and the CLI-equivalent commands.
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <thread>
+
+#include "zg_fastx.h"
+#include "zg_hash.h"
+#include "zg_internal.h"
+
+namespace zg {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+int translate_exception() {
+  try {
+    throw;
+  } catch (const Error &e) {
+    g_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_error = "out of host memory";
+    return ZG_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return ZG_ERR_INVALID;
+  } catch (...) {
+    g_error = "unknown error";
+    return ZG_ERR_INVALID;
+  }
+}
+
+void check_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  int code = (e == hipErrorOutOfMemory) ? ZG_ERR_NOMEM : ZG_ERR_DEVICE;
+  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void DevBuf::alloc(size_t n) {
+  reset();
+  if (!n) n = 8;
+  check_hip(hipMalloc(&p, n), "hipMalloc");
+  bytes = n;
+}
+
+template <class T>
+static void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+  b.alloc(v.size() * sizeof(T));
+  if (!v.empty())
+    check_hip(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync");
+}
+
+std::vector<uint64_t> ocm_inner_table(uint32_t s, uint64_t seed) {
+  std::vector<uint64_t> t(256, 0);
+  for (uint64_t x = 0; x < (1ULL << (2 * s)); x++)
+    t[x] = mixsplit(canonical_encoded(x, (int)s), seed ^ kOcmInnerSeed);
+  return t;
+}
+
+static unsigned host_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  return n ? n : 1;
+}
+
+// Query support of this library for an index (the modes BASELINE targets).
+static void check_query_support(const Manifest &m) {
+  if (m.version < 2) fail(ZG_ERR_UNSUPPORTED, "version-1 indexes are not supported by the GPU query");
+  if (m.has_seed_pattern || m.has_randstrobemers)
+    fail(ZG_ERR_UNSUPPORTED, "spaced seeds and randstrobemers are not supported by the GPU query");
+  if (m.homopolymer_compression)
+    fail(ZG_ERR_UNSUPPORTED, "homopolymer compression is not supported by the GPU query");
+  if (m.index_mode == 0) {
+    if (m.k > 32) fail(ZG_ERR_UNSUPPORTED, "k-mer indexes with k > 32 are not supported by the GPU query");
+  } else if (m.index_mode == 1) {
+    if (m.minimizer_scheme != 1)
+      fail(ZG_ERR_UNSUPPORTED, "only open-closed mod-minimizer indexes are supported by the GPU query");
+  } else {
+    fail(ZG_ERR_UNSUPPORTED, "modimizer and findere indexes are not supported by the GPU query");
+  }
+  for (auto &b : m.banks)
+    if (!b.fingerprint_first)
+      fail(ZG_ERR_UNSUPPORTED, "legacy interleaved banks are not supported by the GPU query");
+}
+
+void upload_index(zg_index *idx, uint64_t budget) {
+  const Manifest &m = idx->m;
+  check_query_support(m);
+  int dev = 0;
+  check_hip(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t prop;
+  check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  idx->device = dev;
+  idx->cu_count = (uint32_t)prop.multiProcessorCount;
+  if (!idx->stream) check_hip(hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const uint32_t fb = (uint32_t)m.fingerprint_bits;
+  const uint64_t pad = 4096;
+  uint64_t need = 0;
+  for (size_t i = 0; i < m.banks.size(); i++) need += idx->payload[i].size() + pad;
+  if (idx->recover && idx->have_rec) need += idx->rec.size() * 24 + idx->rec.origins.size() * 4;
+  if (budget && need > budget)
+    fail(ZG_ERR_BUDGET, "index needs " + std::to_string(need) + " bytes of HBM, budget is " +
+                            std::to_string(budget));
+  hipStream_t s = idx->stream;
+  idx->dbanks.clear();
+  idx->dbanks.resize(m.banks.size());
+  const uint32_t n_colors = (uint32_t)m.colors.size();
+  for (size_t bi = 0; bi < m.banks.size(); bi++) {
+    const BankMeta &meta = m.banks[bi];
+    DeviceBank &db = idx->dbanks[bi];
+    const uint32_t G = (uint32_t)meta.groups.size();
+    uint32_t first_merged = G;
+    for (uint32_t g = 0; g < G; g++)
+      if (meta.groups[g].members.size() > 1) {
+        first_merged = g;
+        break;
+      }
+    for (uint32_t g = first_merged; g < G; g++)
+      if (meta.groups[g].members.size() == 1)
+        fail(ZG_ERR_UNSUPPORTED, "bank columns are not ordered by width");
+    std::vector<uint32_t> col_color(G, kNone), mg_off{0}, mg_members;
+    std::vector<uint64_t> mg_mask_bit;
+    std::vector<uint32_t> color_col(n_colors, kNone), color_bit(n_colors, 0);
+    uint64_t mask_bit = (uint64_t)G * fb;
+    for (uint32_t g = 0; g < G; g++) {
+      const GroupMeta &gm = meta.groups[g];
+      for (size_t j = 0; j < gm.members.size(); j++) {
+        color_col[gm.members[j]] = g;
+        color_bit[gm.members[j]] = (uint32_t)j;
+      }
+      if (g < first_merged) {
+        col_color[g] = gm.members[0];
+      } else {
+        mg_mask_bit.push_back(mask_bit);
+        mask_bit += gm.members.size();
+        for (uint32_t x : gm.members) mg_members.push_back(x);
+        mg_off.push_back((uint32_t)mg_members.size());
+      }
+    }
+    const std::vector<uint8_t> &payload = idx->payload[bi];
+    db.data.alloc(payload.size() + pad);
+    check_hip(hipMemcpyAsync(db.data.p, payload.data(), payload.size(), hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync");
+    check_hip(hipMemsetAsync((uint8_t *)db.data.p + payload.size(), 0, pad, s), "hipMemsetAsync");
+    upload(db.col_color, col_color, s);
+    upload(db.mg_off, mg_off, s);
+    upload(db.mg_members, mg_members, s);
+    upload(db.mg_mask_bit, mg_mask_bit, s);
+    upload(db.color_col, color_col, s);
+    upload(db.color_bit, color_bit, s);
+    BankDev &v = db.view;
+    v.data = db.data.as<uint8_t>();
+    v.row_bytes = meta.row_bytes();
+    v.plane_bytes = ((uint64_t)G * fb + 7) / 8;
+    v.groups = G;
+    v.first_merged = first_merged;
+    v.hashes = (uint32_t)m.hashes;
+    v.partition = (uint32_t)(meta.routed ? meta.partition : 0);
+    v.partitions = (uint32_t)(meta.routed ? meta.partitions : 1);
+    uint32_t tile_fields = 64 * fields_per_lane(fb);
+    v.n_tiles = (G + tile_fields - 1) / tile_fields;
+    v.geo = segment_geometry(meta.slots, (uint32_t)m.hashes,
+                             m.has_segment_length ? m.segment_length : 0);
+    v.col_color = db.col_color.as<uint32_t>();
+    v.mg_off = db.mg_off.as<uint32_t>();
+    v.mg_members = db.mg_members.as<uint32_t>();
+    v.mg_mask_bit = db.mg_mask_bit.as<uint64_t>();
+    v.color_col = db.color_col.as<uint32_t>();
+    v.color_bit = db.color_bit.as<uint32_t>();
+  }
+  // recovery directory (the query-side OriginMap, abandoned.rs:29-206)
+  idx->rec_view = RecDev{};
+  if (idx->recover && idx->have_rec && idx->rec.size()) {
+    const RecoveryMap &r = idx->rec;
+    uint64_t n = r.size();
+    uint64_t cap = 4;
+    while (cap < (4 * n + 2) / 3) cap <<= 1;
+    uint32_t bits = 0;
+    while ((1ULL << bits) < cap) bits++;
+    ensure(bits <= 32, "recovery directory too large", ZG_ERR_UNSUPPORTED);
+    std::vector<uint64_t> dir(cap, (uint64_t)kNone);
+    std::vector<uint32_t> post;
+    post.reserve(n + r.origins.size());
+    for (uint64_t i = 0; i < n; i++) {
+      uint32_t pi = (uint32_t)post.size();
+      post.push_back((uint32_t)(r.off[i + 1] - r.off[i]));
+      for (uint64_t j = r.off[i]; j < r.off[i + 1]; j++) post.push_back(r.origins[j]);
+      uint64_t slot = r.hashes[i] >> (32 - bits);
+      while ((uint32_t)dir[slot] != kNone) slot = (slot + 1) & (cap - 1);
+      dir[slot] = ((uint64_t)r.hashes[i] << 32) | pi;
+    }
+    upload(idx->rec_dir, dir, s);
+    upload(idx->rec_post, post, s);
+    idx->rec_view.dir = idx->rec_dir.as<uint64_t>();
+    idx->rec_view.post = idx->rec_post.as<uint32_t>();
+    idx->rec_view.cap_log2 = bits;
+    idx->rec_view.enabled = 1;
+  }
+  if (m.index_mode == 1) {
+    OcmParams P = ocm_parameters((int)m.k, (int)m.minimizer_size);
+    upload(idx->inner_table, ocm_inner_table((uint32_t)P.s, m.seed), s);
+  }
+  check_hip(hipStreamSynchronize(s), "upload");
+  idx->on_device = true;
+}
+
+}  // namespace zg
+
+using namespace zg;
+
+zg_index::~zg_index() {
+  dbanks.clear();
+  rec_dir.reset();
+  rec_post.reset();
+  inner_table.reset();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+zg_batch::~zg_batch() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+extern "C" const char *zg_last_error(void) { return g_error.c_str(); }
+extern "C" const char *zg_version(void) { return "zindex_amd 0.1.0 (gfx950)"; }
+
+extern "C" int zg_index_open(const char *dir, zg_index **out) {
+  try {
+    ensure(dir && out, "null argument", ZG_ERR_INVALID);
+    auto idx = std::make_unique<zg_index>();
+    idx->m = load_manifest(dir);
+    idx->dir = dir;
+    *out = idx.release();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_close(zg_index *idx) {
+  delete idx;
+  return 0;
+}
+
+extern "C" int zg_index_get_meta(const zg_index *idx, zg_index_meta *o) {
+  try {
+    ensure(idx && o, "null argument", ZG_ERR_INVALID);
+    const Manifest &m = idx->m;
+    memset(o, 0, sizeof *o);
+    o->version = m.version;
+    o->k = (uint32_t)m.k;
+    o->hashes = (uint32_t)m.hashes;
+    o->fingerprint_bits = (uint32_t)m.fingerprint_bits;
+    o->index_mode = (uint32_t)m.index_mode;
+    o->minimizer_size = (uint32_t)m.minimizer_size;
+    o->minimizer_scheme = m.minimizer_scheme;
+    o->n_colors = (uint32_t)m.colors.size();
+    o->n_banks = (uint32_t)m.banks.size();
+    o->n_classes = (uint32_t)m.classes();
+    o->seed = m.seed;
+    o->segment_length = m.has_segment_length ? m.segment_length : 0;
+    o->abandoned_recovery = m.abandoned_recovery;
+    o->abandoned_hashes = m.abandoned_hashes;
+    o->abandoned_origins = m.abandoned_origins;
+    for (auto &b : m.banks) o->payload_bytes += b.payload_bytes();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_color(const zg_index *idx, uint32_t color, const char **name,
+                              uint64_t *unique_kmers) {
+  try {
+    ensure(idx && color < idx->m.colors.size(), "invalid colour", ZG_ERR_INVALID);
+    if (name) *name = idx->m.colors[color].name.c_str();
+    if (unique_kmers) *unique_kmers = idx->m.colors[color].unique_kmers;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_load(zg_index *idx, int recover, uint64_t budget) {
+  try {
+    ensure(idx, "null argument", ZG_ERR_INVALID);
+    check_query_support(idx->m);
+    if (!idx->payload_loaded) {
+      idx->payload.clear();
+      for (auto &b : idx->m.banks)
+        idx->payload.push_back(load_bank_payload(idx->dir, b, (int)host_threads()));
+      idx->payload_loaded = true;
+    }
+    idx->recover = recover && idx->m.abandoned_recovery;
+    if (idx->recover && !idx->have_rec && !idx->dir.empty()) {
+      idx->rec = read_recovery_map(idx->dir, idx->m);
+      idx->have_rec = true;
+    }
+    upload_index(idx, budget);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_bank_payload(const zg_index *idx, uint32_t bank, const uint8_t **data,
+                                     uint64_t *bytes) {
+  try {
+    ensure(idx && data && bytes, "null argument", ZG_ERR_INVALID);
+    ensure(idx->payload_loaded && bank < idx->payload.size(), "bank payload not loaded",
+           ZG_ERR_INVALID);
+    *data = idx->payload[bank].data();
+    *bytes = idx->payload[bank].size();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
+  try {
+    ensure(idx && t, "null argument", ZG_ERR_INVALID);
+    ensure(t->span >= 64 && t->span % 64 == 0, "span must be a positive multiple of 64",
+           ZG_ERR_INVALID);
+    ensure(t->tiles_per_wave == 1 || t->tiles_per_wave == 2, "tiles_per_wave must be 1 or 2",
+           ZG_ERR_INVALID);
+    ensure(t->prefetch == 1 || t->prefetch == 2 || t->prefetch == 4,
+           "prefetch must be 1, 2 or 4", ZG_ERR_INVALID);
+    idx->tuning.span = t->span;
+    idx->tuning.tiles_per_wave = t->tiles_per_wave;
+    idx->tuning.prefetch = t->prefetch;
+    idx->tuning.waves_per_cu = t->waves_per_cu;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_get_tuning(const zg_index *idx, zg_tuning *t) {
+  if (!idx || !t) {
+    set_error("null argument");
+    return ZG_ERR_INVALID;
+  }
+  t->span = idx->tuning.span;
+  t->tiles_per_wave = idx->tuning.tiles_per_wave;
+  t->prefetch = idx->tuning.prefetch;
+  t->waves_per_cu = idx->tuning.waves_per_cu;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches
+// ---------------------------------------------------------------------------
+namespace zg {
+
+static uint32_t max_tiles(const zg_index *idx) {
+  uint32_t t = 1;
+  for (auto &b : idx->dbanks) t = std::max(t, b.view.n_tiles);
+  return t;
+}
+
+// Launch shape for the current tuning.
+static void configure(zg_index *idx, zg_batch *b) {
+  const Tuning &t = idx->tuning;
+  KernelConfig c{};
+  c.bits = (uint32_t)idx->m.fingerprint_bits;
+  c.tiles_per_wave = max_tiles(idx) >= 2 ? t.tiles_per_wave : 1;
+  c.prefetch = t.prefetch;
+  int per_cu = lookup_blocks_per_cu(c);
+  if (t.waves_per_cu) per_cu = std::max(1u, std::min((uint32_t)per_cu, (t.waves_per_cu + 3) / 4));
+  c.blocks = idx->cu_count * (uint32_t)per_cu;
+  b->kcfg = c;
+  b->span = t.span;
+}
+
+static void ensure_partials(zg_batch *b) {
+  uint64_t waves = (uint64_t)b->kcfg.blocks * 4;
+  const uint32_t C = (uint32_t)b->idx->m.colors.size();
+  if (!b->records && b->waves < waves) {
+    b->partial.alloc(waves * C * sizeof(uint64_t));
+    b->waves = waves;
+  }
+}
+
+static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
+  LookupArgs a{};
+  a.bank = idx->dbanks[bank].view;
+  a.rec = idx->rec_view;
+  if (b->flags & ZG_QUERY_NO_RECOVERY) a.rec.enabled = 0;
+  a.source = b->explicit_keys ? kSeedKeys : kSeedKmers;
+  a.k = (uint32_t)idx->m.k;
+  a.codes = b->codes.as<uint64_t>();
+  a.invalid = b->invalid.as<uint64_t>();
+  a.keys = b->keys.as<uint64_t>();
+  a.seg_off = b->explicit_keys ? b->key_off.as<uint64_t>() : b->pos_off.as<uint64_t>();
+  a.n_reads = b->n_reads;
+  a.begin = 0;
+  a.end = b->explicit_keys ? b->n_keys : b->positions;
+  uint32_t tiles = a.bank.n_tiles;
+  uint32_t T = b->kcfg.tiles_per_wave;
+  a.n_tg = (tiles + T - 1) / T;
+  a.span = b->span;
+  uint64_t spans = (a.end - a.begin + a.span - 1) / a.span;
+  a.n_units = spans * a.n_tg;
+  ensure(a.n_units < (1ULL << 31), "batch too large for one launch", ZG_ERR_INVALID);
+  a.queue = b->queue.as<uint32_t>() + bank;
+  a.seed = idx->m.seed;
+  a.bits = (uint32_t)idx->m.fingerprint_bits;
+  a.records = b->records;
+  a.counts = b->counts.as<uint32_t>();
+  a.partial = b->partial.as<uint64_t>();
+  a.n_colors = (uint32_t)idx->m.colors.size();
+  return a;
+}
+
+// The run's stream work: zero outputs, one persistent lookup per bank, reduce.
+static void enqueue_run(zg_index *idx, zg_batch *b, bool with_events) {
+  hipStream_t s = idx->stream;
+  const uint32_t C = (uint32_t)idx->m.colors.size();
+  if (with_events) check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+  if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
+  else launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
+  launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
+  launch_fill_u32(b->queue.as<uint32_t>(), idx->dbanks.size(), 0, s);
+  if (with_events) check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+  bool any = b->explicit_keys ? b->n_keys > 0 : b->positions > 0;
+  if (any)
+    for (size_t i = 0; i < idx->dbanks.size(); i++) launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
+  if (with_events) check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
+  if (!b->records) launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
+  else launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
+  if (with_events) check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
+}
+
+static void drop_graph(zg_batch *b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+static void seed_batch(zg_index *idx, zg_batch *b) {
+  hipStream_t s = idx->stream;
+  const Manifest &m = idx->m;
+  check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+  uint64_t cwords = (b->positions + 31) / 32 + 4, iwords = (b->positions + 63) / 64 + 4;
+  launch_fill_u64(b->codes.as<uint64_t>(), cwords, 0, s);
+  launch_fill_u64(b->invalid.as<uint64_t>(), iwords, ~0ULL, s);
+  launch_pack(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+              b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
+  launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
+  if (m.index_mode == 0) {
+    launch_count_kmers(b->invalid.as<uint64_t>(), b->pos_off.as<uint64_t>(), b->n_reads,
+                       b->positions, (uint32_t)m.k, b->totals.as<uint64_t>(), s);
+    check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+    check_hip(hipEventSynchronize(b->ev[1]), "seed");
+  } else {
+    const uint64_t blocks = (b->positions + ocm_block_positions() - 1) / ocm_block_positions();
+    b->flag.alloc(b->positions + 64);
+    b->delta.alloc(b->positions + 64);
+    b->block_counts.alloc((blocks + 1) * sizeof(uint32_t));
+    b->block_off.alloc((blocks + 1) * sizeof(uint64_t));
+    launch_ocm_select(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+                      b->n_reads, b->positions, (uint32_t)m.k, (uint32_t)m.minimizer_size, m.seed,
+                      idx->inner_table.as<uint64_t>(), b->flag.as<uint8_t>(),
+                      b->delta.as<uint8_t>(), b->block_counts.as<uint32_t>(), nullptr, s);
+    launch_fill_u32(b->block_counts.as<uint32_t>() + blocks, 1, 0, s);
+    size_t tb = scan_temp_bytes_u32_to_u64(blocks + 1);
+    b->scan_temp.alloc(tb);
+    launch_exclusive_scan_u32_to_u64(b->block_counts.as<uint32_t>(), b->block_off.as<uint64_t>(),
+                                     blocks + 1, b->scan_temp.p, tb, s);
+    uint64_t total = 0;
+    check_hip(hipMemcpyAsync(&total, b->block_off.as<uint64_t>() + blocks, 8, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "seed");
+    b->n_keys = total;
+    b->keys.alloc((total + 64) * 8);
+    b->key_win.alloc((total + 64) * 8);
+    launch_ocm_compact(b->codes.as<uint64_t>(), b->flag.as<uint8_t>(), b->delta.as<uint8_t>(),
+                       b->block_off.as<uint64_t>(), b->positions, (uint32_t)m.minimizer_size,
+                       b->keys.as<uint64_t>(), b->key_win.as<uint64_t>(), s);
+    b->key_off.alloc((b->n_reads + 1) * 8);
+    launch_segment_bounds(b->key_win.as<uint64_t>(), total, b->pos_off.as<uint64_t>(), b->n_reads,
+                          b->key_off.as<uint64_t>(), s);
+    launch_diff(b->key_off.as<uint64_t>(), b->n_reads, b->totals.as<uint64_t>(), s);
+    check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+    b->h_seg_off.assign(b->n_reads + 1, 0);
+    check_hip(hipMemcpyAsync(b->h_seg_off.data(), b->key_off.p, (b->n_reads + 1) * 8,
+                             hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "seed");
+    b->explicit_keys = true;
+  }
+  float ms = 0;
+  check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "hipEventElapsedTime");
+  b->last_seed_ms = ms;
+}
+
+}  // namespace zg
+
+extern "C" int zg_batch_create(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
+                               uint64_t n_reads, uint32_t flags, zg_batch **out) {
+  try {
+    ensure(idx && out && (n_reads == 0 || read_off), "null argument", ZG_ERR_INVALID);
+    ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
+           ZG_ERR_INVALID);
+    auto b = std::make_unique<zg_batch>();
+    b->idx = idx;
+    b->flags = flags;
+    b->records = (flags & ZG_QUERY_RECORDS) != 0;
+    b->n_reads = n_reads;
+    hipStream_t s = idx->stream;
+    for (auto &e : b->ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+    uint64_t r0 = n_reads ? read_off[0] : 0;
+    b->bases = n_reads ? read_off[n_reads] - r0 : 0;
+    ensure(b->bases == 0 || seq, "null sequence", ZG_ERR_INVALID);
+    b->positions = b->bases + n_reads;
+    std::vector<uint64_t> roff(n_reads + 1), poff(n_reads + 1);
+    for (uint64_t r = 0; r <= n_reads; r++) {
+      roff[r] = read_off[r] - r0;
+      ensure(r == 0 || roff[r] >= roff[r - 1], "read offsets must be non-decreasing",
+             ZG_ERR_INVALID);
+      poff[r] = roff[r] + r;
+    }
+    b->seq.alloc(b->bases + 64);
+    if (b->bases)
+      check_hip(hipMemcpyAsync(b->seq.p, seq + r0, b->bases, hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+    upload(b->read_off, roff, s);
+    upload(b->pos_off, poff, s);
+    b->codes.alloc(((b->positions + 31) / 32 + 4) * 8);
+    b->invalid.alloc(((b->positions + 63) / 64 + 4) * 8);
+    b->totals.alloc((n_reads + 1) * 8);
+    const uint32_t C = (uint32_t)idx->m.colors.size();
+    if (b->records) b->counts.alloc(std::max<uint64_t>(1, n_reads * C) * 4);
+    b->agg.alloc((uint64_t)C * 8 + 8);
+    b->queue.alloc((idx->dbanks.size() + 1) * 4);
+    b->h_seg_off = poff;
+    zg::seed_batch(idx, b.get());
+    zg::configure(idx, b.get());
+    zg::ensure_partials(b.get());
+    *out = b.release();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_reseed(zg_index *idx, zg_batch *b) {
+  try {
+    ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
+    zg::seed_batch(idx, b);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
+  try {
+    ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
+    hipStream_t s = idx->stream;
+    if (!b->graph_exec && !b->graph_failed) {
+      // Capture once; replays reuse the instantiated graph. If any step of the
+      // capture is refused, fall back to direct launches for this batch.
+      bool captured = false;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        bool enqueued = true;
+        try {
+          zg::enqueue_run(idx, b, true);
+        } catch (...) {
+          enqueued = false;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t end = hipStreamEndCapture(s, &g);
+        if (enqueued && end == hipSuccess && g) {
+          b->graph = g;
+          if (hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0) == hipSuccess)
+            captured = true;
+        } else if (g) {
+          (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+      }
+      if (!captured) {
+        zg::drop_graph(b);
+        b->graph_failed = true;
+      }
+    }
+    if (b->graph_exec) check_hip(hipGraphLaunch(b->graph_exec, s), "hipGraphLaunch");
+    else zg::enqueue_run(idx, b, true);
+    check_hip(hipStreamSynchronize(s), "lookup");
+    float ms = 0;
+    check_hip(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]), "hipEventElapsedTime");
+    b->last_lookup_ms = ms;
+    check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]), "hipEventElapsedTime");
+    b->last_total_ms = ms;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, uint64_t count) {
+  try {
+    ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
+    ensure(first <= b->n_reads && count <= b->n_reads - first, "read range out of bounds",
+           ZG_ERR_INVALID);
+    hipStream_t s = idx->stream;
+    const uint32_t C = (uint32_t)idx->m.colors.size();
+    const size_t nb = idx->dbanks.size();
+    zg::DevBuf queues;
+    queues.alloc((count * nb + 1) * 4);
+    check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+    if (b->records) zg::launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
+    else zg::launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
+    zg::launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
+    zg::launch_fill_u32(queues.as<uint32_t>(), count * nb, 0, s);
+    check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+    for (uint64_t r = first; r < first + count; r++) {
+      uint64_t lo = b->h_seg_off[r], hi = b->h_seg_off[r + 1];
+      if (hi <= lo) continue;
+      for (size_t i = 0; i < nb; i++) {
+        zg::LookupArgs a = zg::lookup_args(idx, b, i);
+        a.begin = lo;
+        a.end = hi;
+        uint64_t spans = (hi - lo + a.span - 1) / a.span;
+        a.n_units = spans * a.n_tg;
+        a.queue = queues.as<uint32_t>() + (r - first) * nb + i;
+        zg::KernelConfig c = b->kcfg;
+        uint64_t want = (a.n_units + 3) / 4;
+        c.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, b->kcfg.blocks));
+        zg::launch_lookup(a, c, s);
+      }
+    }
+    check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
+    if (!b->records)
+      zg::launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
+    else
+      zg::launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
+    check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
+    check_hip(hipStreamSynchronize(s), "plain lookup");
+    float ms = 0;
+    check_hip(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]), "hipEventElapsedTime");
+    b->last_lookup_ms = ms;
+    check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]), "hipEventElapsedTime");
+    b->last_total_ms = ms;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_results(zg_batch *b, uint64_t *totals, uint32_t *counts,
+                                uint64_t *aggregate) {
+  try {
+    ensure(b, "null argument", ZG_ERR_INVALID);
+    hipStream_t s = b->idx->stream;
+    const uint32_t C = (uint32_t)b->idx->m.colors.size();
+    if (totals && b->n_reads)
+      check_hip(hipMemcpyAsync(totals, b->totals.p, b->n_reads * 8, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+    if (counts) {
+      ensure(b->records, "per-read counts need ZG_QUERY_RECORDS", ZG_ERR_INVALID);
+      if (b->n_reads && C)
+        check_hip(hipMemcpyAsync(counts, b->counts.p, b->n_reads * C * 4ULL, hipMemcpyDeviceToHost, s),
+                  "hipMemcpyAsync");
+    }
+    if (aggregate && C)
+      check_hip(hipMemcpyAsync(aggregate, b->agg.p, C * 8ULL, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "results");
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
+  if (!b || !o) {
+    set_error("null argument");
+    return ZG_ERR_INVALID;
+  }
+  try {
+    memset(o, 0, sizeof *o);
+    const zg_index *idx = b->idx;
+    o->n_reads = b->n_reads;
+    o->bases = b->bases;
+    std::vector<uint64_t> totals(b->n_reads);
+    if (b->n_reads)
+      check_hip(hipMemcpy(totals.data(), b->totals.p, b->n_reads * 8, hipMemcpyDeviceToHost),
+                "hipMemcpy");
+    for (uint64_t t : totals) o->seeds += t;
+    uint64_t per_lookup = 0;
+    for (auto &bk : idx->m.banks) {
+      uint64_t G = bk.groups.size();
+      uint64_t parts = bk.routed ? bk.partitions : 1;
+      // every seed probes one partition of each class
+      per_lookup += idx->m.hashes * ((G * idx->m.fingerprint_bits + 7) / 8) / parts;
+    }
+    o->lookups = o->seeds * idx->m.classes();
+    o->algorithmic_bytes = o->seeds * per_lookup;
+    o->last_lookup_ms = b->last_lookup_ms;
+    o->last_seed_ms = b->last_seed_ms;
+    o->last_total_ms = b->last_total_ms;
+    o->tuning.span = b->span;
+    o->tuning.tiles_per_wave = b->kcfg.tiles_per_wave;
+    o->tuning.prefetch = b->kcfg.prefetch;
+    o->tuning.waves_per_cu = b->kcfg.blocks * 4 / std::max(1u, idx->cu_count);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_destroy(zg_batch *b) {
+  delete b;
+  return 0;
+}
+
+extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
+  try {
+    ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
+    zg::Tuning best = idx->tuning;
+    double best_ms = 1e30;
+    std::vector<uint32_t> tiles = {1};
+    if (zg::max_tiles(idx) >= 2) tiles.push_back(2);
+    int base_per_cu = 0;
+    for (uint32_t span : {1024u, 4096u, 16384u})
+      for (uint32_t t : tiles)
+        for (uint32_t d : {1u, 2u, 4u})
+          for (uint32_t frac : {1u, 2u}) {
+            zg::Tuning cand;
+            cand.span = span;
+            cand.tiles_per_wave = t;
+            cand.prefetch = d;
+            zg::KernelConfig kc{(uint32_t)idx->m.fingerprint_bits, t, d, 0};
+            base_per_cu = zg::lookup_blocks_per_cu(kc);
+            cand.waves_per_cu = (uint32_t)std::max(1, base_per_cu / (int)frac) * 4;
+            if (frac == 2 && base_per_cu < 2) continue;
+            idx->tuning = cand;
+            zg::drop_graph(b);
+            zg::configure(idx, b);
+            zg::ensure_partials(b);
+            double ms = 1e30;
+            for (int rep = 0; rep < 2; rep++) {
+              int rc = zg_batch_run(idx, b);
+              if (rc) return rc;
+              ms = std::min(ms, b->last_lookup_ms);
+            }
+            if (ms < best_ms) {
+              best_ms = ms;
+              best = cand;
+            }
+          }
+    idx->tuning = best;
+    zg::drop_graph(b);
+    zg::configure(idx, b);
+    zg::ensure_partials(b);
+    if (chosen) zg_index_get_tuning(idx, chosen);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host-buffer queries
+// ---------------------------------------------------------------------------
+extern "C" int zg_query_reads(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
+                              uint64_t n_reads, uint32_t flags, uint64_t *totals,
+                              uint32_t *counts, uint64_t *aggregate) {
+  try {
+    ensure(idx && (n_reads == 0 || read_off), "null argument", ZG_ERR_INVALID);
+    ensure(!counts || (flags & ZG_QUERY_RECORDS), "per-read counts need ZG_QUERY_RECORDS",
+           ZG_ERR_INVALID);
+    const uint32_t C = (uint32_t)idx->m.colors.size();
+    if (aggregate) memset(aggregate, 0, C * 8ULL);
+    std::vector<uint64_t> agg(C);
+    const uint64_t max_bases = 1ULL << 30;
+    const uint64_t max_cells = 1ULL << 30;  // per-read counter cells per chunk
+    uint64_t r = 0;
+    while (r < n_reads || (n_reads == 0 && r == 0)) {
+      uint64_t e = r;
+      while (e < n_reads && (e == r || (read_off[e + 1] - read_off[r] <= max_bases &&
+                                        (!counts || (e + 1 - r) * (uint64_t)C <= max_cells))))
+        e++;
+      zg_batch *b = nullptr;
+      int rc = zg_batch_create(idx, seq, read_off + r, e - r, flags, &b);
+      if (rc) return rc;
+      rc = zg_batch_run(idx, b);
+      if (!rc)
+        rc = zg_batch_results(b, totals ? totals + r : nullptr,
+                              counts ? counts + r * (uint64_t)C : nullptr,
+                              aggregate ? agg.data() : nullptr);
+      zg_batch_destroy(b);
+      if (rc) return rc;
+      if (aggregate)
+        for (uint32_t c = 0; c < C; c++) aggregate[c] += agg[c];
+      if (n_reads == 0) break;
+      r = e;
+    }
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
+                                uint64_t n_reads, uint64_t *key_off, uint64_t *keys,
+                                uint64_t *positions, uint64_t cap) {
+  try {
+    ensure(idx && key_off && (n_reads == 0 || read_off), "null argument", ZG_ERR_INVALID);
+    zg_batch *b = nullptr;
+    int rc = zg_batch_create(idx, seq, read_off, n_reads, 0, &b);
+    if (rc) return rc;
+    std::unique_ptr<zg_batch> hold(b);
+    hipStream_t s = idx->stream;
+    if (b->explicit_keys) {
+      // key_win holds the window start; position = window start + delta
+      std::vector<uint64_t> win(b->n_keys), kk(b->n_keys);
+      std::vector<uint8_t> delta(b->positions);
+      if (b->n_keys) {
+        check_hip(hipMemcpyAsync(win.data(), b->key_win.p, b->n_keys * 8, hipMemcpyDeviceToHost, s), "copy");
+        check_hip(hipMemcpyAsync(kk.data(), b->keys.p, b->n_keys * 8, hipMemcpyDeviceToHost, s), "copy");
+        check_hip(hipMemcpyAsync(delta.data(), b->delta.p, b->positions, hipMemcpyDeviceToHost, s), "copy");
+      }
+      check_hip(hipStreamSynchronize(s), "copy");
+      for (uint64_t r = 0; r <= n_reads; r++) key_off[r] = b->h_seg_off[r];
+      if (b->n_keys > cap) fail(ZG_ERR_INVALID, "seed buffer too small");
+      // position space (window start + selected offset) -> offset in its read
+      for (uint64_t r = 0; r < n_reads; r++) {
+        uint64_t p0 = (read_off[r] - read_off[0]) + r;
+        for (uint64_t i = key_off[r]; i < key_off[r + 1]; i++) {
+          if (keys) keys[i] = kk[i];
+          if (positions) positions[i] = win[i] + delta[win[i]] - p0;
+        }
+      }
+      return 0;
+    }
+    // k-mers: every valid window in order
+    std::vector<uint64_t> codes((b->positions + 31) / 32 + 4), inv((b->positions + 63) / 64 + 4);
+    check_hip(hipMemcpyAsync(codes.data(), b->codes.p, codes.size() * 8, hipMemcpyDeviceToHost, s), "copy");
+    check_hip(hipMemcpyAsync(inv.data(), b->invalid.p, inv.size() * 8, hipMemcpyDeviceToHost, s), "copy");
+    check_hip(hipStreamSynchronize(s), "copy");
+    const uint32_t k = (uint32_t)idx->m.k;
+    auto bits = [](const std::vector<uint64_t> &w, uint64_t bit) {
+      uint64_t i = bit >> 6, sh = bit & 63;
+      return sh ? (w[i] >> sh) | (w[i + 1] << (64 - sh)) : w[i];
+    };
+    uint64_t n = 0;
+    const uint64_t wmask = k >= 64 ? ~0ULL : ((1ULL << k) - 1);
+    for (uint64_t r = 0; r < n_reads; r++) {
+      key_off[r] = n;
+      uint64_t p0 = (read_off[r] - read_off[0]) + r, len = read_off[r + 1] - read_off[r];
+      for (uint64_t p = p0; p < p0 + len; p++) {
+        if (bits(inv, p) & wmask) continue;
+        if (n < cap) {
+          if (keys) keys[n] = canonical_from_window(bits(codes, 2 * p), (int)k);
+          if (positions) positions[n] = p - p0;
+        }
+        n++;
+      }
+    }
+    key_off[n_reads] = n;
+    if (n > cap) fail(ZG_ERR_INVALID, "seed buffer too small");
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CLI-equivalent commands
+// ---------------------------------------------------------------------------
+namespace zg {
+
+struct FdWriter {
+  int fd;
+  std::string buf;
+  explicit FdWriter(int f) : fd(f) {}
+  ~FdWriter() { flush(); }
+  void put(const std::string &s) {
+    buf += s;
+    if (buf.size() > (1 << 20)) flush();
+  }
+  void flush() {
+    size_t off = 0;
+    while (fd >= 0 && off < buf.size()) {
+      ssize_t w = write(fd, buf.data() + off, buf.size() - off);
+      if (w <= 0) break;
+      off += (size_t)w;
+    }
+    buf.clear();
+  }
+};
+
+static std::string fmt_ratio(double r) {
+  char b[64];
+  snprintf(b, sizeof b, "%.8f", r);
+  return b;
+}
+
+static std::string fmt_secs(double s) {
+  char b[64];
+  snprintf(b, sizeof b, "%.6f", s);
+  return b;
+}
+
+static const char *mode_label(uint64_t m) {
+  switch (m) {
+    case 0: return "kmers";
+    case 1: return "minimizers";
+    case 2: return "modimizers";
+    default: return "findere";
+  }
+}
+
+// TSV field quoting (record_output.rs:67-84)
+static void write_field(std::string &out, const std::string &v) {
+  if (v.find_first_of("\t\r\n\"") == std::string::npos) {
+    out += v;
+    return;
+  }
+  out += '"';
+  for (char c : v) {
+    out += c;
+    if (c == '"') out += '"';
+  }
+  out += '"';
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace zg
+
+extern "C" int zg_query_command(const char *dir, const char *query, double min_ratio,
+                                int recover, int out_fd, int err_fd) {
+  zg_index *idx = nullptr;
+  try {
+    ensure(dir && query, "null argument", ZG_ERR_INVALID);
+    ensure(min_ratio >= 0.0 && min_ratio <= 1.0, "--min-ratio must be in [0, 1]", ZG_ERR_INVALID);
+    double t0 = now_s();
+    int rc = zg_index_open(dir, &idx);
+    if (rc) return rc;
+    std::unique_ptr<zg_index> hold(idx);
+    check_query_support(idx->m);
+    double tp = now_s();
+    Records r = parse_sequences(read_maybe_compressed(query));
+    double prepare = now_s() - tp;
+    double tl = now_s();
+    rc = zg_index_load(idx, recover, 0);
+    if (rc) return rc;
+    double load = now_s() - tl + (tp - t0);
+    const uint32_t C = (uint32_t)idx->m.colors.size();
+    std::vector<uint64_t> totals(r.size()), agg(C);
+    double tq = now_s();
+    rc = zg_query_reads(idx, (const uint8_t *)r.seq.data(), r.off.data(), r.size(), 0,
+                        totals.data(), nullptr, agg.data());
+    if (rc) return rc;
+    double lookup = now_s() - tq;
+    uint64_t total = 0;
+    for (uint64_t t : totals) total += t;
+    const Manifest &m = idx->m;
+    uint64_t classes = m.classes();
+    double to = now_s();
+    {
+      FdWriter out(out_fd);
+      std::string s = "color\tmatches\tratio\tunique_kmers\n";
+      for (uint32_t c = 0; c < C; c++) {
+        double ratio = total == 0 ? 0.0 : (double)agg[c] / (double)total;
+        if (ratio >= min_ratio)
+          s += m.colors[c].name + "\t" + std::to_string(agg[c]) + "\t" + fmt_ratio(ratio) + "\t" +
+               std::to_string(m.colors[c].unique_kmers) + "\n";
+        if (s.size() > (1 << 20)) {
+          out.put(s);
+          s.clear();
+        }
+      }
+      out.put(s);
+    }
+    double output = now_s() - to;
+    FdWriter err(err_fd);
+    err.put(std::string("#query\t") + query + "\tk=" + std::to_string(m.k) + "\tindex_mode=" +
+            mode_label(m.index_mode) + "\ttotal_features=" + std::to_string(total) +
+            "\tstack_probes=" + std::to_string(total) + "\tbank_probes=" +
+            std::to_string(total * classes) + "\telapsed_s=" + fmt_secs(now_s() - t0 - load) + "\n");
+    err.put("query_memory\tmode=gpu-resident\tmax_loaded_subindexes=" +
+            std::to_string(m.banks.size()) + "\tsubindexes=" + std::to_string(m.banks.size()) +
+            "\tresult_storage=HBM\n");
+    err.put("query_preparation\tinput_passes=1\tseed_extraction_passes=1\tabandoned_file_scans=" +
+            std::to_string(idx->recover ? 1 : 0) + "\tquery_features=" + std::to_string(total) +
+            "\tqueried_recovery_hashes=" + std::to_string(idx->recover ? idx->rec.size() : 0) +
+            "\tlookup=gpu-persistent\n");
+    err.put("query_phases\tprepare_s=" + fmt_secs(prepare) + "\trecovery_map_s=0.000000\tlookup_s=" +
+            fmt_secs(lookup) + "\toutput_s=" + fmt_secs(output) + "\n");
+    err.put("query_timings\tload_s=" + fmt_secs(load) + "\tquery_s=" +
+            fmt_secs(now_s() - t0 - load) + "\n");
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_query_records_command(const char *dir, const char *query, double min_ratio,
+                                        int recover, uint64_t batch_size, int format, int out_fd,
+                                        int err_fd) {
+  zg_index *idx = nullptr;
+  try {
+    ensure(dir && query, "null argument", ZG_ERR_INVALID);
+    ensure(min_ratio >= 0.0 && min_ratio <= 1.0, "--min-ratio must be in [0, 1]", ZG_ERR_INVALID);
+    ensure(batch_size > 0, "record batch size must be positive", ZG_ERR_INVALID);
+    ensure(format == 0 || format == 1, "invalid output format", ZG_ERR_INVALID);
+    double t0 = now_s();
+    int rc = zg_index_open(dir, &idx);
+    if (rc) return rc;
+    std::unique_ptr<zg_index> hold(idx);
+    check_query_support(idx->m);
+    Records r = parse_records(read_maybe_compressed(query));
+    rc = zg_index_load(idx, recover, 0);
+    if (rc) return rc;
+    double load = now_s() - t0;
+    const Manifest &m = idx->m;
+    const uint32_t C = (uint32_t)m.colors.size();
+    FdWriter out(out_fd);
+    out.put(format == 0 ? "query_id\tdataset_id\tdataset_name\tmatches\tqueried_seeds\tratio\n"
+                        : "query_id\tvalid_kmers\thits\n");
+    // bounded chunks of reads (per-read counters are n_reads x colours)
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(1ULL << 20, (1ULL << 28) / std::max(1u, C)));
+    std::vector<uint64_t> totals;
+    std::vector<uint32_t> counts;
+    for (uint64_t r0 = 0; r0 < r.size(); r0 += chunk) {
+      uint64_t n = std::min<uint64_t>(chunk, r.size() - r0);
+      totals.assign(n, 0);
+      counts.assign(n * C, 0);
+      rc = zg_query_reads(idx, (const uint8_t *)r.seq.data(), r.off.data() + r0, n,
+                          ZG_QUERY_RECORDS, totals.data(), counts.data(), nullptr);
+      if (rc) return rc;
+      std::string s;
+      for (uint64_t i = 0; i < n; i++) {
+        const std::string &name = r.names[r0 + i];
+        uint64_t total = totals[i];
+        const uint32_t *row = counts.data() + i * C;
+        // hits: all colours at ratio 0, else those passing the threshold
+        // (query_stream.rs:139-161; output sorted by dataset id)
+        auto pass = [&](uint32_t c) {
+          if (min_ratio == 0.0) return true;
+          return total != 0 && (double)row[c] / (double)total >= min_ratio;
+        };
+        if (format == 1) {
+          s += name + "\t" + std::to_string(total) + "\t";
+          bool firsth = true;
+          for (uint32_t c = 0; c < C; c++) {
+            if (!pass(c)) continue;
+            if (!firsth) s += ',';
+            firsth = false;
+            s += std::to_string(c) + ":" + std::to_string(row[c]);
+          }
+          s += '\n';
+        } else {
+          bool any = false;
+          for (uint32_t c = 0; c < C; c++) {
+            if (!pass(c)) continue;
+            any = true;
+            write_field(s, name);
+            s += "\t" + std::to_string(c) + "\t";
+            write_field(s, m.colors[c].name);
+            double ratio = total == 0 ? 0.0 : (double)row[c] / (double)total;
+            s += "\t" + std::to_string(row[c]) + "\t" + std::to_string(total) + "\t" +
+                 fmt_ratio(ratio) + "\n";
+          }
+          if (!any) {
+            write_field(s, name);
+            s += "\t.\t.\t0\t" + std::to_string(total) + "\t0.00000000\n";
+          }
+        }
+        if (s.size() > (1 << 20)) {
+          out.put(s);
+          s.clear();
+        }
+      }
+      out.put(s);
+    }
+    out.flush();
+    FdWriter err(err_fd);
+    err.put("query_memory\tmode=gpu-resident\tmax_loaded_subindexes=" +
+            std::to_string(m.banks.size()) + "\tsubindexes=" + std::to_string(m.banks.size()) +
+            "\tresult_storage=HBM\n");
+    err.put("query_timings\tload_s=" + fmt_secs(load) + "\tquery_s=" + fmt_secs(now_s() - t0 - load) +
+            "\n");
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_info_command(const char *dir, int out_fd) {
+  try {
+    ensure(dir, "null argument", ZG_ERR_INVALID);
+    Manifest m = load_manifest(dir);
+    // validate payloads one at a time (storage.rs:3905-3913)
+    for (auto &b : m.banks) (void)load_bank_payload(dir, b, (int)host_threads());
+    RecoveryMap rec;
+    if (m.abandoned_recovery) rec = read_recovery_map(dir, m);
+    uint64_t groups = 0, logical = 0, stack = 0, abandoned_keys = 0;
+    for (auto &b : m.banks) {
+      groups += b.groups.size();
+      logical += b.slots * b.row_bits;
+      stack += b.payload_bytes();
+      for (auto &g : b.groups) abandoned_keys += g.has_abandoned_keys ? g.abandoned_keys : g.fallback_keys;
+    }
+    uint64_t largest = 0, partitions = 0;
+    for (auto &b : m.banks) {
+      largest = std::max(largest, b.payload_bytes());
+      partitions += b.routed;
+    }
+    uint64_t fallback = m.abandoned_hashes * 8 + m.abandoned_origins * 4;
+    std::string s;
+    auto line = [&](const std::string &k, const std::string &v) { s += k + "\t" + v + "\n"; };
+    size_t docs = std::max<size_t>(m.colors.size(), m.json.get("input_document_count")
+                                                        ? strtoull(m.json.get("input_document_count")->text.c_str(), nullptr, 10)
+                                                        : 0);
+    line("collection_status", docs == m.colors.size() ? "complete" : "incomplete");
+    line("indexed_documents", std::to_string(m.colors.size()));
+    line("index", dir);
+    line("k", std::to_string(m.k));
+    line("hashes", std::to_string(m.hashes));
+    line("seed", std::to_string(m.seed));
+    line("fingerprint_bits", std::to_string(m.fingerprint_bits));
+    line("index_mode", mode_label(m.index_mode));
+    line("technical_layout", "merge-pow2");
+    line("colors", std::to_string(m.colors.size()));
+    line("groups", std::to_string(groups));
+    line("banks", std::to_string(m.banks.size()));
+    line("size_classes", std::to_string(m.classes()));
+    line("max_subindex_size", m.has_max_subindex_size ? std::to_string(m.max_subindex_size) : "unlimited");
+    line("largest_subindex_bytes", std::to_string(largest));
+    line("hash_partitions", std::to_string(partitions));
+    line("input_documents", std::to_string(docs));
+    line("minimizer_size", std::to_string(m.minimizer_size));
+    line("modimizer_sampling", std::to_string(m.modimizer_sampling));
+    line("minimizer_scheme", m.minimizer_scheme ? "open-closed-mod" : "simd");
+    line("homopolymer_compression", m.homopolymer_compression ? "true" : "false");
+    line("merge_planner", m.merge_planner_label());
+    if (m.has_segment_length) line("segment_length", std::to_string(m.segment_length));
+    line("abandoned_recovery", m.abandoned_recovery ? "true" : "false");
+    line("abandoned_hash_bits", "32");
+    line("abandoned_keys", std::to_string(abandoned_keys));
+    line("abandoned_unique_hashes", std::to_string(rec.size()));
+    line("abandoned_dataset_associations", std::to_string(rec.origins.size()));
+    line("logical_bits", std::to_string(logical));
+    line("padding_bits", std::to_string(stack * 8 - logical));
+    line("stack_bytes", std::to_string(stack));
+    line("fallback_bytes", std::to_string(fallback));
+    line("structure_bytes", std::to_string(stack + fallback));
+    s += "bank\tslots\tgroups\trow_bits\tpadding_bits_per_row\tfallback_keys\tdocuments\n";
+    for (auto &b : m.banks) {
+      uint64_t fk = 0, docs_b = 0;
+      for (auto &g : b.groups) {
+        fk += g.fallback_keys;
+        docs_b += g.members.size();
+      }
+      s += b.file + "\t" + std::to_string(b.slots) + "\t" + std::to_string(b.groups.size()) + "\t" +
+           std::to_string(b.row_bits) + "\t" + std::to_string(b.row_bytes() * 8 - b.row_bits) + "\t" +
+           std::to_string(fk) + "\t" + std::to_string(docs_b) + "\n";
+    }
+    s += "group\tslots\twidth\tunion_cardinality\tfallback_keys\tcolors\n";
+    uint64_t id = 0;
+    for (auto &b : m.banks)
+      for (auto &g : b.groups) {
+        s += std::to_string(id++) + "\t" + std::to_string(b.slots) + "\t" + std::to_string(g.width) +
+             "\t" + std::to_string(g.union_cardinality) + "\t" + std::to_string(g.fallback_keys) + "\t";
+        for (size_t j = 0; j < g.members.size(); j++) {
+          if (j) s += ",";
+          s += m.colors[g.members[j]].name;
+        }
+        s += "\n";
+      }
+    FdWriter out(out_fd);
+    out.put(s);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}

@@ -1,0 +1,999 @@
+// zg_build.hip -- host-side index construction in the reference on-disk format.
+//
+// Follows power_merge::build (src/power_merge/storage.rs:500-882) with a fixed
+// plan: every dataset is its own group (or `merge_group_size` consecutive
+// datasets share one), groups of equal slot count form one bank, columns are
+// ordered by width then first member (ColumnOrder::Width, storage.rs:1644-1660),
+// cells are found by ZOR hypergraph peeling (src/bin/zorindex.rs:634-841) with
+// back-substitution (src/power_merge/graph.rs:163-216), and keys the peel
+// abandons go to the recovery map (storage.rs:1311-1343, 2053-2081).
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <mutex>
+#include <queue>
+#include <thread>
+
+#include "zg_fastx.h"
+#include "zg_hash.h"
+#include "zg_internal.h"
+
+namespace zg {
+
+void set_error(const std::string &msg);  // zg_api.hip
+int translate_exception();               // zg_api.hip
+
+// ---------------------------------------------------------------------------
+// Seed extraction on the host (build side).
+// ---------------------------------------------------------------------------
+static inline int enc(uint8_t c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return -1;
+  }
+}
+
+// scan_canonical_kmers, src/lib.rs:645-681 (k <= 32).
+static void host_kmers(const uint8_t *seq, size_t len, int k, std::vector<uint64_t> &out) {
+  if (k <= 0 || len < (size_t)k) return;
+  uint64_t mask = k == 32 ? ~0ULL : ((1ULL << (2 * k)) - 1);
+  int shift = 2 * (k - 1);
+  uint64_t f = 0, r = 0;
+  size_t seen = 0;
+  for (size_t i = 0; i < len; i++) {
+    int c = enc(seq[i]);
+    if (c < 0) {
+      f = r = 0;
+      seen = 0;
+      continue;
+    }
+    f = ((f << 2) | (uint64_t)c) & mask;
+    r = (r >> 2) | ((uint64_t)(c ^ 3) << shift);
+    if (++seen >= (size_t)k) out.push_back(f < r ? f : r);
+  }
+}
+
+// Canonical rolling value of a fixed length (implementation.rs Rolling).
+struct Rolling {
+  uint64_t f = 0, r = 0, mask;
+  int shift;
+  explicit Rolling(int len) : mask(len >= 32 ? ~0ULL : ((1ULL << (2 * len)) - 1)), shift(2 * (len - 1)) {}
+  uint64_t push(int c) {
+    f = ((f << 2) | (uint64_t)c) & mask;
+    r = (r >> 2) | ((uint64_t)(c ^ 3) << shift);
+    return f < r ? f : r;
+  }
+};
+
+// Sliding minimum with explicit window start (implementation.rs `Minimum`).
+struct RankEntry {
+  size_t pos;
+  uint8_t cls;
+  uint64_t hash;
+};
+struct SlidingMin {
+  std::deque<RankEntry> q;
+  bool rightmost;
+  explicit SlidingMin(bool r) : rightmost(r) {}
+  void push(size_t pos, uint8_t cls, uint64_t hash, size_t start) {
+    while (!q.empty() && q.front().pos < start) q.pop_front();
+    while (!q.empty()) {
+      const RankEntry &b = q.back();
+      bool greater = b.cls > cls || (b.cls == cls && b.hash > hash);
+      bool equal = b.cls == cls && b.hash == hash;
+      if (greater || (rightmost && equal)) q.pop_back();
+      else break;
+    }
+    q.push_back({pos, cls, hash});
+  }
+};
+
+// One all-ACGT run, the deque formulation of scan_run_scalar
+// (implementation.rs:349-414).
+static void host_ocm_run(const uint8_t *seq, size_t len, int k, int m, uint64_t seed,
+                         std::vector<uint64_t> &out) {
+  OcmParams P = ocm_parameters(k, m);
+  const int w = P.w, t = P.t, s = P.s;
+  const size_t inner_length = (size_t)(t - s + 1), anchor_length = (size_t)(k - t + 1);
+  Rolling rep(m), anc(t), inner(s);
+  SlidingMin inner_min(false), left(false), right(true);
+  std::vector<uint64_t> inner_hashes(inner_length), reps((size_t)w);
+  size_t gt = 0;
+  bool have_prev = false;
+  size_t prev = 0;
+  for (size_t end = 0; end < len; end++) {
+    int c = enc(seq[end]);
+    gt += c >= 2;
+    if (end >= (size_t)k) gt -= enc(seq[end - k]) >= 2;
+    uint64_t key = rep.push(c), anchor_key = anc.push(c), inner_key = inner.push(c);
+    if (end + 1 >= (size_t)m) reps[(end + 1 - m) % w] = key;
+    if (end + 1 < (size_t)s) continue;
+    size_t inner_pos = end + 1 - s;
+    uint64_t h = mixsplit(inner_key, seed ^ kOcmInnerSeed);
+    inner_hashes[inner_pos % inner_length] = h;
+    size_t inner_start = end + 1 >= (size_t)t ? end + 1 - t : 0;
+    inner_min.push(inner_pos, 0, h, inner_start);
+    if (end + 1 < (size_t)t) continue;
+    size_t apos = end + 1 - t;
+    uint64_t mn = inner_min.q.front().hash;
+    uint8_t cls;
+    if (inner_hashes[(apos + (t - s) / 2) % inner_length] == mn) cls = 0;
+    else if (inner_hashes[apos % inner_length] == mn || h == mn) cls = 1;
+    else cls = 2;
+    size_t start = apos + 1 >= anchor_length ? apos + 1 - anchor_length : 0;
+    uint64_t rh = mixsplit(anchor_key, seed);
+    left.push(apos, cls, rh, start);
+    right.push(apos, cls, rh, start);
+    if (end + 1 < (size_t)k) continue;
+    size_t chosen = gt > (size_t)(k / 2) ? right.q.front().pos : left.q.front().pos;
+    size_t pos = start + (chosen - start) % (size_t)w;
+    if (!have_prev || prev != pos) {
+      out.push_back(reps[pos % w]);
+      prev = pos;
+      have_prev = true;
+    }
+  }
+}
+
+static void host_ocm(const uint8_t *seq, size_t len, int k, int m, uint64_t seed,
+                     std::vector<uint64_t> &out) {
+  if (k == 0 || (k & 1) == 0 || m == 0 || m > 31 || m > k) return;
+  size_t start = 0;
+  while (start < len) {
+    size_t end = start;
+    while (end < len && enc(seq[end]) >= 0) end++;
+    if (end - start >= (size_t)k) host_ocm_run(seq + start, end - start, k, m, seed, out);
+    start = end + 1;
+  }
+}
+
+// LSD radix sort (16-bit digits) followed by deduplication.
+static void sort_unique(std::vector<uint64_t> &v) {
+  if (v.size() < 4096) {
+    std::sort(v.begin(), v.end());
+  } else {
+    std::vector<uint64_t> tmp(v.size());
+    uint64_t all_or = 0;
+    for (uint64_t x : v) all_or |= x;
+    std::vector<size_t> count(65537);
+    for (int pass = 0; pass < 4; pass++) {
+      int sh = 16 * pass;
+      if (((all_or >> sh) & 0xFFFF) == 0) continue;
+      std::fill(count.begin(), count.end(), 0);
+      for (uint64_t x : v) count[((x >> sh) & 0xFFFF) + 1]++;
+      for (size_t i = 1; i < count.size(); i++) count[i] += count[i - 1];
+      for (uint64_t x : v) tmp[count[(x >> sh) & 0xFFFF]++] = x;
+      v.swap(tmp);
+    }
+  }
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+}
+
+// ---------------------------------------------------------------------------
+// ZOR construction for one group.
+// ---------------------------------------------------------------------------
+struct GroupResult {
+  std::vector<uint32_t> members;  // global colour ids
+  uint64_t slots = 0;
+  uint32_t width = 0;
+  uint64_t union_cardinality = 0;
+  uint64_t abandoned = 0;
+  std::vector<uint8_t> cells;     // slots cells of cell_bytes each (little endian)
+  uint32_t cell_bytes = 1;
+  uint64_t cell(uint64_t s) const {
+    uint64_t v = 0;
+    memcpy(&v, cells.data() + s * cell_bytes, cell_bytes);
+    return v;
+  }
+  // recovery records of abandoned keys: (hash32, member mask)
+  std::vector<std::pair<uint32_t, uint64_t>> recovery;
+};
+
+static uint64_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// BuildConfig::slots (src/power_merge/mod.rs:319-337).
+static uint64_t config_slots(const zg_build_config &c, uint64_t cardinality) {
+  double needed = std::ceil(std::max<double>((double)cardinality, 1.0) * c.slot_scale);
+  uint64_t n = (uint64_t)needed;
+  uint64_t h2 = next_pow2(c.hashes);
+  n = std::max<uint64_t>(n, h2);
+  n = std::max<uint64_t>(n, (uint64_t)c.segment_length * h2);
+  return next_pow2(n);
+}
+
+// Peeling with degree counts and XOR-of-incident-edge accumulators; a cyclic
+// core is broken at the minimum-degree cell (highest cell id on ties) by
+// keeping its first active equation and abandoning the others, the default
+// no-heuristic rule of choose_cycle_break_candidate.
+static void construct_group(const zg_build_config &cfg, const std::vector<uint64_t> &keys,
+                            const std::vector<uint64_t> &values, GroupResult &g) {
+  const uint64_t S = g.slots;
+  const uint32_t H = cfg.hashes;
+  const SegmentGeometry geo = segment_geometry(S, H, cfg.segment_length);
+  const size_t n = keys.size();
+  auto slots_of = [&](uint64_t key, uint64_t *out) {
+    uint64_t first = mixsplit(key, cfg.seed), step = splitmix64(first);
+    uint64_t base = slot_base(first, geo), h = first;
+    for (uint32_t i = 0; i < H; i++) {
+      out[i] = slot_at(base, h, i, geo);
+      h += step;
+    }
+  };
+  std::vector<uint32_t> count(S, 0), xr(S, 0);
+  uint64_t sl[kMaxHashes];
+  for (size_t e = 0; e < n; e++) {
+    slots_of(keys[e], sl);
+    for (uint32_t i = 0; i < H; i++) {
+      count[sl[i]]++;
+      xr[sl[i]] ^= (uint32_t)e;
+    }
+  }
+  std::vector<uint8_t> state(n, 0);  // 0 active, 1 peeled, 2 abandoned
+  std::vector<std::pair<uint32_t, uint32_t>> order;
+  order.reserve(n);
+  std::vector<uint32_t> queue;
+  for (uint64_t s = 0; s < S; s++)
+    if (count[s] == 1) queue.push_back((uint32_t)s);
+  size_t remaining = n, qi = 0;
+  std::vector<uint32_t> abandoned;
+  // core-phase structures, built on first stall
+  std::vector<uint64_t> adj_off;
+  std::vector<uint32_t> adj;
+  std::priority_queue<std::pair<int64_t, uint32_t>> heap;  // (-degree, slot)
+  bool core = false;
+  auto remove_edge = [&](uint32_t e) {
+    slots_of(keys[e], sl);
+    for (uint32_t i = 0; i < H; i++) {
+      uint64_t s = sl[i];
+      if (count[s] > 0) {
+        count[s]--;
+        xr[s] ^= e;
+        if (count[s] == 1) queue.push_back((uint32_t)s);
+        if (core && count[s] > 1) heap.push({-(int64_t)count[s], (uint32_t)s});
+      }
+    }
+  };
+  while (remaining > 0) {
+    bool progress = false;
+    while (qi < queue.size()) {
+      uint32_t s = queue[qi++];
+      if (count[s] != 1) continue;
+      uint32_t e = xr[s];
+      if (state[e] != 0) continue;
+      state[e] = 1;
+      remaining--;
+      progress = true;
+      order.push_back({e, s});
+      remove_edge(e);
+    }
+    if (qi > (1u << 20)) {
+      queue.erase(queue.begin(), queue.begin() + qi);
+      qi = 0;
+    }
+    if (remaining == 0) break;
+    if (progress && qi < queue.size()) continue;
+    if (!core) {
+      core = true;
+      adj_off.assign(S + 1, 0);
+      for (size_t e = 0; e < n; e++) {
+        if (state[e] != 0) continue;
+        slots_of(keys[e], sl);
+        for (uint32_t i = 0; i < H; i++) adj_off[sl[i] + 1]++;
+      }
+      for (uint64_t s = 0; s < S; s++) adj_off[s + 1] += adj_off[s];
+      adj.assign(adj_off[S], 0);
+      std::vector<uint64_t> cur(adj_off.begin(), adj_off.end() - 1);
+      for (size_t e = 0; e < n; e++) {
+        if (state[e] != 0) continue;
+        slots_of(keys[e], sl);
+        for (uint32_t i = 0; i < H; i++) adj[cur[sl[i]]++] = (uint32_t)e;
+      }
+      for (uint64_t s = 0; s < S; s++)
+        if (count[s] > 1) heap.push({-(int64_t)count[s], (uint32_t)s});
+    }
+    // choose the minimum-degree cell, highest id on ties
+    uint32_t pick = UINT32_MAX;
+    while (!heap.empty()) {
+      auto top = heap.top();
+      heap.pop();
+      uint32_t s = top.second;
+      if ((int64_t)count[s] == -top.first && count[s] > 1) {
+        pick = s;
+        break;
+      }
+    }
+    if (pick == UINT32_MAX) {
+      // no multi-degree cell left: abandon the first active equation
+      for (size_t e = 0; e < n; e++)
+        if (state[e] == 0) {
+          state[e] = 2;
+          remaining--;
+          abandoned.push_back((uint32_t)e);
+          remove_edge((uint32_t)e);
+          break;
+        }
+      continue;
+    }
+    bool kept = false;
+    for (uint64_t a = adj_off[pick]; a < adj_off[pick + 1]; a++) {
+      uint32_t e = adj[a];
+      if (state[e] != 0) continue;
+      if (!kept) {
+        kept = true;
+        continue;
+      }
+      state[e] = 2;
+      remaining--;
+      abandoned.push_back(e);
+      remove_edge(e);
+    }
+  }
+  // back-substitution in reverse peel order (graph.rs:187-199)
+  std::vector<uint64_t> cells(S, 0);
+  for (size_t i = order.size(); i-- > 0;) {
+    uint32_t e = order[i].first, pivot = order[i].second;
+    uint64_t v = values[e];
+    slots_of(keys[e], sl);
+    for (uint32_t h = 0; h < H; h++)
+      if (sl[h] != pivot) v ^= cells[sl[h]];
+    cells[pivot] = v;
+  }
+  // verification (graph.rs:201-214)
+  std::sort(abandoned.begin(), abandoned.end());
+  for (size_t e = 0; e < n; e++) {
+    if (state[e] == 2) continue;
+    slots_of(keys[e], sl);
+    uint64_t v = 0;
+    for (uint32_t h = 0; h < H; h++) v ^= cells[sl[h]];
+    if (v != values[e]) fail(-1, "incorrect merged payload for key " + std::to_string(keys[e]));
+  }
+  g.cell_bytes = g.width <= 8 ? 1 : g.width <= 16 ? 2 : g.width <= 32 ? 4 : 8;
+  g.cells.resize(S * g.cell_bytes);
+  for (uint64_t s = 0; s < S; s++) memcpy(g.cells.data() + s * g.cell_bytes, &cells[s], g.cell_bytes);
+  std::vector<uint64_t>().swap(cells);
+  g.abandoned = abandoned.size();
+  for (uint32_t e : abandoned) {
+    uint64_t mask = g.members.size() == 1 ? 1 : values[e] >> cfg.fingerprint_bits;
+    g.recovery.push_back({abandoned_hash(keys[e], cfg.seed), mask});
+  }
+}
+
+}  // namespace zg
+
+// ---------------------------------------------------------------------------
+// The builder object
+// ---------------------------------------------------------------------------
+struct zg_builder {
+  zg_build_config cfg{};
+  // deques: workers hold references to elements while the caller appends
+  std::deque<zg::Color> colors;
+  std::deque<std::vector<uint64_t>> pending;  // keys per colour
+  std::deque<zg::GroupResult> groups;         // finished groups (index = group id)
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::function<void()>> jobs;
+  std::vector<std::thread> workers;
+  size_t running = 0;
+  bool stopping = false;
+  std::string error;
+  int error_code = 0;
+  std::chrono::steady_clock::time_point start = std::chrono::steady_clock::now();
+
+  void worker() {
+    while (true) {
+      std::function<void()> job;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stopping || !jobs.empty(); });
+        if (jobs.empty()) return;
+        job = std::move(jobs.front());
+        jobs.pop_front();
+        running++;
+      }
+      try {
+        job();
+      } catch (const zg::Error &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!error_code) { error = e.what(); error_code = e.code; }
+      } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!error_code) { error = e.what(); error_code = -1; }
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        running--;
+      }
+      cv.notify_all();
+    }
+  }
+  void submit(std::function<void()> f) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return jobs.size() < workers.size() * 2; });
+    jobs.push_back(std::move(f));
+    cv.notify_all();
+  }
+  void drain() {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return jobs.empty() && running == 0; });
+    if (error_code) throw zg::Error(error_code, error);
+  }
+  ~zg_builder() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stopping = true;
+    }
+    cv.notify_all();
+    for (auto &t : workers) t.join();
+  }
+};
+
+namespace zg {
+
+static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64_t *off,
+                    uint64_t n, std::vector<uint64_t> &keys) {
+  for (uint64_t r = 0; r < n; r++) {
+    const uint8_t *p = seq + off[r];
+    size_t len = off[r + 1] - off[r];
+    if (cfg.index_mode == 1)
+      host_ocm(p, len, (int)cfg.k, (int)cfg.minimizer_size, cfg.seed, keys);
+    else
+      host_kmers(p, len, (int)cfg.k, keys);
+  }
+}
+
+static void build_group(const zg_build_config &cfg, std::vector<uint32_t> members,
+                        std::vector<std::vector<uint64_t> *> key_sets, GroupResult &g) {
+  const uint32_t b = cfg.fingerprint_bits;
+  g.members = std::move(members);
+  std::vector<uint64_t> keys, values;
+  if (g.members.size() == 1) {
+    keys.swap(*key_sets[0]);
+    values.resize(keys.size());
+    for (size_t i = 0; i < keys.size(); i++) values[i] = fingerprint(keys[i], cfg.seed, b);
+  } else {
+    // union with membership masks (storage.rs:1222-1255, 1301-1308)
+    std::vector<std::pair<uint64_t, uint64_t>> all;
+    for (size_t j = 0; j < key_sets.size(); j++)
+      for (uint64_t key : *key_sets[j]) all.push_back({key, 1ULL << j});
+    std::sort(all.begin(), all.end());
+    for (size_t i = 0; i < all.size();) {
+      uint64_t key = all[i].first, mask = 0;
+      while (i < all.size() && all[i].first == key) mask |= all[i++].second;
+      keys.push_back(key);
+      values.push_back(fingerprint(key, cfg.seed, b) | (mask << b));
+    }
+    for (auto *ks : key_sets) std::vector<uint64_t>().swap(*ks);
+  }
+  g.union_cardinality = keys.size();
+  g.width = b + (g.members.size() == 1 ? 0 : (uint32_t)g.members.size());
+  g.slots = config_slots(cfg, keys.size());
+  construct_group(cfg, keys, values, g);
+}
+
+}  // namespace zg
+
+using namespace zg;
+
+extern "C" void zg_build_config_default(zg_build_config *c) {
+  if (!c) return;
+  memset(c, 0, sizeof *c);
+  c->k = 31;
+  c->hashes = 4;
+  c->fingerprint_bits = 8;
+  c->index_mode = 0;
+  c->minimizer_size = 21;
+  c->segment_length = 64;
+  c->seed = 69;
+  c->slot_scale = 1.0;
+  c->max_subindex_size = 8ULL << 30;
+  c->merge_group_size = 1;
+  c->threads = 0;
+  c->drop_abandoned = 0;
+}
+
+static void validate_config(const zg_build_config &c) {
+  uint32_t b = c.fingerprint_bits;
+  ensure(b == 1 || b == 2 || b == 4 || b == 8 || b == 16 || b == 32,
+         "fingerprint bits must be 1, 2, 4, 8, 16 or 32", -1);
+  ensure(c.hashes >= 2 && c.hashes <= 32, "invalid hash count", -1);
+  ensure(c.segment_length >= 1 && c.segment_length <= 4096 &&
+             !(c.segment_length & (c.segment_length - 1)),
+         "segment length must be a power of two in 1..=4096", -1);
+  ensure(c.slot_scale > 0 && std::isfinite(c.slot_scale), "invalid slot scale", -1);
+  ensure(c.max_subindex_size && !(c.max_subindex_size & (c.max_subindex_size - 1)),
+         "--max-subindex-size must be a non-zero power of two", -1);
+  if (c.index_mode == 0) {
+    ensure(c.k >= 1 && c.k <= 32, "k must be in 1..=32 for k-mer indexes in this build", -6);
+  } else if (c.index_mode == 1) {
+    ensure(c.k >= 1 && c.k <= 255 && c.k % 2 == 1 && c.minimizer_size >= 1 &&
+               c.minimizer_size <= 31 && c.minimizer_size <= c.k,
+           "invalid minimizer window/length", -1);
+  } else {
+    fail(-6, "only kmers and open-closed mod-minimizer indexes are built here");
+  }
+  ensure(c.merge_group_size >= 1 && c.merge_group_size <= 64 - b, "invalid merge group size",
+         -1);
+}
+
+extern "C" int zg_builder_create(const zg_build_config *cfg, zg_builder **out) {
+  try {
+    ensure(cfg && out, "null argument", -1);
+    validate_config(*cfg);
+    auto b = std::make_unique<zg_builder>();
+    b->cfg = *cfg;
+    uint32_t threads = cfg->threads ? cfg->threads : std::thread::hardware_concurrency();
+    if (!threads) threads = 1;
+    for (uint32_t i = 0; i < threads; i++) b->workers.emplace_back([p = b.get()] { p->worker(); });
+    *out = b.release();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8_t *seq,
+                                    const uint64_t *rec_off, uint64_t n_records) {
+  try {
+    ensure(b && name && (n_records == 0 || (seq && rec_off)), "null argument", -1);
+    ensure(*name, "empty color name", -1);
+    {
+      std::lock_guard<std::mutex> lk(b->mu);
+      if (b->error_code) throw Error(b->error_code, b->error);
+    }
+    uint32_t id;
+    {
+      std::lock_guard<std::mutex> lk(b->mu);
+      id = (uint32_t)b->colors.size();
+      Color c;
+      c.name = name;
+      c.path = name;
+      b->colors.push_back(c);
+    }
+    uint64_t base = n_records ? rec_off[0] : 0;
+    uint64_t len = n_records ? rec_off[n_records] - base : 0;
+    auto copy = std::make_shared<std::vector<uint8_t>>(seq ? seq + base : nullptr,
+                                                        seq ? seq + base + len : nullptr);
+    auto offs = std::make_shared<std::vector<uint64_t>>(n_records + 1);
+    for (uint64_t i = 0; i <= n_records; i++) (*offs)[i] = n_records ? rec_off[i] - base : 0;
+    const zg_build_config cfg = b->cfg;
+    const bool single = cfg.merge_group_size == 1;
+    {
+      std::lock_guard<std::mutex> lk(b->mu);
+      b->pending.emplace_back();
+      if (single) b->groups.emplace_back();
+    }
+    b->submit([b, id, copy, offs, n_records, cfg, single] {
+      std::vector<uint64_t> keys;
+      extract(cfg, copy->data(), offs->data(), n_records, keys);
+      uint64_t occurrences = keys.size();
+      sort_unique(keys);
+      uint64_t unique = keys.size();
+      std::vector<uint64_t> *slot;
+      GroupResult *gr = nullptr;
+      {
+        std::lock_guard<std::mutex> lk(b->mu);
+        b->colors[id].occurrences = occurrences;
+        b->colors[id].unique_kmers = unique;
+        slot = &b->pending[id];
+        if (single) gr = &b->groups[id];
+      }
+      *slot = std::move(keys);
+      if (single) build_group(cfg, {id}, {slot}, *gr);
+    });
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+namespace zg {
+// Packs groups into plane-layout banks and assembles the manifest.
+static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
+  const zg_build_config &cfg = b->cfg;
+  const uint32_t fb = cfg.fingerprint_bits;
+  // banks by slot count, ascending (pack_banks_streaming's BTreeMap order)
+  std::vector<uint64_t> sizes;
+  for (auto &g : b->groups) sizes.push_back(g.slots);
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  Manifest &m = idx->m;
+  m = Manifest();
+  uint64_t singleton_bits = 0, logical_bits = 0, abandoned_total = 0;
+  for (uint64_t slots : sizes) {
+    std::vector<size_t> ids;
+    for (size_t i = 0; i < b->groups.size(); i++)
+      if (b->groups[i].slots == slots) ids.push_back(i);
+    std::stable_sort(ids.begin(), ids.end(), [&](size_t x, size_t y) {
+      const GroupResult &a = b->groups[x], &c = b->groups[y];
+      if (a.width != c.width) return a.width < c.width;
+      return a.members[0] < c.members[0];
+    });
+    BankMeta bm;
+    bm.slots = slots;
+    bm.fingerprint_first = true;
+    uint64_t row_bits = 0;
+    for (size_t i : ids) {
+      const GroupResult &g = b->groups[i];
+      GroupMeta gm;
+      gm.members = g.members;
+      gm.width = g.width;
+      gm.offset = row_bits;
+      gm.union_cardinality = g.union_cardinality;
+      gm.fallback_keys = cfg.drop_abandoned ? 0 : g.abandoned;
+      gm.has_abandoned_keys = true;
+      gm.abandoned_keys = g.abandoned;
+      row_bits += g.width;
+      bm.groups.push_back(gm);
+      abandoned_total += g.abandoned;
+    }
+    bm.row_bits = row_bits;
+    bm.file = BankMeta::filename(slots, false, 0, 0, 1);
+    uint64_t rb = bm.row_bytes();
+    ensure((unsigned __int128)slots * rb <= cfg.max_subindex_size,
+           "a subindex exceeds --max-subindex-size; hash-partitioned builds are not "
+           "implemented here",
+           -6);
+    logical_bits += slots * row_bits;
+    std::vector<uint8_t> payload(slots * rb, 0);
+    const uint64_t G = ids.size();
+    std::vector<uint64_t> mask_off(G);
+    uint64_t acc = G * fb;
+    for (size_t i = 0; i < G; i++) {
+      mask_off[i] = acc;
+      acc += b->groups[ids[i]].width - fb;
+    }
+    const uint64_t fp_mask = (1ULL << fb) - 1;
+    const uint32_t threads = (uint32_t)b->workers.size();
+    const uint64_t tile = 4096;
+    const uint64_t tiles = (slots + tile - 1) / tile;
+    std::atomic<uint64_t> next{0};
+    auto work = [&] {
+      for (uint64_t t; (t = next++) < tiles;) {
+        uint64_t lo = t * tile, hi = std::min(slots, lo + tile);
+        for (size_t i = 0; i < G; i++) {
+          const GroupResult &g = b->groups[ids[i]];
+          if (fb == 8 && g.width == 8) {
+            const uint8_t *cells = g.cells.data();
+            for (uint64_t s = lo; s < hi; s++) payload[s * rb + i] = cells[s];
+            continue;
+          }
+          for (uint64_t s = lo; s < hi; s++) {
+            uint8_t *row = payload.data() + s * rb;
+            uint64_t v = g.cell(s);
+            uint64_t fpv = v & fp_mask;
+            uint64_t bit = i * fb;
+            for (uint32_t x = 0; x < fb; x++)
+              if ((fpv >> x) & 1) row[(bit + x) >> 3] |= (uint8_t)(1u << ((bit + x) & 7));
+            if (g.width > fb) {
+              uint64_t mk = v >> fb;
+              uint64_t mb = mask_off[i];
+              for (uint32_t x = 0; x < g.width - fb; x++)
+                if ((mk >> x) & 1) row[(mb + x) >> 3] |= (uint8_t)(1u << ((mb + x) & 7));
+            }
+          }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    for (size_t i : ids) std::vector<uint8_t>().swap(b->groups[i].cells);
+    idx->payload.push_back(std::move(payload));
+    m.banks.push_back(std::move(bm));
+  }
+  for (auto &g : b->groups) singleton_bits += (uint64_t)fb * g.slots * g.members.size();
+  // recovery map: union origins per hash (visit_abandoned)
+  RecoveryMap rec;
+  if (!cfg.drop_abandoned) {
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (auto &g : b->groups)
+      for (auto &r : g.recovery) {
+        uint64_t mask = r.second;
+        while (mask) {
+          int bit = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          pairs.push_back({r.first, g.members[bit]});
+        }
+      }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    for (size_t i = 0; i < pairs.size();) {
+      uint32_t h = pairs[i].first;
+      while (i < pairs.size() && pairs[i].first == h) rec.origins.push_back(pairs[i++].second);
+      rec.hashes.push_back(h);
+      rec.off.push_back(rec.origins.size());
+    }
+  }
+  // manifest fields (storage.rs:787-857)
+  m.version = 8;
+  m.abandoned_recovery = !cfg.drop_abandoned;
+  m.has_abandoned_file = !cfg.drop_abandoned;
+  m.abandoned_file = cfg.drop_abandoned ? "" : kAbandonedFile;
+  m.abandoned_hashes = rec.hashes.size();
+  m.abandoned_origins = rec.origins.size();
+  m.backend = "zor";
+  m.k = cfg.k;
+  m.hashes = cfg.hashes;
+  m.has_segment_length = true;
+  m.segment_length = cfg.segment_length;
+  m.seed = cfg.seed;
+  m.fingerprint_bits = fb;
+  m.index_mode = cfg.index_mode;
+  m.minimizer_size = cfg.minimizer_size;
+  m.minimizer_scheme = 1;
+  m.modimizer_sampling = 16;
+  m.findere_z = 10;
+  m.slot_scale = cfg.slot_scale;
+  m.has_max_subindex_size = true;
+  m.max_subindex_size = cfg.max_subindex_size;
+  m.colors.assign(b->colors.begin(), b->colors.end());
+  Json j = Json::object();
+  j.set("version", Json::u64(m.version));
+  j.set("abandoned_recovery", Json::boolean(m.abandoned_recovery));
+  j.set("abandoned_file", m.has_abandoned_file ? Json::str(m.abandoned_file) : Json::null());
+  j.set("abandoned_hashes", Json::u64(m.abandoned_hashes));
+  j.set("abandoned_origins", Json::u64(m.abandoned_origins));
+  j.set("backend", Json::str("zor"));
+  j.set("k", Json::u64(m.k));
+  j.set("hashes", Json::u64(m.hashes));
+  j.set("segment_length", Json::u64(m.segment_length));
+  j.set("seed", Json::u64(m.seed));
+  j.set("fingerprint_bits", Json::u64(fb));
+  j.set("index_mode", Json::u64(m.index_mode));
+  j.set("minimizer_size", Json::u64(m.minimizer_size));
+  j.set("minimizer_scheme", Json::str("open-closed-mod"));
+  j.set("homopolymer_compression", Json::boolean(false));
+  j.set("seed_pattern", Json::null());
+  j.set("randstrobemers", Json::null());
+  j.set("randstrobe_scheme", Json::null());
+  j.set("modimizer_sampling", Json::u64(16));
+  j.set("findere_z", Json::u64(10));
+  j.set("sketch_size", Json::u64(128));
+  j.set("sketch_method", Json::str("none: fixed consecutive grouping (no sketches)"));
+  j.set("merge_planner", Json::str("naive"));
+  j.set("column_order", Json::str("width"));
+  j.set("column_order_method", Json::str("width-then-dataset-id-v1"));
+  j.set("planning_pair_evaluations", Json::u64(0));
+  j.set("planning_possible_pair_evaluations", Json::u64(0));
+  j.set("merge_reverts_enabled", Json::boolean(false));
+  j.set("construction_cell_storage", Json::str("two-pass-memory-packed-no-scratch"));
+  j.set("construction_abandoned_storage", Json::str(cfg.drop_abandoned ? "omitted" : "memory"));
+  j.set("slot_scale", Json::f64(cfg.slot_scale));
+  j.set("min_subindex_documents", Json::u64(1));
+  j.set("max_subindex_size", Json::u64(cfg.max_subindex_size));
+  j.set("promoted_groups", Json::u64(0));
+  j.set("singleton_logical_bits", Json::u64(singleton_bits));
+  j.set("predicted_logical_bits", Json::u64(logical_bits));
+  j.set("planned_groups", Json::u64(b->groups.size()));
+  j.set("rejected_groups", Json::u64(0));
+  j.set("overflow_groups", Json::u64(0));
+  j.set("exact_group_verifications", Json::u64(b->groups.size()));
+  j.set("sketch_seconds", Json::f64(0.0));
+  j.set("plan_seconds", Json::f64(0.0));
+  j.set("build_seconds", Json::f64(build_seconds));
+  j.set("input_document_count", Json::u64(m.colors.size()));
+  j.set("memory_guard_split_groups", Json::u64(0));
+  Json colors = Json::array();
+  for (auto &c : m.colors) {
+    Json o = Json::object();
+    o.set("name", Json::str(c.name));
+    o.set("path", Json::str(c.path));
+    o.set("occurrences", Json::u64(c.occurrences));
+    o.set("unique_kmers", Json::u64(c.unique_kmers));
+    colors.push(std::move(o));
+  }
+  j.set("colors", std::move(colors));
+  Json banks = Json::array();
+  for (auto &bm : m.banks) {
+    Json o = Json::object();
+    o.set("file", Json::str(bm.file));
+    o.set("slots", Json::u64(bm.slots));
+    o.set("row_bits", Json::u64(bm.row_bits));
+    Json gs = Json::array();
+    for (auto &g : bm.groups) {
+      Json go = Json::object();
+      Json mem = Json::array();
+      for (uint32_t x : g.members) mem.push(Json::u64(x));
+      go.set("members", std::move(mem));
+      go.set("width", Json::u64(g.width));
+      go.set("offset", Json::u64(g.offset));
+      go.set("union_cardinality", Json::u64(g.union_cardinality));
+      go.set("fallback_keys", Json::u64(g.fallback_keys));
+      go.set("abandoned_keys", Json::u64(g.abandoned_keys));
+      gs.push(std::move(go));
+    }
+    o.set("groups", std::move(gs));
+    o.set("fingerprint_first", Json::boolean(true));
+    banks.push(std::move(o));
+  }
+  j.set("banks", std::move(banks));
+  m.json = std::move(j);
+  validate_manifest(m);
+  idx->payload_loaded = true;
+  idx->rec = std::move(rec);
+  idx->have_rec = !cfg.drop_abandoned;
+  (void)abandoned_total;
+}
+}  // namespace zg
+
+extern "C" int zg_builder_finish(zg_builder *b, zg_index **out) {
+  try {
+    ensure(b && out, "null argument", -1);
+    b->drain();
+    ensure(!b->colors.empty(), "no datasets were added", -1);
+    if (b->cfg.merge_group_size > 1) {
+      // merged groups: consecutive datasets, built in parallel
+      uint32_t gsz = b->cfg.merge_group_size;
+      size_t n = b->colors.size();
+      size_t n_groups = (n + gsz - 1) / gsz;
+      b->groups.assign(n_groups, GroupResult());
+      for (size_t gi = 0; gi < n_groups; gi++) {
+        b->submit([b, gi, gsz, n] {
+          std::vector<uint32_t> members;
+          std::vector<std::vector<uint64_t> *> sets;
+          for (size_t c = gi * gsz; c < std::min(n, (gi + 1) * gsz); c++) {
+            members.push_back((uint32_t)c);
+            sets.push_back(&b->pending[c]);
+          }
+          build_group(b->cfg, members, sets, b->groups[gi]);
+        });
+      }
+      b->drain();
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - b->start).count();
+    auto idx = std::make_unique<zg_index>();
+    finish_index(b, idx.get(), secs);
+    delete b;
+    *out = idx.release();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_builder_destroy(zg_builder *b) {
+  delete b;
+  return 0;
+}
+
+// read_fof (src/lib.rs:323-381)
+static std::vector<std::pair<std::string, std::string>> read_fof(const std::string &path) {
+  std::string text = read_file(path);
+  std::vector<std::pair<std::string, std::string>> out;
+  size_t i = 0, line_no = 0;
+  auto trim = [](std::string s) {
+    size_t a = 0, e = s.size();
+    while (a < e && isspace((unsigned char)s[a])) a++;
+    while (e > a && isspace((unsigned char)s[e - 1])) e--;
+    return s.substr(a, e - a);
+  };
+  while (i <= text.size()) {
+    size_t j = text.find('\n', i);
+    if (j == std::string::npos) j = text.size();
+    std::string line = trim(text.substr(i, j - i));
+    line_no++;
+    i = j + 1;
+    if (line.empty() || line[0] == '#') {
+      if (j >= text.size()) break;
+      continue;
+    }
+    std::string name, p;
+    size_t tab = line.find('\t');
+    if (tab != std::string::npos) {
+      name = trim(line.substr(0, tab));
+      size_t tab2 = line.find('\t', tab + 1);
+      p = trim(line.substr(tab + 1, tab2 == std::string::npos ? std::string::npos : tab2 - tab - 1));
+    } else {
+      size_t sp = line.find_first_of(" \t\r\n\f\v");
+      if (sp != std::string::npos) {
+        name = line.substr(0, sp);
+        size_t b2 = line.find_first_not_of(" \t\r\n\f\v", sp);
+        size_t e2 = line.find_first_of(" \t\r\n\f\v", b2);
+        p = line.substr(b2, e2 == std::string::npos ? std::string::npos : e2 - b2);
+      } else {
+        p = line;
+        size_t slash = p.find_last_of('/');
+        name = slash == std::string::npos ? p : p.substr(slash + 1);
+        for (const char *ext : {".zstd", ".zst", ".gzip", ".gz", ".xz", ".bz2", ".bz", ".bzip2",
+                                ".fasta", ".fastq", ".fna", ".fa", ".fq", ".fas"}) {
+          size_t el = strlen(ext);
+          if (name.size() >= el && name.compare(name.size() - el, el, ext) == 0)
+            name.resize(name.size() - el);
+        }
+        if (name.empty()) name = "color";
+      }
+    }
+    ensure(!name.empty(), (path + ":" + std::to_string(line_no) + " has an empty color name").c_str(), -1);
+    out.push_back({name, p});
+    if (j >= text.size()) break;
+  }
+  return out;
+}
+
+extern "C" int zg_build_command(const zg_build_config *cfg, const char *fof, const char *out_dir,
+                                int err_fd) {
+  zg_builder *b = nullptr;
+  zg_index *idx = nullptr;
+  try {
+    ensure(cfg && fof && out_dir, "null argument", -1);
+    struct stat st;
+    ensure(stat(out_dir, &st) != 0, "output directory already exists", -1);
+    auto inputs = read_fof(fof);
+    ensure(!inputs.empty(), "the file-of-files lists no datasets", -1);
+    int rc = zg_builder_create(cfg, &b);
+    if (rc) return rc;
+    for (auto &in : inputs) {
+      Records r = parse_sequences(read_maybe_compressed(in.second));
+      rc = zg_builder_add_color(b, in.first.c_str(), (const uint8_t *)r.seq.data(), r.off.data(),
+                                r.size());
+      if (rc) {
+        zg_builder_destroy(b);
+        return rc;
+      }
+    }
+    rc = zg_builder_finish(b, &idx);
+    b = nullptr;
+    if (rc) return rc;
+    // keep the FOF paths in the manifest
+    Json *colors = nullptr;
+    for (auto &f : idx->m.json.fields)
+      if (f.first == "colors") colors = &f.second;
+    for (size_t i = 0; colors && i < colors->items.size(); i++)
+      colors->items[i].set("path", Json::str(inputs[i].second));
+    rc = zg_index_write(idx, out_dir);
+    if (rc == 0 && err_fd >= 0) {
+      size_t groups = 0;
+      for (auto &bk : idx->m.banks) groups += bk.groups.size();
+      std::string line = "built " + std::string(out_dir) + ": banks=" +
+                         std::to_string(idx->m.banks.size()) + ", groups=" + std::to_string(groups) +
+                         ", abandoned_hashes=" + std::to_string(idx->m.abandoned_hashes) + "\n";
+      ssize_t w = write(err_fd, line.data(), line.size());
+      (void)w;
+    }
+    zg_index_close(idx);
+    return rc;
+  } catch (...) {
+    if (b) zg_builder_destroy(b);
+    if (idx) zg_index_close(idx);
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_write(const zg_index *idx, const char *out_dir) {
+  try {
+    ensure(idx && out_dir, "null argument", -1);
+    ensure(idx->payload_loaded && idx->payload.size() == idx->m.banks.size(),
+           "bank payloads are not in host memory", -1);
+    std::string stage = std::string(out_dir) + ".staging";
+    ensure(mkdir(stage.c_str(), 0755) == 0, "cannot create staging directory", -2);
+    for (size_t i = 0; i < idx->m.banks.size(); i++)
+      write_bank_file(path_join(stage, idx->m.banks[i].file), idx->m.banks[i],
+                      idx->payload[i].data(), idx->payload[i].size(),
+                      (int)std::max(1u, std::thread::hardware_concurrency()));
+    if (idx->m.abandoned_recovery)
+      write_recovery_map(path_join(stage, kAbandonedFile), idx->rec);
+    std::string json;
+    json_write(json, idx->m.json);
+    write_file(path_join(stage, "manifest.json"), json);
+    ensure(rename(stage.c_str(), out_dir) == 0, "cannot publish the index directory", -2);
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}

@@ -1,0 +1,140 @@
+// zg_format.h -- the reference on-disk index format: manifest.json, bank files
+// (.zorbank.zst) and the abandoned-key recovery map (abandoned.zormap.zst).
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "zg_json.h"
+
+namespace zg {
+
+// Error carrying one of the ZG_ERR_* codes of zindex_gpu.h.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+inline void ensure(bool ok, const char *msg, int code = -3) {
+  if (!ok) fail(code, msg);
+}
+
+constexpr char kPlaneMagic[9] = "ZORPOW3";   // + NUL (8 bytes)
+constexpr char kLegacyMagic[9] = "ZORPOW1";
+constexpr char kAbandonedMagic[9] = "ZORABN1";
+constexpr char kAbandonedDeltaMagic[9] = "ZORABN2";
+constexpr const char *kAbandonedFile = "abandoned.zormap.zst";
+constexpr uint64_t kFingerprintSeed = 0x8E2D4F6A9B13C5D7ULL;
+constexpr int kMaxHashes = 32;
+
+// storage.rs:57-63
+struct Color {
+  std::string name, path;
+  uint64_t occurrences = 0;
+  uint64_t unique_kmers = 0;
+};
+
+// storage.rs:65-73
+struct GroupMeta {
+  std::vector<uint32_t> members;
+  uint64_t width = 0, offset = 0, union_cardinality = 0, fallback_keys = 0;
+  bool has_abandoned_keys = false;
+  uint64_t abandoned_keys = 0;
+};
+
+// storage.rs:75-131
+struct BankMeta {
+  uint64_t component = 0;
+  std::string file;
+  uint64_t slots = 0, row_bits = 0;
+  std::vector<GroupMeta> groups;
+  bool fingerprint_first = false;
+  bool routed = false;
+  uint64_t routing_source_slots = 0, partition = 0, partitions = 1;
+
+  uint64_t source_slots() const { return routed ? routing_source_slots : slots; }
+  std::pair<uint64_t, uint64_t> source_group() const {
+    return {component, source_slots()};
+  }
+  uint64_t row_bytes() const { return (row_bits + 7) / 8; }
+  uint64_t payload_bytes() const { return slots * row_bytes(); }
+  std::string expected_filename() const;
+  static std::string filename(uint64_t slots, bool routed, uint64_t source_slots,
+                              uint64_t partition, uint64_t partitions);
+};
+
+// storage.rs:133-258 (fields the query path reads; the rest is kept in `json`
+// so that info and rewrites preserve them).
+struct Manifest {
+  uint32_t version = 0;
+  bool abandoned_recovery = true;
+  bool has_abandoned_file = false;
+  std::string abandoned_file;
+  uint64_t abandoned_hashes = 0, abandoned_origins = 0;
+  std::string backend;
+  uint64_t k = 0, hashes = 0;
+  bool has_segment_length = false;
+  uint64_t segment_length = 0;
+  uint64_t seed = 0;
+  uint64_t fingerprint_bits = 0;
+  uint64_t index_mode = 0;
+  uint64_t minimizer_size = 0;
+  uint32_t minimizer_scheme = 0;  // 0 simd, 1 open-closed-mod
+  bool homopolymer_compression = false;
+  bool has_seed_pattern = false, has_randstrobemers = false;
+  std::string randstrobe_scheme;
+  uint64_t modimizer_sampling = 0, findere_z = 0;
+  double slot_scale = 1.0;
+  bool has_max_subindexes = false;
+  uint64_t max_subindexes = 0;
+  bool has_max_subindex_size = false;
+  uint64_t max_subindex_size = 0;
+  std::vector<Color> colors;
+  std::vector<BankMeta> banks;
+  Json json;  // the parsed document
+
+  size_t classes() const;
+  std::string merge_planner_label() const;
+};
+
+// Parses manifest.json text (serde rules: required fields, types, defaults).
+Manifest parse_manifest(const std::string &text);
+// Index::load_mode validation (storage.rs:2301-2637), metadata only.
+void validate_manifest(const Manifest &m);
+Manifest load_manifest(const std::string &dir);
+// Serializes a manifest built by this library.
+std::string write_manifest_json(const Manifest &m, const Json &extra);
+
+// ---- files ------------------------------------------------------------------
+std::string read_file(const std::string &path);
+void write_file(const std::string &path, const std::string &bytes);
+std::string path_join(const std::string &dir, const std::string &name);
+
+// zstd helpers
+std::string zstd_decompress_all(const std::string &compressed);  // any frames
+std::string zstd_compress(const void *data, size_t n, int level, bool checksum);
+
+// Bank payload: load_modern_bank (storage.rs:2232-2256) with
+// bank_io::read_parallel (bank_io.rs:122-291). Returns the decoded payload
+// (slots * row_bytes bytes) after header validation.
+std::vector<uint8_t> load_bank_payload(const std::string &dir, const BankMeta &meta,
+                                       int threads);
+// bank_io::Writer (bank_io.rs:34-120): 4 MiB checksummed frames + seek table.
+void write_bank_file(const std::string &path, const BankMeta &meta,
+                     const uint8_t *payload, uint64_t bytes, int threads);
+
+// Recovery map: sorted hashes, origin offsets and origins (dataset ids).
+struct RecoveryMap {
+  std::vector<uint32_t> hashes;
+  std::vector<uint64_t> off{0};
+  std::vector<uint32_t> origins;
+  size_t size() const { return hashes.size(); }
+};
+// AbandonedReader (abandoned_io.rs:43-129) over the whole file.
+RecoveryMap read_recovery_map(const std::string &dir, const Manifest &m);
+// write_abandoned_map (storage.rs:2053-2094), ZORABN2.
+void write_recovery_map(const std::string &path, const RecoveryMap &map);
+
+}  // namespace zg

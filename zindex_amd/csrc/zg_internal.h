@@ -1,0 +1,108 @@
+// zg_internal.h -- the objects behind the opaque handles of zindex_gpu.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/zindex_gpu.h"
+#include "zg_format.h"
+#include "zg_kernels.h"
+
+namespace zg {
+
+// Owning device allocation.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { reset(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t n);
+  template <class T> T *as() const { return (T *)p; }
+};
+
+void check_hip(hipError_t e, const char *what);
+
+struct DeviceBank {
+  DevBuf data, col_color, mg_off, mg_members, mg_mask_bit, color_col, color_bit;
+  BankDev view{};
+};
+
+// Index options: the persistent lookup's launch shape.
+struct Tuning {
+  uint32_t span = 4096;
+  uint32_t tiles_per_wave = 1;
+  uint32_t prefetch = 2;
+  uint32_t waves_per_cu = 0;  // 0 = occupancy limit
+};
+
+}  // namespace zg
+
+struct zg_index {
+  zg::Manifest m;
+  std::string dir;                               // empty when built in memory
+  std::vector<std::vector<uint8_t>> payload;     // host bank payloads
+  bool payload_loaded = false;
+  zg::RecoveryMap rec;
+  bool have_rec = false;                         // recovery map present in host memory
+  bool recover = true;                           // use it in queries
+  // device state
+  bool on_device = false;
+  int device = -1;
+  uint32_t cu_count = 0;
+  hipStream_t stream = nullptr;
+  std::vector<zg::DeviceBank> dbanks;
+  zg::DevBuf rec_dir, rec_post, inner_table;
+  zg::RecDev rec_view{};
+  zg::Tuning tuning;
+  ~zg_index();
+};
+
+struct zg_batch {
+  zg_index *idx = nullptr;
+  uint32_t flags = 0;
+  bool records = false;
+  uint64_t n_reads = 0, bases = 0, positions = 0;
+  std::vector<uint64_t> h_seg_off;  // read starts in the seed index space (host copy)
+  zg::DevBuf seq, read_off, pos_off, codes, invalid, totals;
+  // explicit seeds (minimizer mode)
+  bool explicit_keys = false;
+  uint64_t n_keys = 0;
+  zg::DevBuf flag, delta, block_counts, block_off, keys, key_win, key_off, scan_temp;
+  // outputs
+  zg::DevBuf counts, partial, agg, queue;
+  uint64_t waves = 0;
+  uint32_t blocks = 0;
+  zg::KernelConfig kcfg{};
+  uint32_t span = 0, n_tg = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_failed = false;         // capture refused: launch directly
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_lookup_ms = 0, last_seed_ms = 0, last_total_ms = 0;
+  ~zg_batch();
+};
+
+struct zg_builder;
+
+namespace zg {
+// Builds the device copy of idx (banks, column tables, recovery directory).
+void upload_index(zg_index *idx, uint64_t budget);
+// Host-side helpers shared with the builder.
+std::vector<uint64_t> ocm_inner_table(uint32_t s, uint64_t seed);
+}  // namespace zg

@@ -1,0 +1,131 @@
+// zg_kernels.h -- device-side argument blocks and launchers of the gfx950
+// kernels (zg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "zg_hash.h"
+
+namespace zg {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// One bank resident in HBM, in the plane layout (fingerprint_first).
+struct BankDev {
+  const uint8_t *data;        // slots * row_bytes (+ padding)
+  uint64_t row_bytes;
+  uint64_t plane_bytes;       // ceil(G * b / 8)
+  uint32_t groups;            // G
+  uint32_t first_merged;      // columns >= this are multi-member groups
+  uint32_t hashes;
+  uint32_t partition, partitions;
+  uint32_t n_tiles;           // row tiles of 64 lanes x field bytes
+  SegmentGeometry geo;
+  const uint32_t *col_color;  // [G] colour of a singleton column
+  const uint32_t *mg_off;     // [G - first_merged + 1] into mg_members
+  const uint32_t *mg_members; // colours of merged groups, bit order
+  const uint64_t *mg_mask_bit;// [G - first_merged] bit offset of the mask field
+  const uint32_t *color_col;  // [n_colors] column of a colour in this bank or kNone
+  const uint32_t *color_bit;  // [n_colors] member bit inside its merged group
+};
+
+// Recovery directory: open addressing on the high hash bits, 64-bit entries
+// (hash << 32 | posting index), empty = low word kNone. Postings hold
+// [count, origin...] with global colour ids.
+struct RecDev {
+  const uint64_t *dir;
+  const uint32_t *post;
+  uint32_t cap_log2;          // directory capacity 2^cap_log2
+  uint32_t enabled;
+};
+
+enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1 };
+
+struct LookupArgs {
+  BankDev bank;
+  RecDev rec;
+  // seeds
+  uint32_t source;            // SeedSource
+  uint32_t k;
+  const uint64_t *codes;      // packed bases, 32 per word (kSeedKmers)
+  const uint64_t *invalid;    // invalid-base bits, 64 per word (kSeedKmers)
+  const uint64_t *keys;       // explicit keys (kSeedKeys)
+  const uint64_t *seg_off;    // [n_reads + 1] read starts in the seed index space
+  uint64_t n_reads;
+  uint64_t begin, end;        // seed index range of this launch
+  uint64_t n_units;           // spans * tile groups
+  uint32_t span;              // seed indices per work unit (multiple of 64)
+  uint32_t n_tg;              // tile groups per span
+  uint32_t *queue;            // work counter, zero before launch
+  uint64_t seed;
+  uint32_t bits;
+  uint32_t records;           // 1: per-read counts; 0: per-wave partial sums
+  uint32_t *counts;           // [n_reads * n_colors]
+  uint64_t *partial;          // [waves * n_colors]
+  uint32_t n_colors;
+  uint32_t pad;
+};
+
+struct KernelConfig {
+  uint32_t bits;
+  uint32_t tiles_per_wave;  // 1 or 2
+  uint32_t prefetch;        // 1, 2 or 4
+  uint32_t blocks;          // grid size (256-thread blocks)
+};
+
+// Field bytes each lane reads per row and tile for fingerprint width b.
+inline uint32_t lane_bytes_for_bits(uint32_t b) {
+  return b == 1 ? 2 : b == 2 ? 4 : b == 4 ? 8 : 16;
+}
+inline uint32_t fields_per_lane(uint32_t b) { return lane_bytes_for_bits(b) * 8 / b; }
+
+// Launchers (all asynchronous on `stream`).
+void launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s);
+void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+// ASCII reads -> packed codes + invalid bits in position space (one invalid
+// separator after each read). pos_off[r] = read_off[r] + r.
+void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                 uint64_t n_reads, uint64_t positions, uint64_t *codes, uint64_t *invalid,
+                 hipStream_t s);
+// Valid k-mer windows per read (atomically added into totals).
+void launch_count_kmers(const uint64_t *invalid, const uint64_t *pos_off, uint64_t n_reads,
+                        uint64_t positions, uint32_t k, uint64_t *totals, hipStream_t s);
+// Open-closed mod-minimizer selection: per window start, flag + offset, and
+// per-block and per-read counts.
+void launch_ocm_select(const uint64_t *codes, const uint64_t *invalid, const uint64_t *pos_off,
+                       uint64_t n_reads, uint64_t positions, uint32_t k, uint32_t m,
+                       uint64_t seed, const uint64_t *inner_table, uint8_t *flag,
+                       uint8_t *delta, uint32_t *block_counts, uint64_t *totals,
+                       hipStream_t s);
+void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_t *delta,
+                        const uint64_t *block_off, uint64_t positions, uint32_t m,
+                        uint64_t *keys, uint64_t *key_pos, hipStream_t s);
+uint32_t ocm_block_positions();
+// Exclusive prefix sums (hipcub) with a caller-provided temporary buffer.
+size_t scan_temp_bytes_u64(uint64_t n);
+void launch_exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *temp,
+                               size_t temp_bytes, hipStream_t s);
+void launch_exclusive_scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_t n,
+                                      void *temp, size_t temp_bytes, hipStream_t s);
+size_t scan_temp_bytes_u32_to_u64(uint64_t n);
+// The lookup.
+void launch_lookup(const LookupArgs &a, const KernelConfig &c, hipStream_t s);
+// Sum per-wave partials into agg[n_colors] (added).
+void launch_reduce_partials(const uint64_t *partial, uint64_t waves, uint32_t n_colors,
+                            uint64_t *agg, hipStream_t s);
+// Occupancy-derived number of resident 256-thread blocks per CU for a config.
+int lookup_blocks_per_cu(const KernelConfig &c);
+
+}  // namespace zg
+
+namespace zg {
+// key_off[r] = first key whose window start is >= pos_off[r] (r <= n_reads).
+void launch_segment_bounds(const uint64_t *key_win, uint64_t n_keys, const uint64_t *pos_off,
+                           uint64_t n_reads, uint64_t *key_off, hipStream_t s);
+// totals[r] = off[r+1] - off[r]
+void launch_diff(const uint64_t *off, uint64_t n, uint64_t *totals, hipStream_t s);
+// agg[c] += sum over reads of counts[r * n_colors + c]
+void launch_colsum(const uint32_t *counts, uint64_t n_reads, uint32_t n_colors, uint64_t *agg,
+                   hipStream_t s);
+}  // namespace zg

@@ -1,0 +1,892 @@
+// zg_kernels.hip -- gfx950 kernels of the zorindex query path.
+//
+// Data layout in HBM (one query batch):
+//   codes    2-bit bases, 32 per u64, base p at bits 2(p%32) of word p/32
+//   invalid  1 bit per base, 64 per u64; set for non-ACGT bytes, for the one
+//            separator position after every read, and for padding
+//   bank     slots * row_bytes bytes, plane layout: the G fingerprint fields of
+//            a row first (b bits each, little endian), the membership masks of
+//            multi-member groups after them
+// The lookup is HBM-bound random row gathers: every seed reads H rows of the
+// first ceil(G*b/8) bytes. A wave owns a tile of 64 lanes x VB bytes of those
+// rows, so one seed costs H coalesced 64*VB-byte reads per tile, XOR and
+// compare stay in registers and per-read colour counters live in VGPRs.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "zg_kernels.h"
+
+namespace zg {
+
+namespace {
+
+__device__ __forceinline__ int encode_ascii(uint8_t c) {
+  switch (c) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return -1;
+  }
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+  uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t readlane32(uint32_t v, int l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+// 64 bits of a bit/pair stream starting at bit `bit` (words padded by one).
+__device__ __forceinline__ uint64_t stream_bits(const uint64_t *words, uint64_t bit) {
+  uint64_t w = bit >> 6;
+  uint32_t sh = (uint32_t)(bit & 63);
+  uint64_t lo = words[w];
+  if (!sh) return lo;
+  return (lo >> sh) | (words[w + 1] << (64 - sh));
+}
+
+// Largest r in [0, n) with off[r] <= x (off is non-decreasing, off[0] <= x).
+__device__ __forceinline__ uint64_t find_segment(const uint64_t *off, uint64_t n, uint64_t x) {
+  uint64_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Fills
+// ---------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 65536) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return (uint32_t)(g > cap ? cap : g);
+}
+
+void launch_fill_u32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (!n) return;
+  fill_u32_kernel<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
+}
+void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (!n) return;
+  fill_u64_kernel<<<grid_for(n, 256, 16384), 256, 0, s>>>(p, n, v);
+}
+
+// ---------------------------------------------------------------------------
+// Packing: one thread per 64 positions.
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(const uint8_t *seq, const uint64_t *read_off,
+                            const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+                            uint64_t *codes, uint64_t *invalid) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t n_words = (positions + 63) / 64;
+  if (w >= n_words) return;
+  uint64_t p0 = w * 64;
+  uint64_t r = find_segment(pos_off, n_reads, p0);
+  uint64_t inv = 0, c0 = 0, c1 = 0;
+  for (int i = 0; i < 64; i++) {
+    uint64_t p = p0 + i;
+    int code = -1;
+    if (p < positions) {
+      while (pos_off[r + 1] <= p) r++;
+      uint64_t off = p - pos_off[r];
+      uint64_t start = read_off[r], len = read_off[r + 1] - start;
+      if (off < len) code = encode_ascii(seq[start + off]);
+    }
+    if (code < 0) {
+      inv |= 1ULL << i;
+      code = 0;
+    }
+    if (i < 32) c0 |= (uint64_t)code << (2 * i);
+    else c1 |= (uint64_t)code << (2 * (i - 32));
+  }
+  invalid[w] = inv;
+  codes[2 * w] = c0;
+  codes[2 * w + 1] = c1;
+}
+
+void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                 uint64_t n_reads, uint64_t positions, uint64_t *codes, uint64_t *invalid,
+                 hipStream_t s) {
+  uint64_t words = (positions + 63) / 64;
+  if (!words || !n_reads) return;
+  pack_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(seq, read_off, pos_off, n_reads,
+                                                              positions, codes, invalid);
+}
+
+// ---------------------------------------------------------------------------
+// Valid k-mer windows per read (k <= 32): one thread per 64 window starts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invalid, uint64_t w,
+                                                          uint32_t k) {
+  // bit i set iff window [64w+i, 64w+i+k) contains an invalid base
+  uint64_t lo = invalid[w], hi = invalid[w + 1];
+  uint64_t m = 0;
+  for (uint32_t d = 0; d < k; d++) m |= d ? ((lo >> d) | (hi << (64 - d))) : lo;
+  return m;
+}
+
+__global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *pos_off,
+                                   uint64_t n_reads, uint64_t positions, uint32_t k,
+                                   uint64_t *totals) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t n_words = (positions + 63) / 64;
+  if (w >= n_words) return;
+  uint64_t valid = ~invalid_window_starts(invalid, w, k);
+  uint64_t p0 = w * 64;
+  uint64_t r = find_segment(pos_off, n_reads, p0);
+  while (r < n_reads && pos_off[r] < p0 + 64) {
+    uint64_t a = pos_off[r] > p0 ? pos_off[r] - p0 : 0;
+    uint64_t b = pos_off[r + 1] - p0;
+    if (b > 64) b = 64;
+    if (b > a) {
+      uint64_t span = b - a;
+      uint64_t mask = (span == 64 ? ~0ULL : ((1ULL << span) - 1)) << a;
+      uint64_t c = __popcll(valid & mask);
+      if (c) atomicAdd((unsigned long long *)&totals[r], (unsigned long long)c);
+    }
+    if (pos_off[r + 1] >= p0 + 64) break;
+    r++;
+  }
+}
+
+void launch_count_kmers(const uint64_t *invalid, const uint64_t *pos_off, uint64_t n_reads,
+                        uint64_t positions, uint32_t k, uint64_t *totals, hipStream_t s) {
+  uint64_t words = (positions + 63) / 64;
+  if (!words || !n_reads) return;
+  count_kmers_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(invalid, pos_off, n_reads,
+                                                                     positions, k, totals);
+}
+
+// ---------------------------------------------------------------------------
+// Open-closed mod-minimizers (utils/ocm-minimizers/src/implementation.rs).
+// A block evaluates 256 consecutive window starts plus the one before them.
+// Inner s-mer hashes and anchor ranks (class, t-mer hash) for the block's
+// positions are staged in LDS; every thread then takes the minimum rank over
+// the k-t+1 anchors of its window, keeping the leftmost and the rightmost tie,
+// and picks by the GT majority of the window. A window is emitted when it
+// starts a run (the previous window is invalid) or its position differs from
+// the previous window's, which is the reference's `previous != pos` rule.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kOcmTile = 256;
+constexpr uint32_t kOcmSpan = kOcmTile + 1 + 256;  // window starts - 1 .. + k (k <= 255)
+
+uint32_t ocm_block_positions() { return kOcmTile; }
+
+__global__ __launch_bounds__(256) void ocm_select_kernel(
+    const uint64_t *codes, const uint64_t *invalid, uint64_t positions, uint32_t k,
+    uint32_t m, uint64_t seed, const uint64_t *inner_table, uint8_t *flag, uint8_t *delta,
+    uint32_t *block_counts) {
+  __shared__ uint8_t code_s[kOcmSpan];
+  __shared__ uint8_t bad_s[kOcmSpan];
+  __shared__ uint64_t ih_s[kOcmSpan];
+  __shared__ uint64_t ah_s[kOcmSpan];
+  __shared__ uint8_t ac_s[kOcmSpan];
+  __shared__ uint32_t pos_s[kOcmTile + 1];
+  __shared__ uint8_t ok_s[kOcmTile + 1];
+  __shared__ uint32_t wave_counts[4];
+
+  const OcmParams P = ocm_parameters((int)k, (int)m);
+  const uint64_t S0 = (uint64_t)blockIdx.x * kOcmTile;
+  const int64_t base = (int64_t)S0 - 1;  // position of local index 0
+  const uint32_t span = kOcmTile + k;      // local positions needed: base .. base+span-1
+  const uint32_t tid = threadIdx.x;
+
+  for (uint32_t i = tid; i < span; i += blockDim.x) {
+    int64_t p = base + (int64_t)i;
+    uint8_t c = 0, bad = 1;
+    if (p >= 0 && (uint64_t)p < positions) {
+      uint64_t up = (uint64_t)p;
+      c = (uint8_t)((codes[up >> 5] >> (2 * (up & 31))) & 3);
+      bad = (uint8_t)((invalid[up >> 6] >> (up & 63)) & 1);
+    }
+    code_s[i] = c;
+    bad_s[i] = bad;
+  }
+  __syncthreads();
+  const uint32_t s = (uint32_t)P.s, t = (uint32_t)P.t;
+  for (uint32_t i = tid; i + s <= span; i += blockDim.x) {
+    uint32_t x = 0;
+    for (uint32_t j = 0; j < s; j++) x = (x << 2) | code_s[i + j];
+    ih_s[i] = inner_table[x];
+  }
+  __syncthreads();
+  const uint64_t tmask = (1ULL << (2 * t)) - 1;
+  for (uint32_t i = tid; i + t <= span; i += blockDim.x) {
+    uint64_t mn = ~0ULL;
+    for (uint32_t j = 0; j + s <= t; j++) mn = ih_s[i + j] < mn ? ih_s[i + j] : mn;
+    uint8_t cls;
+    if (ih_s[i + (t - s) / 2] == mn) cls = 0;
+    else if (ih_s[i] == mn || ih_s[i + t - s] == mn) cls = 1;
+    else cls = 2;
+    uint64_t f = 0, r = 0;
+    for (uint32_t j = 0; j < t; j++) {
+      f = (f << 2) | code_s[i + j];
+      r |= (uint64_t)(code_s[i + j] ^ 3) << (2 * j);
+    }
+    f &= tmask;
+    ac_s[i] = cls;
+    ah_s[i] = mixsplit(f < r ? f : r, seed);
+  }
+  __syncthreads();
+  // windows: local index w = 0 is start base (S0 - 1), w = 1 + tid is S0 + tid
+  for (uint32_t w = tid; w < kOcmTile + 1; w += blockDim.x) {
+    int64_t start = base + (int64_t)w;
+    uint8_t ok = start >= 0 && (uint64_t)start + k <= positions;
+    uint32_t gt = 0;
+    if (ok) {
+      for (uint32_t j = 0; j < k; j++) {
+        ok &= !bad_s[w + j];
+        gt += code_s[w + j] >= 2;
+      }
+    }
+    uint32_t pos = 0;
+    if (ok) {
+      uint8_t bc = 255;
+      uint64_t bh = 0;
+      uint32_t first = 0, last = 0;
+      for (uint32_t a = 0; a + t <= k; a++) {
+        uint8_t c = ac_s[w + a];
+        uint64_t h = ah_s[w + a];
+        if (c < bc || (c == bc && h < bh)) {
+          bc = c;
+          bh = h;
+          first = last = a;
+        } else if (c == bc && h == bh) {
+          last = a;
+        }
+      }
+      uint32_t anchor = gt > k / 2 ? last : first;
+      pos = anchor % (uint32_t)P.w;
+    }
+    ok_s[w] = ok;
+    pos_s[w] = pos;
+  }
+  __syncthreads();
+  uint64_t S = S0 + tid;
+  bool emit = false;
+  uint8_t d = 0;
+  if (S < positions && ok_s[tid + 1]) {
+    d = (uint8_t)pos_s[tid + 1];
+    // previous window start S-1 selects position (S-1) + pos_s[tid]
+    emit = !ok_s[tid] || (uint64_t)(S - 1 + pos_s[tid]) != S + d;
+  }
+  if (S < positions) {
+    flag[S] = emit;
+    delta[S] = d;
+  }
+  uint64_t b = __ballot(emit);
+  if ((tid & 63) == 0) wave_counts[tid >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  if (tid == 0)
+    block_counts[blockIdx.x] = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
+}
+
+__global__ __launch_bounds__(256) void ocm_compact_kernel(const uint64_t *codes,
+                                                          const uint8_t *flag,
+                                                          const uint8_t *delta,
+                                                          const uint64_t *block_off,
+                                                          uint64_t positions, uint32_t m,
+                                                          uint64_t *keys, uint64_t *key_win) {
+  __shared__ uint32_t wave_counts[4];
+  const uint32_t tid = threadIdx.x;
+  uint64_t S = (uint64_t)blockIdx.x * kOcmTile + tid;
+  bool emit = S < positions && flag[S];
+  uint64_t b = __ballot(emit);
+  uint32_t lane = tid & 63;
+  uint32_t below = (uint32_t)__popcll(b & ((1ULL << lane) - 1));
+  if (lane == 0) wave_counts[tid >> 6] = (uint32_t)__popcll(b);
+  __syncthreads();
+  uint32_t before = 0;
+  for (uint32_t w = 0; w < (tid >> 6); w++) before += wave_counts[w];
+  if (emit) {
+    uint64_t at = block_off[blockIdx.x] + before + below;
+    uint64_t pos = S + delta[S];
+    keys[at] = canonical_from_window(stream_bits(codes, 2 * pos), (int)m);
+    key_win[at] = S;
+  }
+}
+
+void launch_ocm_select(const uint64_t *codes, const uint64_t *invalid, const uint64_t *pos_off,
+                       uint64_t n_reads, uint64_t positions, uint32_t k, uint32_t m,
+                       uint64_t seed, const uint64_t *inner_table, uint8_t *flag,
+                       uint8_t *delta, uint32_t *block_counts, uint64_t *totals,
+                       hipStream_t s) {
+  (void)pos_off;
+  (void)n_reads;
+  (void)totals;
+  uint64_t blocks = (positions + kOcmTile - 1) / kOcmTile;
+  if (!blocks) return;
+  ocm_select_kernel<<<(uint32_t)blocks, 256, 0, s>>>(codes, invalid, positions, k, m, seed,
+                                                     inner_table, flag, delta, block_counts);
+}
+
+void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_t *delta,
+                        const uint64_t *block_off, uint64_t positions, uint32_t m,
+                        uint64_t *keys, uint64_t *key_pos, hipStream_t s) {
+  uint64_t blocks = (positions + kOcmTile - 1) / kOcmTile;
+  if (!blocks) return;
+  ocm_compact_kernel<<<(uint32_t)blocks, 256, 0, s>>>(codes, flag, delta, block_off, positions,
+                                                      m, keys, key_pos);
+}
+
+// ---------------------------------------------------------------------------
+// Scans (hipcub)
+// ---------------------------------------------------------------------------
+size_t scan_temp_bytes_u64(uint64_t n) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t *)nullptr,
+                                   (uint64_t *)nullptr, (int64_t)n);
+  return bytes;
+}
+void launch_exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *temp,
+                               size_t temp_bytes, hipStream_t s) {
+  hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int64_t)n, s);
+}
+size_t scan_temp_bytes_u32_to_u64(uint64_t n) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, (const uint32_t *)nullptr,
+                                    (uint64_t *)nullptr, hipcub::Sum(), (uint64_t)0,
+                                    (int64_t)n);
+  return bytes;
+}
+void launch_exclusive_scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_t n,
+                                      void *temp, size_t temp_bytes, hipStream_t s) {
+  hipcub::DeviceScan::ExclusiveScan(temp, temp_bytes, in, out, hipcub::Sum(), (uint64_t)0,
+                                    (int64_t)n, s);
+}
+
+// ---------------------------------------------------------------------------
+// The lookup (storage.rs:2761-2925 on the plane layout)
+// ---------------------------------------------------------------------------
+template <int B>
+struct FieldCfg {
+  static constexpr int VB = B == 1 ? 2 : B == 2 ? 4 : B == 4 ? 8 : 16;  // bytes per lane
+  static constexpr int F = VB * 8 / B;                                   // fields per lane
+  static constexpr int NW = VB >= 4 ? VB / 4 : 1;                        // dwords per lane
+};
+
+// VB bytes at p (any alignment when !aligned; over-reads stay in padding).
+template <int B>
+__device__ __forceinline__ void load_lane(const uint8_t *p, bool aligned,
+                                          uint32_t (&x)[FieldCfg<B>::NW]) {
+  constexpr int VB = FieldCfg<B>::VB;
+  constexpr int NW = FieldCfg<B>::NW;
+  if (aligned) {
+    if constexpr (VB == 16) {
+      uint4 v = *(const uint4 *)p;
+      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    } else if constexpr (VB == 8) {
+      uint2 v = *(const uint2 *)p;
+      x[0] = v.x; x[1] = v.y;
+    } else if constexpr (VB == 4) {
+      x[0] = *(const uint32_t *)p;
+    } else {
+      x[0] = *(const uint16_t *)p;
+    }
+    return;
+  }
+  const uint32_t *a = (const uint32_t *)((uintptr_t)p & ~(uintptr_t)3);
+  uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+  uint32_t d[NW + 1];
+#pragma unroll
+  for (int i = 0; i <= NW; i++) d[i] = a[i];
+#pragma unroll
+  for (int i = 0; i < NW; i++) x[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+  if constexpr (VB == 2) x[0] &= 0xFFFFu;
+}
+
+// Bit f of the result: field f of the lane's bytes equals fp.
+template <int B>
+__device__ __forceinline__ uint32_t field_hits(const uint32_t (&x)[FieldCfg<B>::NW],
+                                               uint32_t fp) {
+  uint32_t m = 0;
+  if constexpr (B == 8) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      uint32_t v = x[w] ^ (fp * 0x01010101u);
+      uint32_t z = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+      m |= ((z >> 7) & 1u) << (4 * w);
+      m |= ((z >> 15) & 1u) << (4 * w + 1);
+      m |= ((z >> 23) & 1u) << (4 * w + 2);
+      m |= (z >> 31) << (4 * w + 3);
+    }
+  } else if constexpr (B == 16) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+      m |= (uint32_t)((x[w] & 0xFFFFu) == fp) << (2 * w);
+      m |= (uint32_t)((x[w] >> 16) == fp) << (2 * w + 1);
+    }
+  } else if constexpr (B == 32) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) m |= (uint32_t)(x[w] == fp) << w;
+  } else if constexpr (B == 4) {
+#pragma unroll
+    for (int w = 0; w < 2; w++) {
+      uint32_t v = x[w] ^ (fp * 0x11111111u);
+      uint32_t z = ~(((v & 0x77777777u) + 0x77777777u) | v) & 0x88888888u;
+#pragma unroll
+      for (int i = 0; i < 8; i++) m |= ((z >> (4 * i + 3)) & 1u) << (8 * w + i);
+    }
+  } else if constexpr (B == 2) {
+    uint32_t v = x[0] ^ (fp * 0x55555555u);
+    uint32_t z = ~(v | (v >> 1)) & 0x55555555u;
+#pragma unroll
+    for (int i = 0; i < 16; i++) m |= ((z >> (2 * i)) & 1u) << i;
+  } else {
+    m = (fp ? x[0] : ~x[0]) & 0xFFFFu;
+  }
+  return m;
+}
+
+// get_bits (storage.rs:3525-3538) on one row, byte loads (rare paths only).
+__device__ uint64_t row_bits(const uint8_t *row, uint64_t offset, uint32_t width) {
+  uint64_t byte = offset >> 3;
+  uint32_t shift = (uint32_t)(offset & 7);
+  uint32_t count = (shift + width + 7) >> 3;
+  uint64_t v = 0;
+  uint32_t first = count < 8 ? count : 8;
+  for (uint32_t i = 0; i < first; i++) v |= (uint64_t)row[byte + i] << (8 * i);
+  v >>= shift;
+  if (count == 9) v |= (uint64_t)row[byte + 8] << (64 - shift);
+  if (width < 64) v &= (1ULL << width) - 1;
+  return v;
+}
+
+// XOR over the key's H rows of get_bits(row, offset, width).
+__device__ uint64_t xor_rows_bits(const BankDev &bk, uint64_t key, uint64_t seed,
+                                  uint64_t offset, uint32_t width) {
+  uint64_t first = mixsplit(key, seed);
+  uint64_t step = splitmix64(first);
+  uint64_t base = slot_base(first, bk.geo);
+  uint64_t h = first, v = 0;
+  for (uint32_t i = 0; i < bk.hashes; i++) {
+    uint64_t slot = slot_at(base, h, i, bk.geo);
+    v ^= row_bits(bk.data + slot * bk.row_bytes, offset, width);
+    h += step;
+  }
+  return v;
+}
+
+template <int B, int T, int D>
+__global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
+  using C = FieldCfg<B>;
+  constexpr int F = C::F;
+  constexpr int NW = C::NW;
+  constexpr int VB = C::VB;
+  constexpr int HB = 4;  // rows loaded per block of hashes
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave_slot = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const BankDev &bk = a.bank;
+  const bool aligned = (bk.row_bytes % VB) == 0;
+  const uint32_t tile_fields = 64u * F;
+  const uint32_t H = bk.hashes;
+  const uint64_t kmask = a.k >= 32 ? ~0ULL : ((1ULL << (2 * a.k)) - 1);
+  (void)kmask;
+  const uint64_t win_mask = a.k >= 64 ? ~0ULL : ((1ULL << a.k) - 1);
+
+  uint32_t acc[T][F];
+#pragma unroll
+  for (int t = 0; t < T; t++)
+#pragma unroll
+    for (int f = 0; f < F; f++) acc[t][f] = 0;
+  int64_t held_tg = -1;     // aggregate mode: tile group whose sums acc holds
+  uint64_t held_keys = 0;
+
+  uint32_t col0[T], fmask[T], mmask[T];
+
+  auto flush_aggregate = [&]() {
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        if (acc[t][f]) {
+          uint32_t color = bk.col_color[col0[t] + f];
+          atomicAdd((unsigned long long *)&a.partial[wave_slot * a.n_colors + color],
+                    (unsigned long long)acc[t][f]);
+          acc[t][f] = 0;
+        }
+      }
+    }
+  };
+
+  while (true) {
+    uint32_t unit = 0;
+    if (lane == 0) unit = atomicAdd(a.queue, 1u);
+    unit = readlane32(unit, 0);
+    if (unit >= a.n_units) break;
+    const uint64_t sp = unit / a.n_tg;
+    const uint32_t tg = unit % a.n_tg;
+    const uint64_t i0 = a.begin + sp * (uint64_t)a.span;
+    uint64_t i1 = i0 + a.span;
+    if (i1 > a.end) i1 = a.end;
+
+    if (!a.records && held_tg != (int64_t)tg) {
+      if (held_tg >= 0) flush_aggregate();
+      held_tg = tg;
+      held_keys = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+      uint32_t tile = tg * T + t;
+      col0[t] = tile * tile_fields + lane * F;
+      uint32_t fm = 0, mm = 0;
+      if (tile < bk.n_tiles) {
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+          uint32_t col = col0[t] + f;
+          if (col < bk.groups) {
+            fm |= 1u << f;
+            if (col >= bk.first_merged) mm |= 1u << f;
+          }
+        }
+      }
+      fmask[t] = fm;
+      mmask[t] = mm;
+    }
+
+    uint64_t r = 0, seg_end = i1;
+    if (a.records) {
+      r = find_segment(a.seg_off, a.n_reads, i0);
+      seg_end = a.seg_off[r + 1] < i1 ? a.seg_off[r + 1] : i1;
+    }
+    auto flush_read = [&]() {
+#pragma unroll
+      for (int t = 0; t < T; t++) {
+#pragma unroll
+        for (int f = 0; f < F; f++) {
+          if (acc[t][f]) {
+            uint32_t color = bk.col_color[col0[t] + f];
+            atomicAdd(&a.counts[r * a.n_colors + color], acc[t][f]);
+            acc[t][f] = 0;
+          }
+        }
+      }
+    };
+    auto out_add = [&](uint32_t color) {
+      if (a.records) atomicAdd(&a.counts[r * a.n_colors + color], 1u);
+      else atomicAdd((unsigned long long *)&a.partial[wave_slot * a.n_colors + color], 1ULL);
+    };
+
+    for (uint64_t c = i0; c < i1; c += 64) {
+      const uint64_t idx = c + lane;
+      bool v = idx < i1;
+      uint64_t key = 0;
+      if (a.source == kSeedKmers) {
+        if (v) {
+          uint64_t iv = stream_bits(a.invalid, idx);
+          v = (iv & win_mask) == 0;
+          key = canonical_from_window(stream_bits(a.codes, 2 * idx), (int)a.k);
+        }
+      } else if (v) {
+        key = a.keys[idx];
+      }
+      if (v && bk.partitions > 1) v = partition_id(key, a.seed, bk.partitions) == bk.partition;
+      uint32_t fp = v ? (uint32_t)fingerprint(key, a.seed, a.bits) : 0;
+      uint32_t rec_start = 0, rec_len = 0;
+      if (a.rec.enabled && v) {
+        uint32_t h = abandoned_hash(key, a.seed);
+        uint64_t cap_mask = (1ULL << a.rec.cap_log2) - 1;
+        uint64_t slot = a.rec.cap_log2 ? (uint64_t)(h >> (32 - a.rec.cap_log2)) : 0;
+        while (true) {
+          uint64_t e = a.rec.dir[slot];
+          uint32_t pi = (uint32_t)e;
+          if (pi == kNone) break;
+          if ((uint32_t)(e >> 32) == h) {
+            rec_len = a.rec.post[pi];
+            rec_start = pi + 1;
+            break;
+          }
+          slot = (slot + 1) & cap_mask;
+        }
+      }
+      uint64_t vmask = __ballot(v);
+      held_keys += __popcll(vmask);
+
+      while (vmask) {
+        int jj[D];
+        uint64_t kk[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          if (vmask) {
+            jj[d] = __builtin_ctzll(vmask);
+            vmask &= vmask - 1;
+            kk[d] = readlane64(key, jj[d]);
+          } else {
+            jj[d] = -1;
+            kk[d] = 0;
+          }
+        }
+        // Issue the row loads of all D keys before consuming any.
+        uint32_t x[D][T][NW];
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int t = 0; t < T; t++)
+#pragma unroll
+            for (int w = 0; w < NW; w++) x[d][t][w] = 0;
+        uint64_t first[D], step[D], base[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          first[d] = mixsplit(kk[d], a.seed);
+          step[d] = splitmix64(first[d]);
+          base[d] = slot_base(first[d], bk.geo);
+        }
+        for (uint32_t h0 = 0; h0 < H; h0 += HB) {
+          uint32_t raw[D][HB][T][NW];
+#pragma unroll
+          for (int d = 0; d < D; d++) {
+#pragma unroll
+            for (int hh = 0; hh < HB; hh++) {
+              uint32_t i = h0 + hh;
+              uint64_t hv = first[d] + (uint64_t)i * step[d];
+              uint64_t slot = slot_at(base[d], hv, i, bk.geo);
+              const uint8_t *row = bk.data + slot * bk.row_bytes;
+#pragma unroll
+              for (int t = 0; t < T; t++) {
+                if (jj[d] >= 0 && i < H && fmask[t]) {
+                  uint32_t tile = tg * T + t;
+                  load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, aligned,
+                               raw[d][hh][t]);
+                } else {
+#pragma unroll
+                  for (int w = 0; w < NW; w++) raw[d][hh][t][w] = 0;
+                }
+              }
+            }
+          }
+#pragma unroll
+          for (int d = 0; d < D; d++)
+#pragma unroll
+            for (int hh = 0; hh < HB; hh++)
+#pragma unroll
+              for (int t = 0; t < T; t++)
+#pragma unroll
+                for (int w = 0; w < NW; w++) x[d][t][w] ^= raw[d][hh][t][w];
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          if (jj[d] < 0) continue;
+          const uint64_t kidx = c + (uint64_t)jj[d];
+          if (a.records) {
+            while (kidx >= seg_end) {
+              flush_read();
+              r++;
+              seg_end = a.seg_off[r + 1] < i1 ? a.seg_off[r + 1] : i1;
+            }
+          }
+          const uint32_t fpd = readlane32(fp, jj[d]);
+          uint32_t m[T];
+#pragma unroll
+          for (int t = 0; t < T; t++) m[t] = field_hits<B>(x[d][t], fpd) & fmask[t];
+          // Recovery: OR recovered colours into this key's hits
+          // (storage.rs:2771-2787, 2916-2924).
+          const uint32_t rl = readlane32(rec_len, jj[d]);
+          if (rl) {
+            const uint32_t rs = readlane32(rec_start, jj[d]);
+            for (uint32_t q = 0; q < rl; q++) {
+              uint32_t o = a.rec.post[rs + q];
+              uint32_t col = bk.color_col[o];
+              if (col == kNone) continue;
+              uint32_t tile = col / tile_fields;
+              if (tile < tg * T || tile >= tg * T + T) continue;
+              uint32_t within = col % tile_fields;
+              uint32_t ol = within / F, of = within % F;
+              int tt = (int)(tile - tg * T);
+#pragma unroll
+              for (int t = 0; t < T; t++) {
+                if (t != tt || lane != ol) continue;
+                if (col < bk.first_merged) {
+                  m[t] |= 1u << of;
+                } else {
+                  bool hit = false;
+                  if ((m[t] >> of) & 1u) {
+                    uint64_t bit = bk.mg_mask_bit[col - bk.first_merged] + bk.color_bit[o];
+                    hit = xor_rows_bits(bk, kk[d], a.seed, bit, 1) & 1;
+                  }
+                  if (!hit) out_add(o);
+                }
+              }
+            }
+          }
+          // Multi-member groups: decode the XORed membership mask.
+#pragma unroll
+          for (int t = 0; t < T; t++) {
+            uint32_t mh = m[t] & mmask[t];
+            m[t] &= ~mmask[t];
+            while (mh) {
+              int f = __builtin_ctz(mh);
+              mh &= mh - 1;
+              uint32_t g = col0[t] + f - bk.first_merged;
+              uint32_t lo = bk.mg_off[g], width = bk.mg_off[g + 1] - lo;
+              uint64_t mask = xor_rows_bits(bk, kk[d], a.seed, bk.mg_mask_bit[g], width);
+              while (mask) {
+                int b = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                out_add(bk.mg_members[lo + b]);
+              }
+            }
+          }
+#pragma unroll
+          for (int t = 0; t < T; t++)
+#pragma unroll
+            for (int f = 0; f < F; f++) acc[t][f] += (m[t] >> f) & 1u;
+        }
+      }
+    }
+    if (a.records) {
+      flush_read();
+    } else if (held_keys > (1ULL << 30)) {
+      flush_aggregate();
+      held_keys = 0;
+    }
+  }
+  if (!a.records && held_tg >= 0) flush_aggregate();
+}
+
+template <int B, int T, int D>
+static void launch_one(const LookupArgs &a, uint32_t blocks, hipStream_t s) {
+  lookup_kernel<B, T, D><<<blocks, 256, 0, s>>>(a);
+}
+
+template <int B, int T>
+static void launch_d(const LookupArgs &a, const KernelConfig &c, hipStream_t s) {
+  switch (c.prefetch) {
+    case 1: launch_one<B, T, 1>(a, c.blocks, s); break;
+    case 4: launch_one<B, T, 4>(a, c.blocks, s); break;
+    default: launch_one<B, T, 2>(a, c.blocks, s); break;
+  }
+}
+
+template <int B>
+static void launch_t(const LookupArgs &a, const KernelConfig &c, hipStream_t s) {
+  if (c.tiles_per_wave == 2) launch_d<B, 2>(a, c, s);
+  else launch_d<B, 1>(a, c, s);
+}
+
+void launch_lookup(const LookupArgs &a, const KernelConfig &c, hipStream_t s) {
+  switch (c.bits) {
+    case 1: launch_t<1>(a, c, s); break;
+    case 2: launch_t<2>(a, c, s); break;
+    case 4: launch_t<4>(a, c, s); break;
+    case 8: launch_t<8>(a, c, s); break;
+    case 16: launch_t<16>(a, c, s); break;
+    default: launch_t<32>(a, c, s); break;
+  }
+}
+
+template <int B, int T, int D>
+static int occupancy_one() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, lookup_kernel<B, T, D>, 256, 0) !=
+      hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+template <int B, int T>
+static int occupancy_d(uint32_t d) {
+  return d == 1 ? occupancy_one<B, T, 1>() : d == 4 ? occupancy_one<B, T, 4>()
+                                                    : occupancy_one<B, T, 2>();
+}
+
+template <int B>
+static int occupancy_t(const KernelConfig &c) {
+  return c.tiles_per_wave == 2 ? occupancy_d<B, 2>(c.prefetch) : occupancy_d<B, 1>(c.prefetch);
+}
+
+int lookup_blocks_per_cu(const KernelConfig &c) {
+  switch (c.bits) {
+    case 1: return occupancy_t<1>(c);
+    case 2: return occupancy_t<2>(c);
+    case 4: return occupancy_t<4>(c);
+    case 8: return occupancy_t<8>(c);
+    case 16: return occupancy_t<16>(c);
+    default: return occupancy_t<32>(c);
+  }
+}
+
+__global__ void reduce_partials_kernel(const uint64_t *partial, uint64_t waves,
+                                       uint32_t n_colors, uint64_t *agg) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_colors) return;
+  uint64_t s = 0;
+  for (uint64_t w = 0; w < waves; w++) s += partial[w * n_colors + c];
+  agg[c] += s;
+}
+
+void launch_reduce_partials(const uint64_t *partial, uint64_t waves, uint32_t n_colors,
+                            uint64_t *agg, hipStream_t s) {
+  if (!n_colors) return;
+  reduce_partials_kernel<<<(n_colors + 255) / 256, 256, 0, s>>>(partial, waves, n_colors, agg);
+}
+
+}  // namespace zg
+
+namespace zg {
+
+__global__ void segment_bounds_kernel(const uint64_t *key_win, uint64_t n_keys,
+                                      const uint64_t *pos_off, uint64_t n_reads,
+                                      uint64_t *key_off) {
+  uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (r > n_reads) return;
+  uint64_t x = pos_off[r], lo = 0, hi = n_keys;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) >> 1;
+    if (key_win[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  key_off[r] = lo;
+}
+
+void launch_segment_bounds(const uint64_t *key_win, uint64_t n_keys, const uint64_t *pos_off,
+                           uint64_t n_reads, uint64_t *key_off, hipStream_t s) {
+  segment_bounds_kernel<<<(uint32_t)((n_reads + 1 + 255) / 256), 256, 0, s>>>(
+      key_win, n_keys, pos_off, n_reads, key_off);
+}
+
+__global__ void diff_kernel(const uint64_t *off, uint64_t n, uint64_t *totals) {
+  uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (r < n) totals[r] = off[r + 1] - off[r];
+}
+
+void launch_diff(const uint64_t *off, uint64_t n, uint64_t *totals, hipStream_t s) {
+  if (!n) return;
+  diff_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(off, n, totals);
+}
+
+__global__ void colsum_kernel(const uint32_t *counts, uint64_t n_reads, uint32_t n_colors,
+                              uint64_t *agg) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_colors) return;
+  uint64_t s = 0;
+  for (uint64_t r = 0; r < n_reads; r++) s += counts[r * n_colors + c];
+  agg[c] += s;
+}
+
+void launch_colsum(const uint32_t *counts, uint64_t n_reads, uint32_t n_colors, uint64_t *agg,
+                   hipStream_t s) {
+  if (!n_colors) return;
+  colsum_kernel<<<(n_colors + 255) / 256, 256, 0, s>>>(counts, n_reads, n_colors, agg);
+}
+
+}  // namespace zg
