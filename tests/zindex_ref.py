@@ -368,7 +368,11 @@ def parse_int(v) -> int:
 
 
 def cpu_count() -> int:
+    """Host threads for the oracle: affinity count capped at OMP_NUM_THREADS
+    (default 16) so a shared host is not oversubscribed."""
     try:
-        return len(os.sched_getaffinity(0))
+        n = len(os.sched_getaffinity(0))
     except AttributeError:
-        return os.cpu_count() or 1
+        n = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(n, cap))

@@ -68,10 +68,24 @@ std::vector<uint64_t> ocm_inner_table(uint32_t s, uint64_t seed) {
   return t;
 }
 
-static unsigned host_threads() {
+}  // namespace zg
+
+// Host worker threads: the hardware count, capped by OMP_NUM_THREADS (or 16)
+// so that a shared host is not oversubscribed.
+unsigned zg_host_threads() {
   unsigned n = std::thread::hardware_concurrency();
-  return n ? n : 1;
+  unsigned cap = 16;
+  if (const char *e = getenv("OMP_NUM_THREADS")) {
+    long v = strtol(e, nullptr, 10);
+    if (v > 0) cap = (unsigned)v;
+  }
+  n = n ? n : 1;
+  return n < cap ? n : cap;
 }
+
+namespace zg {
+
+static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
 static void check_query_support(const Manifest &m) {

@@ -536,7 +536,7 @@ extern "C" int zg_builder_create(const zg_build_config *cfg, zg_builder **out) {
     validate_config(*cfg);
     auto b = std::make_unique<zg_builder>();
     b->cfg = *cfg;
-    uint32_t threads = cfg->threads ? cfg->threads : std::thread::hardware_concurrency();
+    uint32_t threads = cfg->threads ? cfg->threads : zg_host_threads();
     if (!threads) threads = 1;
     for (uint32_t i = 0; i < threads; i++) b->workers.emplace_back([p = b.get()] { p->worker(); });
     *out = b.release();
@@ -985,7 +985,7 @@ extern "C" int zg_index_write(const zg_index *idx, const char *out_dir) {
     for (size_t i = 0; i < idx->m.banks.size(); i++)
       write_bank_file(path_join(stage, idx->m.banks[i].file), idx->m.banks[i],
                       idx->payload[i].data(), idx->payload[i].size(),
-                      (int)std::max(1u, std::thread::hardware_concurrency()));
+                      (int)zg_host_threads());
     if (idx->m.abandoned_recovery)
       write_recovery_map(path_join(stage, kAbandonedFile), idx->rec);
     std::string json;

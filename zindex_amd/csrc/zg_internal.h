@@ -100,6 +100,9 @@ struct zg_batch {
 
 struct zg_builder;
 
+// Host worker threads (hardware count capped by OMP_NUM_THREADS, default 16).
+unsigned zg_host_threads();
+
 namespace zg {
 // Builds the device copy of idx (banks, column tables, recovery directory).
 void upload_index(zg_index *idx, uint64_t budget);
