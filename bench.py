@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the zorindex query hot path on one MI355X.
+
+Workload (default): the batched query at scale -- 1024 colours of 5 Mbp from a
+seeded substitution tree (0.5-5% substitutions per leaf spread over a depth-10
+balanced tree, 1% indels of length 1-10 per leaf), indexed as dense canonical
+31-mers with H=4 hashes and b=8 fingerprints (one 8 GiB bank of 2^23 rows x
+1024 bytes). Query: 1,000,000 reads with lognormal lengths (median 2 kbp,
+sigma 1.0, clipped to 100 bp-100 kbp) at 2% substitution error, random strand.
+All reads sit in HBM as ASCII; one step packs them to 2 bits, extracts
+k-mers and runs the persistent lookup (HIP graph) over every read, producing
+per-read per-colour counts.
+
+Printed: one JSON line with the throughput (queried k-mers/s), the lookup
+kernel's roofline against HBM bandwidth, the one-launch-per-read path on a
+sample of reads, and the CPU oracle timed on the host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def parse_args():
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--colors", type=int, default=1024)
+    p.add_argument("--color-len", type=int, default=5_000_000)
+    p.add_argument("--reads", type=int, default=1_000_000)
+    p.add_argument("--median", type=float, default=2000.0)
+    p.add_argument("--sigma", type=float, default=1.0)
+    p.add_argument("--min-len", type=int, default=100)
+    p.add_argument("--max-len", type=int, default=100_000)
+    p.add_argument("--error", type=float, default=0.02)
+    p.add_argument("--bits", type=int, default=8)
+    p.add_argument("--hashes", type=int, default=4)
+    p.add_argument("--k", type=int, default=31)
+    p.add_argument("--seed", type=int, default=2026)
+    p.add_argument("--aggregate", action="store_true", help="aggregate counts only")
+    p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--tune-reads", type=int, default=20000)
+    p.add_argument("--plain-reads", type=int, default=20000)
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--parity-reads", type=int, default=2000)
+    return p.parse_args()
+
+
+def generate(args, zindex_amd, synth):
+    rng = np.random.default_rng(args.seed)
+    root = synth.random_genome(rng, args.color_len)
+    builder = zindex_amd.Builder(k=args.k, fingerprint_bits=args.bits, hashes=args.hashes)
+    leaves = [None] * args.colors
+    t0 = time.time()
+    if args.colors & (args.colors - 1) == 0 and args.colors > 1:
+        gen = synth.substitution_tree(rng, root, args.colors)
+    else:
+        gen = ((i, synth.indels(rng, synth.substitute(rng, root, rng.uniform(0.005, 0.05)), 0.01))
+               for i in range(args.colors))
+    for leaf, seq in gen:
+        leaves[leaf] = seq
+    for leaf, seq in enumerate(leaves):
+        builder.add_color(f"leaf{leaf:05d}", [seq.tobytes()])
+        if leaf % 128 == 127:
+            log(f"colours queued {leaf + 1}/{args.colors} ({time.time() - t0:.1f}s)")
+    idx = builder.finish()
+    log(f"index built in {time.time() - t0:.1f}s")
+    lengths = synth.lognormal_lengths(rng, args.reads, args.median, args.sigma, args.min_len,
+                                      args.max_len)
+    t1 = time.time()
+    buf, off, _ = synth.sample_reads_fast(rng, leaves, args.reads, lengths, args.error)
+    log(f"{args.reads} reads, {int(off[-1])} bases sampled in {time.time() - t1:.1f}s")
+    return idx, buf, off
+
+
+def cpu_baseline(args, idx, buf, off, gpu_counts):
+    """The CPU oracle on the host: lookup phase over a bounded sample of the
+    same reads (keys extracted beforehand, as the reference's lookup_s)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import ctypes as C
+    import zindex_ref as zr
+    zr.oracle()
+    man = idx.manifest()
+    payloads = [idx.bank_payload(i, copy=False) for i in range(len(man["banks"]))]
+    orc = zr.OracleIndex(man, payloads, idx.recovery())
+    threads = args.cpu_threads or zr.cpu_count()
+    # parity on the first reads (per-read per-colour counts, bit-exact)
+    n_par = min(args.parity_reads, len(off) - 1)
+    exp_totals, exp_counts = orc.query_reads(buf[: int(off[n_par])], off[: n_par + 1], threads)
+    parity = None
+    if gpu_counts is not None:
+        parity = bool(np.array_equal(gpu_counts[:n_par].astype(np.uint64), exp_counts))
+    # calibrate then time the lookup phase on a sample taking ~cpu_seconds
+    def keys_for(n):
+        parts, koff = [], np.zeros(n + 1, np.uint64)
+        for i in range(n):
+            k = zr.scan_kmers(buf[int(off[i]):int(off[i + 1])].tobytes(), args.k)
+            parts.append(k)
+            koff[i + 1] = koff[i] + len(k)
+        return (np.concatenate(parts) if parts else np.zeros(0, np.uint64)), koff
+    def timed(n):
+        keys, koff = keys_for(n)
+        counts = np.zeros((n, orc.n_colors), np.uint64)
+        t = time.perf_counter()
+        zr.oracle().zo_lookup_keys(C.byref(orc.z), keys.ctypes.data_as(zr.u64p),
+                                   koff.ctypes.data_as(zr.u64p), n,
+                                   counts.ctypes.data_as(zr.u64p), threads)
+        return time.perf_counter() - t, int(koff[-1])
+    n = min(200, len(off) - 1)
+    dt, nk = timed(n)
+    while dt < args.cpu_seconds / 8 and n < len(off) - 1:
+        n = min(len(off) - 1, n * 4)
+        dt, nk = timed(n)
+    if dt < args.cpu_seconds / 2 and n < len(off) - 1:
+        n = min(len(off) - 1, int(n * args.cpu_seconds / max(dt, 1e-3)))
+        dt, nk = timed(n)
+    return {
+        "value": nk / dt, "unit": "k-mers/s", "cores": threads, "kind": "port",
+        "sample": f"lookup phase of the CPU oracle (oracle/oracle.c, sorted keys with "
+                  f"duplicate folding, 64-bit XOR screens) over the first {n} reads "
+                  f"({nk} k-mers) of the same batch and index, {dt:.1f}s, keys pre-extracted",
+    }, parity, n_par
+
+
+def main() -> int:
+    args = parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        return 2
+    import zindex_amd
+    from zindex_amd import synth
+
+    records = not args.aggregate
+    idx, buf, off = generate(args, zindex_amd, synth)
+    meta = idx.meta
+    t0 = time.time()
+    idx.load()
+    log(f"index resident in HBM ({meta['payload_bytes'] / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
+
+    tuning = idx.tuning
+    if not args.no_autotune:
+        n_t = min(args.tune_reads, args.reads)
+        with idx.batch(buf[: int(off[n_t])], off[: n_t + 1], records=records) as tb:
+            tuning = tb.autotune()
+        log(f"autotuned on {n_t} reads: {tuning}")
+
+    t0 = time.time()
+    batch = idx.batch(buf, off, records=records)
+    log(f"batch resident + seeded in {time.time() - t0:.1f}s")
+    for _ in range(args.warmup):
+        batch.reseed()
+        batch.run()
+    t0 = time.perf_counter()
+    lookup_ms = []
+    for _ in range(args.steps):
+        batch.reseed()
+        batch.run()
+        lookup_ms.append(batch.stats()["last_lookup_ms"])
+    elapsed = time.perf_counter() - t0
+    st = batch.stats()
+    seeds = st["seeds"]
+    ms_per_step = elapsed * 1000 / args.steps
+    value = seeds * args.steps / elapsed
+    totals, counts, agg = batch.results()
+    log(f"{seeds} k-mers per step, {ms_per_step:.1f} ms/step, lookup {np.mean(lookup_ms):.1f} ms")
+
+    # roofline of the lookup kernel: algorithmic bytes per launch / launch time
+    positions = int(off[-1]) + args.reads
+    algo = st["algorithmic_bytes"] + positions * 3 // 8
+    lk_ms = float(np.mean(lookup_ms))
+    achieved = algo / (lk_ms / 1000) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "lookup_kernel", "launch_ms": round(lk_ms, 3),
+                "algorithmic_bytes_per_launch": algo}
+
+    plain = None
+    if args.plain_reads:
+        n_p = min(args.plain_reads, args.reads)
+        with idx.batch(buf[: int(off[n_p])], off[: n_p + 1], records=records) as pb:
+            pb.run_plain()
+            pst = pb.stats()
+            pt, pc, _ = pb.results()
+            pb.run()
+            bst = pb.stats()
+            bt, bc, _ = pb.results()
+            same = bool(np.array_equal(pc, bc)) if records else None
+        plain = {"reads": n_p, "kmers": pst["seeds"],
+                 "plain_kmers_per_s": pst["seeds"] / (pst["last_total_ms"] / 1000),
+                 "batched_kmers_per_s_same_reads": bst["seeds"] / (bst["last_total_ms"] / 1000),
+                 "plain_equals_batched": same}
+        log(f"plain path: {plain}")
+
+    cpu, parity, n_par = (None, None, 0)
+    if not args.no_cpu:
+        cpu, parity, n_par = cpu_baseline(args, idx, buf, off, counts)
+        log(f"cpu baseline: {cpu['value']:.3e} k-mers/s on {cpu['cores']} threads; "
+            f"parity on {n_par} reads: {parity}")
+
+    out = {
+        "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
+        "value": value,
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8" if args.bits == 8 else f"u{args.bits}",
+        "data": "synthetic (seeded substitution tree colours and sampled reads)",
+        "config": {
+            "workload": f"batched query: {args.colors} colours x {args.color_len} bp substitution "
+                        f"tree, k={args.k} H={args.hashes} b={args.bits}; {args.reads} reads "
+                        f"lognormal(median {args.median:.0f}, sigma {args.sigma}) clipped "
+                        f"[{args.min_len},{args.max_len}], {args.error:.0%} error; per-read counts"
+                        if records else "aggregate counts",
+            "colors": args.colors, "k": args.k, "hashes": args.hashes, "bits": args.bits,
+            "reads": args.reads, "bases": int(off[-1]), "kmers_per_step": seeds,
+            "bank_bytes": meta["payload_bytes"], "tuning": st["tuning"],
+            "step": "pack ASCII->2-bit + k-mer counts + persistent lookup (HIP graph)",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_path": plain,
+        "speedup_vs_plain": (plain["batched_kmers_per_s_same_reads"] / plain["plain_kmers_per_s"])
+        if plain else None,
+        "parity_sample": {"reads": n_par, "bit_exact": parity} if parity is not None else None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

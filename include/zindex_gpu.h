@@ -90,6 +90,15 @@ int zg_index_load(zg_index *idx, int recover, uint64_t hbm_budget_bytes);
  * an in-memory build). */
 int zg_index_bank_payload(const zg_index *idx, uint32_t bank,
                           const uint8_t **data, uint64_t *bytes);
+/* manifest.json text of the index (storage.rs:133-258 schema); valid for the
+ * lifetime of idx. */
+int zg_index_manifest(const zg_index *idx, const char **json, uint64_t *len);
+/* Host recovery map in memory (after zg_index_load with recovery or an
+ * in-memory build): n hashes ascending, off[n+1] into origins
+ * (QueryRecovery entries, query_stream.rs:8-46). */
+int zg_index_recovery(const zg_index *idx, const uint32_t **hashes,
+                      const uint64_t **off, const uint32_t **origins,
+                      uint64_t *n);
 
 /* ------------------------------------------------------------------------ */
 /* Queries on host reads                                                     */

@@ -115,11 +115,39 @@ class Index:
         check(lib().zg_index_load(self._h, int(bool(recover)), int(hbm_budget)))
         return self
 
-    def bank_payload(self, bank: int) -> np.ndarray:
+    def bank_payload(self, bank: int, copy: bool = True) -> np.ndarray:
+        """Decoded payload of one bank; copy=False returns a view into the
+        library's host copy (valid while the index is open)."""
         p = C.POINTER(C.c_uint8)()
         n = C.c_uint64()
         check(lib().zg_index_bank_payload(self._h, bank, C.byref(p), C.byref(n)))
-        return np.ctypeslib.as_array(p, shape=(n.value,)).copy()
+        if n.value == 0:
+            return np.zeros(0, np.uint8)
+        view = np.ctypeslib.as_array(p, shape=(n.value,))
+        return view.copy() if copy else view
+
+    def manifest(self) -> dict:
+        import json
+        text = C.c_char_p()
+        n = C.c_uint64()
+        check(lib().zg_index_manifest(self._h, C.byref(text), C.byref(n)))
+        return json.loads(C.string_at(text, n.value).decode())
+
+    def recovery(self):
+        """(hashes, offsets, origins) of the host recovery map, or None."""
+        h = C.POINTER(C.c_uint32)()
+        off = C.POINTER(C.c_uint64)()
+        o = C.POINTER(C.c_uint32)()
+        n = C.c_uint64()
+        if lib().zg_index_recovery(self._h, C.byref(h), C.byref(off), C.byref(o), C.byref(n)):
+            return None
+        if n.value == 0:
+            return (np.zeros(0, np.uint32), np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+        hashes = np.ctypeslib.as_array(h, shape=(n.value,)).copy()
+        offs = np.ctypeslib.as_array(off, shape=(n.value + 1,)).copy()
+        origins = np.ctypeslib.as_array(o, shape=(int(offs[-1]),)).copy() if offs[-1] else \
+            np.zeros(0, np.uint32)
+        return hashes, offs, origins
 
     def write(self, out_dir: str) -> None:
         check(lib().zg_index_write(self._h, str(out_dir).encode()))

@@ -75,6 +75,9 @@ SIGNATURES = {
     "zg_index_color": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_char_p), u64p]),
     "zg_index_load": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
     "zg_index_bank_payload": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(u8p), u64p]),
+    "zg_index_manifest": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u64p]),
+    "zg_index_recovery": (C.c_int, [C.c_void_p, C.POINTER(u32p), C.POINTER(u64p),
+                                    C.POINTER(u32p), u64p]),
     "zg_query_reads": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, C.c_uint32,
                                  u64p, u32p, u64p]),
     "zg_extract_seeds": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, u64p, u64p,

@@ -343,6 +343,33 @@ extern "C" int zg_index_bank_payload(const zg_index *idx, uint32_t bank, const u
   }
 }
 
+extern "C" int zg_index_manifest(const zg_index *idx, const char **json, uint64_t *len) {
+  try {
+    ensure(idx && json && len, "null argument", ZG_ERR_INVALID);
+    if (idx->manifest_text.empty()) json_write(idx->manifest_text, idx->m.json);
+    *json = idx->manifest_text.c_str();
+    *len = idx->manifest_text.size();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_index_recovery(const zg_index *idx, const uint32_t **hashes,
+                                 const uint64_t **off, const uint32_t **origins, uint64_t *n) {
+  try {
+    ensure(idx && hashes && off && origins && n, "null argument", ZG_ERR_INVALID);
+    ensure(idx->have_rec, "recovery map not in host memory", ZG_ERR_INVALID);
+    *hashes = idx->rec.hashes.data();
+    *off = idx->rec.off.data();
+    *origins = idx->rec.origins.data();
+    *n = idx->rec.size();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
 extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
   try {
     ensure(idx && t, "null argument", ZG_ERR_INVALID);
@@ -439,23 +466,31 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   return a;
 }
 
-// The run's stream work: zero outputs, one persistent lookup per bank, reduce.
-static void enqueue_run(zg_index *idx, zg_batch *b, bool with_events) {
+// Zero the outputs of a run (direct launches, before the lookups).
+static void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
   hipStream_t s = idx->stream;
   const uint32_t C = (uint32_t)idx->m.colors.size();
-  if (with_events) check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
   if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
   else launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
   launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
+}
+
+// The lookups: reset the work counters, one persistent launch per bank. This
+// is the part captured in the HIP graph.
+static void enqueue_lookups(zg_index *idx, zg_batch *b) {
+  hipStream_t s = idx->stream;
   launch_fill_u32(b->queue.as<uint32_t>(), idx->dbanks.size(), 0, s);
-  if (with_events) check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
   bool any = b->explicit_keys ? b->n_keys > 0 : b->positions > 0;
   if (any)
     for (size_t i = 0; i < idx->dbanks.size(); i++) launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
-  if (with_events) check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
+}
+
+// Per-colour sums after the lookups.
+static void enqueue_reduce(zg_index *idx, zg_batch *b) {
+  hipStream_t s = idx->stream;
+  const uint32_t C = (uint32_t)idx->m.colors.size();
   if (!b->records) launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
   else launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
-  if (with_events) check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
 }
 
 static void drop_graph(zg_batch *b) {
@@ -463,6 +498,17 @@ static void drop_graph(zg_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->graph_exec = nullptr;
   b->graph = nullptr;
+}
+
+// Milliseconds between two completed events; -1 when the runtime cannot time
+// them (timing never fails a query).
+double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1.0;
+  }
+  return ms;
 }
 
 static void seed_batch(zg_index *idx, zg_batch *b) {
@@ -517,9 +563,7 @@ static void seed_batch(zg_index *idx, zg_batch *b) {
     check_hip(hipStreamSynchronize(s), "seed");
     b->explicit_keys = true;
   }
-  float ms = 0;
-  check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]), "hipEventElapsedTime");
-  b->last_seed_ms = ms;
+  b->last_seed_ms = elapsed_ms(b->ev[0], b->ev[1]);
 }
 
 }  // namespace zg
@@ -587,13 +631,13 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
     ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
     hipStream_t s = idx->stream;
     if (!b->graph_exec && !b->graph_failed) {
-      // Capture once; replays reuse the instantiated graph. If any step of the
-      // capture is refused, fall back to direct launches for this batch.
+      // Capture the lookups once; replays reuse the instantiated graph. If any
+      // step of the capture is refused, fall back to direct launches.
       bool captured = false;
       if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
         bool enqueued = true;
         try {
-          zg::enqueue_run(idx, b, true);
+          zg::enqueue_lookups(idx, b);
         } catch (...) {
           enqueued = false;
         }
@@ -613,14 +657,17 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
         b->graph_failed = true;
       }
     }
+    check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+    zg::enqueue_zero_outputs(idx, b);
+    check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
     if (b->graph_exec) check_hip(hipGraphLaunch(b->graph_exec, s), "hipGraphLaunch");
-    else zg::enqueue_run(idx, b, true);
+    else zg::enqueue_lookups(idx, b);
+    check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
+    zg::enqueue_reduce(idx, b);
+    check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
     check_hip(hipStreamSynchronize(s), "lookup");
-    float ms = 0;
-    check_hip(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]), "hipEventElapsedTime");
-    b->last_lookup_ms = ms;
-    check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]), "hipEventElapsedTime");
-    b->last_total_ms = ms;
+    b->last_lookup_ms = zg::elapsed_ms(b->ev[1], b->ev[2]);
+    b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
     return 0;
   } catch (...) {
     return translate_exception();
@@ -666,11 +713,8 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
       zg::launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
     check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
     check_hip(hipStreamSynchronize(s), "plain lookup");
-    float ms = 0;
-    check_hip(hipEventElapsedTime(&ms, b->ev[1], b->ev[2]), "hipEventElapsedTime");
-    b->last_lookup_ms = ms;
-    check_hip(hipEventElapsedTime(&ms, b->ev[0], b->ev[3]), "hipEventElapsedTime");
-    b->last_total_ms = ms;
+    b->last_lookup_ms = zg::elapsed_ms(b->ev[1], b->ev[2]);
+    b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
     return 0;
   } catch (...) {
     return translate_exception();

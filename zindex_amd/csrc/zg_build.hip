@@ -158,24 +158,40 @@ static void host_ocm(const uint8_t *seq, size_t len, int k, int m, uint64_t seed
   }
 }
 
-// LSD radix sort (16-bit digits) followed by deduplication.
-static void sort_unique(std::vector<uint64_t> &v) {
-  if (v.size() < 4096) {
+// The builder keeps every key as h = mixsplit(key, seed): splitmix64 is a
+// bijection, h is PositionHash's `first` (storage.rs:3458-3465), and sorting
+// by h both deduplicates keys and orders them by first segment (the segment
+// is mulhi(h, count), monotone in h), which is the locality the peel wants.
+constexpr uint64_t inverse_odd(uint64_t a) {
+  uint64_t x = a;
+  for (int i = 0; i < 6; i++) x *= 2 - a * x;
+  return x;
+}
+static inline uint64_t unsplitmix64(uint64_t x) {
+  x = x ^ (x >> 31) ^ (x >> 62);
+  x *= inverse_odd(0x94D049BB133111EBULL);
+  x = x ^ (x >> 27) ^ (x >> 54);
+  x *= inverse_odd(0xBF58476D1CE4E5B9ULL);
+  x = x ^ (x >> 30) ^ (x >> 60);
+  return x - 0x9E3779B97F4A7C15ULL;
+}
+static inline uint64_t key_of(uint64_t h, uint64_t seed) { return unsplitmix64(h) - seed; }
+
+// keys -> sorted distinct h values: one MSD pass on the top 16 bits, then
+// small sorts inside buckets (h is uniformly distributed).
+static void hash_sort_unique(std::vector<uint64_t> &v, uint64_t seed) {
+  for (uint64_t &x : v) x = mixsplit(x, seed);
+  if (v.size() < 65536) {
     std::sort(v.begin(), v.end());
   } else {
     std::vector<uint64_t> tmp(v.size());
-    uint64_t all_or = 0;
-    for (uint64_t x : v) all_or |= x;
-    std::vector<size_t> count(65537);
-    for (int pass = 0; pass < 4; pass++) {
-      int sh = 16 * pass;
-      if (((all_or >> sh) & 0xFFFF) == 0) continue;
-      std::fill(count.begin(), count.end(), 0);
-      for (uint64_t x : v) count[((x >> sh) & 0xFFFF) + 1]++;
-      for (size_t i = 1; i < count.size(); i++) count[i] += count[i - 1];
-      for (uint64_t x : v) tmp[count[(x >> sh) & 0xFFFF]++] = x;
-      v.swap(tmp);
-    }
+    std::vector<size_t> count(65537, 0);
+    for (uint64_t x : v) count[(x >> 48) + 1]++;
+    for (size_t i = 1; i < count.size(); i++) count[i] += count[i - 1];
+    std::vector<size_t> cur(count.begin(), count.end() - 1);
+    for (uint64_t x : v) tmp[cur[x >> 48]++] = x;
+    for (size_t b = 0; b < 65536; b++) std::sort(tmp.begin() + count[b], tmp.begin() + count[b + 1]);
+    v.swap(tmp);
   }
   v.erase(std::unique(v.begin(), v.end()), v.end());
 }
@@ -219,30 +235,41 @@ static uint64_t config_slots(const zg_build_config &c, uint64_t cardinality) {
 // Peeling with degree counts and XOR-of-incident-edge accumulators; a cyclic
 // core is broken at the minimum-degree cell (highest cell id on ties) by
 // keeping its first active equation and abandoning the others, the default
-// no-heuristic rule of choose_cycle_break_candidate.
-static void construct_group(const zg_build_config &cfg, const std::vector<uint64_t> &keys,
+// no-heuristic rule of choose_cycle_break_candidate. `hashed` holds the
+// group's distinct keys as ascending h = mixsplit(key, seed), i.e. ordered by
+// first segment, so the degree/XOR updates walk memory nearly sequentially;
+// every key's H cells are computed once.
+static void construct_group(const zg_build_config &cfg, const std::vector<uint64_t> &hashed,
                             const std::vector<uint64_t> &values, GroupResult &g) {
   const uint64_t S = g.slots;
   const uint32_t H = cfg.hashes;
   const SegmentGeometry geo = segment_geometry(S, H, cfg.segment_length);
-  const size_t n = keys.size();
-  auto slots_of = [&](uint64_t key, uint64_t *out) {
-    uint64_t first = mixsplit(key, cfg.seed), step = splitmix64(first);
+  const size_t n = hashed.size();
+  ensure(S <= UINT32_MAX && n < UINT32_MAX, "group too large for 32-bit cell ids", -6);
+  static const bool trace = getenv("ZG_BUILD_TRACE") != nullptr;
+  std::chrono::steady_clock::time_point tp[8];
+  auto T = [&](int i) { if (trace) tp[i] = std::chrono::steady_clock::now(); };
+  T(0);
+  // 1. the H cells of every equation (PositionHash::positions)
+  std::vector<uint32_t> slots((size_t)n * H);
+  for (size_t e = 0; e < n; e++) {
+    uint64_t first = hashed[e], step = splitmix64(first);
     uint64_t base = slot_base(first, geo), h = first;
     for (uint32_t i = 0; i < H; i++) {
-      out[i] = slot_at(base, h, i, geo);
+      slots[e * H + i] = (uint32_t)slot_at(base, h, i, geo);
       h += step;
     }
-  };
-  std::vector<uint32_t> count(S, 0), xr(S, 0);
-  uint64_t sl[kMaxHashes];
-  for (size_t e = 0; e < n; e++) {
-    slots_of(keys[e], sl);
-    for (uint32_t i = 0; i < H; i++) {
-      count[sl[i]]++;
-      xr[sl[i]] ^= (uint32_t)e;
-    }
   }
+  T(1);
+  // 2. degrees and XOR of incident equation ids
+  std::vector<uint32_t> count(S, 0), xr(S, 0);
+  for (size_t e = 0; e < n; e++)
+    for (uint32_t i = 0; i < H; i++) {
+      uint32_t s = slots[e * H + i];
+      count[s]++;
+      xr[s] ^= (uint32_t)e;
+    }
+  T(2);
   std::vector<uint8_t> state(n, 0);  // 0 active, 1 peeled, 2 abandoned
   std::vector<std::pair<uint32_t, uint32_t>> order;
   order.reserve(n);
@@ -251,20 +278,18 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
     if (count[s] == 1) queue.push_back((uint32_t)s);
   size_t remaining = n, qi = 0;
   std::vector<uint32_t> abandoned;
-  // core-phase structures, built on first stall
-  std::vector<uint64_t> adj_off;
-  std::vector<uint32_t> adj;
-  std::priority_queue<std::pair<int64_t, uint32_t>> heap;  // (-degree, slot)
+  // core phase: (cell, equation) pairs of the equations left when peeling stalls
+  std::vector<std::pair<uint32_t, uint32_t>> pairs;
+  std::priority_queue<std::pair<int64_t, uint32_t>> heap;  // (-degree, cell)
   bool core = false;
   auto remove_edge = [&](uint32_t e) {
-    slots_of(keys[e], sl);
     for (uint32_t i = 0; i < H; i++) {
-      uint64_t s = sl[i];
+      uint32_t s = slots[(size_t)e * H + i];
       if (count[s] > 0) {
         count[s]--;
         xr[s] ^= e;
-        if (count[s] == 1) queue.push_back((uint32_t)s);
-        if (core && count[s] > 1) heap.push({-(int64_t)count[s], (uint32_t)s});
+        if (count[s] == 1) queue.push_back(s);
+        if (core && count[s] > 1) heap.push({-(int64_t)count[s], s});
       }
     }
   };
@@ -289,24 +314,16 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
     if (progress && qi < queue.size()) continue;
     if (!core) {
       core = true;
-      adj_off.assign(S + 1, 0);
       for (size_t e = 0; e < n; e++) {
         if (state[e] != 0) continue;
-        slots_of(keys[e], sl);
-        for (uint32_t i = 0; i < H; i++) adj_off[sl[i] + 1]++;
+        for (uint32_t i = 0; i < H; i++) pairs.push_back({slots[e * H + i], (uint32_t)e});
       }
-      for (uint64_t s = 0; s < S; s++) adj_off[s + 1] += adj_off[s];
-      adj.assign(adj_off[S], 0);
-      std::vector<uint64_t> cur(adj_off.begin(), adj_off.end() - 1);
-      for (size_t e = 0; e < n; e++) {
-        if (state[e] != 0) continue;
-        slots_of(keys[e], sl);
-        for (uint32_t i = 0; i < H; i++) adj[cur[sl[i]]++] = (uint32_t)e;
-      }
-      for (uint64_t s = 0; s < S; s++)
-        if (count[s] > 1) heap.push({-(int64_t)count[s], (uint32_t)s});
+      std::sort(pairs.begin(), pairs.end());
+      for (size_t i = 0; i < pairs.size(); i++)
+        if ((i == 0 || pairs[i].first != pairs[i - 1].first) && count[pairs[i].first] > 1)
+          heap.push({-(int64_t)count[pairs[i].first], pairs[i].first});
     }
-    // choose the minimum-degree cell, highest id on ties
+    // the minimum-degree cell, highest id on ties
     uint32_t pick = UINT32_MAX;
     while (!heap.empty()) {
       auto top = heap.top();
@@ -329,9 +346,14 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
         }
       continue;
     }
+    auto range = std::equal_range(pairs.begin(), pairs.end(), std::make_pair(pick, 0u),
+                                  [](const std::pair<uint32_t, uint32_t> &x,
+                                     const std::pair<uint32_t, uint32_t> &y) {
+                                    return x.first < y.first;
+                                  });
     bool kept = false;
-    for (uint64_t a = adj_off[pick]; a < adj_off[pick + 1]; a++) {
-      uint32_t e = adj[a];
+    for (auto it = range.first; it != range.second; ++it) {
+      uint32_t e = it->second;
       if (state[e] != 0) continue;
       if (!kept) {
         kept = true;
@@ -343,33 +365,48 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
       remove_edge(e);
     }
   }
-  // back-substitution in reverse peel order (graph.rs:187-199)
+  T(3);
+  std::vector<uint32_t>().swap(count);
+  std::vector<uint32_t>().swap(xr);
+  // 3. back-substitution in reverse peel order (graph.rs:187-199)
   std::vector<uint64_t> cells(S, 0);
   for (size_t i = order.size(); i-- > 0;) {
     uint32_t e = order[i].first, pivot = order[i].second;
     uint64_t v = values[e];
-    slots_of(keys[e], sl);
-    for (uint32_t h = 0; h < H; h++)
-      if (sl[h] != pivot) v ^= cells[sl[h]];
+    for (uint32_t h = 0; h < H; h++) {
+      uint32_t s = slots[(size_t)e * H + h];
+      if (s != pivot) v ^= cells[s];
+    }
     cells[pivot] = v;
   }
-  // verification (graph.rs:201-214)
-  std::sort(abandoned.begin(), abandoned.end());
+  T(4);
+  // 4. verification (graph.rs:201-214)
   for (size_t e = 0; e < n; e++) {
     if (state[e] == 2) continue;
-    slots_of(keys[e], sl);
     uint64_t v = 0;
-    for (uint32_t h = 0; h < H; h++) v ^= cells[sl[h]];
-    if (v != values[e]) fail(-1, "incorrect merged payload for key " + std::to_string(keys[e]));
+    for (uint32_t h = 0; h < H; h++) v ^= cells[slots[e * H + h]];
+    if (v != values[e]) fail(-1, "incorrect merged payload for key " + std::to_string(key_of(hashed[e], cfg.seed)));
   }
   g.cell_bytes = g.width <= 8 ? 1 : g.width <= 16 ? 2 : g.width <= 32 ? 4 : 8;
   g.cells.resize(S * g.cell_bytes);
-  for (uint64_t s = 0; s < S; s++) memcpy(g.cells.data() + s * g.cell_bytes, &cells[s], g.cell_bytes);
-  std::vector<uint64_t>().swap(cells);
+  if (g.cell_bytes == 1) {
+    for (uint64_t s = 0; s < S; s++) g.cells[s] = (uint8_t)cells[s];
+  } else {
+    for (uint64_t s = 0; s < S; s++)
+      memcpy(g.cells.data() + s * g.cell_bytes, &cells[s], g.cell_bytes);
+  }
+  T(5);
   g.abandoned = abandoned.size();
+  std::sort(abandoned.begin(), abandoned.end());
   for (uint32_t e : abandoned) {
     uint64_t mask = g.members.size() == 1 ? 1 : values[e] >> cfg.fingerprint_bits;
-    g.recovery.push_back({abandoned_hash(keys[e], cfg.seed), mask});
+    g.recovery.push_back({abandoned_hash(key_of(hashed[e], cfg.seed), cfg.seed), mask});
+  }
+  if (trace) {
+    T(6);
+    auto ms = [&](int a, int b) { return std::chrono::duration<double, std::milli>(tp[b] - tp[a]).count(); };
+    fprintf(stderr, "  group: bucket %.0f, degrees %.0f, peel %.0f, assign %.0f, verify %.0f, pack %.0f ms\n",
+            ms(0, 1), ms(1, 2), ms(2, 3), ms(3, 4), ms(4, 5), ms(5, 6));
   }
 }
 
@@ -464,7 +501,7 @@ static void build_group(const zg_build_config &cfg, std::vector<uint32_t> member
   if (g.members.size() == 1) {
     keys.swap(*key_sets[0]);
     values.resize(keys.size());
-    for (size_t i = 0; i < keys.size(); i++) values[i] = fingerprint(keys[i], cfg.seed, b);
+    for (size_t i = 0; i < keys.size(); i++) values[i] = fingerprint(key_of(keys[i], cfg.seed), cfg.seed, b);
   } else {
     // union with membership masks (storage.rs:1222-1255, 1301-1308)
     std::vector<std::pair<uint64_t, uint64_t>> all;
@@ -475,7 +512,7 @@ static void build_group(const zg_build_config &cfg, std::vector<uint32_t> member
       uint64_t key = all[i].first, mask = 0;
       while (i < all.size() && all[i].first == key) mask |= all[i++].second;
       keys.push_back(key);
-      values.push_back(fingerprint(key, cfg.seed, b) | (mask << b));
+      values.push_back(fingerprint(key_of(key, cfg.seed), cfg.seed, b) | (mask << b));
     }
     for (auto *ks : key_sets) std::vector<uint64_t>().swap(*ks);
   }
@@ -578,11 +615,16 @@ extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8
       if (single) b->groups.emplace_back();
     }
     b->submit([b, id, copy, offs, n_records, cfg, single] {
+      static const bool trace = getenv("ZG_BUILD_TRACE") != nullptr;
+      auto t0 = std::chrono::steady_clock::now();
       std::vector<uint64_t> keys;
+      keys.reserve(copy->size());
       extract(cfg, copy->data(), offs->data(), n_records, keys);
+      auto t1 = std::chrono::steady_clock::now();
       uint64_t occurrences = keys.size();
-      sort_unique(keys);
+      hash_sort_unique(keys, cfg.seed);
       uint64_t unique = keys.size();
+      auto t2 = std::chrono::steady_clock::now();
       std::vector<uint64_t> *slot;
       GroupResult *gr = nullptr;
       {
@@ -594,6 +636,12 @@ extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8
       }
       *slot = std::move(keys);
       if (single) build_group(cfg, {id}, {slot}, *gr);
+      if (trace) {
+        auto t3 = std::chrono::steady_clock::now();
+        auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+        fprintf(stderr, "build colour %u: extract %.1f ms, sort %.1f ms, group %.1f ms\n", id,
+                ms(t0, t1), ms(t1, t2), ms(t2, t3));
+      }
     });
     return 0;
   } catch (...) {

@@ -54,6 +54,7 @@ struct Tuning {
 
 struct zg_index {
   zg::Manifest m;
+  mutable std::string manifest_text;             // serialized on first request
   std::string dir;                               // empty when built in memory
   std::vector<std::vector<uint8_t>> payload;     // host bank payloads
   bool payload_loaded = false;

@@ -20,63 +20,69 @@ def random_genome(rng: np.random.Generator, n: int) -> np.ndarray:
 
 
 def substitute(rng: np.random.Generator, seq: np.ndarray, rate: float) -> np.ndarray:
-    """Each base becomes one of the three other bases with probability rate."""
+    """Binomial(len, rate) uniformly placed bases each become one of the three
+    other bases (a repeated position is substituted once)."""
     out = seq.copy()
-    pos = np.nonzero(rng.random(len(out)) < rate)[0]
-    shift = rng.integers(1, 4, len(pos), dtype=np.uint8)
-    out[pos] = ACGT[(_CODE[out[pos]] + shift) % 4]
+    n = int(rng.binomial(len(out), rate)) if len(out) else 0
+    if n:
+        pos = rng.integers(0, len(out), n)
+        shift = rng.integers(1, 4, n, dtype=np.uint8)
+        out[pos] = ACGT[(_CODE[seq[pos]] + shift) % 4]
     return out
 
 
 def indels(rng: np.random.Generator, seq: np.ndarray, rate: float, max_len: int = 10) -> np.ndarray:
-    """Insert or delete runs of 1..max_len bases at a fraction `rate` of positions."""
-    n_events = rng.binomial(len(seq), rate)
+    """Insert or delete runs of 1..max_len bases at a fraction `rate` of
+    positions (half insertions of random bases, half deletions)."""
+    n_events = int(rng.binomial(len(seq), rate)) if len(seq) else 0
     if n_events == 0:
         return seq.copy()
-    pos = np.sort(rng.choice(len(seq), n_events, replace=False))
-    lens = rng.integers(1, max_len + 1, n_events)
-    kind = rng.random(n_events) < 0.5  # True = insertion
-    pieces, at = [], 0
-    for p, ln, ins in zip(pos.tolist(), lens.tolist(), kind.tolist()):
-        if p < at:
-            continue
-        pieces.append(seq[at:p])
-        if ins:
-            pieces.append(random_genome(rng, ln))
-            at = p
-        else:
-            at = p + ln
-    pieces.append(seq[at:])
-    return np.concatenate(pieces)
+    pos = np.unique(rng.integers(0, len(seq), n_events))
+    lens = rng.integers(1, max_len + 1, len(pos))
+    ins = rng.random(len(pos)) < 0.5
+    dp, dl = pos[~ins], lens[~ins]
+    deleted = np.zeros(0, dtype=np.int64)
+    if len(dp):
+        starts = np.repeat(np.cumsum(dl) - dl, dl)
+        idx = np.repeat(dp, dl) + (np.arange(int(dl.sum())) - starts)
+        deleted = np.unique(idx[idx < len(seq)])
+    kept = np.delete(seq, deleted)
+    ip, il = pos[ins], lens[ins]
+    if len(ip) == 0:
+        return kept
+    # insertion before original index ip lands after the kept bases below ip
+    before = ip - np.searchsorted(deleted, ip)
+    return np.insert(kept, np.repeat(before, il), random_genome(rng, int(il.sum())))
 
 
 def substitution_tree(rng: np.random.Generator, root: np.ndarray, leaves: int,
                       sub_range=(0.005, 0.05), indel_rate=0.01):
-    """Yield `leaves` colour sequences (a power of two) of a balanced binary
-    tree: each leaf's total substitution rate is drawn from sub_range and
-    spread evenly over the depth of its path; indels hit `indel_rate` of
-    positions over the path. Internal nodes are shared, so siblings are
-    closer than cousins. Generated depth first: memory is O(depth)."""
+    """Yield (leaf id, sequence) for the `leaves` colours (a power of two) of a
+    balanced binary tree. Each leaf's total substitution rate is drawn
+    uniformly from sub_range and spread evenly over the edges of its path
+    (an edge carries the mean rate of the leaves below it), so siblings share
+    most substitutions. Indels (lengths 1..10) hit `indel_rate` of positions
+    of each leaf, drawn at the leaf. Generated depth first: memory O(depth)."""
     depth = int(round(np.log2(leaves)))
     assert 1 << depth == leaves
     totals = rng.uniform(sub_range[0], sub_range[1], leaves)
     per_edge = totals / max(depth, 1)
-    per_edge_indel = indel_rate / max(depth, 1)
-    seeds = rng.integers(0, 2**63, size=2 * leaves, dtype=np.int64)
+    seeds = rng.integers(0, 2**62, size=2 * leaves, dtype=np.int64)
 
     def walk(node: int, level: int, seq: np.ndarray):
         if level == depth:
-            yield node - leaves, seq
+            leaf = node - leaves
+            s = seq
+            if indel_rate:
+                s = indels(np.random.default_rng(int(seeds[node]) ^ 0x5EED), s, indel_rate)
+            yield leaf, s
             return
         for child in (2 * node, 2 * node + 1):
-            crng = np.random.default_rng(int(seeds[child % len(seeds)]) ^ child)
-            # edge rate: mean of the leaves below this child
-            lo = child << (depth - level - 1)
-            rate = float(per_edge[lo - leaves: lo - leaves + (1 << (depth - level - 1))].mean())
-            s = substitute(crng, seq, rate)
-            if per_edge_indel:
-                s = indels(crng, s, per_edge_indel)
-            yield from walk(child, level + 1, s)
+            crng = np.random.default_rng(int(seeds[child]))
+            span = 1 << (depth - level - 1)
+            lo = child * span - leaves
+            rate = float(per_edge[lo: lo + span].mean())
+            yield from walk(child, level + 1, substitute(crng, seq, rate))
 
     yield from walk(1, 0, root)
 
@@ -103,6 +109,27 @@ def sample_reads(rng: np.random.Generator, colors: list, n: int, lengths: np.nda
         parts.append(read)
         off[i + 1] = off[i] + ln
     buf = np.concatenate(parts) if parts else np.zeros(0, np.uint8)
+    return buf, off, src
+
+
+def sample_reads_fast(rng: np.random.Generator, colors: list, n: int, lengths: np.ndarray,
+                      error: float, rc_fraction: float = 0.5):
+    """As sample_reads, with the substitution errors drawn once over the whole
+    buffer (the per-read loop only copies slices)."""
+    src = rng.integers(0, len(colors), n)
+    clen = np.array([len(c) for c in colors], dtype=np.int64)
+    ln = np.minimum(np.asarray(lengths, dtype=np.int64), clen[src])
+    starts = (rng.random(n) * (clen[src] - ln + 1)).astype(np.int64)
+    rc = rng.random(n) < rc_fraction
+    off = np.zeros(n + 1, dtype=np.uint64)
+    off[1:] = np.cumsum(ln)
+    buf = np.empty(int(off[-1]), dtype=np.uint8)
+    o = off.astype(np.int64)
+    for i in range(n):
+        piece = colors[src[i]][starts[i]:starts[i] + ln[i]]
+        buf[o[i]:o[i + 1]] = _COMP[piece[::-1]] if rc[i] else piece
+    if error and len(buf):
+        buf = substitute(rng, buf, error)
     return buf, off, src
 
 
