@@ -55,8 +55,11 @@ def parse_args():
     p.add_argument("--seed", type=int, default=2026)
     p.add_argument("--aggregate", action="store_true", help="aggregate counts only")
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--tuning", default="",
+                   help="span,tiles_per_wave,prefetch,waves_per_cu: fixed launch shape "
+                        "(skips the autotuner)")
     p.add_argument("--tune-reads", type=int, default=20000)
-    p.add_argument("--plain-reads", type=int, default=20000)
+    p.add_argument("--plain-reads", type=int, default=2000)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
@@ -157,7 +160,12 @@ def main() -> int:
     log(f"index resident in HBM ({meta['payload_bytes'] / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
 
     tuning = idx.tuning
-    if not args.no_autotune:
+    if args.tuning:
+        span, tiles, prefetch, waves = (int(x) for x in args.tuning.split(","))
+        idx.tuning = {"span": span, "tiles_per_wave": tiles, "prefetch": prefetch,
+                      "waves_per_cu": waves}
+        tuning = idx.tuning
+    elif not args.no_autotune:
         n_t = min(args.tune_reads, args.reads)
         with idx.batch(buf[: int(off[n_t])], off[: n_t + 1], records=records) as tb:
             tuning = tb.autotune()
