@@ -64,7 +64,27 @@ def parse_args():
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--parity-reads", type=int, default=2000)
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="first copy this many MiB device-to-device (a plain copy kernel of "
+                        "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return p.parse_args()
+
+
+def calibration_copy(mib: int) -> None:
+    """hipMemcpy device-to-device of `mib` MiB: the runtime's copy kernel reads
+    and writes exactly that many bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"calibration copy: {mib} MiB device-to-device")
 
 
 def generate(args, zindex_amd, synth):
@@ -153,6 +173,8 @@ def main() -> int:
     from zindex_amd import synth
 
     records = not args.aggregate
+    if args.calibrate_copy_mib:
+        calibration_copy(args.calibrate_copy_mib)
     idx, buf, off = generate(args, zindex_amd, synth)
     meta = idx.meta
     t0 = time.time()

@@ -874,19 +874,28 @@ void launch_diff(const uint64_t *off, uint64_t n, uint64_t *totals, hipStream_t 
   diff_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(off, n, totals);
 }
 
+// Column sums over a [n_reads x n_colors] matrix: blockIdx.y picks a slab of
+// `slab` reads, threads of a row of blocks cover the colours (coalesced), one
+// atomic per (slab, colour).
 __global__ void colsum_kernel(const uint32_t *counts, uint64_t n_reads, uint32_t n_colors,
-                              uint64_t *agg) {
+                              uint64_t slab, uint64_t *agg) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_colors) return;
+  uint64_t r0 = (uint64_t)blockIdx.y * slab;
+  uint64_t r1 = r0 + slab < n_reads ? r0 + slab : n_reads;
   uint64_t s = 0;
-  for (uint64_t r = 0; r < n_reads; r++) s += counts[r * n_colors + c];
-  agg[c] += s;
+  for (uint64_t r = r0; r < r1; r++) s += counts[r * n_colors + c];
+  if (s) atomicAdd((unsigned long long *)&agg[c], (unsigned long long)s);
 }
 
 void launch_colsum(const uint32_t *counts, uint64_t n_reads, uint32_t n_colors, uint64_t *agg,
                    hipStream_t s) {
-  if (!n_colors) return;
-  colsum_kernel<<<(n_colors + 255) / 256, 256, 0, s>>>(counts, n_reads, n_colors, agg);
+  if (!n_colors || !n_reads) return;
+  uint64_t slab = 512;
+  if ((n_reads + slab - 1) / slab > 65535) slab = (n_reads + 65534) / 65535;
+  uint64_t slabs = (n_reads + slab - 1) / slab;
+  dim3 grid((n_colors + 255) / 256, (uint32_t)slabs);
+  colsum_kernel<<<grid, 256, 0, s>>>(counts, n_reads, n_colors, slab, agg);
 }
 
 }  // namespace zg
