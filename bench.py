@@ -114,6 +114,13 @@ def generate(args, zindex_amd, synth):
     return idx, buf, off
 
 
+def workload_name(args, records: bool) -> str:
+    return (f"batched query: {args.colors} colours x {args.color_len} bp substitution tree, "
+            f"k={args.k} H={args.hashes} b={args.bits}; {args.reads} reads lognormal(median "
+            f"{args.median:.0f}, sigma {args.sigma}) clipped [{args.min_len},{args.max_len}], "
+            f"{args.error:.0%} error; " + ("per-read counts" if records else "aggregate counts"))
+
+
 def cpu_baseline(args, idx, buf, off, gpu_counts):
     """The CPU oracle on the host: lookup phase over a bounded sample of the
     same reads (keys extracted beforehand, as the reference's lookup_s)."""
@@ -222,6 +229,14 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "lookup_kernel", "launch_ms": round(lk_ms, 3),
                 "algorithmic_bytes_per_launch": algo}
+    workload = workload_name(args, records)
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this
+    # workload (scaled by a copy kernel of known size), when recorded for it
+    for tf in sorted((ROOT / "profiles").glob("*_traffic.json")):
+        rec = json.loads(tf.read_text())
+        if rec.get("workload") == workload and rec.get("algorithmic_bytes") == algo:
+            roofline["traffic"] = rec["traffic_bytes"]
+            roofline["traffic_source"] = str(tf.relative_to(ROOT))
 
     plain = None
     if args.plain_reads:
@@ -260,11 +275,7 @@ def main() -> int:
         "dtype": "u8" if args.bits == 8 else f"u{args.bits}",
         "data": "synthetic (seeded substitution tree colours and sampled reads)",
         "config": {
-            "workload": f"batched query: {args.colors} colours x {args.color_len} bp substitution "
-                        f"tree, k={args.k} H={args.hashes} b={args.bits}; {args.reads} reads "
-                        f"lognormal(median {args.median:.0f}, sigma {args.sigma}) clipped "
-                        f"[{args.min_len},{args.max_len}], {args.error:.0%} error; per-read counts"
-                        if records else "aggregate counts",
+            "workload": workload,
             "colors": args.colors, "k": args.k, "hashes": args.hashes, "bits": args.bits,
             "reads": args.reads, "bases": int(off[-1]), "kmers_per_step": seeds,
             "bank_bytes": meta["payload_bytes"], "tuning": st["tuning"],
