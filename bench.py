@@ -190,9 +190,11 @@ def main() -> int:
 
     tuning = idx.tuning
     if args.tuning:
-        span, tiles, prefetch, waves = (int(x) for x in args.tuning.split(","))
+        vals = [int(x) for x in args.tuning.split(",")]
+        span, tiles, prefetch, waves = vals[:4]
+        nontemporal = vals[4] if len(vals) > 4 else 0
         idx.tuning = {"span": span, "tiles_per_wave": tiles, "prefetch": prefetch,
-                      "waves_per_cu": waves}
+                      "waves_per_cu": waves, "nontemporal": nontemporal}
         tuning = idx.tuning
     elif not args.no_autotune:
         n_t = min(args.tune_reads, args.reads)

@@ -135,11 +135,10 @@ def test_many_tiles_and_tile_grouping():
     idx = build(colors, fingerprint_bits=8, hashes=4)
     idx.load()
     orc = oracle_for(idx)
-    for tiles in (1, 2):
-        for prefetch in (1, 4):
-            idx.tuning = {"span": 1024, "tiles_per_wave": tiles, "prefetch": prefetch,
-                          "waves_per_cu": 0}
-            check_parity(idx, orc, reads, label=f"tiles{tiles} d{prefetch}")
+    for tiles, prefetch, nt in ((1, 1, 0), (1, 4, 1), (2, 1, 1), (2, 4, 0), (2, 2, 1)):
+        idx.tuning = {"span": 1024, "tiles_per_wave": tiles, "prefetch": prefetch,
+                      "waves_per_cu": 0, "nontemporal": nt}
+        check_parity(idx, orc, reads, label=f"tiles{tiles} d{prefetch} nt{nt}")
 
 
 def test_minimizer_positions_and_counts():

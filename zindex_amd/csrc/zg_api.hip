@@ -383,6 +383,7 @@ extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
     idx->tuning.tiles_per_wave = t->tiles_per_wave;
     idx->tuning.prefetch = t->prefetch;
     idx->tuning.waves_per_cu = t->waves_per_cu;
+    idx->tuning.nontemporal = t->nontemporal ? 1 : 0;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -398,6 +399,7 @@ extern "C" int zg_index_get_tuning(const zg_index *idx, zg_tuning *t) {
   t->tiles_per_wave = idx->tuning.tiles_per_wave;
   t->prefetch = idx->tuning.prefetch;
   t->waves_per_cu = idx->tuning.waves_per_cu;
+  t->nontemporal = idx->tuning.nontemporal;
   return 0;
 }
 
@@ -424,6 +426,7 @@ static void configure(zg_index *idx, zg_batch *b) {
   c.blocks = idx->cu_count * (uint32_t)per_cu;
   b->kcfg = c;
   b->span = t.span;
+  b->nontemporal = t.nontemporal;
 }
 
 static void ensure_partials(zg_batch *b) {
@@ -463,6 +466,7 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   a.counts = b->counts.as<uint32_t>();
   a.partial = b->partial.as<uint64_t>();
   a.n_colors = (uint32_t)idx->m.colors.size();
+  a.nontemporal = b->nontemporal;
   return a;
 }
 
@@ -777,6 +781,7 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->tuning.tiles_per_wave = b->kcfg.tiles_per_wave;
     o->tuning.prefetch = b->kcfg.prefetch;
     o->tuning.waves_per_cu = b->kcfg.blocks * 4 / std::max(1u, idx->cu_count);
+    o->tuning.nontemporal = b->nontemporal;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -796,14 +801,18 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
     std::vector<uint32_t> tiles = {1};
     if (zg::max_tiles(idx) >= 2) tiles.push_back(2);
     int base_per_cu = 0;
+    const bool wide = idx->m.fingerprint_bits >= 8;  // 16-byte lane loads
     for (uint32_t span : {1024u, 4096u, 16384u})
       for (uint32_t t : tiles)
         for (uint32_t d : {1u, 2u, 4u})
-          for (uint32_t frac : {1u, 2u}) {
+          for (uint32_t frac : {1u, 2u})
+            for (uint32_t nt : {0u, 1u}) {
+            if (nt && !wide) continue;
             zg::Tuning cand;
             cand.span = span;
             cand.tiles_per_wave = t;
             cand.prefetch = d;
+            cand.nontemporal = nt;
             zg::KernelConfig kc{(uint32_t)idx->m.fingerprint_bits, t, d, 0};
             base_per_cu = zg::lookup_blocks_per_cu(kc);
             cand.waves_per_cu = (uint32_t)std::max(1, base_per_cu / (int)frac) * 4;
@@ -818,6 +827,10 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
               if (rc) return rc;
               ms = std::min(ms, b->last_lookup_ms);
             }
+            if (getenv("ZG_AUTOTUNE_TRACE"))
+              fprintf(stderr, "autotune span=%u tiles=%u prefetch=%u waves/cu=%u nt=%u: %.3f ms\n",
+                      cand.span, cand.tiles_per_wave, cand.prefetch, cand.waves_per_cu,
+                      cand.nontemporal, ms);
             if (ms < best_ms) {
               best_ms = ms;
               best = cand;

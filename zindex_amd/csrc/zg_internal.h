@@ -48,6 +48,7 @@ struct Tuning {
   uint32_t tiles_per_wave = 1;
   uint32_t prefetch = 2;
   uint32_t waves_per_cu = 0;  // 0 = occupancy limit
+  uint32_t nontemporal = 0;   // non-temporal row loads
 };
 
 }  // namespace zg
@@ -90,6 +91,7 @@ struct zg_batch {
   uint32_t blocks = 0;
   zg::KernelConfig kcfg{};
   uint32_t span = 0, n_tg = 0;
+  uint32_t nontemporal = 0;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -64,7 +64,7 @@ struct LookupArgs {
   uint32_t *counts;           // [n_reads * n_colors]
   uint64_t *partial;          // [waves * n_colors]
   uint32_t n_colors;
-  uint32_t pad;
+  uint32_t nontemporal;       // 1: row loads bypass cache retention
 };
 
 struct KernelConfig {

@@ -387,14 +387,21 @@ struct FieldCfg {
 
 // VB bytes at p (any alignment when !aligned; over-reads stay in padding).
 template <int B>
-__device__ __forceinline__ void load_lane(const uint8_t *p, bool aligned,
+__device__ __forceinline__ void load_lane(const uint8_t *p, bool aligned, bool nontemporal,
                                           uint32_t (&x)[FieldCfg<B>::NW]) {
   constexpr int VB = FieldCfg<B>::VB;
   constexpr int NW = FieldCfg<B>::NW;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   if (aligned) {
     if constexpr (VB == 16) {
-      uint4 v = *(const uint4 *)p;
-      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+      if (nontemporal) {
+        // rows are read once per seed: stream them past the caches
+        u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+      } else {
+        uint4 v = *(const uint4 *)p;
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+      }
     } else if constexpr (VB == 8) {
       uint2 v = *(const uint2 *)p;
       x[0] = v.x; x[1] = v.y;
@@ -602,6 +609,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
       }
       if (v && bk.partitions > 1) v = partition_id(key, a.seed, bk.partitions) == bk.partition;
       uint32_t fp = v ? (uint32_t)fingerprint(key, a.seed, a.bits) : 0;
+      // PositionHash::new for this lane's key, broadcast per key below
+      const uint64_t lane_first = mixsplit(key, a.seed);
+      const uint64_t lane_step = splitmix64(lane_first);
+      const uint64_t lane_base = slot_base(lane_first, bk.geo);
       uint32_t rec_start = 0, rec_len = 0;
       if (a.rec.enabled && v) {
         uint32_t h = abandoned_hash(key, a.seed);
@@ -647,9 +658,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
         uint64_t first[D], step[D], base[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
-          first[d] = mixsplit(kk[d], a.seed);
-          step[d] = splitmix64(first[d]);
-          base[d] = slot_base(first[d], bk.geo);
+          int l = jj[d] < 0 ? 0 : jj[d];
+          first[d] = readlane64(lane_first, l);
+          step[d] = readlane64(lane_step, l);
+          base[d] = readlane64(lane_base, l);
         }
         for (uint32_t h0 = 0; h0 < H; h0 += HB) {
           uint32_t raw[D][HB][T][NW];
@@ -666,7 +678,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
                 if (jj[d] >= 0 && i < H && fmask[t]) {
                   uint32_t tile = tg * T + t;
                   load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, aligned,
-                               raw[d][hh][t]);
+                               a.nontemporal != 0, raw[d][hh][t]);
                 } else {
 #pragma unroll
                   for (int w = 0; w < NW; w++) raw[d][hh][t][w] = 0;
