@@ -67,7 +67,27 @@ def parse_args():
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="first copy this many MiB device-to-device (a plain copy kernel of "
                         "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    return p.parse_args()
+    p.add_argument("--mode", choices=("kmers", "minimizers"), default="kmers")
+    p.add_argument("--m", type=int, default=31, help="minimizer size in minimizer mode")
+    p.add_argument("--preset", choices=("batched", "minimizers", "small"), default="batched",
+                   help="batched: 1024 colours x 5 Mbp, 1M lognormal reads (default); "
+                        "minimizers: 4096 colours x 2 Mbp, OCM k=49 m=31, b=16, H=3, "
+                        "200k reads of 10 kbp at 5%% error; small: 10 colours x 2 Mbp, "
+                        "64 reads of 1 kbp")
+    args = p.parse_args()
+    given = {a.lstrip("-").split("=")[0].replace("-", "_") for a in sys.argv[1:] if a.startswith("--")}
+    preset = {
+        "batched": {},
+        "minimizers": dict(colors=4096, color_len=2_000_000, reads=200_000, median=10_000,
+                           sigma=0.0, min_len=10_000, max_len=10_000, error=0.05, bits=16,
+                           hashes=3, k=49, m=31, mode="minimizers", tune_reads=5000),
+        "small": dict(colors=10, color_len=2_000_000, reads=64, median=1000, sigma=0.0,
+                      min_len=1000, max_len=1000, error=0.025, tune_reads=64, plain_reads=64),
+    }[args.preset]
+    for key, value in preset.items():
+        if key not in given:
+            setattr(args, key, value)
+    return args
 
 
 def calibration_copy(mib: int) -> None:
@@ -90,7 +110,8 @@ def calibration_copy(mib: int) -> None:
 def generate(args, zindex_amd, synth):
     rng = np.random.default_rng(args.seed)
     root = synth.random_genome(rng, args.color_len)
-    builder = zindex_amd.Builder(k=args.k, fingerprint_bits=args.bits, hashes=args.hashes)
+    builder = zindex_amd.Builder(k=args.k, fingerprint_bits=args.bits, hashes=args.hashes,
+                                 index_mode=args.mode, minimizer_size=args.m)
     leaves = [None] * args.colors
     t0 = time.time()
     if args.colors & (args.colors - 1) == 0 and args.colors > 1:
@@ -115,8 +136,10 @@ def generate(args, zindex_amd, synth):
 
 
 def workload_name(args, records: bool) -> str:
+    seeds = f"k={args.k}" if args.mode == "kmers" else \
+        f"open-closed mod-minimizers k={args.k} m={args.m}"
     return (f"batched query: {args.colors} colours x {args.color_len} bp substitution tree, "
-            f"k={args.k} H={args.hashes} b={args.bits}; {args.reads} reads lognormal(median "
+            f"{seeds} H={args.hashes} b={args.bits}; {args.reads} reads lognormal(median "
             f"{args.median:.0f}, sigma {args.sigma}) clipped [{args.min_len},{args.max_len}], "
             f"{args.error:.0%} error; " + ("per-read counts" if records else "aggregate counts"))
 
@@ -142,7 +165,11 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
     def keys_for(n):
         parts, koff = [], np.zeros(n + 1, np.uint64)
         for i in range(n):
-            k = zr.scan_kmers(buf[int(off[i]):int(off[i + 1])].tobytes(), args.k)
+            read = buf[int(off[i]):int(off[i + 1])].tobytes()
+            if args.mode == "minimizers":
+                k = zr.scan_ocm(read, args.k, args.m, orc.manifest["seed"])[1]
+            else:
+                k = zr.scan_kmers(read, args.k)
             parts.append(k)
             koff[i + 1] = koff[i] + len(k)
         return (np.concatenate(parts) if parts else np.zeros(0, np.uint64)), koff
@@ -163,7 +190,8 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
         n = min(len(off) - 1, int(n * args.cpu_seconds / max(dt, 1e-3)))
         dt, nk = timed(n)
     return {
-        "value": nk / dt, "unit": "k-mers/s", "cores": threads, "kind": "port",
+        "value": nk / dt, "unit": "k-mers/s" if args.mode == "kmers" else "minimizers/s",
+        "cores": threads, "kind": "port",
         "sample": f"lookup phase of the CPU oracle (oracle/oracle.c, sorted keys with "
                   f"duplicate folding, 64-bit XOR screens) over the first {n} reads "
                   f"({nk} k-mers) of the same batch and index, {dt:.1f}s, keys pre-extracted",
@@ -224,7 +252,10 @@ def main() -> int:
 
     # roofline of the lookup kernel: algorithmic bytes per launch / launch time
     positions = int(off[-1]) + args.reads
-    algo = st["algorithmic_bytes"] + positions * 3 // 8
+    if args.mode == "kmers":  # packed bases and invalid bits read by the lookup
+        algo = st["algorithmic_bytes"] + positions * 3 // 8
+    else:  # explicit minimizer keys
+        algo = st["algorithmic_bytes"] + seeds * 8
     lk_ms = float(np.mean(lookup_ms))
     achieved = algo / (lk_ms / 1000) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -266,7 +297,7 @@ def main() -> int:
     out = {
         "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
         "value": value,
-        "unit": "k-mers/s",
+        "unit": "k-mers/s" if args.mode == "kmers" else "minimizers/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
