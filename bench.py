@@ -312,7 +312,9 @@ def main() -> int:
             "colors": args.colors, "k": args.k, "hashes": args.hashes, "bits": args.bits,
             "reads": args.reads, "bases": int(off[-1]), "kmers_per_step": seeds,
             "bank_bytes": meta["payload_bytes"], "tuning": st["tuning"],
-            "step": "pack ASCII->2-bit + k-mer counts + persistent lookup (HIP graph)",
+            "step": ("pack ASCII->2-bit + k-mer counts" if args.mode == "kmers" else
+                     "pack ASCII->2-bit + open-closed minimizer selection and compaction")
+                    + " + persistent lookup (HIP graph)",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -19,7 +19,6 @@
 #include <deque>
 #include <functional>
 #include <mutex>
-#include <queue>
 #include <thread>
 
 #include "zg_fastx.h"
@@ -232,13 +231,55 @@ static uint64_t config_slots(const zg_build_config &c, uint64_t cardinality) {
   return next_pow2(n);
 }
 
+// Set of cell ids with "largest member" queries: a 64-ary bit tree, so insert,
+// erase and max each touch one word per level.
+class MaxSet {
+ public:
+  explicit MaxSet(uint64_t n) {
+    do {
+      n = (n + 63) / 64;
+      lv_.emplace_back(n, 0);
+    } while (n > 1);
+  }
+  void insert(uint32_t x) {
+    uint64_t v = x;
+    for (auto &l : lv_) {
+      uint64_t &w = l[v >> 6];
+      bool had = w != 0;
+      w |= 1ULL << (v & 63);
+      if (had) return;
+      v >>= 6;
+    }
+  }
+  void erase(uint32_t x) {
+    uint64_t v = x;
+    for (auto &l : lv_) {
+      uint64_t &w = l[v >> 6];
+      w &= ~(1ULL << (v & 63));
+      if (w) return;
+      v >>= 6;
+    }
+  }
+  bool empty() const { return lv_.back()[0] == 0; }
+  uint32_t max() const {  // requires !empty()
+    uint64_t v = 0;
+    for (size_t l = lv_.size(); l-- > 0;) v = v * 64 + 63 - __builtin_clzll(lv_[l][v]);
+    return (uint32_t)v;
+  }
+
+ private:
+  std::vector<std::vector<uint64_t>> lv_;
+};
+
 // Peeling with degree counts and XOR-of-incident-edge accumulators; a cyclic
 // core is broken at the minimum-degree cell (highest cell id on ties) by
 // keeping its first active equation and abandoning the others, the default
 // no-heuristic rule of choose_cycle_break_candidate. `hashed` holds the
 // group's distinct keys as ascending h = mixsplit(key, seed), i.e. ordered by
 // first segment, so the degree/XOR updates walk memory nearly sequentially;
-// every key's H cells are computed once.
+// every key's H cells are computed once. Once peeling stalls, the cells of
+// degree d >= 2 sit in bucket d (a MaxSet), and each cell's equations are
+// listed in ascending id order (counting sort by cell).
 static void construct_group(const zg_build_config &cfg, const std::vector<uint64_t> &hashed,
                             const std::vector<uint64_t> &values, GroupResult &g) {
   const uint64_t S = g.slots;
@@ -278,18 +319,20 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
     if (count[s] == 1) queue.push_back((uint32_t)s);
   size_t remaining = n, qi = 0;
   std::vector<uint32_t> abandoned;
-  // core phase: (cell, equation) pairs of the equations left when peeling stalls
-  std::vector<std::pair<uint32_t, uint32_t>> pairs;
-  std::priority_queue<std::pair<int64_t, uint32_t>> heap;  // (-degree, cell)
+  // core phase state
   bool core = false;
+  std::vector<MaxSet> bucket;          // bucket[d]: cells of degree d >= 2
+  std::vector<uint64_t> adj_off;       // cell -> [adj_off[s], adj_off[s+1]) in adj
+  std::vector<uint32_t> adj;
   auto remove_edge = [&](uint32_t e) {
     for (uint32_t i = 0; i < H; i++) {
       uint32_t s = slots[(size_t)e * H + i];
       if (count[s] > 0) {
+        if (core && count[s] > 1) bucket[count[s]].erase(s);
         count[s]--;
         xr[s] ^= e;
         if (count[s] == 1) queue.push_back(s);
-        if (core && count[s] > 1) heap.push({-(int64_t)count[s], s});
+        if (core && count[s] > 1) bucket[count[s]].insert(s);
       }
     }
   };
@@ -314,26 +357,31 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
     if (progress && qi < queue.size()) continue;
     if (!core) {
       core = true;
-      for (size_t e = 0; e < n; e++) {
-        if (state[e] != 0) continue;
-        for (uint32_t i = 0; i < H; i++) pairs.push_back({slots[e * H + i], (uint32_t)e});
-      }
-      std::sort(pairs.begin(), pairs.end());
-      for (size_t i = 0; i < pairs.size(); i++)
-        if ((i == 0 || pairs[i].first != pairs[i - 1].first) && count[pairs[i].first] > 1)
-          heap.push({-(int64_t)count[pairs[i].first], pairs[i].first});
+      if (trace) fprintf(stderr, "  core: %zu of %zu equations left after %.0f ms\n", remaining, n,
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp[2]).count());
+      uint32_t dmax = 0;
+      for (uint64_t s = 0; s < S; s++) dmax = std::max(dmax, count[s]);
+      for (uint32_t d = 0; d <= dmax; d++) bucket.emplace_back(d >= 2 ? S : 1);
+      for (uint64_t s = 0; s < S; s++)
+        if (count[s] > 1) bucket[count[s]].insert((uint32_t)s);
+      adj_off.assign(S + 1, 0);
+      for (size_t e = 0; e < n; e++)
+        if (state[e] == 0)
+          for (uint32_t i = 0; i < H; i++) adj_off[slots[e * H + i] + 1]++;
+      for (uint64_t s = 0; s < S; s++) adj_off[s + 1] += adj_off[s];
+      adj.resize(adj_off[S]);
+      std::vector<uint64_t> fill(adj_off.begin(), adj_off.end() - 1);
+      for (size_t e = 0; e < n; e++)
+        if (state[e] == 0)
+          for (uint32_t i = 0; i < H; i++) adj[fill[slots[e * H + i]]++] = (uint32_t)e;
     }
     // the minimum-degree cell, highest id on ties
     uint32_t pick = UINT32_MAX;
-    while (!heap.empty()) {
-      auto top = heap.top();
-      heap.pop();
-      uint32_t s = top.second;
-      if ((int64_t)count[s] == -top.first && count[s] > 1) {
-        pick = s;
+    for (size_t d = 2; d < bucket.size(); d++)
+      if (!bucket[d].empty()) {
+        pick = bucket[d].max();
         break;
       }
-    }
     if (pick == UINT32_MAX) {
       // no multi-degree cell left: abandon the first active equation
       for (size_t e = 0; e < n; e++)
@@ -346,14 +394,9 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
         }
       continue;
     }
-    auto range = std::equal_range(pairs.begin(), pairs.end(), std::make_pair(pick, 0u),
-                                  [](const std::pair<uint32_t, uint32_t> &x,
-                                     const std::pair<uint32_t, uint32_t> &y) {
-                                    return x.first < y.first;
-                                  });
     bool kept = false;
-    for (auto it = range.first; it != range.second; ++it) {
-      uint32_t e = it->second;
+    for (uint64_t j = adj_off[pick]; j < adj_off[pick + 1]; j++) {
+      uint32_t e = adj[j];
       if (state[e] != 0) continue;
       if (!kept) {
         kept = true;
