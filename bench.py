@@ -69,11 +69,20 @@ def parse_args():
                         "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     p.add_argument("--mode", choices=("kmers", "minimizers"), default="kmers")
     p.add_argument("--m", type=int, default=31, help="minimizer size in minimizer mode")
-    p.add_argument("--preset", choices=("batched", "minimizers", "small"), default="batched",
+    p.add_argument("--preset", choices=("batched", "minimizers", "serving", "small"),
+                   default="batched",
                    help="batched: 1024 colours x 5 Mbp, 1M lognormal reads (default); "
                         "minimizers: 4096 colours x 2 Mbp, OCM k=49 m=31, b=16, H=3, "
-                        "200k reads of 10 kbp at 5%% error; small: 10 colours x 2 Mbp, "
-                        "64 reads of 1 kbp")
+                        "200k reads of 10 kbp at 5%% error; serving: 8192 colours x 1 Mbp, "
+                        "Poisson requests of U{1..10000} reads (median 1 kbp) through the "
+                        "continuous-batching server and the plain per-request path; "
+                        "small: 10 colours x 2 Mbp, 64 reads of 1 kbp")
+    p.add_argument("--requests", type=int, default=300, help="serving: requests per pass")
+    p.add_argument("--max-request-reads", type=int, default=10000)
+    p.add_argument("--qps", default="8,12",
+                   help="serving: Poisson arrival rates (requests/s), comma-separated; the "
+                        "first is the headline")
+    p.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     args = p.parse_args()
     given = {a.lstrip("-").split("=")[0].replace("-", "_") for a in sys.argv[1:] if a.startswith("--")}
     preset = {
@@ -81,6 +90,8 @@ def parse_args():
         "minimizers": dict(colors=4096, color_len=2_000_000, reads=200_000, median=10_000,
                            sigma=0.0, min_len=10_000, max_len=10_000, error=0.05, bits=16,
                            hashes=3, k=49, m=31, mode="minimizers", tune_reads=5000),
+        "serving": dict(colors=8192, color_len=1_000_000, median=1000, sigma=1.0, min_len=100,
+                        max_len=100_000, error=0.02, bits=8, hashes=4, k=31, tune_reads=20000),
         "small": dict(colors=10, color_len=2_000_000, reads=64, median=1000, sigma=0.0,
                       min_len=1000, max_len=1000, error=0.025, tune_reads=64, plain_reads=64),
     }[args.preset]
@@ -198,6 +209,158 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
     }, parity, n_par
 
 
+def percentile(values, q):
+    return float(np.percentile(np.asarray(values, dtype=np.float64), q)) if len(values) else None
+
+
+def serve_pass(zindex_amd, idx, seq, off, starts, sizes, arrivals, slots, max_requests):
+    """One pass of requests through a server: request i (reads starts[i] ..
+    starts[i]+sizes[i]) is submitted at arrivals[i] seconds after the start.
+    Returns the requests' stats, the wall time and the aggregates."""
+    slot_bases = max(int(off[a + n] - off[a]) for a, n in zip(starts, sizes))
+    srv = zindex_amd.Server(idx, slots=slots, max_requests=max_requests,
+                            slot_reads=max(32768, int(max(sizes))),
+                            slot_bases=max(64 << 20, slot_bases))
+    tickets = []
+    t0 = time.perf_counter()
+    for a, n, at in zip(starts, sizes, arrivals):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(srv.submit(seq, off[a: a + n + 1], records=False, aggregate=True))
+    stats, aggs = [], []
+    for t in tickets:
+        _, _, agg, st = srv.wait(t)
+        stats.append(st)
+        aggs.append(agg)
+    wall = time.perf_counter() - t0
+    sst = srv.stats()
+    srv.close()
+    return stats, wall, aggs, sst
+
+
+def serve_main(args, zindex_amd, synth) -> int:
+    """The serving workload: Poisson arrivals of requests at a fixed rate through
+    the continuous-batching server, and the same arrivals through the plain
+    per-request path (one request per batch, one batch at a time)."""
+    rng = np.random.default_rng(args.seed + 1)
+    sizes = rng.integers(1, args.max_request_reads + 1, args.requests)
+    args.reads = int(sizes.sum())
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    idx, buf, off = generate(args, zindex_amd, synth)
+    t0 = time.time()
+    idx.load()
+    meta = idx.meta
+    log(f"index resident in HBM ({meta['payload_bytes'] / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
+    if args.tuning:
+        vals = [int(x) for x in args.tuning.split(",")]
+        idx.tuning = {"span": vals[0], "tiles_per_wave": vals[1], "prefetch": vals[2],
+                      "waves_per_cu": vals[3], "nontemporal": vals[4] if len(vals) > 4 else 0}
+    elif not args.no_autotune:
+        n_t = min(args.tune_reads, args.reads)
+        with idx.batch(buf[: int(off[n_t])], off[: n_t + 1], records=True) as tb:
+            log(f"autotuned on {n_t} reads: {tb.autotune()}")
+    # the requests' reads in page-locked memory (DMA straight from the caller)
+    pinned = zindex_amd.HostBuffer((len(buf),), np.uint8)
+    pinned.array[:] = buf
+    seq = pinned.array
+    man = idx.manifest()
+    per_lookup = sum(meta["hashes"] * ((len(b["groups"]) * meta["fingerprint_bits"] + 7) // 8)
+                     // ((b.get("routing") or {}).get("partitions", 1)) for b in man["banks"])
+    # warm-up: a few requests through each server shape
+    for slots, mr in ((args.slots, 256), (1, 1)):
+        serve_pass(zindex_amd, idx, seq, off, starts[:4], sizes[:4], np.zeros(4), slots, mr)
+    results = {}
+    zero = np.zeros(args.requests)
+    for name, slots, mr in (("batched", args.slots, 256), ("plain", 1, 1)):
+        st, wall, aggs, sst = serve_pass(zindex_amd, idx, seq, off, starts, sizes, zero, slots, mr)
+        seeds = sum(x["seeds"] for x in st)
+        results[name] = {"saturation_kmers_per_s": seeds / wall, "wall_s": wall,
+                         "seeds": seeds, "batches": sst["batches"],
+                         "mean_requests_per_batch": args.requests / max(1, sst["batches"]),
+                         "lookup_ms_sum": sst["lookup_ms"],
+                         "device_bytes": sst["device_bytes"]}
+        results[name + "_aggs"] = aggs
+        log(f"{name} closed loop: {seeds / wall:.3e} k-mers/s, {sst['batches']} batches, "
+            f"{wall:.2f}s")
+    batched_aggs, plain_aggs = results.pop("batched_aggs"), results.pop("plain_aggs")
+    same = all(np.array_equal(a, b) for a, b in zip(batched_aggs, plain_aggs))
+    # requests alone through the batch query, on a sample, against the mixed pass
+    alone_same = True
+    for i in [int(i) for i in np.linspace(0, args.requests - 1, 6)]:
+        a, n = int(starts[i]), int(sizes[i])
+        _, _, agg = idx.query_reads(buf[int(off[a]): int(off[a + n])], off[a: a + n + 1] - off[a])
+        alone_same &= bool(np.array_equal(agg, batched_aggs[i]))
+    log(f"mixed == plain: {same}; alone == mixed on 6 requests: {alone_same}")
+    sweep = []
+    for q in [float(x) for x in args.qps.split(",")]:
+        arrivals = np.cumsum(np.random.default_rng(args.seed + 2).exponential(1.0 / q, args.requests))
+        row = {"qps": q}
+        for name, slots, mr in (("batched", args.slots, 256), ("plain", 1, 1)):
+            st, wall, aggs, sst = serve_pass(zindex_amd, idx, seq, off, starts, sizes, arrivals,
+                                             slots, mr)
+            lat = [x["latency_ms"] for x in st]
+            seeds = sum(x["seeds"] for x in st)
+            row[name] = {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                         "mean_ms": float(np.mean(lat)), "kmers_per_s": seeds / wall,
+                         "mean_requests_per_batch": args.requests / max(1, sst["batches"]),
+                         "max_requests_per_batch": sst["max_batch_requests"]}
+            log(f"qps {q} {name}: p50 {row[name]['p50_ms']:.1f} ms, p99 {row[name]['p99_ms']:.1f} ms, "
+                f"{seeds / wall:.3e} k-mers/s")
+        sweep.append(row)
+    b = results["batched"]
+    algo = b["seeds"] * per_lookup
+    kernel_gbs = algo / (b["lookup_ms_sum"] / 1000) / 1e9 if b["lookup_ms_sum"] > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        cpu, _, _ = cpu_baseline(args, idx, buf, off, None)
+        log(f"cpu baseline: {cpu['value']:.3e} k-mers/s on {cpu['cores']} threads")
+    pinned.free()
+    head = sweep[0]
+    out = {
+        "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
+        "value": b["saturation_kmers_per_s"],
+        "unit": "k-mers/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 4,
+        "ms_per_step": b["wall_s"] * 1000 / args.requests,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8" if args.bits == 8 else f"u{args.bits}",
+        "data": "synthetic (seeded substitution tree colours and sampled reads)",
+        "config": {
+            "workload": (f"serving: {args.colors} colours x {args.color_len} bp substitution tree, "
+                         f"k={args.k} H={args.hashes} b={args.bits}; {args.requests} requests of "
+                         f"U{{1..{args.max_request_reads}}} reads, lognormal(median "
+                         f"{args.median:.0f}, sigma {args.sigma}) clipped [{args.min_len},"
+                         f"{args.max_len}], {args.error:.0%} error; Poisson arrivals at "
+                         f"{head['qps']} requests/s; per-request aggregate counts"),
+            "colors": args.colors, "reads": args.reads, "bases": int(off[-1]),
+            "bank_bytes": meta["payload_bytes"], "slots": args.slots, "tuning": idx.tuning,
+            "step": "one request: H2D from page-locked memory, pack, k-mers, persistent lookup, "
+                    "per-request sums, D2H",
+        },
+        "latency": {"qps": head["qps"], "batched_p50_ms": head["batched"]["p50_ms"],
+                    "batched_p99_ms": head["batched"]["p99_ms"],
+                    "plain_p50_ms": head["plain"]["p50_ms"], "plain_p99_ms": head["plain"]["p99_ms"]},
+        "qps_sweep": sweep,
+        "saturation": {k: results[k] for k in ("batched", "plain")},
+        "roofline": {"bound": "hbm", "achieved": round(kernel_gbs, 1) if kernel_gbs else None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kernel_gbs / HBM_PEAK_GBS, 4) if kernel_gbs else None,
+                     "traffic": None, "kernel": "lookup_kernel",
+                     "note": "row bytes of the closed-loop batched pass over the summed HIP-event "
+                             "time of its lookups (overlapping slots are summed)"},
+        "cpu_baseline": cpu,
+        "mixed_equals_plain": same,
+        "alone_equals_mixed": alone_same,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
@@ -206,6 +369,8 @@ def main() -> int:
         return 2
     import zindex_amd
     from zindex_amd import synth
+    if args.preset == "serving":
+        return serve_main(args, zindex_amd, synth)
 
     records = not args.aggregate
     if args.calibrate_copy_mib:

@@ -234,6 +234,80 @@ int zg_index_get_tuning(const zg_index *idx, zg_tuning *out);
 /* Times candidate tunings on the batch and keeps the fastest. */
 int zg_autotune(zg_index *idx, zg_batch *batch, zg_tuning *chosen);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of query requests over a resident index      */
+/* ------------------------------------------------------------------------ */
+/* The reference has no server; its query driver (query_ordinary,
+ * query_stream.rs:163-452) answers one file per process. A server keeps the
+ * index resident and answers requests as they arrive. Each of `slots` worker
+ * threads owns a HIP stream and device buffers for one batch; whenever a slot
+ * is free it admits the queued requests, in arrival order, that fit its
+ * capacity, so batches formed while others are on the GPU overlap with them
+ * (copies, seeding and the tail of one persistent lookup with the next). A
+ * request's counts do not depend on what it was batched with. */
+typedef struct zg_server zg_server;
+
+typedef struct zg_server_config {
+  uint32_t slots;         /* batches in flight (one stream each); default 2 */
+  uint32_t max_requests;  /* requests admitted into one batch; default 256 */
+  uint64_t slot_reads;    /* reads per batch; default 32768 */
+  uint64_t slot_bases;    /* bases per batch; default 64 Mi */
+  uint64_t hbm_budget;    /* bytes for the resident index plus all slots (0 = none) */
+  uint32_t flags;         /* ZG_QUERY_NO_RECOVERY applies to every request */
+  uint32_t reserved;
+} zg_server_config;
+
+typedef struct zg_request_stats {
+  uint64_t ticket;
+  uint64_t batch_id;        /* batches are numbered from 1 in admission order */
+  uint32_t batch_requests;  /* requests in that batch */
+  int32_t status;           /* ZG_OK or the ZG_ERR_* code of the batch */
+  uint64_t n_reads;
+  uint64_t seeds;           /* seed occurrences over the request's reads */
+  double queued_ms;         /* submit -> admitted into a batch */
+  double service_ms;        /* admitted -> results written */
+  double latency_ms;        /* submit -> results written */
+} zg_request_stats;
+
+typedef struct zg_server_stats {
+  uint64_t requests;        /* completed */
+  uint64_t batches;
+  uint64_t reads;
+  uint64_t seeds;
+  uint32_t max_batch_requests;
+  uint32_t slots;
+  double lookup_ms;         /* summed HIP-event time of the batches' lookups */
+  uint64_t device_bytes;    /* resident index + slot buffers */
+} zg_server_stats;
+
+void zg_server_config_default(zg_server_config *cfg);
+/* The index must be resident (zg_index_load) and stay open and loaded until
+ * zg_server_destroy returns. Fails with ZG_ERR_BUDGET when the index and the
+ * slots do not fit hbm_budget. */
+int zg_server_create(zg_index *idx, const zg_server_config *cfg, zg_server **out);
+/* Queues one request: reads seq[read_off[i] .. read_off[i+1]) (ASCII). The
+ * input and the outputs stay owned by the caller and must remain valid until
+ * the request is waited for. Outputs, each optional: totals[n_reads] (seed
+ * occurrences per read), counts[n_reads x colours] (u32 matches per read and
+ * colour, row-major) and aggregate[colours] (u64 sums over the reads). A
+ * request larger than one slot fails with ZG_ERR_BUDGET. Never blocks. */
+int zg_server_submit(zg_server *srv, const uint8_t *seq, const uint64_t *read_off,
+                     uint64_t n_reads, uint64_t *totals, uint32_t *counts,
+                     uint64_t *aggregate, uint64_t *ticket);
+/* Blocks until the request is answered; returns its status. Each ticket is
+ * waited for (or polled to completion) once. */
+int zg_server_wait(zg_server *srv, uint64_t ticket, zg_request_stats *st);
+/* 1 when answered (st filled, ticket released), 0 while queued or running. */
+int zg_server_poll(zg_server *srv, uint64_t ticket, zg_request_stats *st);
+int zg_server_get_stats(zg_server *srv, zg_server_stats *out);
+/* Answers every queued request, then releases the slots. */
+int zg_server_destroy(zg_server *srv);
+
+/* Page-locked host memory for inputs and outputs of requests and batches
+ * (DMA without staging copies). */
+int zg_host_alloc(uint64_t bytes, void **out);
+int zg_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

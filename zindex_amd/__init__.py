@@ -14,11 +14,11 @@ from typing import Iterable, Sequence
 
 import numpy as np
 
-from ._lib import (BatchStats, BuildConfig, IndexMeta, Tuning, ZG_QUERY_NO_RECOVERY,
-                   ZG_QUERY_RECORDS, ZindexError, check, lib)
+from ._lib import (BatchStats, BuildConfig, IndexMeta, RequestStats, ServerConfig, ServerStats,
+                   Tuning, ZG_QUERY_NO_RECOVERY, ZG_QUERY_RECORDS, ZindexError, check, lib)
 
 __all__ = [
-    "Index", "Builder", "Batch", "ZindexError", "build", "build_config",
+    "Index", "Builder", "Batch", "Server", "HostBuffer", "ZindexError", "build", "build_config",
     "query_command", "query_command_with_memory", "query_command_with_recovery",
     "query_records_command", "query_records_command_with_memory", "info_command",
     "pack_reads",
@@ -255,6 +255,113 @@ class Batch:
         out = {name: getattr(s, name) for name, _ in BatchStats._fields_ if name != "tuning"}
         out["tuning"] = {name: getattr(s.tuning, name) for name, _ in Tuning._fields_}
         return out
+
+
+class HostBuffer:
+    """Page-locked host memory (zg_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, shape, dtype):
+        dtype = np.dtype(dtype)
+        n = int(np.prod(shape)) * dtype.itemsize
+        self._p = C.c_void_p()
+        check(lib().zg_host_alloc(max(n, 1), C.byref(self._p)))
+        raw = (C.c_uint8 * max(n, 1)).from_address(self._p.value)
+        self.array = np.frombuffer(raw, dtype=np.uint8, count=n).view(dtype).reshape(shape)
+
+    def free(self) -> None:
+        if self._p:
+            self.array = None
+            lib().zg_host_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server over a resident index (zg_server_*).
+
+    submit() queues a request and returns its ticket at once; wait(ticket)
+    blocks until its results are written and returns (totals, counts,
+    aggregate, stats). Buffers of queued requests are kept alive here."""
+
+    def __init__(self, index: Index, slots: int = 2, max_requests: int = 256,
+                 slot_reads: int = 32768, slot_bases: int = 64 << 20, hbm_budget: int = 0,
+                 recover: bool = True):
+        cfg = ServerConfig()
+        lib().zg_server_config_default(C.byref(cfg))
+        cfg.slots, cfg.max_requests = slots, max_requests
+        cfg.slot_reads, cfg.slot_bases, cfg.hbm_budget = slot_reads, slot_bases, hbm_budget
+        cfg.flags = 0 if recover else ZG_QUERY_NO_RECOVERY
+        self.index = index
+        self.colors = index.meta["n_colors"]
+        self._h = C.c_void_p()
+        self._live = {}
+        check(lib().zg_server_create(index._h, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, reads, offsets=None, records: bool = True, aggregate: bool = True,
+               out=None) -> int:
+        """out: optional (totals, counts) arrays to write into (e.g. HostBuffer
+        views); otherwise fresh arrays are allocated."""
+        buf, off = _as_reads(reads, offsets)
+        n = len(off) - 1
+        if out is not None:
+            totals, counts = out
+        else:
+            totals = np.zeros(max(n, 1), np.uint64)
+            counts = np.zeros((max(n, 1), self.colors), np.uint32) if records else None
+        agg = np.zeros(self.colors, np.uint64) if aggregate else None
+        ticket = C.c_uint64()
+        check(lib().zg_server_submit(self._h, buf.ctypes.data_as(C.c_void_p), _u64p(off), n,
+                                     _u64p(totals), _u32p(counts) if counts is not None else None,
+                                     _u64p(agg) if agg is not None else None, C.byref(ticket)))
+        self._live[ticket.value] = (buf, off, n, totals, counts, agg)
+        return ticket.value
+
+    def wait(self, ticket: int):
+        st = RequestStats()
+        rc = lib().zg_server_wait(self._h, ticket, C.byref(st))
+        buf, off, n, totals, counts, agg = self._live.pop(ticket)
+        check(rc)
+        stats = {name: getattr(st, name) for name, _ in RequestStats._fields_}
+        return totals[:n], (counts[:n] if counts is not None else None), agg, stats
+
+    def poll(self, ticket: int):
+        """None while the request is queued or running, else what wait() returns."""
+        st = RequestStats()
+        rc = lib().zg_server_poll(self._h, ticket, C.byref(st))
+        if rc == 0:
+            return None
+        buf, off, n, totals, counts, agg = self._live.pop(ticket)
+        check(0 if rc == 1 else rc)
+        stats = {name: getattr(st, name) for name, _ in RequestStats._fields_}
+        return totals[:n], (counts[:n] if counts is not None else None), agg, stats
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(lib().zg_server_get_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().zg_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Builder:

@@ -66,6 +66,30 @@ class BatchStats(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("slots", C.c_uint32), ("max_requests", C.c_uint32), ("slot_reads", C.c_uint64),
+        ("slot_bases", C.c_uint64), ("hbm_budget", C.c_uint64), ("flags", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64), ("batch_id", C.c_uint64), ("batch_requests", C.c_uint32),
+        ("status", C.c_int32), ("n_reads", C.c_uint64), ("seeds", C.c_uint64),
+        ("queued_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("requests", C.c_uint64), ("batches", C.c_uint64), ("reads", C.c_uint64),
+        ("seeds", C.c_uint64), ("max_batch_requests", C.c_uint32), ("slots", C.c_uint32),
+        ("lookup_ms", C.c_double), ("device_bytes", C.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "zg_last_error": (C.c_char_p, []),
@@ -106,6 +130,16 @@ SIGNATURES = {
     "zg_index_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "zg_index_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "zg_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Tuning)]),
+    "zg_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "zg_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "zg_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, u64p, u32p, u64p,
+                                   u64p]),
+    "zg_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStats)]),
+    "zg_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStats)]),
+    "zg_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "zg_server_destroy": (C.c_int, [C.c_void_p]),
+    "zg_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "zg_host_free": (C.c_int, [C.c_void_p]),
 }
 
 

@@ -228,6 +228,14 @@ void upload_index(zg_index *idx, uint64_t budget) {
   idx->on_device = true;
 }
 
+uint64_t resident_bytes(const zg_index *idx) {
+  uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->inner_table.bytes;
+  for (auto &d : idx->dbanks)
+    n += d.data.bytes + d.col_color.bytes + d.mg_off.bytes + d.mg_members.bytes +
+         d.mg_mask_bit.bytes + d.color_col.bytes + d.color_bit.bytes;
+  return n;
+}
+
 }  // namespace zg
 
 using namespace zg;
@@ -415,7 +423,7 @@ static uint32_t max_tiles(const zg_index *idx) {
 }
 
 // Launch shape for the current tuning.
-static void configure(zg_index *idx, zg_batch *b) {
+void configure(zg_index *idx, zg_batch *b) {
   const Tuning &t = idx->tuning;
   KernelConfig c{};
   c.bits = (uint32_t)idx->m.fingerprint_bits;
@@ -471,8 +479,8 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
 }
 
 // Zero the outputs of a run (direct launches, before the lookups).
-static void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
-  hipStream_t s = idx->stream;
+void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
   const uint32_t C = (uint32_t)idx->m.colors.size();
   if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
   else launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
@@ -481,8 +489,8 @@ static void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
 
 // The lookups: reset the work counters, one persistent launch per bank. This
 // is the part captured in the HIP graph.
-static void enqueue_lookups(zg_index *idx, zg_batch *b) {
-  hipStream_t s = idx->stream;
+void enqueue_lookups(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
   launch_fill_u32(b->queue.as<uint32_t>(), idx->dbanks.size(), 0, s);
   bool any = b->explicit_keys ? b->n_keys > 0 : b->positions > 0;
   if (any)
@@ -491,7 +499,7 @@ static void enqueue_lookups(zg_index *idx, zg_batch *b) {
 
 // Per-colour sums after the lookups.
 static void enqueue_reduce(zg_index *idx, zg_batch *b) {
-  hipStream_t s = idx->stream;
+  hipStream_t s = b->stream;
   const uint32_t C = (uint32_t)idx->m.colors.size();
   if (!b->records) launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
   else launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
@@ -515,8 +523,8 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-static void seed_batch(zg_index *idx, zg_batch *b) {
-  hipStream_t s = idx->stream;
+void seed_batch(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
   const Manifest &m = idx->m;
   check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
   uint64_t cwords = (b->positions + 31) / 32 + 4, iwords = (b->positions + 63) / 64 + 4;
@@ -532,17 +540,17 @@ static void seed_batch(zg_index *idx, zg_batch *b) {
     check_hip(hipEventSynchronize(b->ev[1]), "seed");
   } else {
     const uint64_t blocks = (b->positions + ocm_block_positions() - 1) / ocm_block_positions();
-    b->flag.alloc(b->positions + 64);
-    b->delta.alloc(b->positions + 64);
-    b->block_counts.alloc((blocks + 1) * sizeof(uint32_t));
-    b->block_off.alloc((blocks + 1) * sizeof(uint64_t));
+    b->flag.reserve(b->positions + 64);
+    b->delta.reserve(b->positions + 64);
+    b->block_counts.reserve((blocks + 1) * sizeof(uint32_t));
+    b->block_off.reserve((blocks + 1) * sizeof(uint64_t));
     launch_ocm_select(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), b->pos_off.as<uint64_t>(),
                       b->n_reads, b->positions, (uint32_t)m.k, (uint32_t)m.minimizer_size, m.seed,
                       idx->inner_table.as<uint64_t>(), b->flag.as<uint8_t>(),
                       b->delta.as<uint8_t>(), b->block_counts.as<uint32_t>(), nullptr, s);
     launch_fill_u32(b->block_counts.as<uint32_t>() + blocks, 1, 0, s);
     size_t tb = scan_temp_bytes_u32_to_u64(blocks + 1);
-    b->scan_temp.alloc(tb);
+    b->scan_temp.reserve(tb);
     launch_exclusive_scan_u32_to_u64(b->block_counts.as<uint32_t>(), b->block_off.as<uint64_t>(),
                                      blocks + 1, b->scan_temp.p, tb, s);
     uint64_t total = 0;
@@ -550,12 +558,12 @@ static void seed_batch(zg_index *idx, zg_batch *b) {
               "hipMemcpyAsync");
     check_hip(hipStreamSynchronize(s), "seed");
     b->n_keys = total;
-    b->keys.alloc((total + 64) * 8);
-    b->key_win.alloc((total + 64) * 8);
+    b->keys.reserve((total + 64) * 8);
+    b->key_win.reserve((total + 64) * 8);
     launch_ocm_compact(b->codes.as<uint64_t>(), b->flag.as<uint8_t>(), b->delta.as<uint8_t>(),
                        b->block_off.as<uint64_t>(), b->positions, (uint32_t)m.minimizer_size,
                        b->keys.as<uint64_t>(), b->key_win.as<uint64_t>(), s);
-    b->key_off.alloc((b->n_reads + 1) * 8);
+    b->key_off.reserve((b->n_reads + 1) * 8);
     launch_segment_bounds(b->key_win.as<uint64_t>(), total, b->pos_off.as<uint64_t>(), b->n_reads,
                           b->key_off.as<uint64_t>(), s);
     launch_diff(b->key_off.as<uint64_t>(), b->n_reads, b->totals.as<uint64_t>(), s);
@@ -583,7 +591,8 @@ extern "C" int zg_batch_create(zg_index *idx, const uint8_t *seq, const uint64_t
     b->flags = flags;
     b->records = (flags & ZG_QUERY_RECORDS) != 0;
     b->n_reads = n_reads;
-    hipStream_t s = idx->stream;
+    b->stream = idx->stream;
+    hipStream_t s = b->stream;
     for (auto &e : b->ev) check_hip(hipEventCreate(&e), "hipEventCreate");
     uint64_t r0 = n_reads ? read_off[0] : 0;
     b->bases = n_reads ? read_off[n_reads] - r0 : 0;
@@ -633,7 +642,7 @@ extern "C" int zg_batch_reseed(zg_index *idx, zg_batch *b) {
 extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
   try {
     ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
-    hipStream_t s = idx->stream;
+    hipStream_t s = b->stream;
     if (!b->graph_exec && !b->graph_failed) {
       // Capture the lookups once; replays reuse the instantiated graph. If any
       // step of the capture is refused, fall back to direct launches.
@@ -683,7 +692,7 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
     ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
     ensure(first <= b->n_reads && count <= b->n_reads - first, "read range out of bounds",
            ZG_ERR_INVALID);
-    hipStream_t s = idx->stream;
+    hipStream_t s = b->stream;
     const uint32_t C = (uint32_t)idx->m.colors.size();
     const size_t nb = idx->dbanks.size();
     zg::DevBuf queues;
@@ -729,7 +738,7 @@ extern "C" int zg_batch_results(zg_batch *b, uint64_t *totals, uint32_t *counts,
                                 uint64_t *aggregate) {
   try {
     ensure(b, "null argument", ZG_ERR_INVALID);
-    hipStream_t s = b->idx->stream;
+    hipStream_t s = b->stream;
     const uint32_t C = (uint32_t)b->idx->m.colors.size();
     if (totals && b->n_reads)
       check_hip(hipMemcpyAsync(totals, b->totals.p, b->n_reads * 8, hipMemcpyDeviceToHost, s),

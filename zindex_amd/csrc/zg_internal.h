@@ -32,6 +32,10 @@ struct DevBuf {
     bytes = 0;
   }
   void alloc(size_t n);
+  // Grows to at least n bytes (contents are not preserved when it grows).
+  void reserve(size_t n) {
+    if (!p || bytes < n) alloc(n);
+  }
   template <class T> T *as() const { return (T *)p; }
 };
 
@@ -76,6 +80,7 @@ struct zg_index {
 
 struct zg_batch {
   zg_index *idx = nullptr;
+  hipStream_t stream = nullptr;     // the index's stream, or a server slot's own
   uint32_t flags = 0;
   bool records = false;
   uint64_t n_reads = 0, bases = 0, positions = 0;
@@ -109,6 +114,15 @@ unsigned zg_host_threads();
 namespace zg {
 // Builds the device copy of idx (banks, column tables, recovery directory).
 void upload_index(zg_index *idx, uint64_t budget);
+// Device bytes held by the resident index.
+uint64_t resident_bytes(const zg_index *idx);
+// Batch pipeline pieces, all enqueued on b->stream (zg_api.hip).
+void configure(zg_index *idx, zg_batch *b);
+void seed_batch(zg_index *idx, zg_batch *b);
+void enqueue_zero_outputs(zg_index *idx, zg_batch *b);
+void enqueue_lookups(zg_index *idx, zg_batch *b);
+double elapsed_ms(hipEvent_t a, hipEvent_t b);
+int translate_exception();
 // Host-side helpers shared with the builder.
 std::vector<uint64_t> ocm_inner_table(uint32_t s, uint64_t seed);
 }  // namespace zg

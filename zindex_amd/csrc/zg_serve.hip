@@ -1,0 +1,472 @@
+// zg_serve.hip -- a request server with continuous batching over a resident
+// index. Requests queue in arrival order; each slot (a worker thread with its
+// own HIP stream and batch buffers) takes the queued requests that fit it as
+// soon as its previous batch is answered, runs the batch pipeline
+// (H2D -> pack -> seeds -> persistent lookup per bank -> per-request sums ->
+// D2H into the callers' buffers) and answers them. With two or more slots the
+// next batch is copied, seeded and launched while the previous one still runs.
+//
+// A request's counts are per read: the lookup writes counter cells of the
+// batch's own reads only (the read index space of a batch is the
+// concatenation of its requests), so what a request shares a batch with never
+// changes its result.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "zg_internal.h"
+
+namespace zg {
+
+void set_error(const std::string &msg);  // zg_api.hip
+
+using Clock = std::chrono::steady_clock;
+
+struct ServeRequest {
+  uint64_t ticket = 0;
+  const uint8_t *seq = nullptr;
+  const uint64_t *read_off = nullptr;
+  uint64_t n_reads = 0, bases = 0;
+  uint64_t *totals = nullptr;
+  uint32_t *counts = nullptr;
+  uint64_t *aggregate = nullptr;
+  Clock::time_point t_submit, t_admit, t_done;
+  uint64_t batch_id = 0;
+  uint32_t batch_requests = 0;
+  int status = 1;  // 1 while queued or running
+  std::string error;
+  uint64_t seeds = 0;
+};
+using RequestPtr = std::shared_ptr<ServeRequest>;
+
+struct ServeSlot {
+  zg_batch b;                     // buffers at slot capacity, own stream
+  DevBuf req_agg;                 // per-request colour sums
+  std::vector<uint64_t> h_roff, h_poff, h_totals;
+  std::thread thread;
+  ~ServeSlot() {
+    hipStream_t s = b.stream;
+    b.stream = nullptr;
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+static double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace zg
+
+struct zg_server {
+  zg_index *idx = nullptr;
+  zg_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<zg::RequestPtr> pending;
+  std::unordered_map<uint64_t, zg::RequestPtr> live;  // submitted, not yet waited for
+  uint64_t next_ticket = 1, next_batch = 1;
+  bool stopping = false;
+  std::vector<std::unique_ptr<zg::ServeSlot>> slots;
+  uint64_t device_bytes = 0;
+  // statistics (under mu)
+  uint64_t done_requests = 0, batches = 0, reads = 0, seeds = 0;
+  uint32_t max_batch_requests = 0;
+  double lookup_ms = 0;
+  void worker(zg::ServeSlot *slot);
+  void run_batch(zg::ServeSlot *slot, const std::vector<zg::RequestPtr> &batch);
+};
+
+using namespace zg;
+
+// Device bytes of one slot at the configured capacity.
+static uint64_t slot_bytes(const zg_index *idx, const zg_server_config &c) {
+  const uint64_t C = idx->m.colors.size();
+  const uint64_t pos = c.slot_bases + c.slot_reads;
+  uint64_t n = (c.slot_bases + 64) + 2 * (c.slot_reads + 1) * 8;  // seq, offsets
+  n += ((pos + 31) / 32 + 4) * 8 + ((pos + 63) / 64 + 4) * 8;     // codes, invalid bits
+  n += (c.slot_reads + 1) * 8;                                    // totals
+  n += std::max<uint64_t>(1, c.slot_reads * C) * 4;               // counters
+  n += C * 8 + 8 + (uint64_t)c.max_requests * C * 8;              // sums
+  n += (idx->dbanks.size() + 1) * 4;                              // work counters
+  if (idx->m.index_mode == 1) {
+    const uint64_t blocks = (pos + ocm_block_positions() - 1) / ocm_block_positions();
+    n += 2 * (pos + 64) + (blocks + 1) * 12 + 2 * (pos + 64) * 8 + (c.slot_reads + 1) * 8;
+    n += scan_temp_bytes_u32_to_u64(blocks + 1);
+  }
+  return n;
+}
+
+static void init_slot(zg_index *idx, const zg_server_config &c, ServeSlot &sl) {
+  zg_batch &b = sl.b;
+  b.idx = idx;
+  b.flags = ZG_QUERY_RECORDS | (c.flags & ZG_QUERY_NO_RECOVERY);
+  b.records = true;
+  check_hip(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : b.ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+  const uint64_t C = idx->m.colors.size();
+  const uint64_t pos = c.slot_bases + c.slot_reads;
+  b.seq.alloc(c.slot_bases + 64);
+  b.read_off.alloc((c.slot_reads + 1) * 8);
+  b.pos_off.alloc((c.slot_reads + 1) * 8);
+  b.codes.alloc(((pos + 31) / 32 + 4) * 8);
+  b.invalid.alloc(((pos + 63) / 64 + 4) * 8);
+  b.totals.alloc((c.slot_reads + 1) * 8);
+  b.counts.alloc(std::max<uint64_t>(1, c.slot_reads * C) * 4);
+  b.agg.alloc(C * 8 + 8);
+  b.queue.alloc((idx->dbanks.size() + 1) * 4);
+  sl.req_agg.alloc(std::max<uint64_t>(1, (uint64_t)c.max_requests * C) * 8);
+  if (idx->m.index_mode == 1) {
+    // the OCM buffers at capacity, so batches never allocate
+    const uint64_t blocks = (pos + ocm_block_positions() - 1) / ocm_block_positions();
+    b.flag.alloc(pos + 64);
+    b.delta.alloc(pos + 64);
+    b.block_counts.alloc((blocks + 1) * 4);
+    b.block_off.alloc((blocks + 1) * 8);
+    b.keys.alloc((pos + 64) * 8);
+    b.key_win.alloc((pos + 64) * 8);
+    b.key_off.alloc((c.slot_reads + 1) * 8);
+    b.scan_temp.alloc(scan_temp_bytes_u32_to_u64(blocks + 1));
+  }
+  sl.h_roff.reserve(c.slot_reads + 1);
+  sl.h_poff.reserve(c.slot_reads + 1);
+  sl.h_totals.reserve(c.slot_reads + 1);
+}
+
+void zg_server::run_batch(ServeSlot *slot, const std::vector<RequestPtr> &batch) {
+  zg_batch &b = slot->b;
+  hipStream_t s = b.stream;
+  const uint32_t C = (uint32_t)idx->m.colors.size();
+  uint64_t n_reads = 0, bases = 0;
+  for (auto &r : batch) {
+    n_reads += r->n_reads;
+    bases += r->bases;
+  }
+  b.n_reads = n_reads;
+  b.bases = bases;
+  b.positions = bases + n_reads;
+  b.explicit_keys = false;
+  b.n_keys = 0;
+  if (n_reads == 0) {
+    for (auto &r : batch)
+      if (r->aggregate) memset(r->aggregate, 0, C * 8ULL);
+    return;
+  }
+  // offsets of the batch's concatenated reads (one separator position per read)
+  auto &roff = slot->h_roff, &poff = slot->h_poff;
+  roff.assign(n_reads + 1, 0);
+  poff.assign(n_reads + 1, 0);
+  uint64_t r_at = 0, b_at = 0;
+  for (auto &r : batch) {
+    const uint64_t r0 = r->n_reads ? r->read_off[0] : 0;
+    for (uint64_t i = 0; i < r->n_reads; i++) {
+      roff[r_at + i + 1] = b_at + (r->read_off[i + 1] - r0);
+      poff[r_at + i + 1] = roff[r_at + i + 1] + r_at + i + 1;
+    }
+    if (r->bases)
+      check_hip(hipMemcpyAsync(b.seq.as<uint8_t>() + b_at, r->seq + r0, r->bases,
+                               hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync");
+    r_at += r->n_reads;
+    b_at += r->bases;
+  }
+  check_hip(hipMemcpyAsync(b.read_off.p, roff.data(), (n_reads + 1) * 8, hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+  check_hip(hipMemcpyAsync(b.pos_off.p, poff.data(), (n_reads + 1) * 8, hipMemcpyHostToDevice, s),
+            "hipMemcpyAsync");
+  seed_batch(idx, &b);
+  configure(idx, &b);
+  check_hip(hipEventRecord(b.ev[0], s), "hipEventRecord");
+  enqueue_zero_outputs(idx, &b);
+  check_hip(hipEventRecord(b.ev[1], s), "hipEventRecord");
+  enqueue_lookups(idx, &b);
+  check_hip(hipEventRecord(b.ev[2], s), "hipEventRecord");
+  // per-request results straight into the callers' buffers
+  launch_fill_u64(slot->req_agg.as<uint64_t>(), (uint64_t)batch.size() * C, 0, s);
+  slot->h_totals.assign(n_reads + 1, 0);
+  if (n_reads)
+    check_hip(hipMemcpyAsync(slot->h_totals.data(), b.totals.p, n_reads * 8, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+  r_at = 0;
+  for (size_t j = 0; j < batch.size(); j++) {
+    const ServeRequest &r = *batch[j];
+    const uint32_t *cnt = b.counts.as<uint32_t>() + r_at * C;
+    if (r.counts && r.n_reads && C)
+      check_hip(hipMemcpyAsync(r.counts, cnt, r.n_reads * C * 4ULL, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+    if (r.aggregate && C) {
+      uint64_t *agg = slot->req_agg.as<uint64_t>() + j * C;
+      launch_colsum(cnt, r.n_reads, C, agg, s);
+      check_hip(hipMemcpyAsync(r.aggregate, agg, C * 8ULL, hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+    }
+    r_at += r.n_reads;
+  }
+  check_hip(hipEventRecord(b.ev[3], s), "hipEventRecord");
+  check_hip(hipStreamSynchronize(s), "serve batch");
+  b.last_lookup_ms = elapsed_ms(b.ev[1], b.ev[2]);
+  r_at = 0;
+  for (auto &r : batch) {
+    uint64_t seeds_r = 0;
+    for (uint64_t i = 0; i < r->n_reads; i++) seeds_r += slot->h_totals[r_at + i];
+    if (r->totals && r->n_reads) memcpy(r->totals, slot->h_totals.data() + r_at, r->n_reads * 8);
+    r->seeds = seeds_r;
+    r_at += r->n_reads;
+  }
+}
+
+void zg_server::worker(ServeSlot *slot) {
+  check_hip(hipSetDevice(idx->device), "hipSetDevice");
+  for (;;) {
+    std::vector<RequestPtr> batch;
+    uint64_t id = 0;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
+      if (pending.empty()) return;  // stopping, and every request answered
+      uint64_t n = 0, nb = 0;
+      while (!pending.empty()) {
+        const ServeRequest &r = *pending.front();
+        if (!batch.empty() && (n + r.n_reads > cfg.slot_reads || nb + r.bases > cfg.slot_bases ||
+                               batch.size() >= cfg.max_requests))
+          break;
+        n += r.n_reads;
+        nb += r.bases;
+        batch.push_back(pending.front());
+        pending.pop_front();
+      }
+      id = next_batch++;
+    }
+    const Clock::time_point t_admit = Clock::now();
+    for (auto &r : batch) {
+      r->t_admit = t_admit;
+      r->batch_id = id;
+      r->batch_requests = (uint32_t)batch.size();
+    }
+    int status = 0;
+    std::string error;
+    try {
+      run_batch(slot, batch);
+    } catch (...) {
+      status = translate_exception();
+      error = zg_last_error();
+    }
+    const Clock::time_point t_done = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto &r : batch) {
+        r->t_done = t_done;
+        r->status = status;
+        r->error = error;
+        if (!status) {
+          done_requests++;
+          reads += r->n_reads;
+          seeds += r->seeds;
+        }
+      }
+      batches++;
+      max_batch_requests = std::max(max_batch_requests, (uint32_t)batch.size());
+      if (!status && slot->b.last_lookup_ms > 0) lookup_ms += slot->b.last_lookup_ms;
+    }
+    cv_done.notify_all();
+  }
+}
+
+extern "C" void zg_server_config_default(zg_server_config *c) {
+  if (!c) return;
+  memset(c, 0, sizeof *c);
+  c->slots = 2;
+  c->max_requests = 256;
+  c->slot_reads = 32768;
+  c->slot_bases = 64ULL << 20;
+}
+
+extern "C" int zg_server_create(zg_index *idx, const zg_server_config *cfg, zg_server **out) {
+  try {
+    ensure(idx && out, "null argument", ZG_ERR_INVALID);
+    ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
+           ZG_ERR_INVALID);
+    zg_server_config c;
+    zg_server_config_default(&c);
+    if (cfg) c = *cfg;
+    ensure(c.slots >= 1 && c.slots <= 16, "server slots must be in 1..=16", ZG_ERR_INVALID);
+    ensure(c.max_requests >= 1, "max_requests must be positive", ZG_ERR_INVALID);
+    ensure(c.slot_reads >= 1 && c.slot_bases >= 1, "slot capacity must be positive",
+           ZG_ERR_INVALID);
+    ensure(c.slot_reads * (uint64_t)std::max<size_t>(1, idx->m.colors.size()) < (1ULL << 40),
+           "slot_reads x colours is too large", ZG_ERR_INVALID);
+    ensure(!(c.flags & ~(uint32_t)ZG_QUERY_NO_RECOVERY), "unknown server flags", ZG_ERR_INVALID);
+    auto srv = std::make_unique<zg_server>();
+    srv->idx = idx;
+    srv->cfg = c;
+    const uint64_t per_slot = slot_bytes(idx, c);
+    srv->device_bytes = resident_bytes(idx) + per_slot * c.slots;
+    if (c.hbm_budget && srv->device_bytes > c.hbm_budget)
+      fail(ZG_ERR_BUDGET, "the resident index (" + std::to_string(resident_bytes(idx)) + " B) and " +
+                              std::to_string(c.slots) + " slots of " + std::to_string(per_slot) +
+                              " B exceed the HBM budget of " + std::to_string(c.hbm_budget) + " B");
+    check_hip(hipSetDevice(idx->device), "hipSetDevice");
+    for (uint32_t i = 0; i < c.slots; i++) {
+      srv->slots.push_back(std::make_unique<ServeSlot>());
+      init_slot(idx, c, *srv->slots.back());
+    }
+    for (auto &sl : srv->slots) {
+      ServeSlot *p = sl.get();
+      zg_server *sp = srv.get();
+      sl->thread = std::thread([sp, p] {
+        try {
+          sp->worker(p);
+        } catch (...) {
+          // hipSetDevice failed: nothing can run on this slot; the others serve
+        }
+      });
+    }
+    *out = srv.release();
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_server_submit(zg_server *srv, const uint8_t *seq, const uint64_t *read_off,
+                                uint64_t n_reads, uint64_t *totals, uint32_t *counts,
+                                uint64_t *aggregate, uint64_t *ticket) {
+  try {
+    ensure(srv && ticket && (n_reads == 0 || read_off), "null argument", ZG_ERR_INVALID);
+    auto r = std::make_shared<ServeRequest>();
+    r->seq = seq;
+    r->read_off = read_off;
+    r->n_reads = n_reads;
+    for (uint64_t i = 0; i < n_reads; i++)
+      ensure(read_off[i + 1] >= read_off[i], "read offsets must be non-decreasing", ZG_ERR_INVALID);
+    r->bases = n_reads ? read_off[n_reads] - read_off[0] : 0;
+    ensure(r->bases == 0 || seq, "null sequence", ZG_ERR_INVALID);
+    if (r->n_reads > srv->cfg.slot_reads || r->bases > srv->cfg.slot_bases)
+      fail(ZG_ERR_BUDGET, "request of " + std::to_string(r->n_reads) + " reads and " +
+                              std::to_string(r->bases) + " bases exceeds one server slot (" +
+                              std::to_string(srv->cfg.slot_reads) + " reads, " +
+                              std::to_string(srv->cfg.slot_bases) + " bases)");
+    r->totals = totals;
+    r->counts = counts;
+    r->aggregate = aggregate;
+    r->t_submit = Clock::now();
+    {
+      std::lock_guard<std::mutex> lk(srv->mu);
+      ensure(!srv->stopping, "server is shutting down", ZG_ERR_INVALID);
+      r->ticket = srv->next_ticket++;
+      srv->live[r->ticket] = r;
+      srv->pending.push_back(r);
+    }
+    srv->cv_work.notify_one();
+    *ticket = r->ticket;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+static void fill_stats(const ServeRequest &r, zg_request_stats *st) {
+  if (!st) return;
+  memset(st, 0, sizeof *st);
+  st->ticket = r.ticket;
+  st->batch_id = r.batch_id;
+  st->batch_requests = r.batch_requests;
+  st->status = r.status;
+  st->n_reads = r.n_reads;
+  st->seeds = r.seeds;
+  st->queued_ms = ms_between(r.t_submit, r.t_admit);
+  st->service_ms = ms_between(r.t_admit, r.t_done);
+  st->latency_ms = ms_between(r.t_submit, r.t_done);
+}
+
+static int finish_request(zg_server *srv, std::unordered_map<uint64_t, RequestPtr>::iterator it,
+                          zg_request_stats *st) {
+  RequestPtr r = it->second;
+  srv->live.erase(it);
+  fill_stats(*r, st);
+  if (r->status) set_error(r->error);
+  return r->status;
+}
+
+extern "C" int zg_server_wait(zg_server *srv, uint64_t ticket, zg_request_stats *st) {
+  try {
+    ensure(srv, "null argument", ZG_ERR_INVALID);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->live.find(ticket);
+    ensure(it != srv->live.end(), "unknown or already released ticket", ZG_ERR_INVALID);
+    RequestPtr r = it->second;
+    srv->cv_done.wait(lk, [&] { return r->status != 1; });
+    return finish_request(srv, srv->live.find(ticket), st);
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_server_poll(zg_server *srv, uint64_t ticket, zg_request_stats *st) {
+  try {
+    ensure(srv, "null argument", ZG_ERR_INVALID);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->live.find(ticket);
+    ensure(it != srv->live.end(), "unknown or already released ticket", ZG_ERR_INVALID);
+    if (it->second->status == 1) return 0;
+    int rc = finish_request(srv, it, st);
+    return rc ? rc : 1;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_server_get_stats(zg_server *srv, zg_server_stats *o) {
+  try {
+    ensure(srv && o, "null argument", ZG_ERR_INVALID);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    memset(o, 0, sizeof *o);
+    o->requests = srv->done_requests;
+    o->batches = srv->batches;
+    o->reads = srv->reads;
+    o->seeds = srv->seeds;
+    o->max_batch_requests = srv->max_batch_requests;
+    o->slots = srv->cfg.slots;
+    o->lookup_ms = srv->lookup_ms;
+    o->device_bytes = srv->device_bytes;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_server_destroy(zg_server *srv) {
+  if (!srv) return 0;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stopping = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto &sl : srv->slots)
+    if (sl->thread.joinable()) sl->thread.join();
+  delete srv;
+  return 0;
+}
+
+extern "C" int zg_host_alloc(uint64_t bytes, void **out) {
+  try {
+    ensure(out, "null argument", ZG_ERR_INVALID);
+    *out = nullptr;
+    check_hip(hipHostMalloc(out, bytes ? bytes : 8, hipHostMallocDefault), "hipHostMalloc");
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
+  return 0;
+}
