@@ -121,7 +121,7 @@ void upload_index(zg_index *idx, uint64_t budget) {
   const uint64_t pad = 4096;
   uint64_t need = 0;
   for (size_t i = 0; i < m.banks.size(); i++) need += idx->payload[i].size() + pad;
-  if (idx->recover && idx->have_rec) need += idx->rec.size() * 24 + idx->rec.origins.size() * 4;
+  if (idx->recover && idx->have_rec) need += idx->rec.size() * 28 + idx->rec.origins.size() * 8;
   if (budget && need > budget)
     fail(ZG_ERR_BUDGET, "index needs " + std::to_string(need) + " bytes of HBM, budget is " +
                             std::to_string(budget));
@@ -192,6 +192,22 @@ void upload_index(zg_index *idx, uint64_t budget) {
     v.color_col = db.color_col.as<uint32_t>();
     v.color_bit = db.color_bit.as<uint32_t>();
   }
+  // Column map ids: banks whose colour -> column maps are equal (the
+  // partitions of one size class) share an id.
+  std::vector<std::vector<uint32_t>> maps;
+  {
+    std::vector<std::vector<uint32_t>> cc(m.banks.size());
+    for (size_t bi = 0; bi < m.banks.size(); bi++) {
+      cc[bi].assign(n_colors, kNone);
+      const auto &groups = m.banks[bi].groups;
+      for (uint32_t g = 0; g < groups.size(); g++)
+        for (uint32_t x : groups[g].members) cc[bi][x] = g;
+      size_t id = 0;
+      while (id < maps.size() && maps[id] != cc[bi]) id++;
+      if (id == maps.size()) maps.push_back(cc[bi]);
+      idx->dbanks[bi].view.colmap = (uint32_t)id;
+    }
+  }
   // recovery directory (the query-side OriginMap, abandoned.rs:29-206)
   idx->rec_view = RecDev{};
   if (idx->recover && idx->have_rec && idx->rec.size()) {
@@ -202,21 +218,37 @@ void upload_index(zg_index *idx, uint64_t budget) {
     uint32_t bits = 0;
     while ((1ULL << bits) < cap) bits++;
     ensure(bits <= 32, "recovery directory too large", ZG_ERR_UNSUPPORTED);
+    ensure(maps.size() <= 255, "more than 255 distinct bank column maps", ZG_ERR_UNSUPPORTED);
+    // each colour's (map id, column); a colour sits in one size class
+    std::vector<uint32_t> color_tag(n_colors, kNone);
+    for (size_t id = 0; id < maps.size(); id++)
+      for (uint32_t c = 0; c < n_colors; c++)
+        if (maps[id][c] != kNone) {
+          ensure(maps[id][c] < (1u << 24), "bank with 2^24 or more columns", ZG_ERR_UNSUPPORTED);
+          color_tag[c] = ((uint32_t)id << 24) | maps[id][c];
+        }
     std::vector<uint64_t> dir(cap, (uint64_t)kNone);
-    std::vector<uint32_t> post;
+    std::vector<uint32_t> post, post_col;
     post.reserve(n + r.origins.size());
+    post_col.reserve(n + r.origins.size());
     for (uint64_t i = 0; i < n; i++) {
       uint32_t pi = (uint32_t)post.size();
       post.push_back((uint32_t)(r.off[i + 1] - r.off[i]));
-      for (uint64_t j = r.off[i]; j < r.off[i + 1]; j++) post.push_back(r.origins[j]);
+      post_col.push_back(kNone);
+      for (uint64_t j = r.off[i]; j < r.off[i + 1]; j++) {
+        post.push_back(r.origins[j]);
+        post_col.push_back(r.origins[j] < n_colors ? color_tag[r.origins[j]] : kNone);
+      }
       uint64_t slot = r.hashes[i] >> (32 - bits);
       while ((uint32_t)dir[slot] != kNone) slot = (slot + 1) & (cap - 1);
       dir[slot] = ((uint64_t)r.hashes[i] << 32) | pi;
     }
     upload(idx->rec_dir, dir, s);
     upload(idx->rec_post, post, s);
+    upload(idx->rec_post_col, post_col, s);
     idx->rec_view.dir = idx->rec_dir.as<uint64_t>();
     idx->rec_view.post = idx->rec_post.as<uint32_t>();
+    idx->rec_view.post_col = idx->rec_post_col.as<uint32_t>();
     idx->rec_view.cap_log2 = bits;
     idx->rec_view.enabled = 1;
   }
@@ -229,7 +261,8 @@ void upload_index(zg_index *idx, uint64_t budget) {
 }
 
 uint64_t resident_bytes(const zg_index *idx) {
-  uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->inner_table.bytes;
+  uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->rec_post_col.bytes +
+               idx->inner_table.bytes;
   for (auto &d : idx->dbanks)
     n += d.data.bytes + d.col_color.bytes + d.mg_off.bytes + d.mg_members.bytes +
          d.mg_mask_bit.bytes + d.color_col.bytes + d.color_bit.bytes;
@@ -244,6 +277,7 @@ zg_index::~zg_index() {
   dbanks.clear();
   rec_dir.reset();
   rec_post.reset();
+  rec_post_col.reset();
   inner_table.reset();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -72,7 +72,7 @@ struct zg_index {
   uint32_t cu_count = 0;
   hipStream_t stream = nullptr;
   std::vector<zg::DeviceBank> dbanks;
-  zg::DevBuf rec_dir, rec_post, inner_table;
+  zg::DevBuf rec_dir, rec_post, rec_post_col, inner_table;
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();

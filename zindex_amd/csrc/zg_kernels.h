@@ -28,14 +28,18 @@ struct BankDev {
   const uint64_t *mg_mask_bit;// [G - first_merged] bit offset of the mask field
   const uint32_t *color_col;  // [n_colors] column of a colour in this bank or kNone
   const uint32_t *color_bit;  // [n_colors] member bit inside its merged group
+  uint32_t colmap;            // id shared by banks with the same colour -> column map
 };
 
 // Recovery directory: open addressing on the high hash bits, 64-bit entries
 // (hash << 32 | posting index), empty = low word kNone. Postings hold
-// [count, origin...] with global colour ids.
+// [count, origin...] with global colour ids; post_col is parallel to post and
+// holds each origin's (column map id << 24 | column), kNone where the origin
+// is in no bank.
 struct RecDev {
   const uint64_t *dir;
   const uint32_t *post;
+  const uint32_t *post_col;
   uint32_t cap_log2;          // directory capacity 2^cap_log2
   uint32_t enabled;
 };

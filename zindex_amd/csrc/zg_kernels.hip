@@ -520,6 +520,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   uint64_t held_keys = 0;
 
   uint32_t col0[T], fmask[T], mmask[T];
+  // Recovered singleton colours of the current key, as field bits per lane
+  // and tile (one wave's slice; written and read by that wave only).
+  __shared__ uint32_t rec_bits_all[4 * T * 64];
+  uint32_t *rec_bits = rec_bits_all + (threadIdx.x >> 6) * T * 64;
 
   auto flush_aggregate = [&]() {
 #pragma unroll
@@ -656,12 +660,17 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
 #pragma unroll
             for (int w = 0; w < NW; w++) x[d][t][w] = 0;
         uint64_t first[D], step[D], base[D];
+        uint32_t rl[D], rs[D], pre[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
           int l = jj[d] < 0 ? 0 : jj[d];
           first[d] = readlane64(lane_first, l);
           step[d] = readlane64(lane_step, l);
           base[d] = readlane64(lane_base, l);
+          // the first 64 recovery postings of the key travel with its rows
+          rl[d] = jj[d] < 0 ? 0 : readlane32(rec_len, l);
+          rs[d] = readlane32(rec_start, l);
+          pre[d] = lane < rl[d] ? a.rec.post_col[rs[d] + lane] : kNone;
         }
         for (uint32_t h0 = 0; h0 < H; h0 += HB) {
           uint32_t raw[D][HB][T][NW];
@@ -711,27 +720,36 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
 #pragma unroll
           for (int t = 0; t < T; t++) m[t] = field_hits<B>(x[d][t], fpd) & fmask[t];
           // Recovery: OR recovered colours into this key's hits
-          // (storage.rs:2771-2787, 2916-2924).
-          const uint32_t rl = readlane32(rec_len, jj[d]);
-          if (rl) {
-            const uint32_t rs = readlane32(rec_start, jj[d]);
-            for (uint32_t q = 0; q < rl; q++) {
-              uint32_t o = a.rec.post[rs + q];
-              uint32_t col = bk.color_col[o];
-              if (col == kNone) continue;
-              uint32_t tile = col / tile_fields;
-              if (tile < tg * T || tile >= tg * T + T) continue;
-              uint32_t within = col % tile_fields;
-              uint32_t ol = within / F, of = within % F;
-              int tt = (int)(tile - tg * T);
+          // (storage.rs:2771-2787, 2916-2924). The key's postings are spread
+          // over the lanes; singleton columns set their field bit through the
+          // wave's LDS slice, merged members are settled by the posting's lane.
+          if (rl[d]) {
 #pragma unroll
-              for (int t = 0; t < T; t++) {
-                if (t != tt || lane != ol) continue;
-                if (col < bk.first_merged) {
-                  m[t] |= 1u << of;
-                } else {
+            for (int t = 0; t < T; t++) rec_bits[t * 64 + lane] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t q0 = 0; q0 < rl[d]; q0 += 64) {
+              const uint32_t q = q0 + lane;
+              const uint32_t e =
+                  q0 == 0 ? pre[d] : (q < rl[d] ? a.rec.post_col[rs[d] + q] : kNone);
+              const uint32_t col = e & 0xFFFFFFu;
+              const uint32_t rel = col / tile_fields - tg * T;  // tile within the group
+              const bool mine = e != kNone && (e >> 24) == bk.colmap && rel < (uint32_t)T;
+              const uint32_t within = col % tile_fields;
+              if (mine && col < bk.first_merged)
+                atomicOr(&rec_bits[rel * 64 + within / F], 1u << (within % F));
+              if (__ballot(mine && col >= bk.first_merged)) {
+                // the group's fingerprint hit, from the lane that owns the field
+                uint32_t mo = 0;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                  uint32_t v = __shfl(m[t], (int)(within / F));
+                  if (rel == (uint32_t)t) mo = v;
+                }
+                if (mine && col >= bk.first_merged) {
+                  const uint32_t o = a.rec.post[rs[d] + q];
                   bool hit = false;
-                  if ((m[t] >> of) & 1u) {
+                  if ((mo >> (within % F)) & 1u) {
                     uint64_t bit = bk.mg_mask_bit[col - bk.first_merged] + bk.color_bit[o];
                     hit = xor_rows_bits(bk, kk[d], a.seed, bit, 1) & 1;
                   }
@@ -739,6 +757,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
                 }
               }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int t = 0; t < T; t++) m[t] |= rec_bits[t * 64 + lane];
           }
           // Multi-member groups: decode the XORed membership mask.
 #pragma unroll
