@@ -159,7 +159,12 @@ typedef struct zg_build_config {
                                   consecutive datasets per group */
   uint32_t threads;            /* 0 = all host cores */
   int32_t drop_abandoned;      /* 1 = no recovery map */
-  uint32_t reserved;
+  uint32_t device;             /* 1 = peel on the GPU: parallel peeling rounds,
+                                  back-substitution and verification as HIP
+                                  kernels; a 2-core left by peeling is broken
+                                  on the host by the same rule as the CPU build
+                                  (the core is unique, so the abandoned keys
+                                  are the same); 0 = CPU */
 } zg_build_config;
 
 /* Fills cfg with the reference CLI defaults (src/bin/zorindex.rs:55-159). */

@@ -291,33 +291,62 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
   std::chrono::steady_clock::time_point tp[8];
   auto T = [&](int i) { if (trace) tp[i] = std::chrono::steady_clock::now(); };
   T(0);
-  // 1. the H cells of every equation (PositionHash::positions)
-  std::vector<uint32_t> slots((size_t)n * H);
-  for (size_t e = 0; e < n; e++) {
-    uint64_t first = hashed[e], step = splitmix64(first);
-    uint64_t base = slot_base(first, geo), h = first;
-    for (uint32_t i = 0; i < H; i++) {
-      slots[e * H + i] = (uint32_t)slot_at(base, h, i, geo);
-      h += step;
-    }
-  }
-  T(1);
-  // 2. degrees and XOR of incident equation ids
-  std::vector<uint32_t> count(S, 0), xr(S, 0);
-  for (size_t e = 0; e < n; e++)
-    for (uint32_t i = 0; i < H; i++) {
-      uint32_t s = slots[e * H + i];
-      count[s]++;
-      xr[s] ^= (uint32_t)e;
-    }
-  T(2);
-  std::vector<uint8_t> state(n, 0);  // 0 active, 1 peeled, 2 abandoned
+  std::vector<uint32_t> slots, count, xr;
+  std::vector<uint8_t> state;  // 0 active, 1 peeled, 2 abandoned
   std::vector<std::pair<uint32_t, uint32_t>> order;
-  order.reserve(n);
+  size_t remaining = n;
+  std::vector<uint64_t> cells;
+  bool solved = false;  // cells final and verified (device peel without a core)
+  if (cfg.device) {
+    // parallel peeling rounds on the GPU; a 2-core comes back for the rule below
+    DevicePeel dp;
+    device_peel(hashed, values, S, H, geo, dp);
+    if (trace) fprintf(stderr, "  device peel: %u rounds, %llu equations left\n", dp.rounds,
+                       (unsigned long long)dp.remaining);
+    if (dp.remaining == 0) {
+      cells = std::move(dp.cells);
+      solved = true;
+      remaining = 0;
+    } else {
+      slots = std::move(dp.slots);
+      count = std::move(dp.count);
+      xr = std::move(dp.xr);
+      state = std::move(dp.state);
+      remaining = dp.remaining;
+      order.reserve(n);
+      for (size_t i = 0; i < dp.order_e.size(); i++) order.push_back({dp.order_e[i], dp.order_p[i]});
+    }
+    T(1);
+  } else {
+    // 1. the H cells of every equation (PositionHash::positions)
+    slots.resize((size_t)n * H);
+    for (size_t e = 0; e < n; e++) {
+      uint64_t first = hashed[e], step = splitmix64(first);
+      uint64_t base = slot_base(first, geo), h = first;
+      for (uint32_t i = 0; i < H; i++) {
+        slots[e * H + i] = (uint32_t)slot_at(base, h, i, geo);
+        h += step;
+      }
+    }
+    T(1);
+    // 2. degrees and XOR of incident equation ids
+    count.assign(S, 0);
+    xr.assign(S, 0);
+    for (size_t e = 0; e < n; e++)
+      for (uint32_t i = 0; i < H; i++) {
+        uint32_t s = slots[e * H + i];
+        count[s]++;
+        xr[s] ^= (uint32_t)e;
+      }
+    state.assign(n, 0);
+    order.reserve(n);
+  }
+  T(2);
   std::vector<uint32_t> queue;
-  for (uint64_t s = 0; s < S; s++)
-    if (count[s] == 1) queue.push_back((uint32_t)s);
-  size_t remaining = n, qi = 0;
+  if (!solved)
+    for (uint64_t s = 0; s < S; s++)
+      if (count[s] == 1) queue.push_back((uint32_t)s);
+  size_t qi = 0;
   std::vector<uint32_t> abandoned;
   // core phase state
   bool core = false;
@@ -411,20 +440,22 @@ static void construct_group(const zg_build_config &cfg, const std::vector<uint64
   T(3);
   std::vector<uint32_t>().swap(count);
   std::vector<uint32_t>().swap(xr);
-  // 3. back-substitution in reverse peel order (graph.rs:187-199)
-  std::vector<uint64_t> cells(S, 0);
-  for (size_t i = order.size(); i-- > 0;) {
-    uint32_t e = order[i].first, pivot = order[i].second;
-    uint64_t v = values[e];
-    for (uint32_t h = 0; h < H; h++) {
-      uint32_t s = slots[(size_t)e * H + h];
-      if (s != pivot) v ^= cells[s];
+  if (!solved) {
+    // 3. back-substitution in reverse peel order (graph.rs:187-199)
+    cells.assign(S, 0);
+    for (size_t i = order.size(); i-- > 0;) {
+      uint32_t e = order[i].first, pivot = order[i].second;
+      uint64_t v = values[e];
+      for (uint32_t h = 0; h < H; h++) {
+        uint32_t s = slots[(size_t)e * H + h];
+        if (s != pivot) v ^= cells[s];
+      }
+      cells[pivot] = v;
     }
-    cells[pivot] = v;
   }
   T(4);
-  // 4. verification (graph.rs:201-214)
-  for (size_t e = 0; e < n; e++) {
+  // 4. verification (graph.rs:201-214); the device verified a solved group
+  for (size_t e = 0; !solved && e < n; e++) {
     if (state[e] == 2) continue;
     uint64_t v = 0;
     for (uint32_t h = 0; h < H; h++) v ^= cells[slots[e * H + h]];
@@ -608,6 +639,7 @@ static void validate_config(const zg_build_config &c) {
   }
   ensure(c.merge_group_size >= 1 && c.merge_group_size <= 64 - b, "invalid merge group size",
          -1);
+  ensure(c.device <= 1, "device must be 0 (CPU peel) or 1 (GPU peel)", -1);
 }
 
 extern "C" int zg_builder_create(const zg_build_config *cfg, zg_builder **out) {

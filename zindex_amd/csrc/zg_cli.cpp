@@ -13,7 +13,7 @@ static int usage() {
           "usage: zorindex <build|query|info> [options]\n"
           "  build -f FOF -o OUT [-k K] [-m M] [-H HASHES] [-b BITS] [-s SEED]\n"
           "        [-i kmers|minimizers] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
-          "        [--merge-group-size N] [--drop-abandoned] [-t THREADS]\n"
+          "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
           "        [-M subindex|all] [--no-recovery]\n"
           "  info -i INDEX\n");
@@ -67,6 +67,7 @@ int main(int argc, char **argv) {
       else if (o == "-t" || o == "--threads") cfg.threads = (uint32_t)atoi(value(i).c_str());
       else if (o == "--merge-group-size") cfg.merge_group_size = (uint32_t)atoi(value(i).c_str());
       else if (o == "--drop-abandoned") cfg.drop_abandoned = 1;
+      else if (o == "--gpu-peel") cfg.device = 1;
       else if (o == "-i" || o == "--index-mode") {
         std::string v = value(i);
         if (v == "kmers") cfg.index_mode = 0;

@@ -125,4 +125,18 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b);
 int translate_exception();
 // Host-side helpers shared with the builder.
 std::vector<uint64_t> ocm_inner_table(uint32_t s, uint64_t seed);
+
+// Result of peeling one group's equations on the GPU (zg_peel.hip). With
+// remaining == 0 the group is solved and `cells` holds its verified cell
+// values; otherwise the 2-core is left and slots/count/xr/state/order carry
+// the device state (state: 0 active, 1 peeled; order: peel order).
+struct DevicePeel {
+  uint64_t remaining = 0;
+  uint32_t rounds = 0;
+  std::vector<uint64_t> cells;
+  std::vector<uint32_t> slots, count, xr, order_e, order_p;
+  std::vector<uint8_t> state;
+};
+void device_peel(const std::vector<uint64_t> &hashed, const std::vector<uint64_t> &values,
+                 uint64_t S, uint32_t H, const SegmentGeometry &geo, DevicePeel &out);
 }  // namespace zg
