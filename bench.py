@@ -69,7 +69,7 @@ def parse_args():
                         "known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     p.add_argument("--mode", choices=("kmers", "minimizers"), default="kmers")
     p.add_argument("--m", type=int, default=31, help="minimizer size in minimizer mode")
-    p.add_argument("--preset", choices=("batched", "minimizers", "serving", "small"),
+    p.add_argument("--preset", choices=("batched", "minimizers", "serving", "sweep", "small"),
                    default="batched",
                    help="batched: 1024 colours x 5 Mbp, 1M lognormal reads (default); "
                         "minimizers: 4096 colours x 2 Mbp, OCM k=49 m=31, b=16, H=3, "
@@ -83,6 +83,9 @@ def parse_args():
                    help="serving: Poisson arrival rates (requests/s), comma-separated; the "
                         "first is the headline")
     p.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
+    p.add_argument("--gpu-peel", type=int, default=0, help="build the index with the GPU peel")
+    p.add_argument("--sweep", default="1:2,1:3,1:8,4:2,4:3,4:8,16:2,16:3,16:8,32:2,32:3,32:8",
+                   help="sweep preset: bits:hashes configurations")
     args = p.parse_args()
     given = {a.lstrip("-").split("=")[0].replace("-", "_") for a in sys.argv[1:] if a.startswith("--")}
     preset = {
@@ -90,6 +93,7 @@ def parse_args():
         "minimizers": dict(colors=4096, color_len=2_000_000, reads=200_000, median=10_000,
                            sigma=0.0, min_len=10_000, max_len=10_000, error=0.05, bits=16,
                            hashes=3, k=49, m=31, mode="minimizers", tune_reads=5000),
+        "sweep": dict(reads=200_000, steps=2, tune_reads=20000),
         "serving": dict(colors=8192, color_len=1_000_000, median=1000, sigma=1.0, min_len=100,
                         max_len=100_000, error=0.02, bits=8, hashes=4, k=31, tune_reads=20000),
         "small": dict(colors=10, color_len=2_000_000, reads=64, median=1000, sigma=0.0,
@@ -118,13 +122,10 @@ def calibration_copy(mib: int) -> None:
     log(f"calibration copy: {mib} MiB device-to-device")
 
 
-def generate(args, zindex_amd, synth):
-    rng = np.random.default_rng(args.seed)
+def make_leaves(args, synth, rng):
+    """The colours: a seeded substitution tree over a random root."""
     root = synth.random_genome(rng, args.color_len)
-    builder = zindex_amd.Builder(k=args.k, fingerprint_bits=args.bits, hashes=args.hashes,
-                                 index_mode=args.mode, minimizer_size=args.m)
     leaves = [None] * args.colors
-    t0 = time.time()
     if args.colors & (args.colors - 1) == 0 and args.colors > 1:
         gen = synth.substitution_tree(rng, root, args.colors)
     else:
@@ -132,18 +133,117 @@ def generate(args, zindex_amd, synth):
                for i in range(args.colors))
     for leaf, seq in gen:
         leaves[leaf] = seq
+    return leaves
+
+
+def build_index(args, zindex_amd, leaves, bits, hashes, device=0):
+    t0 = time.time()
+    builder = zindex_amd.Builder(k=args.k, fingerprint_bits=bits, hashes=hashes,
+                                 index_mode=args.mode, minimizer_size=args.m, device=device)
     for leaf, seq in enumerate(leaves):
         builder.add_color(f"leaf{leaf:05d}", [seq.tobytes()])
         if leaf % 128 == 127:
             log(f"colours queued {leaf + 1}/{args.colors} ({time.time() - t0:.1f}s)")
     idx = builder.finish()
-    log(f"index built in {time.time() - t0:.1f}s")
+    log(f"index built in {time.time() - t0:.1f}s" + (" (GPU peel)" if device else ""))
+    return idx
+
+
+def generate(args, zindex_amd, synth):
+    rng = np.random.default_rng(args.seed)
+    leaves = make_leaves(args, synth, rng)
+    idx = build_index(args, zindex_amd, leaves, args.bits, args.hashes, args.gpu_peel)
     lengths = synth.lognormal_lengths(rng, args.reads, args.median, args.sigma, args.min_len,
                                       args.max_len)
     t1 = time.time()
     buf, off, _ = synth.sample_reads_fast(rng, leaves, args.reads, lengths, args.error)
     log(f"{args.reads} reads, {int(off[-1])} bases sampled in {time.time() - t1:.1f}s")
     return idx, buf, off
+
+
+def sweep_main(args, zindex_amd, synth) -> int:
+    """Fingerprint width x hash count on the batched-query colour set: each
+    index is built with the GPU peel; per configuration the query throughput
+    and roofline, false negatives on stored k-mers (recovery on) and the
+    false-positive rate on absent k-mers (recovery off) against 2^-b."""
+    rng = np.random.default_rng(args.seed)
+    leaves = make_leaves(args, synth, rng)
+    lengths = synth.lognormal_lengths(rng, args.reads, args.median, args.sigma, args.min_len,
+                                      args.max_len)
+    buf, off, _ = synth.sample_reads_fast(rng, leaves, args.reads, lengths, args.error)
+    log(f"{args.colors} colours, {args.reads} reads ({int(off[-1])} bases)")
+    # stored k-mers: error-free 2 kbp pieces of 64 colours, one piece each
+    picks = np.linspace(0, args.colors - 1, 64).astype(int)
+    stored = [leaves[c][1000:3000].tobytes() for c in picks]
+    # absent k-mers: random sequence from a stream no colour was drawn from
+    arng = np.random.default_rng(args.seed + 7919)
+    absent = [synth.random_genome(arng, 2000).tobytes() for _ in range(250)]
+    rows = []
+    for spec in args.sweep.split(","):
+        bits, hashes = (int(x) for x in spec.split(":"))
+        t0 = time.time()
+        idx = build_index(args, zindex_amd, leaves, bits, hashes, device=1)
+        build_s = time.time() - t0
+        meta = idx.meta
+        idx.load()
+        tuning = idx.tuning
+        if not args.no_autotune:
+            n_t = min(args.tune_reads, args.reads)
+            with idx.batch(buf[: int(off[n_t])], off[: n_t + 1], records=False) as tb:
+                tuning = tb.autotune()
+        with idx.batch(buf, off, records=False) as b:
+            b.run()
+            lk = []
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                b.run()
+                lk.append(b.stats()["last_lookup_ms"])
+            wall = (time.perf_counter() - t1) / args.steps
+            st = b.stats()
+        algo = st["algorithmic_bytes"] + (int(off[-1]) + args.reads) * 3 // 8
+        gbs = algo / (np.mean(lk) / 1000) / 1e9
+        # false negatives: every stored k-mer hits its own colour
+        totals, counts, _ = idx.query_reads(stored, records=True)
+        fn = int(sum(int(totals[i]) - int(counts[i, c]) for i, c in enumerate(picks)))
+        # false positives: absent k-mers, filter only
+        t_abs, _, agg = idx.query_reads(absent, records=False, recover=False)
+        n_abs = int(t_abs.sum())
+        p = 2.0 ** -bits
+        rates = agg / max(1, n_abs)
+        bound = p + 8 * np.sqrt(p * (1 - p) / max(1, n_abs))
+        row = {"bits": bits, "hashes": hashes, "build_s": round(build_s, 1),
+               "bank_bytes": meta["payload_bytes"], "abandoned_origins": meta["abandoned_origins"],
+               "kmers_per_s": st["seeds"] / wall, "lookup_ms": round(float(np.mean(lk)), 3),
+               "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "achieved_gbs": round(gbs, 1),
+               "bytes_per_kmer": st["algorithmic_bytes"] // max(1, st["seeds"]),
+               "tuning": tuning, "false_negatives": fn,
+               "stored_kmers": int(totals.sum()), "absent_kmers": n_abs,
+               "fpr_mean": float(rates.mean()), "fpr_max": float(rates.max()),
+               "fpr_bound": bound, "fpr_within_bound": bool(np.all(rates <= bound))}
+        log(f"b={bits} H={hashes}: {row}")
+        rows.append(row)
+        idx.close()
+        del idx
+    out = {
+        "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
+        "value": rows[0]["kmers_per_s"], "unit": "k-mers/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": rows[0]["lookup_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u%d" % rows[0]["bits"],
+        "data": "synthetic (seeded substitution tree colours and sampled reads)",
+        "config": {"workload": f"fingerprint width x hash count sweep: {args.colors} colours x "
+                               f"{args.color_len} bp substitution tree, k={args.k}; "
+                               f"{args.reads} reads lognormal(median {args.median:.0f}, sigma "
+                               f"{args.sigma}), {args.error:.0%} error; GPU-peeled indexes; "
+                               f"aggregate counts", "sweep": args.sweep},
+        "sweep": rows,
+        "roofline": {"bound": "hbm", "achieved": rows[0]["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": rows[0]["roofline_frac"], "traffic": None,
+                     "kernel": "lookup_kernel"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
 
 
 def workload_name(args, records: bool) -> str:
@@ -385,6 +485,8 @@ def main() -> int:
     from zindex_amd import synth
     if args.preset == "serving":
         return serve_main(args, zindex_amd, synth)
+    if args.preset == "sweep":
+        return sweep_main(args, zindex_amd, synth)
 
     records = not args.aggregate
     if args.calibrate_copy_mib:

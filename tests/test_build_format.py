@@ -80,6 +80,58 @@ def test_build_round_trip_no_false_negatives(tmp_path, small_colors, bits, hashe
             assert int(scores[c]) == before + 1, (c, int(key))
 
 
+@pytest.mark.parametrize("group,cap", [(1, 1 << 15), (2, 1 << 17), (1, 1 << 12)])
+def test_hash_partitioned_build(tmp_path, small_colors, group, cap):
+    """partition_large_subindexes (storage.rs:1525-1641): a size class over
+    --max-subindex-size is rebuilt as hash partitions of 1/P of the slots; every
+    key is stored in the partition partition_id picks (storage.rs:99-101)."""
+    idx = _build(small_colors, fingerprint_bits=8, hashes=4, merge_group_size=group,
+                 max_subindex_size=cap)
+    man = idx.manifest()
+    assert man["version"] == 9
+    routed = [b for b in man["banks"] if b.get("routing")]
+    source = routed[0]["routing"]["source_slots"]
+    local, rb = source, (routed[0]["row_bits"] + 7) // 8
+    while local * rb > cap:
+        local //= 2
+    parts = source // local
+    assert parts > 1 and len(routed) == parts
+    assert all(b["routing"]["partitions"] == parts for b in routed)
+    for p, b in enumerate(routed):
+        r = b["routing"]
+        assert r["partition"] == p and b["slots"] * parts == r["source_slots"]
+        assert b["file"] == f"bank-{r['source_slots'].bit_length() - 1}-part-{p}-of-{parts}.zorbank.zst"
+        assert b["slots"] * ((b["row_bits"] + 7) // 8) <= cap
+    # the partitions hold each group's keys exactly once between them
+    for g in range(len(routed[0]["groups"])):
+        assert len({tuple(b["groups"][g]["members"]) for b in routed}) == 1
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    with zindex_amd.Index.open(str(out)) as reopened:
+        assert reopened.meta["n_banks"] == parts and reopened.meta["n_classes"] == 1
+    o = zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"])
+    total = 0
+    for c, seq in enumerate(small_colors):
+        keys = np.unique(zr.scan_kmers(seq, 31))
+        unions = sum(b["groups"][[i for i, gg in enumerate(b["groups"]) if c in gg["members"]][0]]
+                     ["union_cardinality"] for b in routed)
+        if group == 1:
+            assert unions == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:1500]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+            total += 1
+    assert total > 0
+
+
+def test_partition_cap_too_small_is_rejected(small_colors):
+    with pytest.raises(zindex_amd.ZindexError, match="too small"):
+        _build(small_colors[:2], fingerprint_bits=8, hashes=4, max_subindex_size=1 << 8)
+
+
 def test_false_positive_rate_within_bound(small_colors):
     bits = 4
     idx = _build(small_colors, fingerprint_bits=bits, hashes=4)

@@ -293,3 +293,19 @@ def test_hbm_budget_rejects_oversized_index(base_case):
     with pytest.raises(zindex_amd.ZindexError) as e:
         small.load(hbm_budget=1024)
     assert e.value.code == -7
+
+
+@pytest.mark.parametrize("group,cap,device", [(1, 1 << 15, 0), (3, 1 << 15, 0), (1, 1 << 14, 1)])
+def test_hash_partitioned_index(group, cap, device):
+    """Banks of an oversized size class split into hash partitions
+    (storage.rs:1525-1641): each key is looked up in the partition partition_id
+    routes it to (storage.rs:99-101, 2717-2745)."""
+    colors = make_colors(110 + group, 8, 30000)
+    reads = make_reads(111, colors, 200)
+    idx = build(colors, fingerprint_bits=8, hashes=4, merge_group_size=group, slot_scale=0.8,
+                max_subindex_size=cap, device=device)
+    man = idx.manifest()
+    assert man["version"] == 9 and idx.meta["n_banks"] > 1
+    assert all(b.get("routing") for b in man["banks"])
+    idx.load()
+    check_parity(idx, oracle_for(idx), reads, label=f"partitioned g{group} cap {cap}")

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <thread>
 
@@ -213,6 +214,34 @@ struct GroupResult {
   }
   // recovery records of abandoned keys: (hash32, member mask)
   std::vector<std::pair<uint32_t, uint64_t>> recovery;
+  // hash partition of an oversized size class (partitions > 1)
+  uint64_t source_slots = 0;
+  uint32_t partition = 0, partitions = 1;
+};
+
+// A dataset's sequence kept at 2 bits per base plus an invalid-base bit, so
+// its keys can be extracted again when its size class is hash-partitioned.
+struct PackedColor {
+  std::vector<uint64_t> codes, invalid, rec_off;
+  void pack(const uint8_t *seq, const uint64_t *off, uint64_t n_records) {
+    const uint64_t n = n_records ? off[n_records] - off[0] : 0;
+    codes.assign((n + 31) / 32, 0);
+    invalid.assign((n + 63) / 64, 0);
+    rec_off.assign(off, off + n_records + 1);
+    for (uint64_t i = 0; i < n; i++) {
+      int c = enc(seq[off[0] + i]);
+      if (c < 0) invalid[i >> 6] |= 1ULL << (i & 63);
+      else codes[i >> 5] |= (uint64_t)c << (2 * (i & 31));
+    }
+  }
+  // ACGT for valid bases, N for the rest (extraction treats both alike).
+  std::vector<uint8_t> unpack() const {
+    const uint64_t n = rec_off.empty() ? 0 : rec_off.back() - rec_off.front();
+    std::vector<uint8_t> out(n);
+    for (uint64_t i = 0; i < n; i++)
+      out[i] = (invalid[i >> 6] >> (i & 63)) & 1 ? 'N' : "ACGT"[(codes[i >> 5] >> (2 * (i & 31))) & 3];
+    return out;
+  }
 };
 
 static uint64_t next_pow2(uint64_t v) {
@@ -494,6 +523,7 @@ struct zg_builder {
   // deques: workers hold references to elements while the caller appends
   std::deque<zg::Color> colors;
   std::deque<std::vector<uint64_t>> pending;  // keys per colour
+  std::deque<zg::PackedColor> packed;         // sequences, for re-extraction
   std::deque<zg::GroupResult> groups;         // finished groups (index = group id)
   std::mutex mu;
   std::condition_variable cv;
@@ -567,12 +597,16 @@ static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64
   }
 }
 
-static void build_group(const zg_build_config &cfg, std::vector<uint32_t> members,
-                        std::vector<std::vector<uint64_t> *> key_sets, GroupResult &g) {
+// The group's equations: distinct keys (as h, ascending) and cell values
+// fp | membership mask << b (storage.rs:1222-1255, 1301-1308). Consumes
+// key_sets.
+static void group_equations(const zg_build_config &cfg, const std::vector<uint32_t> &members,
+                            std::vector<std::vector<uint64_t> *> key_sets,
+                            std::vector<uint64_t> &keys, std::vector<uint64_t> &values) {
   const uint32_t b = cfg.fingerprint_bits;
-  g.members = std::move(members);
-  std::vector<uint64_t> keys, values;
-  if (g.members.size() == 1) {
+  keys.clear();
+  values.clear();
+  if (members.size() == 1) {
     keys.swap(*key_sets[0]);
     values.resize(keys.size());
     for (size_t i = 0; i < keys.size(); i++) values[i] = fingerprint(key_of(keys[i], cfg.seed), cfg.seed, b);
@@ -590,8 +624,15 @@ static void build_group(const zg_build_config &cfg, std::vector<uint32_t> member
     }
     for (auto *ks : key_sets) std::vector<uint64_t>().swap(*ks);
   }
+}
+
+static void build_group(const zg_build_config &cfg, std::vector<uint32_t> members,
+                        std::vector<std::vector<uint64_t> *> key_sets, GroupResult &g) {
+  g.members = std::move(members);
+  std::vector<uint64_t> keys, values;
+  group_equations(cfg, g.members, key_sets, keys, values);
   g.union_cardinality = keys.size();
-  g.width = b + (g.members.size() == 1 ? 0 : (uint32_t)g.members.size());
+  g.width = cfg.fingerprint_bits + (g.members.size() == 1 ? 0 : (uint32_t)g.members.size());
   g.slots = config_slots(cfg, keys.size());
   construct_group(cfg, keys, values, g);
 }
@@ -687,6 +728,7 @@ extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8
     {
       std::lock_guard<std::mutex> lk(b->mu);
       b->pending.emplace_back();
+      b->packed.emplace_back();
       if (single) b->groups.emplace_back();
     }
     b->submit([b, id, copy, offs, n_records, cfg, single] {
@@ -701,14 +743,17 @@ extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8
       uint64_t unique = keys.size();
       auto t2 = std::chrono::steady_clock::now();
       std::vector<uint64_t> *slot;
+      PackedColor *pk;
       GroupResult *gr = nullptr;
       {
         std::lock_guard<std::mutex> lk(b->mu);
         b->colors[id].occurrences = occurrences;
         b->colors[id].unique_kmers = unique;
         slot = &b->pending[id];
+        pk = &b->packed[id];
         if (single) gr = &b->groups[id];
       }
+      pk->pack(copy->data(), offs->data(), n_records);
       *slot = std::move(keys);
       if (single) build_group(cfg, {id}, {slot}, *gr);
       if (trace) {
@@ -725,22 +770,108 @@ extern "C" int zg_builder_add_color(zg_builder *b, const char *name, const uint8
 }
 
 namespace zg {
+// partition_large_subindexes (storage.rs:1525-1641): a size class whose bank
+// would exceed max_subindex_size is rebuilt as P = slots / local hash
+// partitions of `local` slots (halved until the bank fits); each group's keys
+// are extracted again from the kept sequences and split by partition_id.
+static void partition_oversized(zg_builder *b) {
+  const zg_build_config &cfg = b->cfg;
+  std::map<uint64_t, uint64_t> widths;
+  for (auto &g : b->groups) widths[g.slots] += g.width;
+  const uint64_t minimum = (uint64_t)cfg.segment_length * next_pow2(cfg.hashes);
+  std::map<uint64_t, std::pair<uint64_t, uint32_t>> splits;
+  for (auto &[slots, bits] : widths) {
+    const uint64_t rb = (bits + 7) / 8;
+    if ((unsigned __int128)slots * rb <= cfg.max_subindex_size) continue;
+    uint64_t local = slots;
+    while ((unsigned __int128)local * rb > cfg.max_subindex_size) local /= 2;
+    if (local < minimum)
+      fail(-1, "--max-subindex-size=" + std::to_string(cfg.max_subindex_size) +
+                   " bytes is too small for " + std::to_string(rb) + " bytes per slice and minimum " +
+                   std::to_string(minimum) + " slots; raise the cap");
+    splits[slots] = {local, (uint32_t)(slots / local)};
+  }
+  if (splits.empty()) return;
+  std::deque<GroupResult> old;
+  old.swap(b->groups);
+  std::vector<std::vector<GroupResult>> parts(old.size());
+  for (size_t gi = 0; gi < old.size(); gi++) {
+    auto it = splits.find(old[gi].slots);
+    if (it == splits.end()) continue;
+    const uint64_t local = it->second.first;
+    const uint32_t P = it->second.second;
+    parts[gi].resize(P);
+    b->submit([b, gi, local, P, &old, &parts] {
+      const zg_build_config &c = b->cfg;
+      GroupResult &src = old[gi];
+      std::vector<std::vector<uint64_t>> sets(src.members.size());
+      std::vector<std::vector<uint64_t> *> ptrs;
+      for (size_t j = 0; j < src.members.size(); j++) {
+        const PackedColor &pc = b->packed[src.members[j]];
+        std::vector<uint8_t> seq = pc.unpack();
+        std::vector<uint64_t> roff(pc.rec_off.size());
+        for (size_t r = 0; r < roff.size(); r++) roff[r] = pc.rec_off[r] - pc.rec_off[0];
+        extract(c, seq.data(), roff.data(), roff.empty() ? 0 : roff.size() - 1, sets[j]);
+        hash_sort_unique(sets[j], c.seed);
+        ptrs.push_back(&sets[j]);
+      }
+      std::vector<uint64_t> keys, values;
+      group_equations(c, src.members, ptrs, keys, values);
+      std::vector<std::vector<uint64_t>> pk(P), pv(P);
+      for (size_t i = 0; i < keys.size(); i++) {
+        uint32_t p = (uint32_t)partition_id(key_of(keys[i], c.seed), c.seed, P);
+        pk[p].push_back(keys[i]);
+        pv[p].push_back(values[i]);
+      }
+      for (uint32_t p = 0; p < P; p++) {
+        GroupResult &g = parts[gi][p];
+        g.members = src.members;
+        g.width = src.width;
+        g.slots = local;
+        g.source_slots = src.slots;
+        g.partition = p;
+        g.partitions = P;
+        g.union_cardinality = pk[p].size();
+        construct_group(c, pk[p], pv[p], g);
+        std::vector<uint64_t>().swap(pk[p]);
+        std::vector<uint64_t>().swap(pv[p]);
+      }
+      std::vector<uint8_t>().swap(src.cells);
+    });
+  }
+  b->drain();
+  for (size_t gi = 0; gi < old.size(); gi++) {
+    if (parts[gi].empty()) {
+      b->groups.push_back(std::move(old[gi]));
+    } else {
+      for (auto &g : parts[gi]) b->groups.push_back(std::move(g));
+    }
+  }
+}
+
 // Packs groups into plane-layout banks and assembles the manifest.
 static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   const zg_build_config &cfg = b->cfg;
   const uint32_t fb = cfg.fingerprint_bits;
-  // banks by slot count, ascending (pack_banks_streaming's BTreeMap order)
-  std::vector<uint64_t> sizes;
-  for (auto &g : b->groups) sizes.push_back(g.slots);
-  std::sort(sizes.begin(), sizes.end());
-  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  // banks by (source slots, partition), ascending (pack_banks_streaming's
+  // BTreeMap order, storage.rs:1762-1767)
+  auto bank_key = [](const GroupResult &g) {
+    return std::make_pair(g.partitions > 1 ? g.source_slots : g.slots, (uint64_t)g.partition);
+  };
+  std::vector<std::pair<uint64_t, uint64_t>> keys_of_banks;
+  for (auto &g : b->groups) keys_of_banks.push_back(bank_key(g));
+  std::sort(keys_of_banks.begin(), keys_of_banks.end());
+  keys_of_banks.erase(std::unique(keys_of_banks.begin(), keys_of_banks.end()), keys_of_banks.end());
   Manifest &m = idx->m;
   m = Manifest();
   uint64_t singleton_bits = 0, logical_bits = 0, abandoned_total = 0;
-  for (uint64_t slots : sizes) {
+  bool partitioned = false;
+  for (auto bank_k : keys_of_banks) {
     std::vector<size_t> ids;
     for (size_t i = 0; i < b->groups.size(); i++)
-      if (b->groups[i].slots == slots) ids.push_back(i);
+      if (bank_key(b->groups[i]) == bank_k) ids.push_back(i);
+    const GroupResult &g0 = b->groups[ids[0]];
+    const uint64_t slots = g0.slots;
     std::stable_sort(ids.begin(), ids.end(), [&](size_t x, size_t y) {
       const GroupResult &a = b->groups[x], &c = b->groups[y];
       if (a.width != c.width) return a.width < c.width;
@@ -765,12 +896,17 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
       abandoned_total += g.abandoned;
     }
     bm.row_bits = row_bits;
-    bm.file = BankMeta::filename(slots, false, 0, 0, 1);
+    bm.routed = g0.partitions > 1;
+    if (bm.routed) {
+      partitioned = true;
+      bm.routing_source_slots = g0.source_slots;
+      bm.partition = g0.partition;
+      bm.partitions = g0.partitions;
+    }
+    bm.file = BankMeta::filename(slots, bm.routed, g0.source_slots, g0.partition, g0.partitions);
     uint64_t rb = bm.row_bytes();
     ensure((unsigned __int128)slots * rb <= cfg.max_subindex_size,
-           "a subindex exceeds --max-subindex-size; hash-partitioned builds are not "
-           "implemented here",
-           -6);
+           "internal: a bank exceeds --max-subindex-size after partitioning", -1);
     logical_bits += slots * row_bits;
     std::vector<uint8_t> payload(slots * rb, 0);
     const uint64_t G = ids.size();
@@ -844,7 +980,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
     }
   }
   // manifest fields (storage.rs:787-857)
-  m.version = 8;
+  m.version = partitioned ? 9 : 8;
   m.abandoned_recovery = !cfg.drop_abandoned;
   m.has_abandoned_file = !cfg.drop_abandoned;
   m.abandoned_file = cfg.drop_abandoned ? "" : kAbandonedFile;
@@ -906,7 +1042,13 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   j.set("planned_groups", Json::u64(b->groups.size()));
   j.set("rejected_groups", Json::u64(0));
   j.set("overflow_groups", Json::u64(0));
-  j.set("exact_group_verifications", Json::u64(b->groups.size()));
+  uint64_t verifications = b->groups.size();
+  if (partitioned) {
+    verifications = 0;
+    for (auto &g : b->groups)
+      verifications += g.union_cardinality - (cfg.drop_abandoned ? g.abandoned : 0);
+  }
+  j.set("exact_group_verifications", Json::u64(verifications));
   j.set("sketch_seconds", Json::f64(0.0));
   j.set("plan_seconds", Json::f64(0.0));
   j.set("build_seconds", Json::f64(build_seconds));
@@ -943,6 +1085,13 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
     }
     o.set("groups", std::move(gs));
     o.set("fingerprint_first", Json::boolean(true));
+    if (bm.routed) {
+      Json r = Json::object();
+      r.set("source_slots", Json::u64(bm.routing_source_slots));
+      r.set("partition", Json::u64(bm.partition));
+      r.set("partitions", Json::u64(bm.partitions));
+      o.set("routing", std::move(r));
+    }
     banks.push(std::move(o));
   }
   j.set("banks", std::move(banks));
@@ -979,6 +1128,7 @@ extern "C" int zg_builder_finish(zg_builder *b, zg_index **out) {
       }
       b->drain();
     }
+    partition_oversized(b);
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - b->start).count();
     auto idx = std::make_unique<zg_index>();
     finish_index(b, idx.get(), secs);
