@@ -137,6 +137,7 @@ def make_leaves(args, synth, rng):
 
 
 def build_index(args, zindex_amd, leaves, bits, hashes, device=0):
+    os.environ.setdefault("ZG_BUILD_PROGRESS", "1")  # the builder reports its later phases
     t0 = time.time()
     builder = zindex_amd.Builder(k=args.k, fingerprint_bits=bits, hashes=hashes,
                                  index_mode=args.mode, minimizer_size=args.m, device=device)
@@ -208,9 +209,11 @@ def sweep_main(args, zindex_amd, synth) -> int:
         # false positives: absent k-mers, filter only
         t_abs, _, agg = idx.query_reads(absent, records=False, recover=False)
         n_abs = int(t_abs.sum())
+        # per colour: |hits - n p| <= 8 sqrt(n p (1-p)) + 8, the reference's tolerance
+        # (tests/option_coverage_cli.rs:279-285)
         p = 2.0 ** -bits
         rates = agg / max(1, n_abs)
-        bound = p + 8 * np.sqrt(p * (1 - p) / max(1, n_abs))
+        bound = p + (8 * np.sqrt(n_abs * p * (1 - p)) + 8) / max(1, n_abs)
         row = {"bits": bits, "hashes": hashes, "build_s": round(build_s, 1),
                "bank_bytes": meta["payload_bytes"], "abandoned_origins": meta["abandoned_origins"],
                "kmers_per_s": st["seeds"] / wall, "lookup_ms": round(float(np.mean(lk)), 3),
@@ -219,7 +222,9 @@ def sweep_main(args, zindex_amd, synth) -> int:
                "tuning": tuning, "false_negatives": fn,
                "stored_kmers": int(totals.sum()), "absent_kmers": n_abs,
                "fpr_mean": float(rates.mean()), "fpr_max": float(rates.max()),
-               "fpr_bound": bound, "fpr_within_bound": bool(np.all(rates <= bound))}
+               "fpr_bound": bound, "fpr_within_bound": bool(np.all(rates <= bound)),
+               "absent_hits_all_colours": int(agg.sum()),
+               "absent_hits_expected": n_abs * p * len(agg)}
         log(f"b={bits} H={hashes}: {row}")
         rows.append(row)
         idx.close()

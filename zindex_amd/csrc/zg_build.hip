@@ -792,6 +792,13 @@ static void partition_oversized(zg_builder *b) {
     splits[slots] = {local, (uint32_t)(slots / local)};
   }
   if (splits.empty()) return;
+  static const bool progress = getenv("ZG_BUILD_PROGRESS") != nullptr;
+  std::atomic<size_t> rebuilt{0};
+  size_t to_rebuild = 0;
+  for (auto &g : b->groups) to_rebuild += splits.count(g.slots);
+  if (progress)
+    fprintf(stderr, "[build] hash partitioning %zu oversized size classes (%zu groups)\n",
+            splits.size(), to_rebuild);
   std::deque<GroupResult> old;
   old.swap(b->groups);
   std::vector<std::vector<GroupResult>> parts(old.size());
@@ -801,7 +808,7 @@ static void partition_oversized(zg_builder *b) {
     const uint64_t local = it->second.first;
     const uint32_t P = it->second.second;
     parts[gi].resize(P);
-    b->submit([b, gi, local, P, &old, &parts] {
+    b->submit([b, gi, local, P, &old, &parts, &rebuilt, to_rebuild] {
       const zg_build_config &c = b->cfg;
       GroupResult &src = old[gi];
       std::vector<std::vector<uint64_t>> sets(src.members.size());
@@ -837,6 +844,9 @@ static void partition_oversized(zg_builder *b) {
         std::vector<uint64_t>().swap(pv[p]);
       }
       std::vector<uint8_t>().swap(src.cells);
+      size_t done = ++rebuilt;
+      if (progress && (done % 64 == 0 || done == to_rebuild))
+        fprintf(stderr, "[build] partitioned %zu/%zu groups into %u parts\n", done, to_rebuild, P);
     });
   }
   b->drain();
