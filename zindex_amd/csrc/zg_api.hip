@@ -102,9 +102,76 @@ static void check_query_support(const Manifest &m) {
   } else {
     fail(ZG_ERR_UNSUPPORTED, "modimizer and findere indexes are not supported by the GPU query");
   }
-  for (auto &b : m.banks)
-    if (!b.fingerprint_first)
-      fail(ZG_ERR_UNSUPPORTED, "legacy interleaved banks are not supported by the GPU query");
+}
+
+// get_bits / put_bits (storage.rs:3495-3538): little-endian bit fields of up
+// to 64 bits at any bit offset of a row.
+static inline uint64_t get_field(const uint8_t *row, uint64_t off, uint32_t width) {
+  uint64_t v = 0;
+  const uint32_t sh = (uint32_t)(off & 7);
+  const uint32_t n = (sh + width + 7) / 8;
+  const uint8_t *p = row + (off >> 3);
+  for (uint32_t i = 0; i < n && i < 8; i++) v |= (uint64_t)p[i] << (8 * i);
+  v >>= sh;
+  if (n == 9) v |= (uint64_t)p[8] << (64 - sh);
+  return width == 64 ? v : v & ((1ULL << width) - 1);
+}
+static inline void put_field(uint8_t *row, uint64_t off, uint32_t width, uint64_t v) {
+  for (uint32_t i = 0; i < width; i++)
+    if ((v >> i) & 1) row[(off + i) >> 3] |= (uint8_t)(1u << ((off + i) & 7));
+}
+
+// The device keeps every bank in the plane layout with singleton columns
+// first (the lookup's column model). A plane bank already in that order is
+// uploaded as is; otherwise (legacy interleaved rows, fingerprint_first =
+// false, where group g holds fp | mask << b at its offset, storage.rs:2864-2914,
+// or columns not ordered singletons first) each row is rewritten into
+// perm's column order. XOR is bitwise, so the XOR of rewritten rows is the
+// rewrite of the XORed row and the lookup's results are unchanged.
+static std::vector<uint8_t> plane_rows(const BankMeta &meta, const std::vector<uint8_t> &src,
+                                       const std::vector<uint32_t> &perm, uint32_t fb) {
+  const uint64_t rb = meta.row_bytes(), S = meta.slots;
+  const uint32_t G = (uint32_t)meta.groups.size();
+  std::vector<uint64_t> src_fp(G), src_mask(G), dst_mask(G);
+  std::vector<uint32_t> mw(G);
+  uint64_t acc = (uint64_t)G * fb;
+  for (uint32_t g = 0; g < G; g++) {
+    const GroupMeta &gm = meta.groups[g];
+    mw[g] = gm.members.size() > 1 ? (uint32_t)gm.members.size() : 0;
+    if (meta.fingerprint_first) {
+      src_fp[g] = (uint64_t)g * fb;
+      src_mask[g] = acc;
+      acc += mw[g];
+    } else {
+      src_fp[g] = gm.offset;
+      src_mask[g] = gm.offset + fb;
+    }
+  }
+  acc = (uint64_t)G * fb;
+  for (uint32_t c = 0; c < G; c++) {
+    dst_mask[c] = acc;
+    acc += mw[perm[c]];
+  }
+  std::vector<uint8_t> out(src.size(), 0);
+  const uint64_t block = 4096;
+  std::atomic<uint64_t> next{0};
+  auto work = [&] {
+    for (uint64_t b; (b = next++) * block < S;)
+      for (uint64_t r = b * block; r < std::min(S, (b + 1) * block); r++) {
+        const uint8_t *in = src.data() + r * rb;
+        uint8_t *o = out.data() + r * rb;
+        for (uint32_t c = 0; c < G; c++) {
+          const uint32_t g = perm[c];
+          put_field(o, (uint64_t)c * fb, fb, get_field(in, src_fp[g], fb));
+          if (mw[g]) put_field(o, dst_mask[c], mw[g], get_field(in, src_mask[g], mw[g]));
+        }
+      }
+  };
+  std::vector<std::thread> pool;
+  for (unsigned t = 1; t < host_threads(); t++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  return out;
 }
 
 void upload_index(zg_index *idx, uint64_t budget) {
@@ -129,31 +196,37 @@ void upload_index(zg_index *idx, uint64_t budget) {
   idx->dbanks.clear();
   idx->dbanks.resize(m.banks.size());
   const uint32_t n_colors = (uint32_t)m.colors.size();
+  std::vector<std::vector<uint32_t>> bank_color_col(m.banks.size());  // device columns
   for (size_t bi = 0; bi < m.banks.size(); bi++) {
     const BankMeta &meta = m.banks[bi];
     DeviceBank &db = idx->dbanks[bi];
     const uint32_t G = (uint32_t)meta.groups.size();
-    uint32_t first_merged = G;
+    // device column order: singleton groups, then multi-member groups
+    std::vector<uint32_t> perm;
     for (uint32_t g = 0; g < G; g++)
-      if (meta.groups[g].members.size() > 1) {
-        first_merged = g;
-        break;
-      }
-    for (uint32_t g = first_merged; g < G; g++)
-      if (meta.groups[g].members.size() == 1)
-        fail(ZG_ERR_UNSUPPORTED, "bank columns are not ordered by width");
+      if (meta.groups[g].members.size() == 1) perm.push_back(g);
+    const uint32_t first_merged = (uint32_t)perm.size();
+    for (uint32_t g = 0; g < G; g++)
+      if (meta.groups[g].members.size() > 1) perm.push_back(g);
+    bool as_is = true;
+    for (uint32_t c = 0; c < G; c++) {
+      const GroupMeta &gm = meta.groups[perm[c]];
+      as_is &= perm[c] == c;
+      // legacy rows of singletons only coincide with the plane layout
+      if (!meta.fingerprint_first) as_is &= gm.members.size() == 1 && gm.offset == (uint64_t)c * fb;
+    }
     std::vector<uint32_t> col_color(G, kNone), mg_off{0}, mg_members;
     std::vector<uint64_t> mg_mask_bit;
     std::vector<uint32_t> color_col(n_colors, kNone), color_bit(n_colors, 0);
     uint64_t mask_bit = (uint64_t)G * fb;
-    for (uint32_t g = 0; g < G; g++) {
-      const GroupMeta &gm = meta.groups[g];
+    for (uint32_t c = 0; c < G; c++) {
+      const GroupMeta &gm = meta.groups[perm[c]];
       for (size_t j = 0; j < gm.members.size(); j++) {
-        color_col[gm.members[j]] = g;
+        color_col[gm.members[j]] = c;
         color_bit[gm.members[j]] = (uint32_t)j;
       }
-      if (g < first_merged) {
-        col_color[g] = gm.members[0];
+      if (c < first_merged) {
+        col_color[c] = gm.members[0];
       } else {
         mg_mask_bit.push_back(mask_bit);
         mask_bit += gm.members.size();
@@ -161,7 +234,9 @@ void upload_index(zg_index *idx, uint64_t budget) {
         mg_off.push_back((uint32_t)mg_members.size());
       }
     }
-    const std::vector<uint8_t> &payload = idx->payload[bi];
+    std::vector<uint8_t> rewritten;
+    if (!as_is) rewritten = plane_rows(meta, idx->payload[bi], perm, fb);
+    const std::vector<uint8_t> &payload = as_is ? idx->payload[bi] : rewritten;
     db.data.alloc(payload.size() + pad);
     check_hip(hipMemcpyAsync(db.data.p, payload.data(), payload.size(), hipMemcpyHostToDevice, s),
               "hipMemcpyAsync");
@@ -171,6 +246,7 @@ void upload_index(zg_index *idx, uint64_t budget) {
     upload(db.mg_members, mg_members, s);
     upload(db.mg_mask_bit, mg_mask_bit, s);
     upload(db.color_col, color_col, s);
+    bank_color_col[bi] = color_col;
     upload(db.color_bit, color_bit, s);
     BankDev &v = db.view;
     v.data = db.data.as<uint8_t>();
@@ -191,23 +267,19 @@ void upload_index(zg_index *idx, uint64_t budget) {
     v.mg_mask_bit = db.mg_mask_bit.as<uint64_t>();
     v.color_col = db.color_col.as<uint32_t>();
     v.color_bit = db.color_bit.as<uint32_t>();
+    // the host tables of this bank (and a rewritten payload) end with the loop
+    check_hip(hipStreamSynchronize(s), "upload");
   }
   // Column map ids: banks whose colour -> column maps are equal (the
   // partitions of one size class) share an id.
   std::vector<std::vector<uint32_t>> maps;
-  {
-    std::vector<std::vector<uint32_t>> cc(m.banks.size());
-    for (size_t bi = 0; bi < m.banks.size(); bi++) {
-      cc[bi].assign(n_colors, kNone);
-      const auto &groups = m.banks[bi].groups;
-      for (uint32_t g = 0; g < groups.size(); g++)
-        for (uint32_t x : groups[g].members) cc[bi][x] = g;
-      size_t id = 0;
-      while (id < maps.size() && maps[id] != cc[bi]) id++;
-      if (id == maps.size()) maps.push_back(cc[bi]);
-      idx->dbanks[bi].view.colmap = (uint32_t)id;
-    }
+  for (size_t bi = 0; bi < m.banks.size(); bi++) {
+    size_t id = 0;
+    while (id < maps.size() && maps[id] != bank_color_col[bi]) id++;
+    if (id == maps.size()) maps.push_back(bank_color_col[bi]);
+    idx->dbanks[bi].view.colmap = (uint32_t)id;
   }
+  std::vector<std::vector<uint32_t>>().swap(bank_color_col);
   // recovery directory (the query-side OriginMap, abandoned.rs:29-206)
   idx->rec_view = RecDev{};
   if (idx->recover && idx->have_rec && idx->rec.size()) {

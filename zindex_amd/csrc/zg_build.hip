@@ -256,7 +256,7 @@ static uint64_t config_slots(const zg_build_config &c, uint64_t cardinality) {
   uint64_t n = (uint64_t)needed;
   uint64_t h2 = next_pow2(c.hashes);
   n = std::max<uint64_t>(n, h2);
-  n = std::max<uint64_t>(n, (uint64_t)c.segment_length * h2);
+  n = std::max<uint64_t>(n, (uint64_t)std::max<uint32_t>(c.segment_length, 1) * h2);
   return next_pow2(n);
 }
 
@@ -663,9 +663,10 @@ static void validate_config(const zg_build_config &c) {
   ensure(b == 1 || b == 2 || b == 4 || b == 8 || b == 16 || b == 32,
          "fingerprint bits must be 1, 2, 4, 8, 16 or 32", -1);
   ensure(c.hashes >= 2 && c.hashes <= 32, "invalid hash count", -1);
-  ensure(c.segment_length >= 1 && c.segment_length <= 4096 &&
-             !(c.segment_length & (c.segment_length - 1)),
-         "segment length must be a power of two in 1..=4096", -1);
+  // 0 = no fixed segment length: the legacy geometry (manifest version 7)
+  ensure(c.segment_length == 0 || (c.segment_length <= 4096 &&
+                                   !(c.segment_length & (c.segment_length - 1))),
+         "segment length must be 0 (legacy geometry) or a power of two in 1..=4096", -1);
   ensure(c.slot_scale > 0 && std::isfinite(c.slot_scale), "invalid slot scale", -1);
   ensure(c.max_subindex_size && !(c.max_subindex_size & (c.max_subindex_size - 1)),
          "--max-subindex-size must be a non-zero power of two", -1);
@@ -778,7 +779,7 @@ static void partition_oversized(zg_builder *b) {
   const zg_build_config &cfg = b->cfg;
   std::map<uint64_t, uint64_t> widths;
   for (auto &g : b->groups) widths[g.slots] += g.width;
-  const uint64_t minimum = (uint64_t)cfg.segment_length * next_pow2(cfg.hashes);
+  const uint64_t minimum = (uint64_t)std::max<uint32_t>(cfg.segment_length, 1) * next_pow2(cfg.hashes);
   std::map<uint64_t, std::pair<uint64_t, uint32_t>> splits;
   for (auto &[slots, bits] : widths) {
     const uint64_t rb = (bits + 7) / 8;
@@ -990,7 +991,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
     }
   }
   // manifest fields (storage.rs:787-857)
-  m.version = partitioned ? 9 : 8;
+  m.version = partitioned ? 9 : cfg.segment_length ? 8 : 7;
   m.abandoned_recovery = !cfg.drop_abandoned;
   m.has_abandoned_file = !cfg.drop_abandoned;
   m.abandoned_file = cfg.drop_abandoned ? "" : kAbandonedFile;
@@ -999,7 +1000,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   m.backend = "zor";
   m.k = cfg.k;
   m.hashes = cfg.hashes;
-  m.has_segment_length = true;
+  m.has_segment_length = cfg.segment_length != 0;
   m.segment_length = cfg.segment_length;
   m.seed = cfg.seed;
   m.fingerprint_bits = fb;
@@ -1021,7 +1022,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   j.set("backend", Json::str("zor"));
   j.set("k", Json::u64(m.k));
   j.set("hashes", Json::u64(m.hashes));
-  j.set("segment_length", Json::u64(m.segment_length));
+  j.set("segment_length", m.has_segment_length ? Json::u64(m.segment_length) : Json::null());
   j.set("seed", Json::u64(m.seed));
   j.set("fingerprint_bits", Json::u64(fb));
   j.set("index_mode", Json::u64(m.index_mode));
