@@ -127,6 +127,35 @@ def test_hash_partitioned_build(tmp_path, small_colors, group, cap):
     assert total > 0
 
 
+@pytest.mark.parametrize("bits,group,reverse", [(8, 1, False), (8, 3, True), (4, 2, False),
+                                               (2, 5, True)])
+def test_legacy_interleaved_layout_same_answers(tmp_path, small_colors, bits, group, reverse):
+    """Legacy interleaved rows (fingerprint_first = false) answer like the plane
+    layout (storage.rs:5343-5423): the same index rewritten interleaved, with
+    its groups optionally reversed, gives the oracle the same counts, and the
+    product's validation accepts it."""
+    idx = _build(small_colors, fingerprint_bits=bits, hashes=4, merge_group_size=group,
+                 slot_scale=0.8)
+    plane, legacy = tmp_path / "plane", tmp_path / "legacy"
+    idx.write(str(plane))
+    zr.to_legacy_layout(plane, legacy,
+                        (lambda gs: range(len(gs) - 1, -1, -1)) if reverse else None)
+    with zindex_amd.Index.open(str(legacy)) as z:
+        assert z.meta["n_colors"] == len(small_colors)
+    dp, dl = zr.read_index_dir(plane), zr.read_index_dir(legacy)
+    assert not dl["manifest"]["banks"][0]["fingerprint_first"]
+    op = zr.OracleIndex(dp["manifest"], dp["banks"], dp["recovery"])
+    ol = zr.OracleIndex(dl["manifest"], dl["banks"], dl["recovery"])
+    rng = np.random.default_rng(3)
+    reads = [c[i:i + 700] for c in small_colors for i in range(0, len(c) - 700, 3000)]
+    reads += [_random_genome(rng, 700) for _ in range(10)]
+    buf, off = zindex_amd.pack_reads(reads)
+    tp, cp = op.query_reads(buf, off, threads=zr.cpu_count())
+    tl, cl = ol.query_reads(buf, off, threads=zr.cpu_count())
+    assert np.array_equal(tp, tl) and np.array_equal(cp, cl)
+    assert cp.sum() > 0
+
+
 def test_partition_cap_too_small_is_rejected(small_colors):
     with pytest.raises(zindex_amd.ZindexError, match="too small"):
         _build(small_colors[:2], fingerprint_bits=8, hashes=4, max_subindex_size=1 << 8)

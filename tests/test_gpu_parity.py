@@ -309,3 +309,34 @@ def test_hash_partitioned_index(group, cap, device):
     assert all(b.get("routing") for b in man["banks"])
     idx.load()
     check_parity(idx, oracle_for(idx), reads, label=f"partitioned g{group} cap {cap}")
+
+
+@pytest.mark.parametrize("bits,group,reverse", [(8, 3, True), (4, 1, False), (16, 2, True)])
+def test_legacy_interleaved_banks(tmp_path, bits, group, reverse):
+    """Legacy interleaved rows (fingerprint_first = false, storage.rs:2864-2914),
+    groups optionally out of width order: rewritten to the device plane layout
+    on upload, answers equal the oracle's on the legacy files."""
+    colors = make_colors(120 + bits, 7, 25000, rates=(0.0, 0.05))
+    reads = make_reads(121 + bits, colors, 150)
+    idx = build(colors, fingerprint_bits=bits, hashes=4, merge_group_size=group, slot_scale=0.8)
+    plane, legacy = tmp_path / "plane", tmp_path / "legacy"
+    idx.write(str(plane))
+    zr.to_legacy_layout(plane, legacy,
+                        (lambda gs: range(len(gs) - 1, -1, -1)) if reverse else None)
+    disk = zr.read_index_dir(legacy)
+    orc = zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"])
+    with zindex_amd.Index.open(str(legacy)) as li:
+        li.load()
+        check_parity(li, orc, reads, label=f"legacy b{bits} g{group}")
+
+
+def test_legacy_segment_geometry():
+    """No fixed segment length (manifest version 7): PositionHash's legacy
+    geometry, length 2^floor(log2(clamp(slots/H, 1, 4096))) (storage.rs:3452-3481)."""
+    colors = make_colors(130, 6, 30000)
+    reads = make_reads(131, colors, 150)
+    idx = build(colors, fingerprint_bits=8, hashes=3, segment_length=0, slot_scale=0.8)
+    man = idx.manifest()
+    assert man["version"] == 7 and man["segment_length"] is None
+    idx.load()
+    check_parity(idx, oracle_for(idx), reads, label="legacy geometry")

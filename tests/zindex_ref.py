@@ -191,6 +191,59 @@ def zstd_compress(data: bytes, level: int = 3) -> bytes:
     return dst.raw[:n]
 
 
+def to_legacy_layout(src_dir, dst_dir, order=None) -> None:
+    """Rewrite an index directory with legacy interleaved banks
+    (fingerprint_first = false; storage.rs:2864-2914): group g of a row holds
+    fp | mask << b at bits [offset_g, offset_g + width_g), offsets cumulative
+    in manifest order. `order(groups) -> permutation` reorders each bank's
+    groups (default: as they are). Bank files are one zstd frame without a
+    seek table, behind the legacy magic ZORPOW1 (the old single-stream
+    container, bank_io.rs:351-366)."""
+    import shutil
+    src_dir, dst_dir = Path(src_dir), Path(dst_dir)
+    dst_dir.mkdir(parents=True)
+    disk = read_index_dir(src_dir)
+    man = json.loads(json.dumps(disk["manifest"]))
+    b = man["fingerprint_bits"]
+    for bi, bank in enumerate(man["banks"]):
+        groups = bank["groups"]
+        G = len(groups)
+        rb = (bank["row_bits"] + 7) // 8
+        payload = disk["banks"][bi]
+        # source plane layout: fp at g*b, masks after G*b in group order
+        mw = [len(g["members"]) if len(g["members"]) > 1 else 0 for g in groups]
+        mask_at, acc = [], G * b
+        for w in mw:
+            mask_at.append(acc)
+            acc += w
+        perm = list(range(G)) if order is None else list(order(groups))
+        new_groups, offsets, off = [], [], 0
+        for g in perm:
+            ng = dict(groups[g])
+            ng["offset"] = off
+            offsets.append(off)
+            off += ng["width"]
+            new_groups.append(ng)
+        out = bytearray(len(payload))
+        fpm = (1 << b) - 1
+        for r in range(bank["slots"]):
+            row = int.from_bytes(payload[r * rb:(r + 1) * rb].tobytes(), "little")
+            v = 0
+            for g, o in zip(perm, offsets):
+                fp = (row >> (g * b)) & fpm
+                mask = (row >> mask_at[g]) & ((1 << mw[g]) - 1) if mw[g] else 0
+                v |= (fp | (mask << b)) << o
+            out[r * rb:(r + 1) * rb] = v.to_bytes(rb, "little")
+        bank["groups"] = new_groups
+        bank["fingerprint_first"] = False
+        header = b"ZORPOW1\0" + np.array([bank["slots"], bank["row_bits"], G, len(out)],
+                                         dtype="<u8").tobytes()
+        (dst_dir / bank["file"]).write_bytes(zstd_compress(header + bytes(out)))
+    if man.get("abandoned_file"):
+        shutil.copy(src_dir / man["abandoned_file"], dst_dir / man["abandoned_file"])
+    (dst_dir / "manifest.json").write_text(json.dumps(man, indent=2))
+
+
 def read_varint(buf: bytes, at: int):
     value = 0
     for shift in range(0, 35, 7):
