@@ -84,6 +84,8 @@ def parse_args():
                         "first is the headline")
     p.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     p.add_argument("--gpu-peel", type=int, default=0, help="build the index with the GPU peel")
+    p.add_argument("--gather-sweep", action="store_true",
+                   help="also measure random-row gather rates at 512..8192 B rows")
     p.add_argument("--sweep", default="1:2,1:3,1:8,4:2,4:3,4:8,16:2,16:3,16:8,32:2,32:3,32:8",
                    help="sweep preset: bits:hashes configurations")
     args = p.parse_args()
@@ -314,6 +316,17 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
     }, parity, n_par
 
 
+def gather_ceiling(idx, algo_bytes):
+    """GB/s of random whole-row reads at the bank's row size (zg_gather_probe),
+    over as many bytes as one lookup launch reads (at most 2 TiB)."""
+    man = idx.manifest()
+    rb = (man["banks"][0]["row_bits"] + 7) // 8
+    if rb not in (512, 1024, 2048, 4096, 8192):
+        return None
+    n = max(1 << 20, min(algo_bytes, 1 << 41) // rb)
+    return {"gather_row_bytes": rb, "gather_ceiling_gbs": round(idx.gather_probe(rb, n), 1)}
+
+
 def percentile(values, q):
     return float(np.percentile(np.asarray(values, dtype=np.float64), q)) if len(values) else None
 
@@ -429,6 +442,7 @@ def serve_main(args, zindex_amd, synth) -> int:
     lk_ms = float(np.mean(lk))
     kernel_gbs = algo / (lk_ms / 1000) / 1e9
     log(f"lookup kernel on {n_k} reads: {lk_ms:.1f} ms, {kernel_gbs:.0f} GB/s")
+    ceiling = gather_ceiling(idx, algo)
     cpu = None
     if not args.no_cpu:
         cpu, _, _ = cpu_baseline(args, idx, buf, off, None)
@@ -470,7 +484,9 @@ def serve_main(args, zindex_amd, synth) -> int:
                      "traffic": None, "kernel": "lookup_kernel", "launch_ms": round(lk_ms, 3),
                      "algorithmic_bytes_per_launch": algo,
                      "launch": f"one persistent launch over the first {n_k} reads, aggregate "
-                               f"counts, recovery map on"},
+                               f"counts, recovery map on", **(ceiling or {}),
+                     "frac_of_gather_ceiling": round(kernel_gbs / ceiling["gather_ceiling_gbs"], 4)
+                     if ceiling else None},
         "closed_loop_row_gbs": round(b["seeds"] * per_lookup / b["wall_s"] / 1e9, 1),
         "cpu_baseline": cpu,
         "mixed_equals_plain": same,
@@ -548,6 +564,19 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "lookup_kernel", "launch_ms": round(lk_ms, 3),
                 "algorithmic_bytes_per_launch": algo}
+    # the ceiling of the row gathers alone: random whole rows of the bank's row
+    # size, the same bytes per launch, no other work
+    ceiling = gather_ceiling(idx, algo)
+    if ceiling:
+        roofline.update(ceiling)
+        roofline["frac_of_gather_ceiling"] = round(achieved / ceiling["gather_ceiling_gbs"], 4)
+        log(f"gather ceiling at {ceiling['gather_row_bytes']} B rows: "
+            f"{ceiling['gather_ceiling_gbs']:.0f} GB/s; lookup at "
+            f"{roofline['frac_of_gather_ceiling']:.1%} of it")
+    if args.gather_sweep:
+        roofline["gather_sweep_gbs"] = {rb: round(idx.gather_probe(rb, (1 << 40) // rb), 1)
+                                        for rb in (512, 1024, 2048, 4096, 8192)}
+        log(f"gather sweep: {roofline['gather_sweep_gbs']}")
     workload = workload_name(args, records)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this
     # workload (scaled by a copy kernel of known size), when recorded for it

@@ -308,6 +308,14 @@ int zg_server_get_stats(zg_server *srv, zg_server_stats *out);
 /* Answers every queued request, then releases the slots. */
 int zg_server_destroy(zg_server *srv);
 
+/* Device characterisation (no reference counterpart): reads n_rows whole rows
+ * of row_bytes (512, 1024, 2048, 4096 or 8192) at uniformly random row
+ * indices of resident bank `bank`, four rows per wave in flight and no other
+ * work, and returns the rate in GB/s: the ceiling of random row gathers at
+ * that row size, which the lookup's own rate is compared with. */
+int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
+                    uint64_t n_rows, double *gbs);
+
 /* Page-locked host memory for inputs and outputs of requests and batches
  * (DMA without staging copies). */
 int zg_host_alloc(uint64_t bytes, void **out);

@@ -186,6 +186,12 @@ class Index:
                                     len(off) - 1, flags, C.byref(h)))
         return Batch(self, h, len(off) - 1, records)
 
+    def gather_probe(self, row_bytes: int, n_rows: int, bank: int = 0) -> float:
+        """GB/s of random whole-row reads of row_bytes over a resident bank."""
+        gbs = C.c_double()
+        check(lib().zg_gather_probe(self._h, bank, row_bytes, n_rows, C.byref(gbs)))
+        return gbs.value
+
     @property
     def tuning(self) -> dict:
         t = Tuning()

@@ -962,6 +962,43 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
   }
 }
 
+extern "C" int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
+                               uint64_t n_rows, double *gbs) {
+  try {
+    ensure(idx && gbs, "null argument", ZG_ERR_INVALID);
+    ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
+           ZG_ERR_INVALID);
+    ensure(bank < idx->dbanks.size(), "bank out of range", ZG_ERR_INVALID);
+    const DeviceBank &db = idx->dbanks[bank];
+    const uint64_t bytes = idx->m.banks[bank].payload_bytes();
+    ensure(n_rows > 0 && bytes >= row_bytes, "empty probe", ZG_ERR_INVALID);
+    hipStream_t s = idx->stream;
+    DevBuf sink;
+    sink.alloc(16);
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    const uint32_t blocks = idx->cu_count * 8;  // 32 waves per CU
+    bool ok = launch_gather_probe(db.data.as<uint8_t>(), bytes, row_bytes, n_rows / 8, blocks,
+                                  sink.as<uint32_t>(), s);  // warm-up
+    if (ok) {
+      check_hip(hipEventRecord(e0, s), "hipEventRecord");
+      launch_gather_probe(db.data.as<uint8_t>(), bytes, row_bytes, n_rows, blocks,
+                          sink.as<uint32_t>(), s);
+      check_hip(hipEventRecord(e1, s), "hipEventRecord");
+    }
+    check_hip(hipStreamSynchronize(s), "gather probe");
+    double ms = ok ? elapsed_ms(e0, e1) : -1;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ensure(ok, "row_bytes must be 512, 1024, 2048, 4096 or 8192", ZG_ERR_INVALID);
+    *gbs = ms > 0 ? (double)n_rows * row_bytes / (ms * 1e6) : 0.0;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-buffer queries
 // ---------------------------------------------------------------------------

@@ -120,6 +120,10 @@ void launch_reduce_partials(const uint64_t *partial, uint64_t waves, uint32_t n_
                             uint64_t *agg, hipStream_t s);
 // Occupancy-derived number of resident 256-thread blocks per CU for a config.
 int lookup_blocks_per_cu(const KernelConfig &c);
+// n random whole-row reads of row_bytes (512..8192, powers of two) over
+// data[0, bytes); false for an unsupported row size.
+bool launch_gather_probe(const uint8_t *data, uint64_t bytes, uint32_t row_bytes, uint64_t n,
+                         uint32_t blocks, uint32_t *sink, hipStream_t s);
 
 }  // namespace zg
 

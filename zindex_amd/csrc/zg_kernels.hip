@@ -884,6 +884,73 @@ int lookup_blocks_per_cu(const KernelConfig &c) {
   }
 }
 
+// Random-row gather probe: whole rows of ROW = CH * 64 * LB bytes at
+// uniformly random row indices, RPW rows per wave in flight, with no other
+// work. Its rate is the ceiling the lookup's row gathers are measured against.
+template <int LB, int CH, int RPW>
+__global__ __launch_bounds__(256) void gather_probe_kernel(const uint8_t *data, uint64_t rows,
+                                                           uint64_t n, uint64_t seed,
+                                                           uint32_t *sink) {
+  constexpr uint32_t ROW = CH * 64 * LB;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint32_t acc = 0;
+  for (uint64_t i0 = wave * RPW; i0 < n; i0 += waves * RPW) {
+    uint32_t v[RPW][CH][LB / 4];
+#pragma unroll
+    for (int r = 0; r < RPW; r++) {
+      const uint64_t i = i0 + r;
+      const uint64_t row = mulhi64(splitmix64(i ^ seed), rows);
+      const uint8_t *p = data + row * ROW + lane * LB;
+#pragma unroll
+      for (int c = 0; c < CH; c++) {
+        if (i < n) {
+          if constexpr (LB == 16) {
+            u32x4 x = __builtin_nontemporal_load((const u32x4 *)(p + c * 64 * LB));
+            v[r][c][0] = x.x; v[r][c][1] = x.y; v[r][c][2] = x.z; v[r][c][3] = x.w;
+          } else {
+            u32x2 x = __builtin_nontemporal_load((const u32x2 *)(p + c * 64 * LB));
+            v[r][c][0] = x.x; v[r][c][1] = x.y;
+          }
+        } else {
+#pragma unroll
+          for (int w = 0; w < LB / 4; w++) v[r][c][w] = 0;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int c = 0; c < CH; c++)
+#pragma unroll
+        for (int w = 0; w < LB / 4; w++) acc ^= v[r][c][w];
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads live
+}
+
+template <int LB, int CH>
+static void launch_probe(const uint8_t *data, uint64_t rows, uint64_t n, uint32_t blocks,
+                         uint32_t *sink, hipStream_t s) {
+  gather_probe_kernel<LB, CH, 4><<<blocks, 256, 0, s>>>(data, rows, n, 0x5DEECE66DULL, sink);
+}
+
+bool launch_gather_probe(const uint8_t *data, uint64_t bytes, uint32_t row_bytes, uint64_t n,
+                         uint32_t blocks, uint32_t *sink, hipStream_t s) {
+  const uint64_t rows = bytes / row_bytes;
+  if (!rows) return false;
+  switch (row_bytes) {
+    case 512: launch_probe<8, 1>(data, rows, n, blocks, sink, s); return true;
+    case 1024: launch_probe<16, 1>(data, rows, n, blocks, sink, s); return true;
+    case 2048: launch_probe<16, 2>(data, rows, n, blocks, sink, s); return true;
+    case 4096: launch_probe<16, 4>(data, rows, n, blocks, sink, s); return true;
+    case 8192: launch_probe<16, 8>(data, rows, n, blocks, sink, s); return true;
+    default: return false;
+  }
+}
+
 __global__ void reduce_partials_kernel(const uint64_t *partial, uint64_t waves,
                                        uint32_t n_colors, uint64_t *agg) {
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
