@@ -509,9 +509,6 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   const uint64_t kmask = a.k >= 32 ? ~0ULL : ((1ULL << (2 * a.k)) - 1);
   (void)kmask;
   const uint64_t win_mask = a.k >= 64 ? ~0ULL : ((1ULL << a.k) - 1);
-  // slot indices fit 32 bits (the precomputed per-lane slots apply)
-  const bool small_bank = ((uint64_t)bk.geo.count + H - 1) * bk.geo.length <= (1ULL << 32);
-  const bool chain_rows = !small_bank || H > 2 * HB;
 
   uint32_t acc[T][F];
 #pragma unroll
@@ -619,18 +616,6 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
       const uint64_t lane_first = mixsplit(key, a.seed);
       const uint64_t lane_step = splitmix64(lane_first);
       const uint64_t lane_base = slot_base(lane_first, bk.geo);
-      // The first 2*HB row slots of this lane's key: a key's row address is
-      // then one register read per row instead of a scalar hash chain per row
-      // (rows past 2*HB, or banks of 2^32 slots or more, use the chain).
-      uint32_t lane_slot[2 * HB];
-      {
-        uint64_t hv = lane_first;
-#pragma unroll
-        for (int i = 0; i < 2 * HB; i++) {
-          lane_slot[i] = (uint32_t)slot_at(lane_base, hv, (uint32_t)i, bk.geo);
-          hv += lane_step;
-        }
-      }
       uint32_t rec_start = 0, rec_len = 0;
       if (a.rec.enabled && v) {
         uint32_t h = abandoned_hash(key, a.seed);
@@ -675,18 +660,12 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
             for (int w = 0; w < NW; w++) x[d][t][w] = 0;
         uint64_t first[D], step[D], base[D];
         uint32_t rl[D], rs[D], pre[D];
-        int kl[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
           int l = jj[d] < 0 ? 0 : jj[d];
-          kl[d] = l;
-          if (chain_rows) {
-            first[d] = readlane64(lane_first, l);
-            step[d] = readlane64(lane_step, l);
-            base[d] = readlane64(lane_base, l);
-          } else {
-            first[d] = step[d] = base[d] = 0;
-          }
+          first[d] = readlane64(lane_first, l);
+          step[d] = readlane64(lane_step, l);
+          base[d] = readlane64(lane_base, l);
           // the first 64 recovery postings of the key travel with its rows
           rl[d] = jj[d] < 0 ? 0 : readlane32(rec_len, l);
           rs[d] = readlane32(rec_start, l);
@@ -699,13 +678,8 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
 #pragma unroll
             for (int hh = 0; hh < HB; hh++) {
               uint32_t i = h0 + hh;
-              uint64_t slot;
-              if (small_bank && h0 < 2 * HB) {
-                slot = readlane32(h0 == 0 ? lane_slot[hh] : lane_slot[HB + hh], kl[d]);
-              } else {
-                uint64_t hv = first[d] + (uint64_t)i * step[d];
-                slot = slot_at(base[d], hv, i, bk.geo);
-              }
+              uint64_t hv = first[d] + (uint64_t)i * step[d];
+              uint64_t slot = slot_at(base[d], hv, i, bk.geo);
               const uint8_t *row = bk.data + slot * bk.row_bytes;
 #pragma unroll
               for (int t = 0; t < T; t++) {
