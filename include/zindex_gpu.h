@@ -38,6 +38,12 @@ extern "C" {
 #define ZG_QUERY_RECORDS 0x1u     /* per-read counts (else aggregate only) */
 #define ZG_QUERY_NO_RECOVERY 0x2u /* ignore the abandoned-key recovery map */
 
+/* Query memory modes (QueryMemory, src/power_merge/mod.rs:52-59). */
+#define ZG_MEMORY_ALL 0u      /* every subindex resident in HBM */
+#define ZG_MEMORY_SUBINDEX 1u /* subindexes resident while they fit the HBM
+                                 budget; the rest stream through HBM one at a
+                                 time during each query */
+
 /* Thread-local message for the last failing call on this thread. */
 const char *zg_last_error(void);
 /* Library version string. */
@@ -86,6 +92,15 @@ int zg_index_color(const zg_index *idx, uint32_t color, const char **name,
  * recovery map, then makes them resident in HBM. Fails with ZG_ERR_BUDGET
  * when the payload exceeds hbm_budget_bytes (0 = no budget). */
 int zg_index_load(zg_index *idx, int recover, uint64_t hbm_budget_bytes);
+/* zg_index_load with a query memory mode. Bank files are decoded frame by
+ * frame in parallel into page-locked chunks that are copied to HBM while the
+ * next chunk decodes (bank_io.rs:235-291); host copies are not kept. With
+ * ZG_MEMORY_SUBINDEX and an index larger than the budget, the banks that do
+ * not fit are decoded and streamed in per query (the reference's default
+ * subindex-at-a-time mode); the budget must hold the largest bank beside the
+ * resident ones. Servers need every bank resident. */
+int zg_index_load_ex(zg_index *idx, int recover, uint64_t hbm_budget_bytes,
+                     uint32_t memory);
 /* Host copy of one decoded bank payload (valid after zg_index_load, or after
  * an in-memory build). */
 int zg_index_bank_payload(const zg_index *idx, uint32_t bank,
@@ -139,6 +154,15 @@ int zg_query_records_command(const char *dir, const char *query,
                              double min_ratio, int recover,
                              uint64_t batch_size, int format, int out_fd,
                              int err_fd);
+/* The commands above with a query memory mode and an HBM budget (0 = none). */
+int zg_query_command_ex(const char *dir, const char *query, double min_ratio,
+                        int recover, uint32_t memory, uint64_t hbm_budget,
+                        int out_fd, int err_fd);
+int zg_query_records_command_ex(const char *dir, const char *query,
+                                double min_ratio, int recover,
+                                uint64_t batch_size, int format,
+                                uint32_t memory, uint64_t hbm_budget,
+                                int out_fd, int err_fd);
 /* info_command (storage.rs:3901-4092). */
 int zg_info_command(const char *dir, int out_fd);
 

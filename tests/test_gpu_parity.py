@@ -340,3 +340,79 @@ def test_legacy_segment_geometry():
     assert man["version"] == 7 and man["segment_length"] is None
     idx.load()
     check_parity(idx, oracle_for(idx), reads, label="legacy geometry")
+
+
+@pytest.fixture(scope="module")
+def disk_index(tmp_path_factory):
+    """An index on disk whose banks span several 4 MiB frames and 64 MiB upload
+    chunks (80 colours x 1 Mbp, one 80 MiB bank), plus its reads."""
+    colors = make_colors(140, 80, 1_000_000, rates=(0.005, 0.05))
+    reads = make_reads(141, colors, 300)
+    idx = build(colors, fingerprint_bits=8, hashes=4)
+    out = tmp_path_factory.mktemp("disk") / "idx"
+    idx.write(str(out))
+    idx.load()
+    expect = idx.query_reads(reads, records=True)
+    return out, reads, expect
+
+
+def test_streamed_load_from_disk(disk_index):
+    """Bank files decode frame by frame into page-locked chunks copied to HBM
+    (bank_io.rs:235-291); answers equal those of the index built in memory."""
+    out, reads, expect = disk_index
+    with zindex_amd.Index.open(str(out)) as z:
+        z.load()
+        got = z.query_reads(reads, records=True)
+        for g, e in zip(got, expect):
+            assert np.array_equal(g, e)
+
+
+def test_subindex_mode_streams_banks_beyond_budget(tmp_path):
+    """ZG_MEMORY_SUBINDEX: banks that do not fit the HBM budget stream in per
+    query (QueryMemory::Subindex, mod.rs:52-59); results equal all-resident."""
+    colors = make_colors(150, 10, 200_000)
+    reads = make_reads(151, colors, 200)
+    idx = build(colors, fingerprint_bits=8, hashes=4, max_subindex_size=1 << 19)
+    man = idx.manifest()
+    assert idx.meta["n_banks"] >= 4
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    bank = man["banks"][0]["slots"] * ((man["banks"][0]["row_bits"] + 7) // 8)
+    with zindex_amd.Index.open(str(out)) as full:
+        full.load()
+        expect = full.query_reads(reads, records=True)
+        rec_bytes = full.meta["abandoned_hashes"] * 28 + full.meta["abandoned_origins"] * 8
+    # room for the recovery map, the streaming buffer and one resident bank
+    budget = rec_bytes + 2 * (bank + 4096) + 1024
+    with zindex_amd.Index.open(str(out)) as z:
+        with pytest.raises(zindex_amd.ZindexError) as e:
+            z.load(hbm_budget=budget, memory="all")
+        assert e.value.code == -7
+        with pytest.raises(zindex_amd.ZindexError) as e:
+            z.load(hbm_budget=bank, memory="subindex")
+        assert e.value.code == -7
+        z.load(hbm_budget=budget, memory="subindex")
+        for _ in range(2):  # the second query starts from the bank left in HBM
+            got = z.query_reads(reads, records=True)
+            for g, x in zip(got, expect):
+                assert np.array_equal(g, x)
+        with z.batch(*zindex_amd.pack_reads(reads), records=True) as b:
+            b.run()
+            b.run()
+            assert np.array_equal(b.results()[1], expect[1])
+            b.run_plain()
+            assert np.array_equal(b.results()[1], expect[1])
+        with pytest.raises(zindex_amd.ZindexError) as e:
+            zindex_amd.Server(z)
+        assert e.value.code == -7
+    # the CLI's -M subindex under the same budget prints what -M all prints
+    q = tmp_path / "q.fa"
+    q.write_bytes(b"".join(b">r%d\n%s\n" % (i, r) for i, r in enumerate(reads) if r))
+    cli = str(ROOT / "zindex_amd" / "zorindex")
+    for extra in ([], ["-r"]):
+        a = subprocess.run([cli, "query", "-i", str(out), "-q", str(q), "-M", "all"] + extra,
+                           capture_output=True, check=True).stdout
+        b = subprocess.run([cli, "query", "-i", str(out), "-q", str(q), "-M", "subindex",
+                            "--hbm-budget", str(budget)] + extra,
+                           capture_output=True, check=True).stdout
+        assert a == b and len(a) > 0

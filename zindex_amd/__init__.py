@@ -24,7 +24,7 @@ __all__ = [
     "pack_reads",
 ]
 
-QUERY_MEMORY = ("subindex", "all")
+QUERY_MEMORY = ("all", "subindex")  # ZG_MEMORY_ALL, ZG_MEMORY_SUBINDEX
 RECORD_FORMATS = {"tsv": 0, "compact": 1}
 
 
@@ -111,8 +111,13 @@ class Index:
             out.append((name.value.decode(), uk.value))
         return out
 
-    def load(self, recover: bool = True, hbm_budget: int = 0) -> "Index":
-        check(lib().zg_index_load(self._h, int(bool(recover)), int(hbm_budget)))
+    def load(self, recover: bool = True, hbm_budget: int = 0, memory: str = "all") -> "Index":
+        """Makes the index resident. memory="subindex" with a budget below the
+        index streams the banks that do not fit through HBM per query."""
+        if memory not in QUERY_MEMORY:
+            raise ValueError(f"invalid query memory mode {memory}")
+        check(lib().zg_index_load_ex(self._h, int(bool(recover)), int(hbm_budget),
+                                     QUERY_MEMORY.index(memory)))
         return self
 
     def bank_payload(self, bank: int, copy: bool = True) -> np.ndarray:
@@ -409,12 +414,13 @@ def build(fof: str, output: str, err_fd: int = 2, **config) -> None:
 
 def query_command_with_memory(directory: str, query: str, min_ratio: float = 0.0,
                               recover: bool = True, memory: str = "subindex",
-                              out_fd: int = 1, err_fd: int = 2) -> None:
+                              out_fd: int = 1, err_fd: int = 2, hbm_budget: int = 0) -> None:
     """storage.rs:3731-3767: aggregate query, TSV to out_fd, diagnostics to err_fd."""
     if memory not in QUERY_MEMORY:
         raise ValueError(f"invalid query memory mode {memory}")
-    check(lib().zg_query_command(str(directory).encode(), str(query).encode(), float(min_ratio),
-                                 int(bool(recover)), out_fd, err_fd))
+    check(lib().zg_query_command_ex(str(directory).encode(), str(query).encode(),
+                                    float(min_ratio), int(bool(recover)),
+                                    QUERY_MEMORY.index(memory), int(hbm_budget), out_fd, err_fd))
 
 
 def query_command_with_recovery(directory: str, query: str, min_ratio: float,
@@ -429,13 +435,16 @@ def query_command(directory: str, query: str, min_ratio: float = 0.0, **kw) -> N
 def query_records_command_with_memory(directory: str, query: str, min_ratio: float = 0.0,
                                       recover: bool = True, batch_size: int = 4096,
                                       memory: str = "subindex", format: str = "tsv",
-                                      out_fd: int = 1, err_fd: int = 2) -> None:
+                                      out_fd: int = 1, err_fd: int = 2,
+                                      hbm_budget: int = 0) -> None:
     """storage.rs:3626-3720: per-record output (tsv or compact)."""
     if memory not in QUERY_MEMORY:
         raise ValueError(f"invalid query memory mode {memory}")
-    check(lib().zg_query_records_command(str(directory).encode(), str(query).encode(),
-                                         float(min_ratio), int(bool(recover)), int(batch_size),
-                                         RECORD_FORMATS[format], out_fd, err_fd))
+    check(lib().zg_query_records_command_ex(str(directory).encode(), str(query).encode(),
+                                            float(min_ratio), int(bool(recover)),
+                                            int(batch_size), RECORD_FORMATS[format],
+                                            QUERY_MEMORY.index(memory), int(hbm_budget),
+                                            out_fd, err_fd))
 
 
 def query_records_command(directory: str, query: str, min_ratio: float, recover: bool,

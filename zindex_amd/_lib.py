@@ -22,6 +22,8 @@ ZG_ERR_BUDGET = -7
 
 ZG_QUERY_RECORDS = 0x1
 ZG_QUERY_NO_RECOVERY = 0x2
+ZG_MEMORY_ALL = 0
+ZG_MEMORY_SUBINDEX = 1
 
 u8p = C.POINTER(C.c_uint8)
 u32p = C.POINTER(C.c_uint32)
@@ -99,6 +101,7 @@ SIGNATURES = {
     "zg_index_get_meta": (C.c_int, [C.c_void_p, C.POINTER(IndexMeta)]),
     "zg_index_color": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_char_p), u64p]),
     "zg_index_load": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64]),
+    "zg_index_load_ex": (C.c_int, [C.c_void_p, C.c_int, C.c_uint64, C.c_uint32]),
     "zg_index_bank_payload": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(u8p), u64p]),
     "zg_index_manifest": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), u64p]),
     "zg_index_recovery": (C.c_int, [C.c_void_p, C.POINTER(u32p), C.POINTER(u64p),
@@ -111,6 +114,11 @@ SIGNATURES = {
                                    C.c_int]),
     "zg_query_records_command": (C.c_int, [C.c_char_p, C.c_char_p, C.c_double, C.c_int,
                                            C.c_uint64, C.c_int, C.c_int, C.c_int]),
+    "zg_query_command_ex": (C.c_int, [C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_uint32,
+                                      C.c_uint64, C.c_int, C.c_int]),
+    "zg_query_records_command_ex": (C.c_int, [C.c_char_p, C.c_char_p, C.c_double, C.c_int,
+                                              C.c_uint64, C.c_int, C.c_uint32, C.c_uint64,
+                                              C.c_int, C.c_int]),
     "zg_info_command": (C.c_int, [C.c_char_p, C.c_int]),
     "zg_build_config_default": (None, [C.POINTER(BuildConfig)]),
     "zg_build_command": (C.c_int, [C.POINTER(BuildConfig), C.c_char_p, C.c_char_p, C.c_int]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <set>
 #include <thread>
 
@@ -174,9 +175,138 @@ static std::vector<uint8_t> plane_rows(const BankMeta &meta, const std::vector<u
   return out;
 }
 
-void upload_index(zg_index *idx, uint64_t budget) {
+constexpr uint64_t kStageBytes = 64ULL << 20;  // one page-locked upload chunk
+constexpr uint64_t kBankPad = 4096;             // zeroed tail of every bank buffer
+
+static void ensure_staging(zg_index *idx) {
+  for (int k = 0; k < 2; k++) {
+    if (!idx->staging[k])
+      check_hip(hipHostMalloc(&idx->staging[k], kStageBytes, hipHostMallocDefault), "hipHostMalloc");
+    if (!idx->staging_ev[k])
+      check_hip(hipEventCreateWithFlags(&idx->staging_ev[k], hipEventDisableTiming), "hipEventCreate");
+  }
+}
+
+template <class F>
+static void parallel_ranges(uint64_t n, uint64_t grain, F &&f) {
+  const uint64_t parts = std::max<uint64_t>(1, std::min<uint64_t>(host_threads(), n / grain));
+  if (parts == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (uint64_t t = 0; t < parts; t++)
+    pool.emplace_back([&, t] { f(n * t / parts, n * (t + 1) / parts); });
+  for (auto &th : pool) th.join();
+}
+
+// Host bytes -> device through the two page-locked chunks: one chunk is
+// filled (a parallel memcpy) while the other is in flight.
+static void staged_copy(zg_index *idx, uint8_t *dst, const uint8_t *src, uint64_t n,
+                        hipStream_t s) {
+  ensure_staging(idx);
+  int k = 0;
+  for (uint64_t at = 0; at < n; at += kStageBytes, k ^= 1) {
+    const uint64_t len = std::min(kStageBytes, n - at);
+    check_hip(hipEventSynchronize(idx->staging_ev[k]), "staging");
+    uint8_t *buf = (uint8_t *)idx->staging[k];
+    parallel_ranges(len, 4ULL << 20, [&](uint64_t a, uint64_t b) { memcpy(buf + a, src + at + a, b - a); });
+    check_hip(hipMemcpyAsync(dst + at, buf, len, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    check_hip(hipEventRecord(idx->staging_ev[k], s), "hipEventRecord");
+  }
+}
+
+// A bank file -> device, frame by frame (bank_io::read_parallel,
+// bank_io.rs:235-291): the frames of one chunk decode in parallel into a
+// page-locked chunk, which is copied while the next chunk decodes.
+static void stream_frames(zg_index *idx, const BankFrames &bf, const BankMeta &meta, uint8_t *dst,
+                          hipStream_t s) {
+  ensure_staging(idx);
+  const auto &fr = bf.frames;
+  int k = 0;
+  for (size_t i = 0; i < fr.size(); k ^= 1) {
+    const uint64_t r0 = fr[i].raw;
+    size_t j = i;
+    while (j < fr.size() && fr[j].raw + fr[j].raw_len - r0 <= kStageBytes) j++;
+    ensure(j > i, "bank frame larger than an upload chunk", ZG_ERR_FORMAT);
+    check_hip(hipEventSynchronize(idx->staging_ev[k]), "staging");
+    uint8_t *buf = (uint8_t *)idx->staging[k];
+    std::atomic<size_t> next{i};
+    std::exception_ptr error;
+    std::mutex mu;
+    auto work = [&] {
+      try {
+        for (size_t f; (f = next++) < j;) decode_bank_frame(bf, f, buf + (fr[f].raw - r0));
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!error) error = std::current_exception();
+      }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < std::min<size_t>(host_threads(), j - i); t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (error) std::rethrow_exception(error);
+    const uint64_t r1 = fr[j - 1].raw + fr[j - 1].raw_len;
+    uint64_t skip = 0;
+    if (r0 == 0) {  // the raw stream opens with the bank header
+      check_bank_header(buf, meta);
+      skip = kBankHeaderBytes;
+    }
+    check_hip(hipMemcpyAsync(dst + (r0 + skip - kBankHeaderBytes), buf + skip, r1 - r0 - skip,
+                             hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync");
+    check_hip(hipEventRecord(idx->staging_ev[k], s), "hipEventRecord");
+    i = j;
+  }
+}
+
+// Bank bi's payload in the device layout into dst (plus a zeroed pad), from
+// the host copy when the index has one, else from the bank file.
+static void put_bank(zg_index *idx, size_t bi, uint8_t *dst, hipStream_t s) {
+  const BankMeta &meta = idx->m.banks[bi];
+  const uint64_t bytes = meta.payload_bytes();
+  const bool host = bi < idx->payload.size() && idx->payload[bi].size() == bytes;
+  if (!idx->as_is[bi]) {
+    std::vector<uint8_t> tmp;
+    if (!host) tmp = load_bank_payload(idx->dir, meta, (int)host_threads());
+    std::vector<uint8_t> rows = plane_rows(meta, host ? idx->payload[bi] : tmp, idx->perm[bi],
+                                           (uint32_t)idx->m.fingerprint_bits);
+    staged_copy(idx, dst, rows.data(), bytes, s);
+    check_hip(hipStreamSynchronize(s), "upload");  // rows end here
+  } else if (host) {
+    staged_copy(idx, dst, idx->payload[bi].data(), bytes, s);
+  } else {
+    BankFrames bf = open_bank_frames(idx->dir, meta);
+    if (bf.frames.empty()) {  // a single-stream bank decodes whole
+      std::vector<uint8_t> tmp = load_bank_payload(idx->dir, meta, (int)host_threads());
+      staged_copy(idx, dst, tmp.data(), bytes, s);
+      check_hip(hipStreamSynchronize(s), "upload");
+    } else {
+      stream_frames(idx, bf, meta, dst, s);
+    }
+  }
+  check_hip(hipMemsetAsync(dst + bytes, 0, kBankPad, s), "hipMemsetAsync");
+}
+
+bool any_streamed(const zg_index *idx) {
+  for (bool b : idx->streamed)
+    if (b) return true;
+  return false;
+}
+
+void make_resident(zg_index *idx, size_t i, hipStream_t s) {
+  if (!idx->streamed[i] || idx->stream_holds == (int)i) return;
+  idx->stream_holds = -1;
+  put_bank(idx, i, idx->stream_data.as<uint8_t>(), s);
+  idx->stream_holds = (int)i;
+}
+
+void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   const Manifest &m = idx->m;
   check_query_support(m);
+  ensure(memory == ZG_MEMORY_ALL || memory == ZG_MEMORY_SUBINDEX, "invalid query memory mode",
+         ZG_ERR_INVALID);
   int dev = 0;
   check_hip(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
@@ -185,24 +315,47 @@ void upload_index(zg_index *idx, uint64_t budget) {
   idx->cu_count = (uint32_t)prop.multiProcessorCount;
   if (!idx->stream) check_hip(hipStreamCreateWithFlags(&idx->stream, hipStreamNonBlocking), "hipStreamCreate");
   const uint32_t fb = (uint32_t)m.fingerprint_bits;
-  const uint64_t pad = 4096;
-  uint64_t need = 0;
-  for (size_t i = 0; i < m.banks.size(); i++) need += idx->payload[i].size() + pad;
-  if (idx->recover && idx->have_rec) need += idx->rec.size() * 28 + idx->rec.origins.size() * 8;
-  if (budget && need > budget)
-    fail(ZG_ERR_BUDGET, "index needs " + std::to_string(need) + " bytes of HBM, budget is " +
-                            std::to_string(budget));
+  const size_t nb = m.banks.size();
+  // residency plan
+  uint64_t payload_total = 0, largest = 0;
+  for (auto &b : m.banks) {
+    payload_total += b.payload_bytes() + kBankPad;
+    largest = std::max(largest, b.payload_bytes() + kBankPad);
+  }
+  uint64_t rec_bytes = 0;
+  if (idx->recover && idx->have_rec) rec_bytes = idx->rec.size() * 28 + idx->rec.origins.size() * 8;
+  idx->streamed.assign(nb, false);
+  if (budget && payload_total + rec_bytes > budget) {
+    if (memory != ZG_MEMORY_SUBINDEX)
+      fail(ZG_ERR_BUDGET, "index needs " + std::to_string(payload_total + rec_bytes) +
+                              " bytes of HBM, budget is " + std::to_string(budget));
+    if (rec_bytes + largest > budget)
+      fail(ZG_ERR_BUDGET, "the recovery map and the largest subindex need " +
+                              std::to_string(rec_bytes + largest) + " bytes of HBM, budget is " +
+                              std::to_string(budget));
+    uint64_t left = budget - rec_bytes - largest;  // beside one streaming buffer
+    for (size_t bi = 0; bi < nb; bi++) {
+      const uint64_t b = m.banks[bi].payload_bytes() + kBankPad;
+      if (b <= left) left -= b;
+      else idx->streamed[bi] = true;
+    }
+  }
   hipStream_t s = idx->stream;
   idx->dbanks.clear();
-  idx->dbanks.resize(m.banks.size());
+  idx->dbanks.resize(nb);
+  idx->perm.assign(nb, {});
+  idx->as_is.assign(nb, true);
+  idx->stream_holds = -1;
+  idx->stream_data.reset();
+  if (any_streamed(idx)) idx->stream_data.alloc(largest);
   const uint32_t n_colors = (uint32_t)m.colors.size();
-  std::vector<std::vector<uint32_t>> bank_color_col(m.banks.size());  // device columns
-  for (size_t bi = 0; bi < m.banks.size(); bi++) {
+  std::vector<std::vector<uint32_t>> bank_color_col(nb);  // device columns
+  for (size_t bi = 0; bi < nb; bi++) {
     const BankMeta &meta = m.banks[bi];
     DeviceBank &db = idx->dbanks[bi];
     const uint32_t G = (uint32_t)meta.groups.size();
     // device column order: singleton groups, then multi-member groups
-    std::vector<uint32_t> perm;
+    std::vector<uint32_t> &perm = idx->perm[bi];
     for (uint32_t g = 0; g < G; g++)
       if (meta.groups[g].members.size() == 1) perm.push_back(g);
     const uint32_t first_merged = (uint32_t)perm.size();
@@ -215,6 +368,7 @@ void upload_index(zg_index *idx, uint64_t budget) {
       // legacy rows of singletons only coincide with the plane layout
       if (!meta.fingerprint_first) as_is &= gm.members.size() == 1 && gm.offset == (uint64_t)c * fb;
     }
+    idx->as_is[bi] = as_is;
     std::vector<uint32_t> col_color(G, kNone), mg_off{0}, mg_members;
     std::vector<uint64_t> mg_mask_bit;
     std::vector<uint32_t> color_col(n_colors, kNone), color_bit(n_colors, 0);
@@ -234,13 +388,14 @@ void upload_index(zg_index *idx, uint64_t budget) {
         mg_off.push_back((uint32_t)mg_members.size());
       }
     }
-    std::vector<uint8_t> rewritten;
-    if (!as_is) rewritten = plane_rows(meta, idx->payload[bi], perm, fb);
-    const std::vector<uint8_t> &payload = as_is ? idx->payload[bi] : rewritten;
-    db.data.alloc(payload.size() + pad);
-    check_hip(hipMemcpyAsync(db.data.p, payload.data(), payload.size(), hipMemcpyHostToDevice, s),
-              "hipMemcpyAsync");
-    check_hip(hipMemsetAsync((uint8_t *)db.data.p + payload.size(), 0, pad, s), "hipMemsetAsync");
+    uint8_t *home;
+    if (idx->streamed[bi]) {
+      home = idx->stream_data.as<uint8_t>();
+    } else {
+      db.data.alloc(meta.payload_bytes() + kBankPad);
+      home = db.data.as<uint8_t>();
+      put_bank(idx, bi, home, s);
+    }
     upload(db.col_color, col_color, s);
     upload(db.mg_off, mg_off, s);
     upload(db.mg_members, mg_members, s);
@@ -249,7 +404,7 @@ void upload_index(zg_index *idx, uint64_t budget) {
     bank_color_col[bi] = color_col;
     upload(db.color_bit, color_bit, s);
     BankDev &v = db.view;
-    v.data = db.data.as<uint8_t>();
+    v.data = home;
     v.row_bytes = meta.row_bytes();
     v.plane_bytes = ((uint64_t)G * fb + 7) / 8;
     v.groups = G;
@@ -267,7 +422,7 @@ void upload_index(zg_index *idx, uint64_t budget) {
     v.mg_mask_bit = db.mg_mask_bit.as<uint64_t>();
     v.color_col = db.color_col.as<uint32_t>();
     v.color_bit = db.color_bit.as<uint32_t>();
-    // the host tables of this bank (and a rewritten payload) end with the loop
+    // the host tables of this bank end with the loop
     check_hip(hipStreamSynchronize(s), "upload");
   }
   // Column map ids: banks whose colour -> column maps are equal (the
@@ -346,7 +501,13 @@ uint64_t resident_bytes(const zg_index *idx) {
 using namespace zg;
 
 zg_index::~zg_index() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (int k = 0; k < 2; k++) {
+    if (staging_ev[k]) (void)hipEventDestroy(staging_ev[k]);
+    if (staging[k]) (void)hipHostFree(staging[k]);
+  }
   dbanks.clear();
+  stream_data.reset();
   rec_dir.reset();
   rec_post.reset();
   rec_post_col.reset();
@@ -421,36 +582,48 @@ extern "C" int zg_index_color(const zg_index *idx, uint32_t color, const char **
   }
 }
 
-extern "C" int zg_index_load(zg_index *idx, int recover, uint64_t budget) {
+extern "C" int zg_index_load_ex(zg_index *idx, int recover, uint64_t budget, uint32_t memory) {
   try {
     ensure(idx, "null argument", ZG_ERR_INVALID);
     check_query_support(idx->m);
-    if (!idx->payload_loaded) {
-      idx->payload.clear();
-      for (auto &b : idx->m.banks)
-        idx->payload.push_back(load_bank_payload(idx->dir, b, (int)host_threads()));
-      idx->payload_loaded = true;
-    }
     idx->recover = recover && idx->m.abandoned_recovery;
     if (idx->recover && !idx->have_rec && !idx->dir.empty()) {
       idx->rec = read_recovery_map(idx->dir, idx->m);
       idx->have_rec = true;
     }
-    upload_index(idx, budget);
+    upload_index(idx, budget, memory);
     return 0;
   } catch (...) {
     return translate_exception();
   }
 }
 
+extern "C" int zg_index_load(zg_index *idx, int recover, uint64_t budget) {
+  return zg_index_load_ex(idx, recover, budget, ZG_MEMORY_ALL);
+}
+
+namespace zg {
+// The decoded payload of one bank in host memory (decoded from the bank file
+// on first use when the index was opened from disk).
+const std::vector<uint8_t> &host_payload(zg_index *idx, size_t bank) {
+  ensure(bank < idx->m.banks.size(), "bank out of range", ZG_ERR_INVALID);
+  if (idx->payload.size() < idx->m.banks.size()) idx->payload.resize(idx->m.banks.size());
+  std::vector<uint8_t> &p = idx->payload[bank];
+  if (p.size() != idx->m.banks[bank].payload_bytes()) {
+    ensure(!idx->dir.empty(), "bank payload not in host memory", ZG_ERR_INVALID);
+    p = load_bank_payload(idx->dir, idx->m.banks[bank], (int)host_threads());
+  }
+  return p;
+}
+}  // namespace zg
+
 extern "C" int zg_index_bank_payload(const zg_index *idx, uint32_t bank, const uint8_t **data,
                                      uint64_t *bytes) {
   try {
     ensure(idx && data && bytes, "null argument", ZG_ERR_INVALID);
-    ensure(idx->payload_loaded && bank < idx->payload.size(), "bank payload not loaded",
-           ZG_ERR_INVALID);
-    *data = idx->payload[bank].data();
-    *bytes = idx->payload[bank].size();
+    const std::vector<uint8_t> &p = host_payload(const_cast<zg_index *>(idx), bank);
+    *data = p.data();
+    *bytes = p.size();
     return 0;
   } catch (...) {
     return translate_exception();
@@ -595,12 +768,29 @@ void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
 
 // The lookups: reset the work counters, one persistent launch per bank. This
 // is the part captured in the HIP graph.
+// Banks in lookup order: resident ones, then (subindex mode) the streamed one
+// already in HBM, then the other streamed ones, each loaded just before its
+// lookups (the reference reuses its first loaded bank the same way,
+// query_stream.rs:255-265).
+static std::vector<size_t> bank_order(const zg_index *idx) {
+  std::vector<size_t> order;
+  for (size_t i = 0; i < idx->dbanks.size(); i++)
+    if (!idx->streamed[i]) order.push_back(i);
+  if (idx->stream_holds >= 0) order.push_back((size_t)idx->stream_holds);
+  for (size_t i = 0; i < idx->dbanks.size(); i++)
+    if (idx->streamed[i] && (int)i != idx->stream_holds) order.push_back(i);
+  return order;
+}
+
 void enqueue_lookups(zg_index *idx, zg_batch *b) {
   hipStream_t s = b->stream;
   launch_fill_u32(b->queue.as<uint32_t>(), idx->dbanks.size(), 0, s);
   bool any = b->explicit_keys ? b->n_keys > 0 : b->positions > 0;
   if (any)
-    for (size_t i = 0; i < idx->dbanks.size(); i++) launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
+    for (size_t i : bank_order(idx)) {
+      make_resident(idx, i, s);
+      launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
+    }
 }
 
 // Per-colour sums after the lookups.
@@ -749,7 +939,9 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
   try {
     ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
     hipStream_t s = b->stream;
-    if (!b->graph_exec && !b->graph_failed) {
+    // banks streamed per query cannot sit in a graph: launch directly
+    if (any_streamed(idx)) zg::drop_graph(b);
+    if (!b->graph_exec && !b->graph_failed && !any_streamed(idx)) {
       // Capture the lookups once; replays reuse the instantiated graph. If any
       // step of the capture is refused, fall back to direct launches.
       bool captured = false;
@@ -809,10 +1001,11 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
     zg::launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
     zg::launch_fill_u32(queues.as<uint32_t>(), count * nb, 0, s);
     check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
-    for (uint64_t r = first; r < first + count; r++) {
-      uint64_t lo = b->h_seg_off[r], hi = b->h_seg_off[r + 1];
-      if (hi <= lo) continue;
-      for (size_t i = 0; i < nb; i++) {
+    for (size_t i : zg::bank_order(idx)) {
+      zg::make_resident(idx, i, s);
+      for (uint64_t r = first; r < first + count; r++) {
+        uint64_t lo = b->h_seg_off[r], hi = b->h_seg_off[r + 1];
+        if (hi <= lo) continue;
         zg::LookupArgs a = zg::lookup_args(idx, b, i);
         a.begin = lo;
         a.end = hi;
@@ -969,6 +1162,7 @@ extern "C" int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
     ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
            ZG_ERR_INVALID);
     ensure(bank < idx->dbanks.size(), "bank out of range", ZG_ERR_INVALID);
+    make_resident(idx, bank, idx->stream);
     const DeviceBank &db = idx->dbanks[bank];
     const uint64_t bytes = idx->m.banks[bank].payload_bytes();
     ensure(n_rows > 0 && bytes >= row_bytes, "empty probe", ZG_ERR_INVALID);
@@ -979,12 +1173,12 @@ extern "C" int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
     check_hip(hipEventCreate(&e0), "hipEventCreate");
     check_hip(hipEventCreate(&e1), "hipEventCreate");
     const uint32_t blocks = idx->cu_count * 8;  // 32 waves per CU
-    bool ok = launch_gather_probe(db.data.as<uint8_t>(), bytes, row_bytes, n_rows / 8, blocks,
-                                  sink.as<uint32_t>(), s);  // warm-up
+    const uint8_t *data = db.view.data;
+    bool ok = launch_gather_probe(data, bytes, row_bytes, n_rows / 8, blocks, sink.as<uint32_t>(),
+                                  s);  // warm-up
     if (ok) {
       check_hip(hipEventRecord(e0, s), "hipEventRecord");
-      launch_gather_probe(db.data.as<uint8_t>(), bytes, row_bytes, n_rows, blocks,
-                          sink.as<uint32_t>(), s);
+      launch_gather_probe(data, bytes, row_bytes, n_rows, blocks, sink.as<uint32_t>(), s);
       check_hip(hipEventRecord(e1, s), "hipEventRecord");
     }
     check_hip(hipStreamSynchronize(s), "gather probe");
@@ -1173,6 +1367,12 @@ static double now_s() {
 
 extern "C" int zg_query_command(const char *dir, const char *query, double min_ratio,
                                 int recover, int out_fd, int err_fd) {
+  return zg_query_command_ex(dir, query, min_ratio, recover, ZG_MEMORY_ALL, 0, out_fd, err_fd);
+}
+
+extern "C" int zg_query_command_ex(const char *dir, const char *query, double min_ratio,
+                                   int recover, uint32_t memory, uint64_t hbm_budget, int out_fd,
+                                   int err_fd) {
   zg_index *idx = nullptr;
   try {
     ensure(dir && query, "null argument", ZG_ERR_INVALID);
@@ -1186,7 +1386,7 @@ extern "C" int zg_query_command(const char *dir, const char *query, double min_r
     Records r = parse_sequences(read_maybe_compressed(query));
     double prepare = now_s() - tp;
     double tl = now_s();
-    rc = zg_index_load(idx, recover, 0);
+    rc = zg_index_load_ex(idx, recover, hbm_budget, memory);
     if (rc) return rc;
     double load = now_s() - tl + (tp - t0);
     const uint32_t C = (uint32_t)idx->m.colors.size();
@@ -1242,6 +1442,14 @@ extern "C" int zg_query_command(const char *dir, const char *query, double min_r
 extern "C" int zg_query_records_command(const char *dir, const char *query, double min_ratio,
                                         int recover, uint64_t batch_size, int format, int out_fd,
                                         int err_fd) {
+  return zg_query_records_command_ex(dir, query, min_ratio, recover, batch_size, format,
+                                     ZG_MEMORY_ALL, 0, out_fd, err_fd);
+}
+
+extern "C" int zg_query_records_command_ex(const char *dir, const char *query, double min_ratio,
+                                           int recover, uint64_t batch_size, int format,
+                                           uint32_t memory, uint64_t hbm_budget, int out_fd,
+                                           int err_fd) {
   zg_index *idx = nullptr;
   try {
     ensure(dir && query, "null argument", ZG_ERR_INVALID);
@@ -1254,7 +1462,7 @@ extern "C" int zg_query_records_command(const char *dir, const char *query, doub
     std::unique_ptr<zg_index> hold(idx);
     check_query_support(idx->m);
     Records r = parse_records(read_maybe_compressed(query));
-    rc = zg_index_load(idx, recover, 0);
+    rc = zg_index_load_ex(idx, recover, hbm_budget, memory);
     if (rc) return rc;
     double load = now_s() - t0;
     const Manifest &m = idx->m;

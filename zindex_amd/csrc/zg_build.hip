@@ -1262,14 +1262,19 @@ extern "C" int zg_build_command(const zg_build_config *cfg, const char *fof, con
 extern "C" int zg_index_write(const zg_index *idx, const char *out_dir) {
   try {
     ensure(idx && out_dir, "null argument", -1);
-    ensure(idx->payload_loaded && idx->payload.size() == idx->m.banks.size(),
-           "bank payloads are not in host memory", -1);
+    zg_index *ix = const_cast<zg_index *>(idx);  // host copies are decoded on first use
+    if (ix->m.abandoned_recovery && !ix->have_rec) {
+      ensure(!ix->dir.empty(), "recovery map not in host memory", -1);
+      ix->rec = read_recovery_map(ix->dir, ix->m);
+      ix->have_rec = true;
+    }
     std::string stage = std::string(out_dir) + ".staging";
     ensure(mkdir(stage.c_str(), 0755) == 0, "cannot create staging directory", -2);
-    for (size_t i = 0; i < idx->m.banks.size(); i++)
-      write_bank_file(path_join(stage, idx->m.banks[i].file), idx->m.banks[i],
-                      idx->payload[i].data(), idx->payload[i].size(),
+    for (size_t i = 0; i < idx->m.banks.size(); i++) {
+      const std::vector<uint8_t> &p = host_payload(ix, i);
+      write_bank_file(path_join(stage, idx->m.banks[i].file), idx->m.banks[i], p.data(), p.size(),
                       (int)zg_host_threads());
+    }
     if (idx->m.abandoned_recovery)
       write_recovery_map(path_join(stage, kAbandonedFile), idx->rec);
     std::string json;

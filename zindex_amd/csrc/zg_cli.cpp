@@ -15,7 +15,7 @@ static int usage() {
           "        [-i kmers|minimizers] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
           "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
-          "        [-M subindex|all] [--no-recovery]\n"
+          "        [-M subindex|all] [--hbm-budget BYTES] [--no-recovery]\n"
           "  info -i INDEX\n");
   return 2;
 }
@@ -87,7 +87,8 @@ int main(int argc, char **argv) {
     double ratio = 0.0;
     bool records = false, recover = true;
     int format = 0;
-    uint64_t batch = 4096;
+    uint64_t batch = 4096, budget = 0;
+    uint32_t memory = ZG_MEMORY_SUBINDEX;  // the reference's default query memory mode
     for (size_t i = 0; i < a.size(); i++) {
       const std::string &o = a[i];
       if (o == "-i" || o == "--index") index = value(i);
@@ -100,16 +101,22 @@ int main(int argc, char **argv) {
         else if (v == "compact") format = 1;
         else return usage();
       } else if (o == "-B" || o == "--record-batch-size") batch = strtoull(value(i).c_str(), nullptr, 10);
-      else if (o == "-M" || o == "--query-memory") (void)value(i);  // both modes keep banks in HBM
+      else if (o == "-M" || o == "--query-memory") {
+        std::string v = value(i);
+        if (v == "subindex") memory = ZG_MEMORY_SUBINDEX;
+        else if (v == "all") memory = ZG_MEMORY_ALL;
+        else return usage();
+      } else if (o == "--hbm-budget") budget = strtoull(value(i).c_str(), nullptr, 10);
       else if (o == "-t" || o == "--threads") (void)value(i);
       else if (o == "--no-recovery") recover = false;
       else return usage();
     }
     if (index.empty() || query.empty()) return usage();
     if (records)
-      return report(zg_query_records_command(index.c_str(), query.c_str(), ratio, recover, batch,
-                                             format, 1, 2));
-    return report(zg_query_command(index.c_str(), query.c_str(), ratio, recover, 1, 2));
+      return report(zg_query_records_command_ex(index.c_str(), query.c_str(), ratio, recover,
+                                                batch, format, memory, budget, 1, 2));
+    return report(zg_query_command_ex(index.c_str(), query.c_str(), ratio, recover, memory,
+                                      budget, 1, 2));
   }
   if (cmd == "info") {
     std::string index;

@@ -593,6 +593,32 @@ std::vector<uint8_t> load_bank_payload(const std::string &dir, const BankMeta &m
   return data;
 }
 
+BankFrames open_bank_frames(const std::string &dir, const BankMeta &meta) {
+  BankFrames b;
+  b.file = read_file(path_join(dir, meta.file));
+  std::vector<Frame> frames;
+  if (read_table(b.file, meta.payload_bytes() + kHeader, frames) && frames.size() > 1)
+    for (const Frame &f : frames) b.frames.push_back({f.compressed, f.compressed_len, f.raw, f.raw_len});
+  return b;
+}
+
+void decode_bank_frame(const BankFrames &b, size_t i, uint8_t *dst) {
+  const BankFrames::Frame &f = b.frames[i];
+  const char *src = b.file.data() + f.compressed;
+  ensure(ZSTD_findFrameCompressedSize(src, f.compressed_len) == f.compressed_len,
+         "frame contains trailing data", -2);
+  ensure(ZSTD_getFrameContentSize(src, f.compressed_len) == f.raw_len,
+         "frame content size differs from seek table", -2);
+  ZSTD_DCtx *d = ZSTD_createDCtx();
+  size_t r = ZSTD_decompressDCtx(d, dst, f.raw_len, src, f.compressed_len);
+  ZSTD_freeDCtx(d);
+  ensure(!ZSTD_isError(r) && r == f.raw_len, "short bank frame", -2);
+}
+
+void check_bank_header(const uint8_t *header, const BankMeta &meta) {
+  validate_header(header, meta, meta.payload_bytes());
+}
+
 void write_bank_file(const std::string &path, const BankMeta &meta, const uint8_t *payload,
                      uint64_t bytes, int threads) {
   // The raw stream is header + payload, cut into 4 MiB frames.

@@ -121,6 +121,23 @@ std::string zstd_compress(const void *data, size_t n, int level, bool checksum);
 // (slots * row_bytes bytes) after header validation.
 std::vector<uint8_t> load_bank_payload(const std::string &dir, const BankMeta &meta,
                                        int threads);
+// A bank file opened for streamed decoding (bank_io::read_parallel,
+// bank_io.rs:235-291): the compressed file and its seek-table frames, whose
+// raw offsets count the 40-byte header. No frames: a single-stream bank,
+// decoded whole by load_bank_payload.
+constexpr size_t kBankHeaderBytes = 40;
+struct BankFrames {
+  struct Frame {
+    uint64_t compressed, compressed_len, raw, raw_len;
+  };
+  std::string file;
+  std::vector<Frame> frames;
+};
+BankFrames open_bank_frames(const std::string &dir, const BankMeta &meta);
+// Decodes frame i (raw_len bytes) into dst; frame 0 starts with the header.
+void decode_bank_frame(const BankFrames &b, size_t i, uint8_t *dst);
+// The decoded 40-byte header against the manifest (storage.rs:2184-2231).
+void check_bank_header(const uint8_t *header, const BankMeta &meta);
 // bank_io::Writer (bank_io.rs:34-120): 4 MiB checksummed frames + seek table.
 void write_bank_file(const std::string &path, const BankMeta &meta,
                      const uint8_t *payload, uint64_t bytes, int threads);

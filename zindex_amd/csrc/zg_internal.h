@@ -72,6 +72,16 @@ struct zg_index {
   uint32_t cu_count = 0;
   hipStream_t stream = nullptr;
   std::vector<zg::DeviceBank> dbanks;
+  // Bank residency: a bank either owns its HBM payload (dbanks[i].data) or,
+  // in subindex mode beyond the budget, is streamed into `stream_data` right
+  // before its lookups (one bank at a time).
+  std::vector<bool> streamed;                   // per bank
+  std::vector<std::vector<uint32_t>> perm;      // device column order per bank
+  std::vector<bool> as_is;                      // payload is already the device layout
+  zg::DevBuf stream_data;
+  int stream_holds = -1;                        // bank in stream_data, -1 none
+  void *staging[2] = {nullptr, nullptr};        // page-locked upload chunks
+  hipEvent_t staging_ev[2] = {nullptr, nullptr};
   zg::DevBuf rec_dir, rec_post, rec_post_col, inner_table;
   zg::RecDev rec_view{};
   zg::Tuning tuning;
@@ -113,9 +123,16 @@ unsigned zg_host_threads();
 
 namespace zg {
 // Builds the device copy of idx (banks, column tables, recovery directory).
-void upload_index(zg_index *idx, uint64_t budget);
+void upload_index(zg_index *idx, uint64_t budget, uint32_t memory);
 // Device bytes held by the resident index.
 uint64_t resident_bytes(const zg_index *idx);
+// Makes bank i's payload readable by lookups enqueued next on stream s
+// (streams it in when the bank is not resident).
+void make_resident(zg_index *idx, size_t i, hipStream_t s);
+// Some bank streams per query (subindex mode beyond the budget).
+bool any_streamed(const zg_index *idx);
+// Bank payload in host memory, decoded from the bank file on first use.
+const std::vector<uint8_t> &host_payload(zg_index *idx, size_t bank);
 // Batch pipeline pieces, all enqueued on b->stream (zg_api.hip).
 void configure(zg_index *idx, zg_batch *b);
 void seed_batch(zg_index *idx, zg_batch *b);

@@ -293,6 +293,9 @@ extern "C" int zg_server_create(zg_index *idx, const zg_server_config *cfg, zg_s
     ensure(idx && out, "null argument", ZG_ERR_INVALID);
     ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
            ZG_ERR_INVALID);
+    ensure(!any_streamed(idx),
+           "a server needs every subindex resident (load with ZG_MEMORY_ALL or a larger budget)",
+           ZG_ERR_BUDGET);
     zg_server_config c;
     zg_server_config_default(&c);
     if (cfg) c = *cfg;
