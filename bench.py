@@ -84,6 +84,8 @@ def parse_args():
                         "first is the headline")
     p.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
     p.add_argument("--gpu-peel", type=int, default=0, help="build the index with the GPU peel")
+    p.add_argument("--disk-load", default="",
+                   help="directory to write the index to and time loading it back from")
     p.add_argument("--gather-sweep", action="store_true",
                    help="also measure random-row gather rates at 512..8192 B rows")
     p.add_argument("--sweep", default="1:2,1:3,1:8,4:2,4:3,4:8,16:2,16:3,16:8,32:2,32:3,32:8",
@@ -516,7 +518,24 @@ def main() -> int:
     meta = idx.meta
     t0 = time.time()
     idx.load()
-    log(f"index resident in HBM ({meta['payload_bytes'] / 2**30:.2f} GiB) in {time.time() - t0:.1f}s")
+    load_s = time.time() - t0
+    log(f"index resident in HBM ({meta['payload_bytes'] / 2**30:.2f} GiB) in {load_s:.1f}s")
+    load = {"bank_bytes": meta["payload_bytes"], "from_host_s": round(load_s, 3)}
+    if args.disk_load:
+        # the reference's load phase: bank files -> memory (here, decoded
+        # frame by frame into page-locked chunks copied to HBM)
+        import tempfile
+        with tempfile.TemporaryDirectory(dir=args.disk_load) as d:
+            t0 = time.time()
+            idx.write(os.path.join(d, "idx"))
+            load["write_s"] = round(time.time() - t0, 3)
+            load["file_bytes"] = sum(f.stat().st_size for f in Path(d, "idx").iterdir())
+            t0 = time.time()
+            with zindex_amd.Index.open(os.path.join(d, "idx")) as z:
+                z.load()
+                load["from_disk_s"] = round(time.time() - t0, 3)
+        load["from_disk_gbs"] = round(meta["payload_bytes"] / load["from_disk_s"] / 1e9, 2)
+        log(f"load: {load}")
 
     tuning = idx.tuning
     if args.tuning:
@@ -637,6 +656,7 @@ def main() -> int:
         "speedup_vs_plain": (plain["batched_kmers_per_s_same_reads"] / plain["plain_kmers_per_s"])
         if plain else None,
         "parity_sample": {"reads": n_par, "bit_exact": parity} if parity is not None else None,
+        "load": load,
     }
     print(json.dumps(out))
     return 0

@@ -2,7 +2,10 @@
 // the reference on-disk format.
 #include "zg_format.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -386,6 +389,8 @@ std::string read_file(const std::string &path) {
   FILE *f = fopen(path.c_str(), "rb");
   if (!f) fail(-2, "opening " + path + ": " + strerror(errno));
   std::string out;
+  struct stat st;
+  if (fstat(fileno(f), &st) == 0 && S_ISREG(st.st_mode)) out.reserve((size_t)st.st_size);
   char buf[1 << 16];
   size_t n;
   while ((n = fread(buf, 1, sizeof buf, f)) > 0) out.append(buf, n);
@@ -412,11 +417,15 @@ Manifest load_manifest(const std::string &dir) {
 
 // ---- zstd ---------------------------------------------------------------------
 std::string zstd_decompress_all(const std::string &compressed) {
+  return zstd_decompress_all(compressed.data(), compressed.size());
+}
+
+std::string zstd_decompress_all(const void *data, size_t n) {
   ZSTD_DCtx *d = ZSTD_createDCtx();
   if (!d) fail(-5, "zstd: out of memory");
   std::string out;
   std::vector<char> buf(ZSTD_DStreamOutSize());
-  ZSTD_inBuffer in{compressed.data(), compressed.size(), 0};
+  ZSTD_inBuffer in{data, n, 0};
   // Decode until all input is consumed and the last call left output room
   // (nothing buffered); a final return of 0 means the last frame completed.
   size_t last = 1;
@@ -481,10 +490,8 @@ struct Frame {
 };
 
 // bank_io::table (bank_io.rs:122-197). Returns false when no table exists.
-bool read_table(const std::string &file, uint64_t raw_total, std::vector<Frame> &frames) {
-  uint64_t length = file.size();
+bool read_table(const uint8_t *p, uint64_t length, uint64_t raw_total, std::vector<Frame> &frames) {
   if (length < 16) return false;
-  const uint8_t *p = (const uint8_t *)file.data();
   if (memcmp(p + length - 16, "ZORSEEK1", 8) != 0) return false;
   uint64_t len = le64(p + length - 8);
   ensure(len >= 40 && len <= kMaxTable && (len - 40) % 32 == 0 && len + 8 <= length,
@@ -549,42 +556,27 @@ void parallel_for(size_t n, int threads, F &&f) {
 
 std::vector<uint8_t> load_bank_payload(const std::string &dir, const BankMeta &meta,
                                        int threads) {
-  uint64_t expected = meta.payload_bytes();
-  std::string path = path_join(dir, meta.file);
-  std::string file = read_file(path);
-  std::vector<Frame> frames;
+  const uint64_t expected = meta.payload_bytes();
+  BankFrames bf = open_bank_frames(dir, meta);
   std::vector<uint8_t> data;
-  if (read_table(file, expected + kHeader, frames) && frames.size() > 1) {
+  if (!bf.frames.empty()) {
     data.assign(expected, 0);
     uint8_t header[kHeader];
-    parallel_for(frames.size(), threads, [&](size_t i) {
-      const Frame &f = frames[i];
-      const char *src = file.data() + f.compressed;
-      ensure(ZSTD_findFrameCompressedSize(src, f.compressed_len) == f.compressed_len,
-             "frame contains trailing data", -2);
-      ensure(ZSTD_getFrameContentSize(src, f.compressed_len) == f.raw_len,
-             "frame content size differs from seek table", -2);
-      std::vector<uint8_t> tmp;
-      uint8_t *dst;
-      if (i == 0) {
-        tmp.resize(f.raw_len);
-        dst = tmp.data();
-      } else {
-        dst = data.data() + (f.raw - kHeader);
-      }
-      ZSTD_DCtx *d = ZSTD_createDCtx();
-      size_t r = ZSTD_decompressDCtx(d, dst, f.raw_len, src, f.compressed_len);
-      ZSTD_freeDCtx(d);
-      ensure(!ZSTD_isError(r) && r == f.raw_len, "short bank frame", -2);
-      if (i == 0) {
+    parallel_for(bf.frames.size(), threads, [&](size_t i) {
+      const BankFrames::Frame &f = bf.frames[i];
+      if (i == 0) {  // the raw stream opens with the header
+        std::vector<uint8_t> tmp(f.raw_len);
+        decode_bank_frame(bf, 0, tmp.data());
         memcpy(header, tmp.data(), kHeader);
         memcpy(data.data(), tmp.data() + kHeader, f.raw_len - kHeader);
+      } else {
+        decode_bank_frame(bf, i, data.data() + (f.raw - kHeader));
       }
     });
     validate_header(header, meta, expected);
     return data;
   }
-  std::string raw = zstd_decompress_all(file);
+  std::string raw = zstd_decompress_all(bf.data, bf.size);
   ensure(raw.size() >= kHeader, "truncated bank", -2);
   validate_header((const uint8_t *)raw.data(), meta, expected);
   ensure(raw.size() - kHeader >= expected, "truncated bank", -2);
@@ -595,16 +587,37 @@ std::vector<uint8_t> load_bank_payload(const std::string &dir, const BankMeta &m
 
 BankFrames open_bank_frames(const std::string &dir, const BankMeta &meta) {
   BankFrames b;
-  b.file = read_file(path_join(dir, meta.file));
+  const std::string path = path_join(dir, meta.file);
+  int fd = open(path.c_str(), O_RDONLY);
+  if (fd < 0) fail(-2, "opening " + path + ": " + strerror(errno));
+  struct stat st;
+  if (fstat(fd, &st) != 0) {
+    close(fd);
+    fail(-2, "reading " + path + ": " + strerror(errno));
+  }
+  b.size = (size_t)st.st_size;
+  if (b.size) {
+    void *p = mmap(nullptr, b.size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (p == MAP_FAILED) {
+      close(fd);
+      fail(-2, "mapping " + path + ": " + strerror(errno));
+    }
+    b.data = (const uint8_t *)p;
+  }
+  close(fd);
   std::vector<Frame> frames;
-  if (read_table(b.file, meta.payload_bytes() + kHeader, frames) && frames.size() > 1)
+  if (read_table(b.data, b.size, meta.payload_bytes() + kHeader, frames) && frames.size() > 1)
     for (const Frame &f : frames) b.frames.push_back({f.compressed, f.compressed_len, f.raw, f.raw_len});
   return b;
 }
 
+BankFrames::~BankFrames() {
+  if (data) munmap((void *)data, size);
+}
+
 void decode_bank_frame(const BankFrames &b, size_t i, uint8_t *dst) {
   const BankFrames::Frame &f = b.frames[i];
-  const char *src = b.file.data() + f.compressed;
+  const char *src = (const char *)b.data + f.compressed;
   ensure(ZSTD_findFrameCompressedSize(src, f.compressed_len) == f.compressed_len,
          "frame contains trailing data", -2);
   ensure(ZSTD_getFrameContentSize(src, f.compressed_len) == f.raw_len,

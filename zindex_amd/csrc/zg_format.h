@@ -114,6 +114,7 @@ std::string path_join(const std::string &dir, const std::string &name);
 
 // zstd helpers
 std::string zstd_decompress_all(const std::string &compressed);  // any frames
+std::string zstd_decompress_all(const void *data, size_t n);
 std::string zstd_compress(const void *data, size_t n, int level, bool checksum);
 
 // Bank payload: load_modern_bank (storage.rs:2232-2256) with
@@ -130,8 +131,17 @@ struct BankFrames {
   struct Frame {
     uint64_t compressed, compressed_len, raw, raw_len;
   };
-  std::string file;
+  const uint8_t *data = nullptr;  // the bank file, memory-mapped read-only
+  size_t size = 0;
   std::vector<Frame> frames;
+  BankFrames() = default;
+  BankFrames(const BankFrames &) = delete;
+  BankFrames &operator=(const BankFrames &) = delete;
+  BankFrames(BankFrames &&o) noexcept : data(o.data), size(o.size), frames(std::move(o.frames)) {
+    o.data = nullptr;
+    o.size = 0;
+  }
+  ~BankFrames();
 };
 BankFrames open_bank_frames(const std::string &dir, const BankMeta &meta);
 // Decodes frame i (raw_len bytes) into dst; frame 0 starts with the header.
