@@ -192,15 +192,37 @@ constexpr uint32_t kOcmSpan = kOcmTile + 1 + 256;  // window starts - 1 .. + k (
 
 uint32_t ocm_block_positions() { return kOcmTile; }
 
+// An anchor's rank with the leftmost and rightmost local index holding it;
+// combining two ranges keeps the smaller (class, hash) and, on a tie, the
+// union of their index spans (ScalarRank::combine with first/last ties,
+// implementation.rs:266-346).
+struct OcmMin {
+  uint64_t h;
+  uint32_t c, first, last;
+};
+__device__ __forceinline__ OcmMin ocm_combine(const OcmMin &a, const OcmMin &b) {
+  if (a.c < b.c || (a.c == b.c && a.h < b.h)) return a;
+  if (b.c < a.c || (b.c == a.c && b.h < a.h)) return b;
+  return {a.h, a.c, a.first < b.first ? a.first : b.first, a.last > b.last ? a.last : b.last};
+}
+
+// Open-closed mod-minimizer selection for kOcmTile window starts per block.
+// Per window in O(1): invalid-base and GT counts come from 64-position ballot
+// masks with popcount prefixes; the minimum anchor over the window's
+// W = k - t + 1 anchors comes from a sparse table of range minima (two
+// overlapping power-of-two ranges cover the window).
 __global__ __launch_bounds__(256) void ocm_select_kernel(
     const uint64_t *codes, const uint64_t *invalid, uint64_t positions, uint32_t k,
     uint32_t m, uint64_t seed, const uint64_t *inner_table, uint8_t *flag, uint8_t *delta,
     uint32_t *block_counts) {
+  constexpr uint32_t kWords = (kOcmSpan + 63) / 64 + 1;
   __shared__ uint8_t code_s[kOcmSpan];
-  __shared__ uint8_t bad_s[kOcmSpan];
+  __shared__ uint64_t bad_m[kWords], gt_m[kWords];
+  __shared__ uint32_t bad_c[kWords], gt_c[kWords];
   __shared__ uint64_t ih_s[kOcmSpan];
-  __shared__ uint64_t ah_s[kOcmSpan];
-  __shared__ uint8_t ac_s[kOcmSpan];
+  __shared__ uint64_t th[2][kOcmSpan];
+  __shared__ uint8_t tc[2][kOcmSpan];
+  __shared__ uint16_t tf[2][kOcmSpan], tl[2][kOcmSpan];
   __shared__ uint32_t pos_s[kOcmTile + 1];
   __shared__ uint8_t ok_s[kOcmTile + 1];
   __shared__ uint32_t wave_counts[4];
@@ -210,19 +232,41 @@ __global__ __launch_bounds__(256) void ocm_select_kernel(
   const int64_t base = (int64_t)S0 - 1;  // position of local index 0
   const uint32_t span = kOcmTile + k;      // local positions needed: base .. base+span-1
   const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63;
+  const uint32_t words = (span + 63) / 64;
 
-  for (uint32_t i = tid; i < span; i += blockDim.x) {
-    int64_t p = base + (int64_t)i;
-    uint8_t c = 0, bad = 1;
-    if (p >= 0 && (uint64_t)p < positions) {
-      uint64_t up = (uint64_t)p;
+  // codes, and the invalid / GT bits of 64 positions per ballot (positions
+  // outside the batch count as invalid)
+  for (uint32_t i0 = tid & ~63u; i0 < words * 64; i0 += blockDim.x) {
+    const uint32_t i = i0 + lane;
+    const int64_t p = base + (int64_t)i;
+    uint8_t c = 0;
+    bool bad = true;
+    if (i < span && p >= 0 && (uint64_t)p < positions) {
+      const uint64_t up = (uint64_t)p;
       c = (uint8_t)((codes[up >> 5] >> (2 * (up & 31))) & 3);
-      bad = (uint8_t)((invalid[up >> 6] >> (up & 63)) & 1);
+      bad = (invalid[up >> 6] >> (up & 63)) & 1;
     }
-    code_s[i] = c;
-    bad_s[i] = bad;
+    if (i < span) code_s[i] = c;
+    const uint64_t bm = __ballot(bad), gm = __ballot(!bad && c >= 2);
+    if (lane == 0) {
+      bad_m[i0 >> 6] = bm;
+      gt_m[i0 >> 6] = gm;
+    }
   }
   __syncthreads();
+  if (tid == 0) {
+    uint32_t b = 0, g = 0;
+    for (uint32_t w = 0; w < words; w++) {
+      bad_c[w] = b;
+      gt_c[w] = g;
+      b += (uint32_t)__popcll(bad_m[w]);
+      g += (uint32_t)__popcll(gt_m[w]);
+    }
+    bad_c[words] = b;
+    gt_c[words] = g;
+    bad_m[words] = gt_m[words] = 0;
+  }
   const uint32_t s = (uint32_t)P.s, t = (uint32_t)P.t;
   for (uint32_t i = tid; i + s <= span; i += blockDim.x) {
     uint32_t x = 0;
@@ -230,52 +274,60 @@ __global__ __launch_bounds__(256) void ocm_select_kernel(
     ih_s[i] = inner_table[x];
   }
   __syncthreads();
-  const uint64_t tmask = (1ULL << (2 * t)) - 1;
-  for (uint32_t i = tid; i + t <= span; i += blockDim.x) {
+  // anchors: rank (class, mixsplit of the canonical t-mer), sparse-table level 0
+  const uint32_t n_anchor = span - t + 1;
+  for (uint32_t i = tid; i < n_anchor; i += blockDim.x) {
     uint64_t mn = ~0ULL;
     for (uint32_t j = 0; j + s <= t; j++) mn = ih_s[i + j] < mn ? ih_s[i + j] : mn;
     uint8_t cls;
     if (ih_s[i + (t - s) / 2] == mn) cls = 0;
     else if (ih_s[i] == mn || ih_s[i + t - s] == mn) cls = 1;
     else cls = 2;
-    uint64_t f = 0, r = 0;
-    for (uint32_t j = 0; j < t; j++) {
-      f = (f << 2) | code_s[i + j];
-      r |= (uint64_t)(code_s[i + j] ^ 3) << (2 * j);
-    }
-    f &= tmask;
-    ac_s[i] = cls;
-    ah_s[i] = mixsplit(f < r ? f : r, seed);
+    // the canonical t-mer straight from the packed codes (anchors reaching
+    // past the batch belong to no valid window)
+    const int64_t p = base + (int64_t)i;
+    const uint64_t canon = p >= 0 && (uint64_t)p + t <= positions
+                               ? canonical_from_window(stream_bits(codes, 2 * (uint64_t)p), (int)t)
+                               : 0;
+    th[0][i] = mixsplit(canon, seed);
+    tc[0][i] = cls;
+    tf[0][i] = tl[0][i] = (uint16_t)i;
   }
   __syncthreads();
+  // range minima over 2^j anchors, j = 1 .. L with 2^L <= W
+  const uint32_t W = k - t + 1;
+  uint32_t L = 0;
+  while ((2u << L) <= W) L++;
+  for (uint32_t j = 1; j <= L; j++) {
+    const uint32_t src = (j - 1) & 1, dst = j & 1, half = 1u << (j - 1);
+    for (uint32_t i = tid; i + (1u << j) <= n_anchor; i += blockDim.x) {
+      OcmMin a{th[src][i], tc[src][i], tf[src][i], tl[src][i]};
+      OcmMin b{th[src][i + half], tc[src][i + half], tf[src][i + half], tl[src][i + half]};
+      OcmMin c = ocm_combine(a, b);
+      th[dst][i] = c.h;
+      tc[dst][i] = (uint8_t)c.c;
+      tf[dst][i] = (uint16_t)c.first;
+      tl[dst][i] = (uint16_t)c.last;
+    }
+    __syncthreads();
+  }
+  const uint32_t top = L & 1;
+  auto ones = [&](const uint64_t *msk, const uint32_t *cum, uint32_t x) {
+    return cum[x >> 6] + (uint32_t)__popcll(msk[x >> 6] & ((1ULL << (x & 63)) - 1));
+  };
   // windows: local index w = 0 is start base (S0 - 1), w = 1 + tid is S0 + tid
   for (uint32_t w = tid; w < kOcmTile + 1; w += blockDim.x) {
-    int64_t start = base + (int64_t)w;
-    uint8_t ok = start >= 0 && (uint64_t)start + k <= positions;
-    uint32_t gt = 0;
-    if (ok) {
-      for (uint32_t j = 0; j < k; j++) {
-        ok &= !bad_s[w + j];
-        gt += code_s[w + j] >= 2;
-      }
-    }
+    const int64_t start = base + (int64_t)w;
+    bool ok = start >= 0 && (uint64_t)start + k <= positions &&
+              ones(bad_m, bad_c, w + k) == ones(bad_m, bad_c, w);
     uint32_t pos = 0;
     if (ok) {
-      uint8_t bc = 255;
-      uint64_t bh = 0;
-      uint32_t first = 0, last = 0;
-      for (uint32_t a = 0; a + t <= k; a++) {
-        uint8_t c = ac_s[w + a];
-        uint64_t h = ah_s[w + a];
-        if (c < bc || (c == bc && h < bh)) {
-          bc = c;
-          bh = h;
-          first = last = a;
-        } else if (c == bc && h == bh) {
-          last = a;
-        }
-      }
-      uint32_t anchor = gt > k / 2 ? last : first;
+      const uint32_t gt = ones(gt_m, gt_c, w + k) - ones(gt_m, gt_c, w);
+      const uint32_t j2 = w + W - (1u << L);
+      OcmMin a{th[top][w], tc[top][w], tf[top][w], tl[top][w]};
+      OcmMin b{th[top][j2], tc[top][j2], tf[top][j2], tl[top][j2]};
+      OcmMin c = ocm_combine(a, b);
+      const uint32_t anchor = (gt > k / 2 ? c.last : c.first) - w;
       pos = anchor % (uint32_t)P.w;
     }
     ok_s[w] = ok;
