@@ -326,7 +326,10 @@ def gather_ceiling(idx, algo_bytes):
     if rb not in (512, 1024, 2048, 4096, 8192):
         return None
     n = max(1 << 20, min(algo_bytes, 1 << 41) // rb)
-    return {"gather_row_bytes": rb, "gather_ceiling_gbs": round(idx.gather_probe(rb, n), 1)}
+    out = {"gather_row_bytes": rb, "gather_ceiling_gbs": round(idx.gather_probe(rb, n), 1)}
+    if rb <= 2048:  # the lookup's own row pattern: H rows of a key in H consecutive segments
+        out["key_rows_ceiling_gbs"] = round(idx.gather_probe_keys(n // man["hashes"]), 1)
+    return out
 
 
 def percentile(values, q):
@@ -589,8 +592,11 @@ def main() -> int:
     if ceiling:
         roofline.update(ceiling)
         roofline["frac_of_gather_ceiling"] = round(achieved / ceiling["gather_ceiling_gbs"], 4)
+        if "key_rows_ceiling_gbs" in ceiling:
+            roofline["frac_of_key_rows_ceiling"] = round(achieved / ceiling["key_rows_ceiling_gbs"], 4)
         log(f"gather ceiling at {ceiling['gather_row_bytes']} B rows: "
-            f"{ceiling['gather_ceiling_gbs']:.0f} GB/s; lookup at "
+            f"{ceiling['gather_ceiling_gbs']:.0f} GB/s (key-row pattern "
+            f"{ceiling.get('key_rows_ceiling_gbs')}); lookup at "
             f"{roofline['frac_of_gather_ceiling']:.1%} of it")
     if args.gather_sweep:
         roofline["gather_sweep_gbs"] = {rb: round(idx.gather_probe(rb, (1 << 40) // rb), 1)

@@ -339,6 +339,12 @@ int zg_server_destroy(zg_server *srv);
  * that row size, which the lookup's own rate is compared with. */
 int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
                     uint64_t n_rows, double *gbs);
+/* The same for the lookup's own row pattern: the H rows (H consecutive
+ * segments) of n_keys random keys under bank `bank`'s PositionHash geometry,
+ * one key per wave in flight, whole rows; banks of 512, 1024 or 2048-byte
+ * rows. GB/s of row bytes. */
+int zg_gather_probe_keys(zg_index *idx, uint32_t bank, uint64_t n_keys,
+                         double *gbs);
 
 /* Page-locked host memory for inputs and outputs of requests and batches
  * (DMA without staging copies). */

@@ -197,6 +197,12 @@ class Index:
         check(lib().zg_gather_probe(self._h, bank, row_bytes, n_rows, C.byref(gbs)))
         return gbs.value
 
+    def gather_probe_keys(self, n_keys: int, bank: int = 0) -> float:
+        """GB/s of the H rows of random keys under the bank's PositionHash geometry."""
+        gbs = C.c_double()
+        check(lib().zg_gather_probe_keys(self._h, bank, n_keys, C.byref(gbs)))
+        return gbs.value
+
     @property
     def tuning(self) -> dict:
         t = Tuning()

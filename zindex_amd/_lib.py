@@ -148,6 +148,8 @@ SIGNATURES = {
     "zg_server_destroy": (C.c_int, [C.c_void_p]),
     "zg_gather_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64,
                                   C.POINTER(C.c_double)]),
+    "zg_gather_probe_keys": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64,
+                                       C.POINTER(C.c_double)]),
     "zg_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
     "zg_host_free": (C.c_int, [C.c_void_p]),
 }

@@ -1193,6 +1193,41 @@ extern "C" int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
   }
 }
 
+extern "C" int zg_gather_probe_keys(zg_index *idx, uint32_t bank, uint64_t n_keys, double *gbs) {
+  try {
+    ensure(idx && gbs, "null argument", ZG_ERR_INVALID);
+    ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
+           ZG_ERR_INVALID);
+    ensure(bank < idx->dbanks.size(), "bank out of range", ZG_ERR_INVALID);
+    ensure(n_keys > 0, "empty probe", ZG_ERR_INVALID);
+    make_resident(idx, bank, idx->stream);
+    const BankDev &v = idx->dbanks[bank].view;
+    hipStream_t s = idx->stream;
+    DevBuf sink;
+    sink.alloc(16);
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    const uint32_t blocks = idx->cu_count * 8;
+    bool ok = launch_gather_keys(v.data, v.row_bytes, v.geo, v.hashes, n_keys / 8, blocks,
+                                 sink.as<uint32_t>(), s);  // warm-up
+    if (ok) {
+      check_hip(hipEventRecord(e0, s), "hipEventRecord");
+      launch_gather_keys(v.data, v.row_bytes, v.geo, v.hashes, n_keys, blocks, sink.as<uint32_t>(), s);
+      check_hip(hipEventRecord(e1, s), "hipEventRecord");
+    }
+    check_hip(hipStreamSynchronize(s), "gather probe");
+    double ms = ok ? elapsed_ms(e0, e1) : -1;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ensure(ok, "key probe needs rows of 512, 1024 or 2048 bytes", ZG_ERR_INVALID);
+    *gbs = ms > 0 ? (double)n_keys * v.hashes * v.row_bytes / (ms * 1e6) : 0.0;
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-buffer queries
 // ---------------------------------------------------------------------------

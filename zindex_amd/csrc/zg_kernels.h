@@ -124,6 +124,10 @@ int lookup_blocks_per_cu(const KernelConfig &c);
 // data[0, bytes); false for an unsupported row size.
 bool launch_gather_probe(const uint8_t *data, uint64_t bytes, uint32_t row_bytes, uint64_t n,
                          uint32_t blocks, uint32_t *sink, hipStream_t s);
+// The H rows of n_keys random keys under a bank's PositionHash geometry
+// (whole rows of row_bytes 512, 1024 or 2048); false for other row sizes.
+bool launch_gather_keys(const uint8_t *data, uint64_t row_bytes, SegmentGeometry geo, uint32_t H,
+                        uint64_t n_keys, uint32_t blocks, uint32_t *sink, hipStream_t s);
 
 }  // namespace zg
 

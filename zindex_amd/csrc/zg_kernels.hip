@@ -957,6 +957,75 @@ __global__ __launch_bounds__(256) void gather_probe_kernel(const uint8_t *data, 
   if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads live
 }
 
+// The rows of random keys under the bank's PositionHash geometry (H rows in H
+// consecutive segments per key, storage.rs:3452-3481), KPW keys per wave in
+// flight, no other work: the ceiling of the lookup's own row pattern.
+template <int LB, int CH, int KPW>
+__global__ __launch_bounds__(256) void gather_keys_kernel(const uint8_t *data, uint64_t row_bytes,
+                                                          SegmentGeometry geo, uint32_t H,
+                                                          uint64_t n_keys, uint64_t seed,
+                                                          uint32_t *sink) {
+  constexpr int HB = 8;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t waves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  uint32_t acc = 0;
+  for (uint64_t i0 = wave * KPW; i0 < n_keys; i0 += waves * KPW) {
+    for (uint32_t h0 = 0; h0 < H; h0 += HB) {
+      uint32_t v[KPW][HB][CH][LB / 4];
+#pragma unroll
+      for (int q = 0; q < KPW; q++) {
+        const uint64_t first = splitmix64((i0 + q) ^ seed), step = splitmix64(first);
+        const uint64_t base = slot_base(first, geo);
+#pragma unroll
+        for (int hh = 0; hh < HB; hh++) {
+          const uint32_t h = h0 + hh;
+          const bool live = i0 + q < n_keys && h < H;
+          const uint64_t slot = slot_at(base, first + (uint64_t)h * step, h, geo);
+          const uint8_t *p = data + slot * row_bytes + lane * LB;
+#pragma unroll
+          for (int c = 0; c < CH; c++) {
+            if (live) {
+              if constexpr (LB == 16) {
+                u32x4 x = __builtin_nontemporal_load((const u32x4 *)(p + c * 64 * LB));
+                v[q][hh][c][0] = x.x; v[q][hh][c][1] = x.y; v[q][hh][c][2] = x.z; v[q][hh][c][3] = x.w;
+              } else {
+                u32x2 x = __builtin_nontemporal_load((const u32x2 *)(p + c * 64 * LB));
+                v[q][hh][c][0] = x.x; v[q][hh][c][1] = x.y;
+              }
+            } else {
+#pragma unroll
+              for (int w = 0; w < LB / 4; w++) v[q][hh][c][w] = 0;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KPW; q++)
+#pragma unroll
+        for (int hh = 0; hh < HB; hh++)
+#pragma unroll
+          for (int c = 0; c < CH; c++)
+#pragma unroll
+            for (int w = 0; w < LB / 4; w++) acc ^= v[q][hh][c][w];
+    }
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the loads live
+}
+
+bool launch_gather_keys(const uint8_t *data, uint64_t row_bytes, SegmentGeometry geo, uint32_t H,
+                        uint64_t n_keys, uint32_t blocks, uint32_t *sink, hipStream_t s) {
+  const uint64_t seed = 0x2545F4914F6CDD1DULL;
+  switch (row_bytes) {
+    case 512: gather_keys_kernel<8, 1, 1><<<blocks, 256, 0, s>>>(data, row_bytes, geo, H, n_keys, seed, sink); return true;
+    case 1024: gather_keys_kernel<16, 1, 1><<<blocks, 256, 0, s>>>(data, row_bytes, geo, H, n_keys, seed, sink); return true;
+    case 2048: gather_keys_kernel<16, 2, 1><<<blocks, 256, 0, s>>>(data, row_bytes, geo, H, n_keys, seed, sink); return true;
+    default: return false;
+  }
+}
+
 template <int LB, int CH>
 static void launch_probe(const uint8_t *data, uint64_t rows, uint64_t n, uint32_t blocks,
                          uint32_t *sink, hipStream_t s) {
