@@ -201,6 +201,13 @@ void zg_build_config_default(zg_build_config *cfg);
 int zg_build_command(const zg_build_config *cfg, const char *fof,
                      const char *out_dir, int err_fd);
 
+/* fuse_many_indexes (src/power_merge/fusion.rs:233-480): concatenates two
+ * or more compatible indexes into out_dir (which must not exist) without
+ * rereading their datasets: banks copied as components 1..n, dataset ids of
+ * later inputs offset, recovery maps merged, manifest version 10. */
+int zg_fusion_command(const char *const *inputs, uint32_t n_inputs,
+                      const char *out_dir, int err_fd);
+
 typedef struct zg_builder zg_builder;
 int zg_builder_create(const zg_build_config *cfg, zg_builder **out);
 /* Adds one dataset: records are seq[rec_off[i] .. rec_off[i+1]). The dataset

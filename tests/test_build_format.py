@@ -156,6 +156,62 @@ def test_legacy_interleaved_layout_same_answers(tmp_path, small_colors, bits, gr
     assert cp.sum() > 0
 
 
+def test_fusion_concatenates_indexes(tmp_path, small_colors):
+    """fuse_many_indexes (fusion.rs:233-480): components 1..n, dataset ids of
+    later inputs offset, recovery maps merged. The oracle's counts on the fused
+    index are each input's counts side by side; the CLI and the product's
+    validation accept the result."""
+    parts = [small_colors[:2], small_colors[2:4], small_colors[4:]]
+    dirs = []
+    for i, cols in enumerate(parts):
+        idx = _build(cols, fingerprint_bits=8, hashes=4, slot_scale=0.8,
+                     merge_group_size=2 if i == 1 else 1)
+        d = tmp_path / f"in{i}"
+        idx.write(str(d))
+        dirs.append(d)
+    out = tmp_path / "fused"
+    zindex_amd.fuse_many_indexes(dirs, out)
+    with zindex_amd.Index.open(str(out)) as z:
+        assert z.meta["version"] == 10 and z.meta["n_colors"] == len(small_colors)
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert [b["component"] for b in man["banks"]] == [1, 2, 3]
+    assert all(b["file"].startswith(f"component-{b['component']}-") for b in man["banks"])
+    assert man["source_document_ids"] == list(range(len(small_colors)))
+    fused = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    rng = np.random.default_rng(4)
+    reads = [c[i:i + 700] for c in small_colors for i in range(0, len(c) - 700, 4000)]
+    reads += [_random_genome(rng, 700) for _ in range(8)]
+    buf, off = zindex_amd.pack_reads(reads)
+    tf, cf = fused.query_reads(buf, off, threads=zr.cpu_count())
+    col = 0
+    for d, cols in zip(dirs, parts):
+        di = zr.read_index_dir(d)
+        t, c = zr.OracleIndex(di["manifest"], di["banks"], di["recovery"]).query_reads(
+            buf, off, threads=zr.cpu_count())
+        assert np.array_equal(t, tf)
+        assert np.array_equal(c, cf[:, col:col + len(cols)])
+        col += len(cols)
+    # CLI, and refusal to overwrite
+    cli = str(ROOT / "zindex_amd" / "zorindex")
+    r = subprocess.run([cli, "fusion", "-i", str(dirs[0]), str(dirs[1]), "-o",
+                        str(tmp_path / "f2")], capture_output=True)
+    assert r.returncode == 0, r.stderr
+    with zindex_amd.Index.open(str(tmp_path / "f2")) as z:
+        assert z.meta["n_colors"] == 4
+    with pytest.raises(zindex_amd.ZindexError, match="already exists"):
+        zindex_amd.fuse_indexes(dirs[0], dirs[1], out)
+
+
+def test_fusion_rejects_incompatible_indexes(tmp_path, small_colors):
+    a = _build(small_colors[:2], fingerprint_bits=8, hashes=4)
+    b = _build(small_colors[2:4], fingerprint_bits=4, hashes=4)
+    a.write(str(tmp_path / "a"))
+    b.write(str(tmp_path / "b"))
+    with pytest.raises(zindex_amd.ZindexError, match="fingerprint_bits"):
+        zindex_amd.fuse_indexes(tmp_path / "a", tmp_path / "b", tmp_path / "out")
+
+
 def test_partition_cap_too_small_is_rejected(small_colors):
     with pytest.raises(zindex_amd.ZindexError, match="too small"):
         _build(small_colors[:2], fingerprint_bits=8, hashes=4, max_subindex_size=1 << 8)

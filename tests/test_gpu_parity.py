@@ -416,3 +416,32 @@ def test_subindex_mode_streams_banks_beyond_budget(tmp_path):
                             "--hbm-budget", str(budget)] + extra,
                            capture_output=True, check=True).stdout
         assert a == b and len(a) > 0
+
+
+def test_fused_index_components(tmp_path):
+    """Version-10 fused indexes (fusion.rs:233-480): each input a component,
+    dataset ids offset, recovery maps merged. GPU counts equal the oracle's
+    and each input's own GPU counts side by side."""
+    colors = make_colors(160, 9, 30000)
+    reads = make_reads(161, colors, 200)
+    parts = [colors[:3], colors[3:6], colors[6:]]
+    dirs, alone = [], []
+    for i, cols in enumerate(parts):
+        idx = build(cols, fingerprint_bits=8, hashes=4, slot_scale=0.75,
+                    merge_group_size=3 if i == 2 else 1)
+        d = tmp_path / f"in{i}"
+        idx.write(str(d))
+        dirs.append(d)
+        idx.load()
+        alone.append(idx.query_reads(reads, records=True))
+    out = tmp_path / "fused"
+    zindex_amd.fuse_many_indexes(dirs, out)
+    disk = zr.read_index_dir(out)
+    orc = zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"])
+    with zindex_amd.Index.open(str(out)) as z:
+        assert z.meta["version"] == 10
+        z.load()
+        check_parity(z, orc, reads, label="fused")
+        t, c, _ = z.query_reads(reads, records=True)
+        assert np.array_equal(c, np.concatenate([a[1] for a in alone], axis=1))
+        assert all(np.array_equal(t, a[0]) for a in alone)

@@ -21,7 +21,7 @@ __all__ = [
     "Index", "Builder", "Batch", "Server", "HostBuffer", "ZindexError", "build", "build_config",
     "query_command", "query_command_with_memory", "query_command_with_recovery",
     "query_records_command", "query_records_command_with_memory", "info_command",
-    "pack_reads",
+    "pack_reads", "fuse_indexes", "fuse_many_indexes",
 ]
 
 QUERY_MEMORY = ("all", "subindex")  # ZG_MEMORY_ALL, ZG_MEMORY_SUBINDEX
@@ -416,6 +416,16 @@ def build(fof: str, output: str, err_fd: int = 2, **config) -> None:
     """zorindex build --fof FOF --output OUT (naive planner, width column order)."""
     cfg = build_config(**config)
     check(lib().zg_build_command(C.byref(cfg), str(fof).encode(), str(output).encode(), err_fd))
+
+
+def fuse_many_indexes(inputs, output: str, err_fd: int = 2) -> None:
+    """fusion.rs:233-480: concatenate compatible indexes into `output`."""
+    arr = (C.c_char_p * len(inputs))(*[str(p).encode() for p in inputs])
+    check(lib().zg_fusion_command(arr, len(inputs), str(output).encode(), err_fd))
+
+
+def fuse_indexes(first: str, second: str, output: str, err_fd: int = 2) -> None:
+    fuse_many_indexes([first, second], output, err_fd)
 
 
 def query_command_with_memory(directory: str, query: str, min_ratio: float = 0.0,

@@ -121,6 +121,7 @@ SIGNATURES = {
                                               C.c_int, C.c_int]),
     "zg_info_command": (C.c_int, [C.c_char_p, C.c_int]),
     "zg_build_config_default": (None, [C.POINTER(BuildConfig)]),
+    "zg_fusion_command": (C.c_int, [C.POINTER(C.c_char_p), C.c_uint32, C.c_char_p, C.c_int]),
     "zg_build_command": (C.c_int, [C.POINTER(BuildConfig), C.c_char_p, C.c_char_p, C.c_int]),
     "zg_builder_create": (C.c_int, [C.POINTER(BuildConfig), C.POINTER(C.c_void_p)]),
     "zg_builder_add_color": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, u64p, C.c_uint64]),

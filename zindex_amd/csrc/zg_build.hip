@@ -1210,6 +1210,208 @@ static std::vector<std::pair<std::string, std::string>> read_fof(const std::stri
   return out;
 }
 
+// fuse_many_indexes (src/power_merge/fusion.rs:233-480): concatenates
+// compatible indexes without rereading their datasets. Bank files are copied
+// under component-c- names (components numbered from 1 in input order),
+// dataset ids of later inputs are offset, recovery maps are merged by hash
+// (equal hashes keep input order, so origins stay sorted), and the manifest
+// becomes version 10.
+extern "C" int zg_fusion_command(const char *const *inputs, uint32_t n_inputs,
+                                 const char *out_dir, int err_fd) {
+  std::vector<zg_index *> opened;
+  auto close_all = [&] {
+    for (zg_index *x : opened) zg_index_close(x);
+    opened.clear();
+  };
+  try {
+    ensure(inputs && out_dir, "null argument", -1);
+    ensure(n_inputs >= 2, "fusion requires at least two input indexes", -1);
+    struct stat st;
+    if (stat(out_dir, &st) == 0)
+      fail(-1, std::string("output ") + out_dir + " already exists, choose a new directory");
+    auto start = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < n_inputs; i++) {
+      ensure(inputs[i] != nullptr, "null input", -1);
+      zg_index *x = nullptr;
+      int rc = zg_index_open(inputs[i], &x);
+      if (rc) throw Error(rc, std::string("reading input index ") + inputs[i] + ": " + zg_last_error());
+      opened.push_back(x);
+    }
+    const Manifest &m0 = opened[0]->m;
+    for (zg_index *x : opened) {
+      const Manifest &m = x->m;
+      ensure(m.version >= 2, "fusion requires separate recovery files", -1);
+      auto same = [&](bool eq, const char *field) {
+        if (!eq) fail(-1, std::string("incompatible indexes: ") + field + " differs");
+      };
+      same(m.backend == m0.backend, "backend");
+      same(m.k == m0.k, "k");
+      same(m.index_mode == m0.index_mode, "index_mode");
+      same(m.minimizer_size == m0.minimizer_size, "minimizer_size");
+      same(m.minimizer_scheme == m0.minimizer_scheme, "minimizer_scheme");
+      same(m.homopolymer_compression == m0.homopolymer_compression, "homopolymer_compression");
+      for (const char *f : {"seed_pattern", "randstrobemers", "randstrobe_scheme"}) {
+        std::string a, b;
+        if (const Json *v = m.json.get(f)) json_write(a, *v);
+        if (const Json *v = m0.json.get(f)) json_write(b, *v);
+        same(a == b, f);
+      }
+      same(m.modimizer_sampling == m0.modimizer_sampling, "modimizer_sampling");
+      same(m.findere_z == m0.findere_z, "findere_z");
+      same(m.hashes == m0.hashes, "hashes");
+      same(m.seed == m0.seed, "seed");
+      same(m.fingerprint_bits == m0.fingerprint_bits, "fingerprint_bits");
+      same(m.has_segment_length == m0.has_segment_length &&
+               (!m.has_segment_length || m.segment_length == m0.segment_length),
+           "segment_length");
+      same(m.abandoned_recovery == m0.abandoned_recovery, "abandoned_recovery");
+    }
+    std::string stage = std::string(out_dir) + ".staging";
+    ensure(mkdir(stage.c_str(), 0755) == 0, "cannot create staging directory", -2);
+    Json out = m0.json;
+    Json banks = Json::array(), colors = Json::array(), source_ids = Json::array(),
+         skipped = Json::array();
+    uint64_t offset = 0, source_offset = 0;
+    uint64_t component = 0;
+    std::vector<uint64_t> offsets;
+    for (zg_index *x : opened) {
+      const Manifest &m = x->m;
+      offsets.push_back(offset);
+      int64_t previous = -1;
+      const Json *jb = m.json.get("banks");
+      for (size_t bi = 0; bi < m.banks.size(); bi++) {
+        BankMeta meta = m.banks[bi];
+        if ((int64_t)meta.component != previous) {
+          component++;
+          previous = (int64_t)meta.component;
+        }
+        meta.component = component;
+        const std::string name = meta.expected_filename();
+        write_file(path_join(stage, name), read_file(path_join(x->dir, m.banks[bi].file)));
+        Json b = jb->items[bi];
+        Json nb = Json::object();
+        nb.set("component", Json::u64(component));
+        for (auto &f : b.fields)
+          if (f.first != "component") nb.set(f.first, f.second);
+        nb.set("file", Json::str(name));
+        Json &groups = nb.set("groups", nb.get("groups") ? *nb.get("groups") : Json::array());
+        for (Json &g : groups.items) {
+          Json mem = Json::array();
+          for (const Json &id : g.get("members")->items)
+            mem.push(Json::u64(std::stoull(id.text) + offset));
+          g.set("members", std::move(mem));
+        }
+        banks.push(std::move(nb));
+      }
+      if (const Json *c = m.json.get("colors"))
+        for (const Json &cj : c->items) colors.push(cj);
+      const Json *ids = m.json.get("source_document_ids");
+      if (ids && ids->kind == Json::Array && !ids->items.empty()) {
+        for (const Json &id : ids->items) source_ids.push(Json::u64(std::stoull(id.text) + source_offset));
+      } else {
+        for (size_t c = 0; c < m.colors.size(); c++) source_ids.push(Json::u64(c + source_offset));
+      }
+      if (const Json *sk = m.json.get("skipped_documents"))
+        for (const Json &e : sk->items) {
+          Json ne = e;
+          if (const Json *sid = e.get("source_document_id"))
+            ne.set("source_document_id", Json::u64(std::stoull(sid->text) + source_offset));
+          skipped.push(std::move(ne));
+        }
+      uint64_t docs = m.colors.size();
+      if (const Json *idc = m.json.get("input_document_count"))
+        docs = std::max<uint64_t>(docs, std::stoull(idc->text));
+      offset += m.colors.size();
+      source_offset += docs;
+      ensure(offset <= UINT32_MAX, "dataset IDs must fit in 32 bits", -1);
+    }
+    // recovery maps merged by hash
+    uint64_t rec_hashes = 0, rec_origins = 0;
+    if (m0.abandoned_recovery) {
+      std::vector<RecoveryMap> maps;
+      for (zg_index *x : opened) maps.push_back(read_recovery_map(x->dir, x->m));
+      RecoveryMap merged;
+      std::vector<size_t> at(maps.size(), 0);
+      while (true) {
+        uint64_t best = UINT64_MAX;
+        for (size_t i = 0; i < maps.size(); i++)
+          if (at[i] < maps[i].size()) best = std::min<uint64_t>(best, maps[i].hashes[at[i]]);
+        if (best == UINT64_MAX) break;
+        for (size_t i = 0; i < maps.size(); i++)
+          if (at[i] < maps[i].size() && maps[i].hashes[at[i]] == best) {
+            for (uint64_t j = maps[i].off[at[i]]; j < maps[i].off[at[i] + 1]; j++)
+              merged.origins.push_back((uint32_t)(maps[i].origins[j] + offsets[i]));
+            at[i]++;
+          }
+        merged.hashes.push_back((uint32_t)best);
+        merged.off.push_back(merged.origins.size());
+      }
+      write_recovery_map(path_join(stage, kAbandonedFile), merged);
+      rec_hashes = merged.size();
+      rec_origins = merged.origins.size();
+    }
+    // manifest (fusion.rs:384-447)
+    for (size_t i = 1; i < opened.size(); i++) {
+      const Json &r = opened[i]->m.json;
+      for (const char *f : {"planning_pair_evaluations", "planning_possible_pair_evaluations",
+                            "promoted_groups", "singleton_logical_bits", "predicted_logical_bits",
+                            "planned_groups", "rejected_groups", "overflow_groups",
+                            "exact_group_verifications", "memory_guard_split_groups"}) {
+        const Json *a = out.get(f), *b = r.get(f);
+        if (a && b && a->kind == Json::Number && b->kind == Json::Number)
+          out.set(f, Json::u64(std::stoull(a->text) + std::stoull(b->text)));
+      }
+      for (const char *f : {"sketch_seconds", "plan_seconds"}) {
+        const Json *a = out.get(f), *b = r.get(f);
+        if (a && b && a->kind == Json::Number && b->kind == Json::Number)
+          out.set(f, Json::f64(std::stod(a->text) + std::stod(b->text)));
+      }
+      const Json *a = out.get("max_subindex_size"), *b = r.get("max_subindex_size");
+      if (a && b && a->kind == Json::Number && b->kind == Json::Number)
+        out.set("max_subindex_size",
+                Json::u64(std::max(std::stoull(a->text), std::stoull(b->text))));
+      else
+        out.set("max_subindex_size", Json::null());
+    }
+    out.set("version", Json::u64(10));
+    out.set("abandoned_hashes", Json::u64(rec_hashes));
+    out.set("abandoned_origins", Json::u64(rec_origins));
+    out.set("colors", std::move(colors));
+    out.set("banks", std::move(banks));
+    out.set("source_document_ids", std::move(source_ids));
+    out.set("skipped_documents", std::move(skipped));
+    out.set("input_document_count", Json::u64(source_offset));
+    out.set("max_subindexes", Json::null());
+    out.set("min_subindex_documents", Json::u64(1));
+    out.set("merge_memory_budget_bytes", Json::null());
+    out.set("construction_cell_storage", Json::str("copied-compressed-banks"));
+    out.set("column_order_method", Json::null());
+    out.set("construction_abandoned_storage",
+            Json::str(m0.abandoned_recovery ? "streaming-merge" : "omitted"));
+    out.set("build_seconds",
+            Json::f64(std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count()));
+    std::string text;
+    json_write(text, out);
+    write_file(path_join(stage, "manifest.json"), text);
+    close_all();
+    ensure(rename(stage.c_str(), out_dir) == 0, "cannot publish the index directory", -2);
+    // the fused index must pass the full validation
+    zg_index *check = nullptr;
+    int rc = zg_index_open(out_dir, &check);
+    if (check) zg_index_close(check);
+    if (rc) return rc;
+    if (err_fd >= 0) {
+      std::string msg = "fused " + std::to_string(n_inputs) + " indexes: " + std::to_string(offset) +
+                        " datasets, " + std::to_string(component) + " components\n";
+      (void)!write(err_fd, msg.data(), msg.size());
+    }
+    return 0;
+  } catch (...) {
+    close_all();
+    return translate_exception();
+  }
+}
+
 extern "C" int zg_build_command(const zg_build_config *cfg, const char *fof, const char *out_dir,
                                 int err_fd) {
   zg_builder *b = nullptr;

@@ -10,13 +10,14 @@
 
 static int usage() {
   fprintf(stderr,
-          "usage: zorindex <build|query|info> [options]\n"
+          "usage: zorindex <build|query|info|fusion> [options]\n"
           "  build -f FOF -o OUT [-k K] [-m M] [-H HASHES] [-b BITS] [-s SEED]\n"
           "        [-i kmers|minimizers] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
           "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
           "        [-M subindex|all] [--hbm-budget BYTES] [--no-recovery]\n"
-          "  info -i INDEX\n");
+          "  info -i INDEX\n"
+          "  fusion -i INDEX INDEX... -o OUT\n");
   return 2;
 }
 
@@ -117,6 +118,23 @@ int main(int argc, char **argv) {
                                                 batch, format, memory, budget, 1, 2));
     return report(zg_query_command_ex(index.c_str(), query.c_str(), ratio, recover, memory,
                                       budget, 1, 2));
+  }
+  if (cmd == "fusion") {
+    std::vector<std::string> ins;
+    std::string out;
+    for (size_t i = 0; i < a.size(); i++) {
+      if (a[i] == "-i" || a[i] == "--index") {
+        while (i + 1 < a.size() && a[i + 1].rfind("-", 0) != 0) ins.push_back(a[++i]);
+      } else if (a[i] == "-o" || a[i] == "--output") {
+        out = value(i);
+      } else {
+        return usage();
+      }
+    }
+    if (ins.size() < 2 || out.empty()) return usage();
+    std::vector<const char *> ptrs;
+    for (auto &x : ins) ptrs.push_back(x.c_str());
+    return report(zg_fusion_command(ptrs.data(), (uint32_t)ptrs.size(), out.c_str(), 2));
   }
   if (cmd == "info") {
     std::string index;
