@@ -212,6 +212,80 @@ def test_fusion_rejects_incompatible_indexes(tmp_path, small_colors):
         zindex_amd.fuse_indexes(tmp_path / "a", tmp_path / "b", tmp_path / "out")
 
 
+def _v1_case(tmp_path, small_colors, group=2):
+    """A version-1 index (no fixed segment length, interleaved banks, fallback
+    keys in the bank streams) whose fallback lists hold k-mers of the reads."""
+    idx = _build(small_colors, fingerprint_bits=8, hashes=4, merge_group_size=group,
+                 segment_length=0, slot_scale=0.9, drop_abandoned=1)
+    plane = tmp_path / "plane"
+    idx.write(str(plane))
+    man = idx.manifest()
+    rng = np.random.default_rng(5)
+    reads = [c[i:i + 600] for c in small_colors for i in range(0, len(c) - 600, 5000)]
+    keys = np.unique(np.concatenate([zr.scan_kmers(r, 31) for r in reads]))
+    fallback = {}
+    for key in rng.choice(keys, 300, replace=False):
+        bi = int(rng.integers(len(man["banks"])))
+        gi = int(rng.integers(len(man["banks"][bi]["groups"])))
+        n = len(man["banks"][bi]["groups"][gi]["members"])
+        mask = int(rng.integers(1, 1 << n))
+        fallback.setdefault((bi, gi), []).append((int(key), mask))
+    v1 = tmp_path / "v1"
+    zr.to_v1(plane, v1, fallback)
+    return plane, v1, reads, fallback
+
+
+def test_version1_index_reads_fallback_keys(tmp_path, small_colors):
+    """Version 1 (storage.rs:2555-2612, 2665-2674): the recovery map comes from
+    the fallback lists inside the bank files. The oracle on the v1 files equals
+    the oracle on the plane files plus a recovery map made of the same keys."""
+    plane, v1, reads, fallback = _v1_case(tmp_path, small_colors)
+    with zindex_amd.Index.open(str(v1)) as z:
+        assert z.meta["version"] == 1 and z.meta["segment_length"] == 0
+        assert len(z.bank_payload(0)) > 0
+    dp, dl = zr.read_index_dir(plane), zr.read_index_dir(v1)
+    man = dp["manifest"]
+    by = {}
+    for (bi, gi), entries in fallback.items():
+        members = man["banks"][bi]["groups"][gi]["members"]
+        for key, mask in entries:
+            h = zr.oracle().zo_abandoned_hash(key, man["seed"])
+            by.setdefault(h, set()).update(members[i] for i in range(len(members)) if mask >> i & 1)
+    hashes = sorted(by)
+    off = np.cumsum([0] + [len(by[h]) for h in hashes]).astype(np.uint64)
+    origins = np.array([o for h in hashes for o in sorted(by[h])], np.uint32)
+    rec = (np.array(hashes, np.uint32), off, origins)
+    op = zr.OracleIndex(man, dp["banks"], rec)
+    ol = zr.OracleIndex(dl["manifest"], dl["banks"], dl["recovery"])
+    buf, off = zindex_amd.pack_reads(reads)
+    tp, cp = op.query_reads(buf, off, threads=zr.cpu_count())
+    tl, cl = ol.query_reads(buf, off, threads=zr.cpu_count())
+    assert np.array_equal(tp, tl) and np.array_equal(cp, cl)
+    op0 = zr.OracleIndex(man, dp["banks"], None)
+    assert (cl >= op0.query_reads(buf, off, threads=zr.cpu_count())[1]).all()
+    assert cl.sum() > op0.query_reads(buf, off, threads=zr.cpu_count())[1].sum()
+
+
+def test_version1_bad_fallback_fingerprint_is_rejected(tmp_path, small_colors):
+    plane, v1, reads, fallback = _v1_case(tmp_path, small_colors)
+    man = json.loads((v1 / "manifest.json").read_text())
+    bank = man["banks"][0]
+    raw = bytearray(zr.zstd_decompress((v1 / bank["file"]).read_bytes()))
+    n = 40 + bank["slots"] * ((bank["row_bits"] + 7) // 8)
+    # the first group with a fallback key: flip its value's fingerprint
+    at = n
+    for g in bank["groups"]:
+        cnt = int.from_bytes(raw[at:at + 8], "little")
+        at += 8
+        if cnt:
+            raw[at + 8] ^= 1
+            break
+    (v1 / bank["file"]).write_bytes(zr.zstd_compress(bytes(raw)))
+    with zindex_amd.Index.open(str(v1)) as z:
+        with pytest.raises(zindex_amd.ZindexError, match="fallback fingerprint"):
+            z.load()
+
+
 def test_partition_cap_too_small_is_rejected(small_colors):
     with pytest.raises(zindex_amd.ZindexError, match="too small"):
         _build(small_colors[:2], fingerprint_bits=8, hashes=4, max_subindex_size=1 << 8)

@@ -445,3 +445,33 @@ def test_fused_index_components(tmp_path):
         t, c, _ = z.query_reads(reads, records=True)
         assert np.array_equal(c, np.concatenate([a[1] for a in alone], axis=1))
         assert all(np.array_equal(t, a[0]) for a in alone)
+
+
+@pytest.mark.parametrize("group", [1, 3])
+def test_version1_index(tmp_path, group):
+    """Manifest version 1 (storage.rs:2555-2612, 2665-2674): interleaved banks
+    with the groups' fallback keys after the payload; their recovery map is
+    built at load. GPU counts equal the oracle's on the v1 files."""
+    colors = make_colors(170 + group, 7, 25000)
+    reads = make_reads(171, colors, 200)
+    idx = build(colors, fingerprint_bits=8, hashes=4, merge_group_size=group,
+                segment_length=0, slot_scale=0.9, drop_abandoned=1)
+    plane = tmp_path / "plane"
+    idx.write(str(plane))
+    man = idx.manifest()
+    rng = np.random.default_rng(172)
+    keys = np.unique(np.concatenate([zr.scan_kmers(r, 31) for r in reads if len(r) >= 31]))
+    fallback = {}
+    for key in rng.choice(keys, 400, replace=False):
+        bi = int(rng.integers(len(man["banks"])))
+        gi = int(rng.integers(len(man["banks"][bi]["groups"])))
+        n = len(man["banks"][bi]["groups"][gi]["members"])
+        fallback.setdefault((bi, gi), []).append((int(key), int(rng.integers(1, 1 << n))))
+    v1 = tmp_path / "v1"
+    zr.to_v1(plane, v1, fallback)
+    disk = zr.read_index_dir(v1)
+    orc = zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"])
+    with zindex_amd.Index.open(str(v1)) as z:
+        z.load()
+        assert z.meta["abandoned_hashes"] == 0  # v1 keeps no separate map
+        check_parity(z, orc, reads, label=f"version 1 g{group}")

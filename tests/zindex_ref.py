@@ -191,7 +191,15 @@ def zstd_compress(data: bytes, level: int = 3) -> bytes:
     return dst.raw[:n]
 
 
-def to_legacy_layout(src_dir, dst_dir, order=None) -> None:
+def to_v1(src_dir, dst_dir, fallback) -> None:
+    """Rewrite an index directory (built without a fixed segment length) as
+    manifest version 1: legacy interleaved banks whose streams end with every
+    group's fallback list (storage.rs:2555-2612), no separate recovery file.
+    fallback: {(bank, group): [(key, member mask), ...]} with the caller's keys."""
+    to_legacy_layout(src_dir, dst_dir, fallback=fallback)
+
+
+def to_legacy_layout(src_dir, dst_dir, order=None, fallback=None) -> None:
     """Rewrite an index directory with legacy interleaved banks
     (fingerprint_first = false; storage.rs:2864-2914): group g of a row holds
     fp | mask << b at bits [offset_g, offset_g + width_g), offsets cumulative
@@ -238,8 +246,25 @@ def to_legacy_layout(src_dir, dst_dir, order=None) -> None:
         bank["fingerprint_first"] = False
         header = b"ZORPOW1\0" + np.array([bank["slots"], bank["row_bits"], G, len(out)],
                                          dtype="<u8").tobytes()
-        (dst_dir / bank["file"]).write_bytes(zstd_compress(header + bytes(out)))
-    if man.get("abandoned_file"):
+        trailer = b""
+        if fallback is not None:
+            for gi, g in enumerate(new_groups):
+                entries = sorted(fallback.get((bi, gi), []))
+                g.pop("abandoned_keys", None)
+                g["fallback_keys"] = len(entries)
+                trailer += np.array([len(entries)], dtype="<u8").tobytes()
+                for key, mask in entries:
+                    value = oracle().zo_fingerprint(int(key), man["seed"], b) | (
+                        0 if len(g["members"]) == 1 else mask << b)
+                    trailer += np.array([key, value], dtype="<u8").tobytes()
+        (dst_dir / bank["file"]).write_bytes(zstd_compress(header + bytes(out) + trailer))
+    if fallback is not None:
+        assert man.get("segment_length") is None
+        man["version"] = 1
+        for f in ("abandoned_recovery", "abandoned_file", "abandoned_hashes",
+                  "abandoned_origins"):  # newer fields; recovery defaults on
+            man.pop(f, None)
+    elif man.get("abandoned_file"):
         shutil.copy(src_dir / man["abandoned_file"], dst_dir / man["abandoned_file"])
     (dst_dir / "manifest.json").write_text(json.dumps(man, indent=2))
 
@@ -261,16 +286,45 @@ def read_index_dir(path) -> dict:
     path = Path(path)
     man = json.loads((path / "manifest.json").read_text())
     banks = []
+    records = []  # version 1: (abandoned hash, origins) of every fallback key
     for b in man["banks"]:
         raw = zstd_decompress((path / b["file"]).read_bytes())
         magic = raw[:8]
         assert magic in (b"ZORPOW3\0", b"ZORPOW1\0"), magic
         slots, row_bits, groups, n = np.frombuffer(raw[8:40], dtype="<u8")
         assert (slots, row_bits, groups) == (b["slots"], b["row_bits"], len(b["groups"]))
-        payload = np.frombuffer(raw[40:], dtype=np.uint8)
-        assert len(payload) == n == slots * ((row_bits + 7) // 8)
+        assert n == slots * ((row_bits + 7) // 8)
+        payload = np.frombuffer(raw[40:40 + int(n)], dtype=np.uint8)
+        assert len(payload) == n
+        tail = raw[40 + int(n):]
+        if man["version"] == 1:
+            # per group: u64 count, then count x (u64 key, u64 value) (storage.rs:2564-2612)
+            at, fb = 0, man["fingerprint_bits"]
+            for g in b["groups"]:
+                (cnt,) = np.frombuffer(tail[at:at + 8], dtype="<u8")
+                at += 8
+                assert cnt == g["fallback_keys"]
+                kv = np.frombuffer(tail[at:at + 16 * int(cnt)], dtype="<u8").reshape(-1, 2)
+                at += 16 * int(cnt)
+                for key, value in kv:
+                    mask = 1 if len(g["members"]) == 1 else int(value) >> fb
+                    origins = [g["members"][i] for i in range(64) if mask >> i & 1]
+                    records.append((oracle().zo_abandoned_hash(int(key), man["seed"]), origins))
+            assert at == len(tail)
+        else:
+            assert len(tail) == 0
         banks.append(payload)
     rec = None
+    if man["version"] == 1 and man.get("abandoned_recovery", True):
+        by = {}
+        for h, o in records:
+            by.setdefault(h, set()).update(o)
+        hashes = sorted(by)
+        off, origins = [0], []
+        for h in hashes:
+            origins += sorted(by[h])
+            off.append(len(origins))
+        rec = (np.array(hashes, np.uint32), np.array(off, np.uint64), np.array(origins, np.uint32))
     if man.get("abandoned_recovery", True) and man.get("abandoned_file"):
         raw = zstd_decompress((path / man["abandoned_file"]).read_bytes())
         assert raw[:8] == b"ZORABN2\0"

@@ -90,7 +90,6 @@ static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
 static void check_query_support(const Manifest &m) {
-  if (m.version < 2) fail(ZG_ERR_UNSUPPORTED, "version-1 indexes are not supported by the GPU query");
   if (m.has_seed_pattern || m.has_randstrobemers)
     fail(ZG_ERR_UNSUPPORTED, "spaced seeds and randstrobemers are not supported by the GPU query");
   if (m.homopolymer_compression)
@@ -582,11 +581,43 @@ extern "C" int zg_index_color(const zg_index *idx, uint32_t color, const char **
   }
 }
 
+namespace zg {
+// Version 1 (storage.rs:2555-2612, 2665-2674): banks carry their groups'
+// fallback keys after the payload, and the recovery map is built from them.
+static void load_v1(zg_index *idx) {
+  const Manifest &m = idx->m;
+  const uint32_t b = (uint32_t)m.fingerprint_bits;
+  const uint64_t fp_mask = (1ULL << b) - 1;
+  std::vector<FallbackRecord> records;
+  std::vector<std::vector<uint32_t>> members;
+  idx->payload.assign(m.banks.size(), {});
+  for (size_t bi = 0; bi < m.banks.size(); bi++) {
+    std::vector<std::vector<std::pair<uint64_t, uint64_t>>> fallback;
+    idx->payload[bi] = load_v1_bank(idx->dir, m.banks[bi], fallback);
+    for (size_t g = 0; g < m.banks[bi].groups.size(); g++) {
+      const GroupMeta &gm = m.banks[bi].groups[g];
+      const uint32_t gid = (uint32_t)members.size();
+      members.push_back(gm.members);
+      for (auto &[key, value] : fallback[g]) {
+        ensure((value & fp_mask) == fingerprint(key, m.seed, b), "invalid fallback fingerprint");
+        const uint64_t mask = gm.members.size() == 1 ? 1 : value >> b;
+        ensure(mask != 0, "empty fallback membership");
+        records.push_back({abandoned_hash(key, m.seed), gid, mask});
+      }
+    }
+  }
+  idx->payload_loaded = true;
+  idx->rec = recovery_from_records(records, members);
+  idx->have_rec = true;
+}
+}  // namespace zg
+
 extern "C" int zg_index_load_ex(zg_index *idx, int recover, uint64_t budget, uint32_t memory) {
   try {
     ensure(idx, "null argument", ZG_ERR_INVALID);
     check_query_support(idx->m);
     idx->recover = recover && idx->m.abandoned_recovery;
+    if (idx->m.version == 1 && !idx->payload_loaded && !idx->dir.empty()) load_v1(idx);
     if (idx->recover && !idx->have_rec && !idx->dir.empty()) {
       idx->rec = read_recovery_map(idx->dir, idx->m);
       idx->have_rec = true;
@@ -611,7 +642,12 @@ const std::vector<uint8_t> &host_payload(zg_index *idx, size_t bank) {
   std::vector<uint8_t> &p = idx->payload[bank];
   if (p.size() != idx->m.banks[bank].payload_bytes()) {
     ensure(!idx->dir.empty(), "bank payload not in host memory", ZG_ERR_INVALID);
-    p = load_bank_payload(idx->dir, idx->m.banks[bank], (int)host_threads());
+    if (idx->m.version == 1) {
+      std::vector<std::vector<std::pair<uint64_t, uint64_t>>> fallback;
+      p = load_v1_bank(idx->dir, idx->m.banks[bank], fallback);
+    } else {
+      p = load_bank_payload(idx->dir, idx->m.banks[bank], (int)host_threads());
+    }
   }
   return p;
 }

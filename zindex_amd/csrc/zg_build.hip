@@ -1464,6 +1464,7 @@ extern "C" int zg_build_command(const zg_build_config *cfg, const char *fof, con
 extern "C" int zg_index_write(const zg_index *idx, const char *out_dir) {
   try {
     ensure(idx && out_dir, "null argument", -1);
+    ensure(idx->m.version != 1, "writing version-1 indexes is not supported", -6);
     zg_index *ix = const_cast<zg_index *>(idx);  // host copies are decoded on first use
     if (ix->m.abandoned_recovery && !ix->have_rec) {
       ensure(!ix->dir.empty(), "recovery map not in host memory", -1);

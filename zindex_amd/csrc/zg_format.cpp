@@ -632,6 +632,63 @@ void check_bank_header(const uint8_t *header, const BankMeta &meta) {
   validate_header(header, meta, meta.payload_bytes());
 }
 
+std::vector<uint8_t> load_v1_bank(const std::string &dir, const BankMeta &meta,
+                                  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> &fallback) {
+  const uint64_t expected = meta.payload_bytes();
+  BankFrames bf = open_bank_frames(dir, meta);
+  std::string raw = zstd_decompress_all(bf.data, bf.size);
+  ensure(raw.size() >= kHeader + expected, "truncated bank", -2);
+  validate_header((const uint8_t *)raw.data(), meta, expected);
+  const uint8_t *p = (const uint8_t *)raw.data() + kHeader + expected;
+  const uint8_t *end = (const uint8_t *)raw.data() + raw.size();
+  auto read_u64 = [&]() {
+    ensure(end - p >= 8, "truncated bank", -2);
+    uint64_t v = le64(p);
+    p += 8;
+    return v;
+  };
+  fallback.assign(meta.groups.size(), {});
+  for (size_t g = 0; g < meta.groups.size(); g++) {
+    const GroupMeta &gm = meta.groups[g];
+    ensure(read_u64() == gm.fallback_keys, "invalid fallback length");
+    for (uint64_t i = 0; i < gm.fallback_keys; i++) {
+      const uint64_t key = read_u64(), value = read_u64();
+      ensure(gm.width == 64 || value < (1ULL << gm.width), "invalid fallback payload width");
+      ensure(fallback[g].empty() || fallback[g].back().first < key, "unsorted fallback keys");
+      fallback[g].push_back({key, value});
+    }
+  }
+  ensure(p == end, "unexpected bank data");
+  return std::vector<uint8_t>(raw.begin() + kHeader, raw.begin() + kHeader + expected);
+}
+
+RecoveryMap recovery_from_records(std::vector<FallbackRecord> &records,
+                                  const std::vector<std::vector<uint32_t>> &group_members) {
+  std::sort(records.begin(), records.end(),
+            [](const FallbackRecord &a, const FallbackRecord &b) { return a.hash < b.hash; });
+  RecoveryMap map;
+  std::vector<uint32_t> origins;
+  for (size_t i = 0; i < records.size();) {
+    const uint32_t h = records[i].hash;
+    origins.clear();
+    for (; i < records.size() && records[i].hash == h; i++) {
+      const std::vector<uint32_t> &members = group_members[records[i].group];
+      for (uint64_t mask = records[i].mask; mask; mask &= mask - 1) {
+        const int bit = __builtin_ctzll(mask);
+        ensure((size_t)bit < members.size(), "invalid abandoned membership bit");
+        origins.push_back(members[bit]);
+      }
+    }
+    std::sort(origins.begin(), origins.end());
+    origins.erase(std::unique(origins.begin(), origins.end()), origins.end());
+    ensure(!origins.empty(), "empty abandoned membership");
+    map.hashes.push_back(h);
+    map.origins.insert(map.origins.end(), origins.begin(), origins.end());
+    map.off.push_back(map.origins.size());
+  }
+  return map;
+}
+
 void write_bank_file(const std::string &path, const BankMeta &meta, const uint8_t *payload,
                      uint64_t bytes, int threads) {
   // The raw stream is header + payload, cut into 4 MiB frames.

@@ -148,10 +148,6 @@ BankFrames open_bank_frames(const std::string &dir, const BankMeta &meta);
 void decode_bank_frame(const BankFrames &b, size_t i, uint8_t *dst);
 // The decoded 40-byte header against the manifest (storage.rs:2184-2231).
 void check_bank_header(const uint8_t *header, const BankMeta &meta);
-// bank_io::Writer (bank_io.rs:34-120): 4 MiB checksummed frames + seek table.
-void write_bank_file(const std::string &path, const BankMeta &meta,
-                     const uint8_t *payload, uint64_t bytes, int threads);
-
 // Recovery map: sorted hashes, origin offsets and origins (dataset ids).
 struct RecoveryMap {
   std::vector<uint32_t> hashes;
@@ -159,6 +155,29 @@ struct RecoveryMap {
   std::vector<uint32_t> origins;
   size_t size() const { return hashes.size(); }
 };
+
+// A version-1 bank (storage.rs:2555-2612): the decoded stream holds the
+// legacy header and payload, then for each group its fallback list (u64
+// count == fallback_keys, then count x (u64 key, u64 value), keys strictly
+// ascending, value below 2^width). Returns the payload; fallback[g] gets
+// group g's (key, value) pairs (the caller checks fingerprints).
+std::vector<uint8_t> load_v1_bank(const std::string &dir, const BankMeta &meta,
+                                  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> &fallback);
+// One recovered key of a version-1 group: (abandoned hash, global group id,
+// member mask).
+struct FallbackRecord {
+  uint32_t hash;
+  uint32_t group;
+  uint64_t mask;
+};
+// visit_abandoned (storage.rs:1882-1923): records sorted by hash, origins =
+// the members their masks select, sorted and deduplicated, per hash.
+RecoveryMap recovery_from_records(std::vector<FallbackRecord> &records,
+                                  const std::vector<std::vector<uint32_t>> &group_members);
+// bank_io::Writer (bank_io.rs:34-120): 4 MiB checksummed frames + seek table.
+void write_bank_file(const std::string &path, const BankMeta &meta,
+                     const uint8_t *payload, uint64_t bytes, int threads);
+
 // AbandonedReader (abandoned_io.rs:43-129) over the whole file.
 RecoveryMap read_recovery_map(const std::string &dir, const Manifest &m);
 // write_abandoned_map (storage.rs:2053-2094), ZORABN2.
