@@ -173,7 +173,7 @@ typedef struct zg_build_config {
   uint32_t k;                  /* default 31 */
   uint32_t hashes;             /* 2..32, default 4 */
   uint32_t fingerprint_bits;   /* 1,2,4,8,16,32, default 8 */
-  uint32_t index_mode;         /* 0 kmers, 1 minimizers (open-closed mod) */
+  uint32_t index_mode;         /* 0 kmers, 1 minimizers (open-closed mod), 2 modimizers */
   uint32_t minimizer_size;     /* m for minimizers */
   uint32_t segment_length;     /* default 64 */
   uint64_t seed;               /* default 69 */
@@ -183,6 +183,9 @@ typedef struct zg_build_config {
                                   consecutive datasets per group */
   uint32_t threads;            /* 0 = all host cores */
   int32_t drop_abandoned;      /* 1 = no recovery map */
+  uint32_t modimizer_sampling; /* index_mode 2 (modimizers): keep canonical
+                                  k-mers with mixsplit % sampling == 0;
+                                  default 16 */
   uint32_t device;             /* 1 = peel on the GPU: parallel peeling rounds,
                                   back-substitution and verification as HIP
                                   kernels; a 2-core left by peeling is broken

@@ -640,7 +640,16 @@ size_t zo_read_features(const zo_index *idx, const uint8_t *seq, size_t len,
                         uint64_t *out) {
   if (idx->mode == 1)
     return zo_scan_ocm(seq, len, idx->k, idx->m, idx->seed, NULL, out);
-  return zo_scan_kmers(seq, len, idx->k, out);
+  size_t n = zo_scan_kmers(seq, len, idx->k, out);
+  if (idx->mode == 2) {
+    /* scan_modimizers (src/lib.rs:796-806): canonical k-mers whose
+     * mixsplit(kmer, seed) is a multiple of the sampling */
+    size_t kept = 0;
+    for (size_t i = 0; i < n; i++)
+      if (zo_mixsplit(out[i], idx->seed) % idx->sampling == 0) out[kept++] = out[i];
+    n = kept;
+  }
+  return n;
 }
 
 static int cmp_u64(const void *a, const void *b) {

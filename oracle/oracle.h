@@ -88,7 +88,7 @@ typedef struct zo_index {
   int k;
   int hashes;
   int bits;
-  int mode;                   /* 0 = kmers, 1 = minimizers (open-closed mod) */
+  int mode;                   /* 0 = kmers, 1 = minimizers (open-closed mod), 2 = modimizers */
   int m;                      /* minimizer size */
   uint64_t seed;
   uint64_t segment_length;    /* 0 = legacy None */
@@ -96,6 +96,7 @@ typedef struct zo_index {
   const zo_bank *banks;
   uint32_t n_banks;
   const zo_recovery *recovery; /* NULL = no recovery */
+  uint64_t sampling;          /* modimizer sampling (mode 2) */
 } zo_index;
 
 /* src/power_merge/storage.rs:2761-2925 (lookup_selected, ranges = None),

@@ -80,6 +80,34 @@ def test_build_round_trip_no_false_negatives(tmp_path, small_colors, bits, hashe
             assert int(scores[c]) == before + 1, (c, int(key))
 
 
+@pytest.mark.parametrize("sampling", [1, 5])
+def test_modimizer_build_inserts_sampled_kmers(tmp_path, small_colors, sampling):
+    idx = _build(small_colors, fingerprint_bits=16, hashes=3, index_mode="modimizers",
+                 modimizer_sampling=sampling)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["index_mode"] == 2 and man["modimizer_sampling"] == sampling
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        every = np.unique(zr.scan_kmers(seq, 31))
+        keys = np.unique(zr.read_features(seq, 31, man["seed"], mode=2, sampling=sampling))
+        if sampling == 1:
+            assert np.array_equal(keys, every)
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:2000]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+
+
+def test_modimizer_sampling_zero_is_rejected(small_colors):
+    with pytest.raises(Exception, match="sampling"):
+        _build(small_colors[:1], index_mode=2, modimizer_sampling=0)
+
+
 @pytest.mark.parametrize("group,cap", [(1, 1 << 15), (2, 1 << 17), (1, 1 << 12)])
 def test_hash_partitioned_build(tmp_path, small_colors, group, cap):
     """partition_large_subindexes (storage.rs:1525-1641): a size class over

@@ -172,6 +172,21 @@ def test_minimizer_parameter_edges(k, m):
     check_parity(idx, oracle_for(idx), reads, label=f"ocm k{k} m{m}")
 
 
+@pytest.mark.parametrize("sampling,bits", [(1, 8), (5, 4), (16, 16)])
+def test_modimizer_seeds_and_counts(sampling, bits):
+    colors = make_colors(120 + sampling, 6, 30000)
+    reads = make_reads(121 + sampling, colors, 150, lo=20, hi=2500)
+    idx = build(colors, fingerprint_bits=bits, hashes=3, index_mode=2,
+                modimizer_sampling=sampling)
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, _ = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.read_features(r, 31, 69, mode=2, sampling=sampling)
+        assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
+    check_parity(idx, oracle_for(idx), reads, label=f"modimizers s{sampling} b{bits}")
+
+
 def test_kmer_seeds_match_oracle_scan(base_case):
     colors, reads, idx, orc = base_case
     buf, off = zindex_amd.pack_reads(reads)

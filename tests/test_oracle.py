@@ -145,6 +145,35 @@ def test_ocm_splits_runs_and_rejects_even_windows():
     assert len(zr.scan_ocm(seq, 49, 32, 69)[0]) == 0
 
 
+# src/lib.rs:1089-1113: with sampling 1 the modimizers are every canonical k-mer
+@pytest.mark.parametrize("seq,k,seed", [(b"ACGTACGT", 3, 1), (b"AAAACCCC", 4, 2),
+                                        (b"TTTTGGGG", 5, 3), (b"GATTACA", 3, 4),
+                                        (b"ACGTNACGT", 4, 5), (b"CCCCCCCC", 2, 6)])
+def test_modimizers_sampling_one_are_all_kmers(seq, k, seed):
+    assert zr.read_features(seq, k, seed, mode=2, sampling=1).tolist() == \
+        zr.scan_kmers(seq, k).tolist()
+
+
+def test_modimizers_nest_and_thin_by_sampling():
+    # sampling s keeps the k-mers whose splitmix64(kmer + seed) is a multiple of
+    # s (src/lib.rs:796-806), in scan order: the sample at 8 is a subsequence of
+    # the one at 4, and each keeps about 1/s of the k-mers
+    seq = _sequence(20_000, 11)[:9_000] + b"N" + _sequence(20_000, 11)[9_001:]
+    k, seed = 31, 42
+    every = zr.scan_kmers(seq, k).tolist()
+    for s in (2, 4, 8, 5):
+        got = zr.read_features(seq, k, seed, mode=2, sampling=s).tolist()
+        assert got == [x for x in every if sm64(x + seed) % s == 0]
+        n, p = len(every), 1.0 / s
+        assert abs(len(got) - n * p) < 6 * (n * p * (1 - p)) ** 0.5
+    s4 = zr.read_features(seq, k, seed, mode=2, sampling=4).tolist()
+    s8 = iter(s4)
+    assert all(x in s8 for x in zr.read_features(seq, k, seed, mode=2, sampling=8).tolist())
+    # canonical k-mers: the reverse complement samples the same multiset
+    rc = zr.read_features(_rc(seq), k, seed, mode=2, sampling=4).tolist()
+    assert sorted(rc) == sorted(s4)
+
+
 def test_positions_closed_form():
     # storage.rs:4800-4826 (legacy segment length) plus fixed-length geometry
     for hashes in (2, 3, 4, 8, 32):

@@ -40,7 +40,7 @@ class ZoIndex(C.Structure):
         ("k", C.c_int), ("hashes", C.c_int), ("bits", C.c_int), ("mode", C.c_int),
         ("m", C.c_int), ("seed", C.c_uint64), ("segment_length", C.c_uint64),
         ("n_colors", C.c_uint32), ("banks", C.POINTER(ZoBank)), ("n_banks", C.c_uint32),
-        ("recovery", C.POINTER(ZoRecovery)),
+        ("recovery", C.POINTER(ZoRecovery)), ("sampling", C.c_uint64),
     ]
 
 
@@ -87,6 +87,8 @@ def oracle() -> C.CDLL:
         o.zo_scan_ocm_bruteforce.restype = C.c_size_t
         o.zo_scan_ocm_bruteforce.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
                                              u64p, u64p]
+        o.zo_read_features.restype = C.c_size_t
+        o.zo_read_features.argtypes = [C.POINTER(ZoIndex), C.c_char_p, C.c_size_t, u64p]
         o.zo_lookup.restype = None
         o.zo_lookup.argtypes = [C.POINTER(ZoIndex), C.c_uint64, C.c_uint64, u64p]
         o.zo_query_reads.restype = C.c_int
@@ -109,6 +111,15 @@ def oracle() -> C.CDLL:
 def scan_kmers(seq: bytes, k: int) -> np.ndarray:
     out = np.zeros(max(len(seq), 1), np.uint64)
     n = oracle().zo_scan_kmers(seq, len(seq), k, out.ctypes.data_as(u64p))
+    return out[:n]
+
+
+def read_features(seq: bytes, k: int, seed: int, mode: int = 0, m: int = 0,
+                  sampling: int = 16) -> np.ndarray:
+    """The query-side features of one read (k-mers, OCM or modimizers)."""
+    idx = ZoIndex(k=k, mode=mode, m=m, seed=seed, sampling=sampling)
+    out = np.zeros(max(len(seq), 1), np.uint64)
+    n = oracle().zo_read_features(C.byref(idx), seq, len(seq), out.ctypes.data_as(u64p))
     return out[:n]
 
 
@@ -381,7 +392,8 @@ class OracleIndex:
                          manifest["minimizer_size"], manifest["seed"],
                          manifest.get("segment_length") or 0, len(manifest["colors"]), zb,
                          len(manifest["banks"]),
-                         C.pointer(self._rec) if self._rec is not None else None)
+                         C.pointer(self._rec) if self._rec is not None else None,
+                         manifest.get("modimizer_sampling") or 16)
 
     @classmethod
     def from_dir(cls, path, recover: bool = True) -> "OracleIndex":

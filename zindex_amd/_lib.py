@@ -49,7 +49,8 @@ class BuildConfig(C.Structure):
         ("index_mode", C.c_uint32), ("minimizer_size", C.c_uint32),
         ("segment_length", C.c_uint32), ("seed", C.c_uint64), ("slot_scale", C.c_double),
         ("max_subindex_size", C.c_uint64), ("merge_group_size", C.c_uint32),
-        ("threads", C.c_uint32), ("drop_abandoned", C.c_int32), ("device", C.c_uint32),
+        ("threads", C.c_uint32), ("drop_abandoned", C.c_int32),
+        ("modimizer_sampling", C.c_uint32), ("device", C.c_uint32),
     ]
 
 

@@ -94,13 +94,15 @@ static void check_query_support(const Manifest &m) {
     fail(ZG_ERR_UNSUPPORTED, "spaced seeds and randstrobemers are not supported by the GPU query");
   if (m.homopolymer_compression)
     fail(ZG_ERR_UNSUPPORTED, "homopolymer compression is not supported by the GPU query");
-  if (m.index_mode == 0) {
+  if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32) fail(ZG_ERR_UNSUPPORTED, "k-mer indexes with k > 32 are not supported by the GPU query");
+    if (m.index_mode == 2 && m.modimizer_sampling == 0)
+      fail(ZG_ERR_FORMAT, "modimizer sampling must be positive");
   } else if (m.index_mode == 1) {
     if (m.minimizer_scheme != 1)
       fail(ZG_ERR_UNSUPPORTED, "only open-closed mod-minimizer indexes are supported by the GPU query");
   } else {
-    fail(ZG_ERR_UNSUPPORTED, "modimizer and findere indexes are not supported by the GPU query");
+    fail(ZG_ERR_UNSUPPORTED, "findere indexes are not supported by the GPU query");
   }
 }
 
@@ -790,6 +792,7 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   a.partial = b->partial.as<uint64_t>();
   a.n_colors = (uint32_t)idx->m.colors.size();
   a.nontemporal = b->nontemporal;
+  a.sampling = (idx->m.index_mode == 2 && !b->explicit_keys) ? idx->m.modimizer_sampling : 0;
   return a;
 }
 
@@ -865,9 +868,10 @@ void seed_batch(zg_index *idx, zg_batch *b) {
   launch_pack(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
               b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
   launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
-  if (m.index_mode == 0) {
-    launch_count_kmers(b->invalid.as<uint64_t>(), b->pos_off.as<uint64_t>(), b->n_reads,
-                       b->positions, (uint32_t)m.k, b->totals.as<uint64_t>(), s);
+  if (m.index_mode != 1) {  // k-mers, or modimizers (sampled k-mers)
+    launch_count_kmers(b->invalid.as<uint64_t>(), b->codes.as<uint64_t>(), m.seed,
+                       m.index_mode == 2 ? m.modimizer_sampling : 0, b->pos_off.as<uint64_t>(),
+                       b->n_reads, b->positions, (uint32_t)m.k, b->totals.as<uint64_t>(), s);
     check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
     check_hip(hipEventSynchronize(b->ev[1]), "seed");
   } else {
@@ -1355,8 +1359,11 @@ extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_
       uint64_t p0 = (read_off[r] - read_off[0]) + r, len = read_off[r + 1] - read_off[r];
       for (uint64_t p = p0; p < p0 + len; p++) {
         if (bits(inv, p) & wmask) continue;
+        const uint64_t key = canonical_from_window(bits(codes, 2 * p), (int)k);
+        if (idx->m.index_mode == 2 && mixsplit(key, idx->m.seed) % idx->m.modimizer_sampling)
+          continue;  // not a modimizer
         if (n < cap) {
-          if (keys) keys[n] = canonical_from_window(bits(codes, 2 * p), (int)k);
+          if (keys) keys[n] = key;
           if (positions) positions[n] = p - p0;
         }
         n++;

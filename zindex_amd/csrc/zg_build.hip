@@ -590,10 +590,18 @@ static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64
   for (uint64_t r = 0; r < n; r++) {
     const uint8_t *p = seq + off[r];
     size_t len = off[r + 1] - off[r];
-    if (cfg.index_mode == 1)
+    if (cfg.index_mode == 1) {
       host_ocm(p, len, (int)cfg.k, (int)cfg.minimizer_size, cfg.seed, keys);
-    else
+    } else {
+      const size_t at = keys.size();
       host_kmers(p, len, (int)cfg.k, keys);
+      if (cfg.index_mode == 2) {  // scan_modimizers (src/lib.rs:796-806)
+        size_t kept = at;
+        for (size_t i = at; i < keys.size(); i++)
+          if (mixsplit(keys[i], cfg.seed) % cfg.modimizer_sampling == 0) keys[kept++] = keys[i];
+        keys.resize(kept);
+      }
+    }
   }
 }
 
@@ -656,6 +664,7 @@ extern "C" void zg_build_config_default(zg_build_config *c) {
   c->merge_group_size = 1;
   c->threads = 0;
   c->drop_abandoned = 0;
+  c->modimizer_sampling = 16;
 }
 
 static void validate_config(const zg_build_config &c) {
@@ -670,14 +679,16 @@ static void validate_config(const zg_build_config &c) {
   ensure(c.slot_scale > 0 && std::isfinite(c.slot_scale), "invalid slot scale", -1);
   ensure(c.max_subindex_size && !(c.max_subindex_size & (c.max_subindex_size - 1)),
          "--max-subindex-size must be a non-zero power of two", -1);
-  if (c.index_mode == 0) {
+  if (c.index_mode == 0 || c.index_mode == 2) {
+    ensure(c.index_mode == 0 || c.modimizer_sampling >= 1, "modimizer sampling must be positive",
+           -1);
     ensure(c.k >= 1 && c.k <= 32, "k must be in 1..=32 for k-mer indexes in this build", -6);
   } else if (c.index_mode == 1) {
     ensure(c.k >= 1 && c.k <= 255 && c.k % 2 == 1 && c.minimizer_size >= 1 &&
                c.minimizer_size <= 31 && c.minimizer_size <= c.k,
            "invalid minimizer window/length", -1);
   } else {
-    fail(-6, "only kmers and open-closed mod-minimizer indexes are built here");
+    fail(-6, "only k-mer, open-closed mod-minimizer and modimizer indexes are built here");
   }
   ensure(c.merge_group_size >= 1 && c.merge_group_size <= 64 - b, "invalid merge group size",
          -1);
@@ -1007,7 +1018,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   m.index_mode = cfg.index_mode;
   m.minimizer_size = cfg.minimizer_size;
   m.minimizer_scheme = 1;
-  m.modimizer_sampling = 16;
+  m.modimizer_sampling = cfg.modimizer_sampling ? cfg.modimizer_sampling : 16;
   m.findere_z = 10;
   m.slot_scale = cfg.slot_scale;
   m.has_max_subindex_size = true;
@@ -1032,7 +1043,7 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   j.set("seed_pattern", Json::null());
   j.set("randstrobemers", Json::null());
   j.set("randstrobe_scheme", Json::null());
-  j.set("modimizer_sampling", Json::u64(16));
+  j.set("modimizer_sampling", Json::u64(m.modimizer_sampling));
   j.set("findere_z", Json::u64(10));
   j.set("sketch_size", Json::u64(128));
   j.set("sketch_method", Json::str("none: fixed consecutive grouping (no sketches)"));

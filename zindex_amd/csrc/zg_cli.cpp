@@ -12,7 +12,7 @@ static int usage() {
   fprintf(stderr,
           "usage: zorindex <build|query|info|fusion> [options]\n"
           "  build -f FOF -o OUT [-k K] [-m M] [-H HASHES] [-b BITS] [-s SEED]\n"
-          "        [-i kmers|minimizers] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
+          "        [-i kmers|minimizers|modimizers] [--modimizer-sampling N] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
           "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
           "        [-M subindex|all] [--hbm-budget BYTES] [--no-recovery]\n"
@@ -73,7 +73,10 @@ int main(int argc, char **argv) {
         std::string v = value(i);
         if (v == "kmers") cfg.index_mode = 0;
         else if (v == "minimizers") cfg.index_mode = 1;
+        else if (v == "modimizers") cfg.index_mode = 2;
         else { fprintf(stderr, "unsupported index mode %s\n", v.c_str()); return 2; }
+      } else if (o == "--modimizer-sampling") {
+        cfg.modimizer_sampling = (uint32_t)atoi(value(i).c_str());
       } else return usage();
     }
     if (fof.empty() || out.empty()) return usage();

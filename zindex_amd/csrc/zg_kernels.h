@@ -69,6 +69,7 @@ struct LookupArgs {
   uint64_t *partial;          // [waves * n_colors]
   uint32_t n_colors;
   uint32_t nontemporal;       // 1: row loads bypass cache retention
+  uint64_t sampling;          // modimizers: keep k-mers with mixsplit % sampling == 0 (0 = all)
 };
 
 struct KernelConfig {
@@ -93,7 +94,9 @@ void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *p
                  uint64_t n_reads, uint64_t positions, uint64_t *codes, uint64_t *invalid,
                  hipStream_t s);
 // Valid k-mer windows per read (atomically added into totals).
-void launch_count_kmers(const uint64_t *invalid, const uint64_t *pos_off, uint64_t n_reads,
+// sampling > 0 counts only modimizers (mixsplit(canonical, seed) % sampling == 0).
+void launch_count_kmers(const uint64_t *invalid, const uint64_t *codes, uint64_t seed,
+                        uint64_t sampling, const uint64_t *pos_off, uint64_t n_reads,
                         uint64_t positions, uint32_t k, uint64_t *totals, hipStream_t s);
 // Open-closed mod-minimizer selection: per window start, flag + offset, and
 // per-block and per-read counts.

@@ -145,7 +145,8 @@ __device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invali
   return m;
 }
 
-__global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *pos_off,
+__global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *codes, uint64_t seed,
+                                   uint64_t sampling, const uint64_t *pos_off,
                                    uint64_t n_reads, uint64_t positions, uint32_t k,
                                    uint64_t *totals) {
   uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -153,6 +154,12 @@ __global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *pos_
   if (w >= n_words) return;
   uint64_t valid = ~invalid_window_starts(invalid, w, k);
   uint64_t p0 = w * 64;
+  if (sampling)  // scan_modimizers (src/lib.rs:796-806)
+    for (uint64_t m = valid; m; m &= m - 1) {
+      const int j = __builtin_ctzll(m);
+      const uint64_t key = canonical_from_window(stream_bits(codes, 2 * (p0 + j)), (int)k);
+      if (mixsplit(key, seed) % sampling) valid &= ~(1ULL << j);
+    }
   uint64_t r = find_segment(pos_off, n_reads, p0);
   while (r < n_reads && pos_off[r] < p0 + 64) {
     uint64_t a = pos_off[r] > p0 ? pos_off[r] - p0 : 0;
@@ -169,12 +176,13 @@ __global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *pos_
   }
 }
 
-void launch_count_kmers(const uint64_t *invalid, const uint64_t *pos_off, uint64_t n_reads,
+void launch_count_kmers(const uint64_t *invalid, const uint64_t *codes, uint64_t seed,
+                        uint64_t sampling, const uint64_t *pos_off, uint64_t n_reads,
                         uint64_t positions, uint32_t k, uint64_t *totals, hipStream_t s) {
   uint64_t words = (positions + 63) / 64;
   if (!words || !n_reads) return;
-  count_kmers_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(invalid, pos_off, n_reads,
-                                                                     positions, k, totals);
+  count_kmers_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      invalid, codes, seed, sampling, pos_off, n_reads, positions, k, totals);
 }
 
 // ---------------------------------------------------------------------------
@@ -668,6 +676,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
       const uint64_t lane_first = mixsplit(key, a.seed);
       const uint64_t lane_step = splitmix64(lane_first);
       const uint64_t lane_base = slot_base(lane_first, bk.geo);
+      if (a.sampling && lane_first % a.sampling) v = false;  // not a modimizer
       uint32_t rec_start = 0, rec_len = 0;
       if (a.rec.enabled && v) {
         uint32_t h = abandoned_hash(key, a.seed);
