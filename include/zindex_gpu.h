@@ -192,6 +192,21 @@ typedef struct zg_build_config {
                                   on the host by the same rule as the CPU build
                                   (the core is unique, so the abandoned keys
                                   are the same); 0 = CPU */
+  uint32_t homopolymer_compression; /* 1 = collapse runs of one base (case-
+                                       insensitive; other bytes kept as
+                                       breaks) before seeding, at build and
+                                       query (src/lib.rs:209-222) */
+  uint32_t findere_z;          /* index_mode 3 (findere): index (k - z)-mers and
+                                  count a k-mer when its z + 1 s-mers all hit;
+                                  default 10 */
+  char seed_pattern[32];       /* spaced seed: a symmetric '#'/'_' pattern of
+                                  width k (m for minimizers), NUL-terminated;
+                                  "" = none (src/anchors.rs:10-90) */
+  uint32_t randstrobemers;     /* 1 = link two k-mers or selected minimizers
+                                  into randstrobemers (src/anchors.rs:136-256) */
+  uint32_t strobe_min;         /* inclusive strobe-start distance bounds;
+                                  0 = the strobe length (k or m) and three */
+  uint32_t strobe_max;         /*   times it */
 } zg_build_config;
 
 /* Fills cfg with the reference CLI defaults (src/bin/zorindex.rs:55-159). */

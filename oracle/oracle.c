@@ -30,6 +30,17 @@
  *                        (storage.rs:5460-5468), plane == legacy layout
  *                        (storage.rs:5343-5423); no false negatives and
  *                        FPR ~ 2^-b on built indexes (option_coverage_cli.rs)
+ *   zo_homopolymer_compress
+ *                        known answer AaAACccNNNTttAA -> ACNNNTA and the
+ *                        repeated-run feature equality (minimizer_schemes_cli.rs:
+ *                        248-282)
+ *   zo_spaced_project / zo_spaced_canonical
+ *                        projection known answer, equal canonical keys and
+ *                        reverse-complement equality (src/anchors.rs:750-773);
+ *                        all-retained pattern leaves keys unchanged
+ *                        (src/anchors.rs:839-852); bit-run extraction
+ *   zo_read_features     modimizer sampling-one cases (src/lib.rs:1089-1113),
+ *                        nesting and density of sampled sets
  *   zo_query_reads       README known answers (970/970) on bundled data;
  *                        exact k-mer membership lower bound
  *                        (option_coverage_cli.rs:137-244)
@@ -636,19 +647,58 @@ void zo_lookup(const zo_index *idx, uint64_t key, uint64_t weight,
   if (snap != snap_small) free(snap);
 }
 
+size_t zo_homopolymer_compress(const uint8_t *seq, size_t len, uint8_t *out) {
+  size_t n = 0;
+  int prev = -1;
+  for (size_t i = 0; i < len; i++) {
+    int c = zo_encode_base(seq[i]);
+    if (c < 0 || c != prev) out[n++] = c < 0 ? seq[i] : (uint8_t)"ACGT"[c];
+    prev = c;
+  }
+  return n;
+}
+
+uint64_t zo_spaced_project(uint64_t encoded, uint32_t width, uint64_t mask) {
+  uint64_t r = 0;
+  for (int i = (int)width - 1; i >= 0; i--)
+    if (mask & (3ULL << (2 * i))) r = (r << 2) | ((encoded >> (2 * i)) & 3);
+  return r;
+}
+
+uint64_t zo_spaced_canonical(uint64_t encoded, uint32_t width, uint64_t mask) {
+  int weight = 0;
+  for (uint32_t i = 0; i < width; i++) weight += (mask >> (2 * i)) & 1;
+  return zo_canonical_encoded(zo_spaced_project(encoded, width, mask), weight);
+}
+
 size_t zo_read_features(const zo_index *idx, const uint8_t *seq, size_t len,
                         uint64_t *out) {
-  if (idx->mode == 1)
-    return zo_scan_ocm(seq, len, idx->k, idx->m, idx->seed, NULL, out);
-  size_t n = zo_scan_kmers(seq, len, idx->k, out);
-  if (idx->mode == 2) {
-    /* scan_modimizers (src/lib.rs:796-806): canonical k-mers whose
-     * mixsplit(kmer, seed) is a multiple of the sampling */
-    size_t kept = 0;
-    for (size_t i = 0; i < n; i++)
-      if (zo_mixsplit(out[i], idx->seed) % idx->sampling == 0) out[kept++] = out[i];
-    n = kept;
+  uint8_t *compressed = NULL;
+  if (idx->hpc) {
+    compressed = malloc(len ? len : 1);
+    len = zo_homopolymer_compress(seq, len, compressed);
+    seq = compressed;
   }
+  size_t n;
+  if (idx->mode == 1) {
+    n = zo_scan_ocm(seq, len, idx->k, idx->m, idx->seed, NULL, out);
+  } else {
+    n = zo_scan_kmers(seq, len, idx->k, out);
+    if (idx->mode == 2) {
+      /* scan_modimizers (src/lib.rs:796-806): canonical k-mers whose
+       * mixsplit(kmer, seed) is a multiple of the sampling */
+      size_t kept = 0;
+      for (size_t i = 0; i < n; i++)
+        if (zo_mixsplit(out[i], idx->seed) % idx->sampling == 0) out[kept++] = out[i];
+      n = kept;
+    }
+  }
+  /* anchors::scan (src/anchors.rs:265-283): the pattern masks each plain
+   * feature (minimizers are selected before masking) */
+  if (idx->seed_width)
+    for (size_t i = 0; i < n; i++)
+      out[i] = zo_spaced_canonical(out[i], idx->seed_width, idx->seed_mask);
+  free(compressed);
   return n;
 }
 

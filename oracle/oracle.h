@@ -97,7 +97,17 @@ typedef struct zo_index {
   uint32_t n_banks;
   const zo_recovery *recovery; /* NULL = no recovery */
   uint64_t sampling;          /* modimizer sampling (mode 2) */
+  int hpc;                    /* homopolymer compression before seeding */
+  uint32_t seed_width;        /* spaced seed width (0 = none) */
+  uint64_t seed_mask;         /* 3 per retained base, first base most significant */
 } zo_index;
+
+/* homopolymer_compress (src/lib.rs:209-222); returns the output length. */
+size_t zo_homopolymer_compress(const uint8_t *seq, size_t len, uint8_t *out);
+
+/* SpacedSeed::project_portable and ::canonical (src/anchors.rs:67-90). */
+uint64_t zo_spaced_project(uint64_t encoded, uint32_t width, uint64_t mask);
+uint64_t zo_spaced_canonical(uint64_t encoded, uint32_t width, uint64_t mask);
 
 /* src/power_merge/storage.rs:2761-2925 (lookup_selected, ranges = None),
  * with WeightedScores (3123-3140). Adds weight to scores[color]. */

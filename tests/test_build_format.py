@@ -103,6 +103,65 @@ def test_modimizer_build_inserts_sampled_kmers(tmp_path, small_colors, sampling)
             assert int(scores[c]) == before + 1, (c, int(key))
 
 
+K31_PATTERN = "###_##_#_##_#_###_#_##_#_##_###"
+M19_PATTERN = "#####_#_###_#_#####"
+
+
+@pytest.mark.parametrize("mode,hpc,pattern", [
+    (0, True, None), (1, True, None), (2, True, None),
+    (0, False, K31_PATTERN), (1, False, M19_PATTERN), (0, True, K31_PATTERN)])
+def test_build_with_compression_and_spaced_seeds(tmp_path, small_colors, mode, hpc, pattern):
+    cfg = dict(fingerprint_bits=16, hashes=3, index_mode=mode, homopolymer_compression=int(hpc),
+               modimizer_sampling=3)
+    if mode == 1:
+        cfg.update(k=31, minimizer_size=19)
+    if pattern:
+        cfg["seed_pattern"] = pattern
+    idx = _build(small_colors, **cfg)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["homopolymer_compression"] is hpc and man.get("seed_pattern") == pattern
+    with zindex_amd.Index.open(str(out)) as reopened:
+        assert reopened.meta["n_colors"] == len(small_colors)
+    info = tmp_path / "info.txt"
+    fd = os.open(info, os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+    try:
+        zindex_amd.info_command(str(out), out_fd=fd)
+    finally:
+        os.close(fd)
+    text = info.read_text()
+    # tests/minimizer_schemes_cli.rs:157, src/anchors.rs:112-126
+    assert f"homopolymer_compression\t{str(hpc).lower()}\n" in text
+    assert f"seed_pattern\t{pattern or 'none'}\n" in text and "randstrobemers\tfalse\n" in text
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        keys = np.unique(zr.read_features(seq, man["k"], man["seed"], mode=mode,
+                                          m=man["minimizer_size"], sampling=3, hpc=hpc,
+                                          pattern=pattern))
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:1500]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+
+
+@pytest.mark.parametrize("cfg,msg", [
+    (dict(seed_pattern="##_"), "symmetric"),
+    (dict(seed_pattern="#x#"), "only"),
+    (dict(seed_pattern="___"), "at least one"),
+    (dict(seed_pattern="#" * 21), "must match k"),
+    (dict(index_mode=1, k=31, minimizer_size=21, seed_pattern="#" * 31), "minimizer size m"),
+    (dict(index_mode=2, seed_pattern="#" * 31), "require --index-mode"),
+    (dict(randstrobemers=1), "randstrobemers"),
+])
+def test_seed_option_validation(small_colors, cfg, msg):
+    with pytest.raises(Exception, match=msg):
+        _build(small_colors[:1], **cfg)
+
+
 def test_modimizer_sampling_zero_is_rejected(small_colors):
     with pytest.raises(Exception, match="sampling"):
         _build(small_colors[:1], index_mode=2, modimizer_sampling=0)

@@ -187,6 +187,63 @@ def test_modimizer_seeds_and_counts(sampling, bits):
     check_parity(idx, oracle_for(idx), reads, label=f"modimizers s{sampling} b{bits}")
 
 
+K31_PATTERN = "###_##_#_##_#_###_#_##_#_##_###"
+M19_PATTERN = "#####_#_###_#_#####"
+
+
+def _with_runs(reads):
+    """Long homopolymer runs, mixed case and N breaks inside some reads."""
+    out = list(reads)
+    for i in range(5, len(out), 7):
+        r = bytearray(out[i])
+        if len(r) > 120:
+            r[30:50] = b"AAAAAAaaaaCCCCccccGG"
+            r[70:75] = b"TtTtN"
+            r[100:104] = b"gggg"
+        out[i] = bytes(r)
+    return out
+
+
+@pytest.mark.parametrize("mode,hpc,pattern", [
+    (0, True, None), (1, True, None), (2, True, None),
+    (0, False, K31_PATTERN), (1, False, M19_PATTERN), (1, True, M19_PATTERN)])
+def test_compression_and_spaced_seed_parity(mode, hpc, pattern):
+    colors = make_colors(140 + mode, 6, 30000)
+    reads = _with_runs(make_reads(141 + mode, colors, 160, lo=20, hi=2500))
+    idx = build(colors, fingerprint_bits=8, hashes=3, index_mode=mode, k=31, minimizer_size=19,
+                modimizer_sampling=3, homopolymer_compression=int(hpc),
+                seed_pattern=pattern or "")
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, _ = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.read_features(r, 31, 69, mode=mode, m=19, sampling=3, hpc=hpc, pattern=pattern)
+        assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
+    check_parity(idx, oracle_for(idx), reads, label=f"mode{mode} hpc{hpc} {pattern}")
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_compressed_index_answers_compressed_and_expanded_queries(mode):
+    # tests/minimizer_schemes_cli.rs:102-177: an index of an expanded sequence
+    # built with compression finds every seed of the compressed sequence, of its
+    # reverse complement and of the expanded sequence itself
+    rng = np.random.default_rng(150 + mode)
+    raw = synth.random_genome(rng, 1800).tobytes()
+    compressed = zr.homopolymer_compress(raw)
+    expanded = b"".join(bytes([c]) * int(n) for c, n in
+                        zip(compressed, rng.integers(1, 4, len(compressed))))
+    rc = bytes(b"TGCA"[b"ACGT".index(c)] for c in reversed(compressed))
+    idx = build([expanded], fingerprint_bits=8, hashes=3, index_mode=mode, k=31,
+                minimizer_size=21, modimizer_sampling=16, homopolymer_compression=1)
+    idx.load()
+    expected = len(zr.read_features(compressed, 31, 69, mode=mode, m=21, sampling=16))
+    assert expected > 0
+    for query in (compressed, rc, expanded):
+        buf, off = zindex_amd.pack_reads([query])
+        totals, counts, agg = idx.query_reads(buf, off, records=True)
+        assert int(totals[0]) == expected and int(counts[0][0]) == expected
+
+
 def test_kmer_seeds_match_oracle_scan(base_case):
     colors, reads, idx, orc = base_case
     buf, off = zindex_amd.pack_reads(reads)

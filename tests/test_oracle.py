@@ -145,6 +145,78 @@ def test_ocm_splits_runs_and_rejects_even_windows():
     assert len(zr.scan_ocm(seq, 49, 32, 69)[0]) == 0
 
 
+def test_homopolymer_compress_known_answer():
+    # tests/minimizer_schemes_cli.rs:250-254
+    assert zr.homopolymer_compress(b"AaAACccNNNTttAA") == b"ACNNNTA"
+    assert zr.homopolymer_compress(b"") == b""
+    assert zr.homopolymer_compress(b"acgtNNacgt") == b"ACGTNNACGT"
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_homopolymer_runs_give_the_compressed_features(mode):
+    # tests/minimizer_schemes_cli.rs:255-282: two copies of ACGTA behind runs of
+    # four or five equal bases, split by an N run
+    expanded = b"AAAACCCCGGGGTTTTAAAAANNNAAAACCCCGGGGTTTTAAAAA"
+    got = zr.read_features(expanded, 5, 69, mode=mode, m=3, sampling=1, hpc=True).tolist()
+    once = zr.read_features(b"ACGTA", 5, 69, mode=mode, m=3, sampling=1).tolist()
+    assert got == once * 2 and len(once) > 0
+
+
+def _pext(x, mask):
+    out, o = 0, 0
+    for b in range(64):
+        if (mask >> b) & 1:
+            out |= ((x >> b) & 1) << o
+            o += 1
+    return out
+
+
+def test_spaced_seed_projection_known_answers():
+    # src/anchors.rs:750-773
+    width, mask = zr.spaced_seed("##_##_##")
+    assert width == 8 and bin(mask).count("1") // 2 == 6
+    enc = _encode
+    assert oracle().zo_spaced_project(enc(b"ACGTAACG"), width, mask) == enc(b"ACTACG")
+    a, b = b"ACATACCG", b"ACGTAGCG"
+    ca = oracle().zo_spaced_canonical(enc(a), width, mask)
+    assert ca == oracle().zo_spaced_canonical(enc(b), width, mask)
+    assert ca == oracle().zo_spaced_canonical(enc(_rc(a)), width, mask)
+    # src/anchors.rs:776-791: projection is the bit extraction for every width
+    for w in range(1, 32):
+        pat = ["_"] * w
+        for i in range(w):
+            if i == 0 or sm64(min(i, w - 1 - i)) & 1:
+                pat[i] = pat[w - 1 - i] = "#"
+        width, mask = zr.spaced_seed("".join(pat))
+        for i in range(100):
+            key = sm64(i)
+            assert oracle().zo_spaced_project(key, width, mask) == _pext(key, mask)
+
+
+def _encode(s: bytes) -> int:
+    v = 0
+    for ch in s:
+        v = (v << 2) | b"ACGT".index(ch)
+    return v
+
+
+@pytest.mark.parametrize("mode,k,m", [(0, 9, 3), (1, 31, 9)])
+def test_spaced_seeds_mask_plain_features(mode, k, m):
+    seq = _sequence(4000, 17)
+    plain = zr.read_features(seq, k, 69, mode=mode, m=m)
+    # src/anchors.rs:839-852: an all-retained pattern leaves keys unchanged
+    full = "#" * (k if mode == 0 else m)
+    assert zr.read_features(seq, k, 69, mode=mode, m=m, pattern=full).tolist() == plain.tolist()
+    # src/anchors.rs:854-866: minimizers are selected before masking
+    pat = "##__#__##" if mode == 1 else "##_#_#_##"
+    width, mask = zr.spaced_seed(pat)
+    got = zr.read_features(seq, k, 69, mode=mode, m=m, pattern=pat).tolist()
+    assert got == [oracle().zo_spaced_canonical(int(x), width, mask) for x in plain]
+    # src/anchors.rs:869-908: reverse complements keep the anchor multiset
+    rc = zr.read_features(_rc(seq), k, 69, mode=mode, m=m, pattern=pat).tolist()
+    assert sorted(rc) == sorted(got)
+
+
 # src/lib.rs:1089-1113: with sampling 1 the modimizers are every canonical k-mer
 @pytest.mark.parametrize("seq,k,seed", [(b"ACGTACGT", 3, 1), (b"AAAACCCC", 4, 2),
                                         (b"TTTTGGGG", 5, 3), (b"GATTACA", 3, 4),

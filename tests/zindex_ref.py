@@ -41,7 +41,19 @@ class ZoIndex(C.Structure):
         ("m", C.c_int), ("seed", C.c_uint64), ("segment_length", C.c_uint64),
         ("n_colors", C.c_uint32), ("banks", C.POINTER(ZoBank)), ("n_banks", C.c_uint32),
         ("recovery", C.POINTER(ZoRecovery)), ("sampling", C.c_uint64),
+        ("hpc", C.c_int), ("seed_width", C.c_uint32), ("seed_mask", C.c_uint64),
     ]
+
+
+def spaced_seed(pattern):
+    """(width, mask) of a '#'/'_' pattern: two set bits per retained base, the
+    first base most significant (src/anchors.rs:15-43); (0, 0) for None."""
+    if not pattern:
+        return 0, 0
+    mask = 0
+    for ch in pattern:
+        mask = (mask << 2) | (3 if ch == "#" else 0)
+    return len(pattern), mask
 
 
 _oracle = None
@@ -87,6 +99,12 @@ def oracle() -> C.CDLL:
         o.zo_scan_ocm_bruteforce.restype = C.c_size_t
         o.zo_scan_ocm_bruteforce.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint64,
                                              u64p, u64p]
+        o.zo_homopolymer_compress.restype = C.c_size_t
+        o.zo_homopolymer_compress.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p]
+        o.zo_spaced_project.restype = C.c_uint64
+        o.zo_spaced_project.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+        o.zo_spaced_canonical.restype = C.c_uint64
+        o.zo_spaced_canonical.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
         o.zo_read_features.restype = C.c_size_t
         o.zo_read_features.argtypes = [C.POINTER(ZoIndex), C.c_char_p, C.c_size_t, u64p]
         o.zo_lookup.restype = None
@@ -114,10 +132,18 @@ def scan_kmers(seq: bytes, k: int) -> np.ndarray:
     return out[:n]
 
 
+def homopolymer_compress(seq: bytes) -> bytes:
+    out = C.create_string_buffer(max(len(seq), 1))
+    n = oracle().zo_homopolymer_compress(seq, len(seq), out)
+    return out.raw[:n]
+
+
 def read_features(seq: bytes, k: int, seed: int, mode: int = 0, m: int = 0,
-                  sampling: int = 16) -> np.ndarray:
+                  sampling: int = 16, hpc: bool = False, pattern=None) -> np.ndarray:
     """The query-side features of one read (k-mers, OCM or modimizers)."""
-    idx = ZoIndex(k=k, mode=mode, m=m, seed=seed, sampling=sampling)
+    width, mask = spaced_seed(pattern)
+    idx = ZoIndex(k=k, mode=mode, m=m, seed=seed, sampling=sampling, hpc=int(hpc),
+                  seed_width=width, seed_mask=mask)
     out = np.zeros(max(len(seq), 1), np.uint64)
     n = oracle().zo_read_features(C.byref(idx), seq, len(seq), out.ctypes.data_as(u64p))
     return out[:n]
@@ -393,7 +419,9 @@ class OracleIndex:
                          manifest.get("segment_length") or 0, len(manifest["colors"]), zb,
                          len(manifest["banks"]),
                          C.pointer(self._rec) if self._rec is not None else None,
-                         manifest.get("modimizer_sampling") or 16)
+                         manifest.get("modimizer_sampling") or 16,
+                         int(bool(manifest.get("homopolymer_compression"))),
+                         *spaced_seed(manifest.get("seed_pattern")))
 
     @classmethod
     def from_dir(cls, path, recover: bool = True) -> "OracleIndex":

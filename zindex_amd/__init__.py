@@ -60,7 +60,9 @@ def build_config(**kw) -> BuildConfig:
     lib().zg_build_config_default(C.byref(cfg))
     for key, value in kw.items():
         if key == "index_mode" and isinstance(value, str):
-            value = {"kmers": 0, "minimizers": 1, "modimizers": 2}[value]
+            value = {"kmers": 0, "minimizers": 1, "modimizers": 2, "findere": 3}[value]
+        if key == "seed_pattern" and isinstance(value, str):
+            value = value.encode()
         if not hasattr(cfg, key):
             raise TypeError(f"unknown build option {key}")
         setattr(cfg, key, value)

@@ -51,6 +51,9 @@ class BuildConfig(C.Structure):
         ("max_subindex_size", C.c_uint64), ("merge_group_size", C.c_uint32),
         ("threads", C.c_uint32), ("drop_abandoned", C.c_int32),
         ("modimizer_sampling", C.c_uint32), ("device", C.c_uint32),
+        ("homopolymer_compression", C.c_uint32), ("findere_z", C.c_uint32),
+        ("seed_pattern", C.c_char * 32), ("randstrobemers", C.c_uint32),
+        ("strobe_min", C.c_uint32), ("strobe_max", C.c_uint32),
     ]
 
 

@@ -90,10 +90,8 @@ static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
 static void check_query_support(const Manifest &m) {
-  if (m.has_seed_pattern || m.has_randstrobemers)
-    fail(ZG_ERR_UNSUPPORTED, "spaced seeds and randstrobemers are not supported by the GPU query");
-  if (m.homopolymer_compression)
-    fail(ZG_ERR_UNSUPPORTED, "homopolymer compression is not supported by the GPU query");
+  if (m.has_randstrobemers)
+    fail(ZG_ERR_UNSUPPORTED, "randstrobemers are not supported by the GPU query");
   if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32) fail(ZG_ERR_UNSUPPORTED, "k-mer indexes with k > 32 are not supported by the GPU query");
     if (m.index_mode == 2 && m.modimizer_sampling == 0)
@@ -308,6 +306,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   check_query_support(m);
   ensure(memory == ZG_MEMORY_ALL || memory == ZG_MEMORY_SUBINDEX, "invalid query memory mode",
          ZG_ERR_INVALID);
+  idx->spaced = m.has_seed_pattern ? spaced_runs(m.seed_width, m.seed_mask) : SpacedRuns{};
   int dev = 0;
   check_hip(hipGetDevice(&dev), "hipGetDevice");
   hipDeviceProp_t prop;
@@ -793,6 +792,7 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   a.n_colors = (uint32_t)idx->m.colors.size();
   a.nontemporal = b->nontemporal;
   a.sampling = (idx->m.index_mode == 2 && !b->explicit_keys) ? idx->m.modimizer_sampling : 0;
+  a.spaced = b->explicit_keys ? SpacedRuns{} : idx->spaced;
   return a;
 }
 
@@ -865,8 +865,28 @@ void seed_batch(zg_index *idx, zg_batch *b) {
   uint64_t cwords = (b->positions + 31) / 32 + 4, iwords = (b->positions + 63) / 64 + 4;
   launch_fill_u64(b->codes.as<uint64_t>(), cwords, 0, s);
   launch_fill_u64(b->invalid.as<uint64_t>(), iwords, ~0ULL, s);
-  launch_pack(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
-              b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
+  if (m.homopolymer_compression) {
+    // each read compressed in place: its bases first, the rest invalid
+    const uint64_t words = (b->positions + 63) / 64;
+    b->hpc_keep.reserve((words + 1) * 8);
+    b->hpc_counts.reserve((words + 1) * 4);
+    b->hpc_off.reserve((words + 1) * 8);
+    launch_hpc_keep(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+                    b->n_reads, b->positions, b->hpc_keep.as<uint64_t>(),
+                    b->hpc_counts.as<uint32_t>(), s);
+    launch_fill_u32(b->hpc_counts.as<uint32_t>() + words, 1, 0, s);
+    const size_t tb = scan_temp_bytes_u32_to_u64(words + 1);
+    b->scan_temp.reserve(tb);
+    launch_exclusive_scan_u32_to_u64(b->hpc_counts.as<uint32_t>(), b->hpc_off.as<uint64_t>(),
+                                     words + 1, b->scan_temp.p, tb, s);
+    launch_hpc_place(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+                     b->n_reads, b->positions, b->hpc_keep.as<uint64_t>(),
+                     b->hpc_off.as<uint64_t>(), b->codes.as<uint64_t>(),
+                     b->invalid.as<uint64_t>(), s);
+  } else {
+    launch_pack(b->seq.as<uint8_t>(), b->read_off.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+                b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
+  }
   launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
   if (m.index_mode != 1) {  // k-mers, or modimizers (sampled k-mers)
     launch_count_kmers(b->invalid.as<uint64_t>(), b->codes.as<uint64_t>(), m.seed,
@@ -898,7 +918,7 @@ void seed_batch(zg_index *idx, zg_batch *b) {
     b->key_win.reserve((total + 64) * 8);
     launch_ocm_compact(b->codes.as<uint64_t>(), b->flag.as<uint8_t>(), b->delta.as<uint8_t>(),
                        b->block_off.as<uint64_t>(), b->positions, (uint32_t)m.minimizer_size,
-                       b->keys.as<uint64_t>(), b->key_win.as<uint64_t>(), s);
+                       idx->spaced, b->keys.as<uint64_t>(), b->key_win.as<uint64_t>(), s);
     b->key_off.reserve((b->n_reads + 1) * 8);
     launch_segment_bounds(b->key_win.as<uint64_t>(), total, b->pos_off.as<uint64_t>(), b->n_reads,
                           b->key_off.as<uint64_t>(), s);
@@ -1359,7 +1379,8 @@ extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_
       uint64_t p0 = (read_off[r] - read_off[0]) + r, len = read_off[r + 1] - read_off[r];
       for (uint64_t p = p0; p < p0 + len; p++) {
         if (bits(inv, p) & wmask) continue;
-        const uint64_t key = canonical_from_window(bits(codes, 2 * p), (int)k);
+        uint64_t key = canonical_from_window(bits(codes, 2 * p), (int)k);
+        if (idx->spaced.n) key = spaced_canonical(key, idx->spaced);
         if (idx->m.index_mode == 2 && mixsplit(key, idx->m.seed) % idx->m.modimizer_sampling)
           continue;  // not a modimizer
         if (n < cap) {
@@ -1660,10 +1681,21 @@ extern "C" int zg_info_command(const char *dir, int out_fd) {
     line("largest_subindex_bytes", std::to_string(largest));
     line("hash_partitions", std::to_string(partitions));
     line("input_documents", std::to_string(docs));
+    if (m.index_mode == 3) {
+      line("findere_z", std::to_string(m.findere_z));
+      line("indexed_k", std::to_string(m.k - m.findere_z));
+    }
     line("minimizer_size", std::to_string(m.minimizer_size));
     line("modimizer_sampling", std::to_string(m.modimizer_sampling));
     line("minimizer_scheme", m.minimizer_scheme ? "open-closed-mod" : "simd");
     line("homopolymer_compression", m.homopolymer_compression ? "true" : "false");
+    // anchors::print_settings (src/anchors.rs:112-126)
+    line("seed_pattern", m.has_seed_pattern ? m.seed_pattern : "none");
+    line("randstrobemers", m.has_randstrobemers ? "true" : "false");
+    if (m.has_randstrobemers) {
+      line("strobe_min", std::to_string(m.strobe_min));
+      line("strobe_max", std::to_string(m.strobe_max));
+    }
     line("merge_planner", m.merge_planner_label());
     if (m.has_segment_length) line("segment_length", std::to_string(m.segment_length));
     line("abandoned_recovery", m.abandoned_recovery ? "true" : "false");

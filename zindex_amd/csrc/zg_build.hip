@@ -585,15 +585,55 @@ struct zg_builder {
 
 namespace zg {
 
+// homopolymer_compress (src/lib.rs:209-222): drops a base equal to the one
+// before it (case-insensitive); other bytes are kept and break runs. Kept
+// bases are written upper case. Returns the compressed length.
+static size_t homopolymer_compress(const uint8_t *seq, size_t len, uint8_t *out) {
+  size_t n = 0;
+  int prev = -1;
+  for (size_t i = 0; i < len; i++) {
+    const int c = enc(seq[i]);
+    if (c < 0 || c != prev) out[n++] = c < 0 ? seq[i] : (uint8_t)"ACGT"[c];
+    prev = c;
+  }
+  return n;
+}
+
+static SpacedRuns config_spaced(const zg_build_config &cfg) {
+  if (!cfg.seed_pattern[0]) return SpacedRuns{};
+  uint32_t width = 0;
+  uint64_t mask = 0;
+  std::string err = parse_spaced_seed(cfg.seed_pattern, &width, &mask);
+  ensure(err.empty(), err.c_str(), -1);
+  return spaced_runs(width, mask);
+}
+
+// scan_index_features (src/lib.rs:573-642, anchors.rs:265-283) per record.
 static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64_t *off,
                     uint64_t n, std::vector<uint64_t> &keys) {
+  const SpacedRuns spaced = config_spaced(cfg);
+  std::vector<uint8_t> compressed;
   for (uint64_t r = 0; r < n; r++) {
     const uint8_t *p = seq + off[r];
     size_t len = off[r + 1] - off[r];
+    if (cfg.homopolymer_compression) {
+      compressed.resize(len);
+      len = homopolymer_compress(p, len, compressed.data());
+      p = compressed.data();
+    }
+    const size_t at = keys.size();
     if (cfg.index_mode == 1) {
       host_ocm(p, len, (int)cfg.k, (int)cfg.minimizer_size, cfg.seed, keys);
+    } else if (cfg.index_mode == 3) {
+      // findere: canonical (k - z)-mers of the runs of at least k valid bases
+      for (size_t a = 0; a < len;) {
+        if (enc(p[a]) < 0) { a++; continue; }
+        size_t b = a;
+        while (b < len && enc(p[b]) >= 0) b++;
+        if (b - a >= cfg.k) host_kmers(p + a, b - a, (int)(cfg.k - cfg.findere_z), keys);
+        a = b;
+      }
     } else {
-      const size_t at = keys.size();
       host_kmers(p, len, (int)cfg.k, keys);
       if (cfg.index_mode == 2) {  // scan_modimizers (src/lib.rs:796-806)
         size_t kept = at;
@@ -602,6 +642,8 @@ static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64
         keys.resize(kept);
       }
     }
+    if (spaced.n)
+      for (size_t i = at; i < keys.size(); i++) keys[i] = spaced_canonical(keys[i], spaced);
   }
 }
 
@@ -665,6 +707,7 @@ extern "C" void zg_build_config_default(zg_build_config *c) {
   c->threads = 0;
   c->drop_abandoned = 0;
   c->modimizer_sampling = 16;
+  c->findere_z = 10;
 }
 
 static void validate_config(const zg_build_config &c) {
@@ -687,9 +730,31 @@ static void validate_config(const zg_build_config &c) {
     ensure(c.k >= 1 && c.k <= 255 && c.k % 2 == 1 && c.minimizer_size >= 1 &&
                c.minimizer_size <= 31 && c.minimizer_size <= c.k,
            "invalid minimizer window/length", -1);
+  } else if (c.index_mode == 3) {
+    // findere::indexed_length (src/findere.rs:17-21)
+    ensure(c.k > 0, "findere query k must be positive", -1);
+    ensure(c.findere_z < c.k, "--findere-z must be smaller than query k", -1);
+    ensure(c.k - c.findere_z <= 32, "findere s-mers longer than 32 are not built here", -6);
   } else {
-    fail(-6, "only k-mer, open-closed mod-minimizer and modimizer indexes are built here");
+    fail(-1, "unsupported index mode");
   }
+  if (c.seed_pattern[0] || c.randstrobemers)
+    ensure(c.index_mode == 0 || c.index_mode == 1,
+           "spaced seeds and randstrobemers require --index-mode kmers or minimizers", -1);
+  if (c.seed_pattern[0]) {
+    ensure(memchr(c.seed_pattern, 0, sizeof c.seed_pattern) != nullptr,
+           "seed pattern width must be in 1..=31", -1);
+    uint32_t width = 0;
+    uint64_t mask = 0;
+    const std::string err = parse_spaced_seed(c.seed_pattern, &width, &mask);
+    ensure(err.empty(), err.c_str(), -1);
+    const uint32_t length = c.index_mode == 1 ? c.minimizer_size : c.k;
+    if (width != length)
+      fail(-1, "seed pattern width " + std::to_string(width) + " must match " +
+                   (c.index_mode == 1 ? "minimizer size m" : "k") + " (" +
+                   std::to_string(length) + ")");
+  }
+  ensure(!c.randstrobemers, "randstrobemers are not built here", -6);
   ensure(c.merge_group_size >= 1 && c.merge_group_size <= 64 - b, "invalid merge group size",
          -1);
   ensure(c.device <= 1, "device must be 0 (CPU peel) or 1 (GPU peel)", -1);
@@ -1019,7 +1084,13 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   m.minimizer_size = cfg.minimizer_size;
   m.minimizer_scheme = 1;
   m.modimizer_sampling = cfg.modimizer_sampling ? cfg.modimizer_sampling : 16;
-  m.findere_z = 10;
+  m.findere_z = cfg.findere_z;
+  m.homopolymer_compression = cfg.homopolymer_compression != 0;
+  if (cfg.seed_pattern[0]) {
+    m.has_seed_pattern = true;
+    m.seed_pattern = cfg.seed_pattern;
+    parse_spaced_seed(m.seed_pattern, &m.seed_width, &m.seed_mask);
+  }
   m.slot_scale = cfg.slot_scale;
   m.has_max_subindex_size = true;
   m.max_subindex_size = cfg.max_subindex_size;
@@ -1039,12 +1110,12 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   j.set("index_mode", Json::u64(m.index_mode));
   j.set("minimizer_size", Json::u64(m.minimizer_size));
   j.set("minimizer_scheme", Json::str("open-closed-mod"));
-  j.set("homopolymer_compression", Json::boolean(false));
-  j.set("seed_pattern", Json::null());
+  j.set("homopolymer_compression", Json::boolean(m.homopolymer_compression));
+  j.set("seed_pattern", cfg.seed_pattern[0] ? Json::str(cfg.seed_pattern) : Json::null());
   j.set("randstrobemers", Json::null());
   j.set("randstrobe_scheme", Json::null());
   j.set("modimizer_sampling", Json::u64(m.modimizer_sampling));
-  j.set("findere_z", Json::u64(10));
+  j.set("findere_z", Json::u64(m.findere_z));
   j.set("sketch_size", Json::u64(128));
   j.set("sketch_method", Json::str("none: fixed consecutive grouping (no sketches)"));
   j.set("merge_planner", Json::str("naive"));

@@ -111,6 +111,20 @@ bool opt_bool(const Json &o, const std::string &n, bool dflt) {
 
 }  // namespace
 
+std::string parse_spaced_seed(const std::string &pattern, uint32_t *width, uint64_t *mask) {
+  if (pattern.empty() || pattern.size() > 31) return "seed pattern width must be in 1..=31";
+  for (char c : pattern)
+    if (c != '#' && c != '_') return "seed pattern accepts only '#' and '_'";
+  if (pattern.find('#') == std::string::npos) return "seed pattern must contain at least one '#'";
+  if (!std::equal(pattern.begin(), pattern.end(), pattern.rbegin()))
+    return "seed pattern must be symmetric under reversal for reverse-complement indexing";
+  uint64_t mk = 0;
+  for (char c : pattern) mk = (mk << 2) | (c == '#' ? 3u : 0u);
+  *width = (uint32_t)pattern.size();
+  *mask = mk;
+  return "";
+}
+
 Manifest parse_manifest(const std::string &text) {
   Manifest m;
   try {
@@ -148,12 +162,16 @@ Manifest parse_manifest(const std::string &text) {
   }
   m.homopolymer_compression = opt_bool(o, "homopolymer_compression", false);
   if (const Json *s = o.get("seed_pattern"); s && !s->is_null()) {
-    as_str(*s, "seed_pattern");
+    m.seed_pattern = as_str(*s, "seed_pattern");
     m.has_seed_pattern = true;
+    std::string err = parse_spaced_seed(m.seed_pattern, &m.seed_width, &m.seed_mask);
+    if (!err.empty()) fail(-3, err);
   }
   if (const Json *s = o.get("randstrobemers"); s && !s->is_null()) {
     if (s->kind != Json::Object) field_error("randstrobemers", "expected an object");
     m.has_randstrobemers = true;
+    m.strobe_min = req_u64(*s, "min_distance");
+    m.strobe_max = req_u64(*s, "max_distance");
   }
   if (const Json *s = o.get("randstrobe_scheme"); s && !s->is_null())
     m.randstrobe_scheme = as_str(*s, "randstrobe_scheme");
@@ -268,6 +286,19 @@ void validate_manifest(const Manifest &m) {
   if ((m.has_seed_pattern || m.has_randstrobemers))
     ensure(m.index_mode == 0 || m.index_mode == 1,
            "spaced seeds and randstrobemers require --index-mode kmers or minimizers");
+  // FeatureConfig::validate_anchors (src/lib.rs:177-205)
+  if (m.has_seed_pattern) {
+    const uint64_t length = m.index_mode == 1 ? m.minimizer_size : m.k;
+    if (m.seed_width != length)
+      fail(-3, "seed pattern width " + std::to_string(m.seed_width) + " must match " +
+                              (m.index_mode == 1 ? "minimizer size m" : "k") + " (" +
+                              std::to_string(length) + ")");
+  }
+  if (m.has_randstrobemers) {
+    ensure(m.strobe_min > 0, "randstrobemer minimum distance must be greater than zero");
+    ensure(m.strobe_min <= m.strobe_max,
+           "randstrobemer minimum distance must not exceed maximum distance");
+  }
   switch (m.index_mode) {
     case 3:
       ensure(m.findere_z < m.k, "invalid findere z");

@@ -84,6 +84,10 @@ struct Manifest {
   uint32_t minimizer_scheme = 0;  // 0 simd, 1 open-closed-mod
   bool homopolymer_compression = false;
   bool has_seed_pattern = false, has_randstrobemers = false;
+  std::string seed_pattern;       // as written
+  uint32_t seed_width = 0;        // parsed pattern (SpacedSeed)
+  uint64_t seed_mask = 0;
+  uint64_t strobe_min = 0, strobe_max = 0;  // RandstrobeConfig
   std::string randstrobe_scheme;
   uint64_t modimizer_sampling = 0, findere_z = 0;
   double slot_scale = 1.0;
@@ -100,6 +104,12 @@ struct Manifest {
 };
 
 // Parses manifest.json text (serde rules: required fields, types, defaults).
+// SpacedSeed::from_str (src/anchors.rs:15-43): a symmetric '#'/'_' pattern
+// of 1..=31 bases with at least one '#'. mask holds 3 in the two bits of each
+// retained base, the pattern's first base most significant. Returns "" or the
+// reference's error message.
+std::string parse_spaced_seed(const std::string &pattern, uint32_t *width, uint64_t *mask);
+
 Manifest parse_manifest(const std::string &text);
 // Index::load_mode validation (storage.rs:2301-2637), metadata only.
 void validate_manifest(const Manifest &m);

@@ -100,6 +100,50 @@ __host__ __device__ inline uint64_t canonical_from_window(uint64_t w, int k) {
   return fwd < rc ? fwd : rc;
 }
 
+// min(x, reverse complement of x) over w <= 32 bases.
+__host__ __device__ inline uint64_t canonical_weight(uint64_t x, uint32_t w) {
+  uint64_t mask = w >= 32 ? ~0ULL : ((1ULL << (2 * w)) - 1);
+  uint64_t rc = reverse_pairs(~x & mask) >> (64 - 2 * w);
+  return x < rc ? x : rc;
+}
+
+// A spaced seed (SpacedSeed, src/anchors.rs:10-90): the retained bases of a
+// key in their order (pext with the pattern's two-bit mask) taken as runs of
+// consecutive retained bases, then made canonical over the seed's weight.
+// Runs count bases from the least significant end of the key.
+struct SpacedRuns {
+  uint32_t n = 0;  // runs; 0 = no pattern
+  uint32_t weight = 0;
+  uint8_t src[16] = {}, dst[16] = {}, len[16] = {};
+};
+__host__ __device__ inline uint64_t spaced_canonical(uint64_t key, const SpacedRuns &s) {
+  uint64_t r = 0;
+  for (uint32_t i = 0; i < s.n; i++) {
+    uint64_t m = s.len[i] >= 32 ? ~0ULL : ((1ULL << (2 * s.len[i])) - 1);
+    r |= ((key >> (2 * s.src[i])) & m) << (2 * s.dst[i]);
+  }
+  return canonical_weight(r, s.weight);
+}
+// The runs of a pattern of `width` bases whose retained base i (from the
+// least significant end) has bits 2i..2i+1 of mask set.
+inline SpacedRuns spaced_runs(uint32_t width, uint64_t mask) {
+  SpacedRuns s;
+  uint32_t out = 0;
+  for (uint32_t i = 0; i < width;) {
+    if (!((mask >> (2 * i)) & 3)) { i++; continue; }
+    uint32_t j = i;
+    while (j < width && ((mask >> (2 * j)) & 3)) j++;
+    s.src[s.n] = (uint8_t)i;
+    s.dst[s.n] = (uint8_t)out;
+    s.len[s.n] = (uint8_t)(j - i);
+    s.n++;
+    out += j - i;
+    i = j;
+  }
+  s.weight = out;
+  return s;
+}
+
 // Open-closed mod-minimizer parameters (implementation.rs:89-98).
 struct OcmParams {
   int w, t, s;

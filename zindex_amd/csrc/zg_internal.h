@@ -83,6 +83,7 @@ struct zg_index {
   void *staging[2] = {nullptr, nullptr};        // page-locked upload chunks
   hipEvent_t staging_ev[2] = {nullptr, nullptr};
   zg::DevBuf rec_dir, rec_post, rec_post_col, inner_table;
+  zg::SpacedRuns spaced;  // the manifest's seed pattern (n = 0: none)
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();
@@ -100,6 +101,8 @@ struct zg_batch {
   bool explicit_keys = false;
   uint64_t n_keys = 0;
   zg::DevBuf flag, delta, block_counts, block_off, keys, key_win, key_off, scan_temp;
+  // homopolymer compression: keep flags, counts and offsets per 64 positions
+  zg::DevBuf hpc_keep, hpc_counts, hpc_off;
   // outputs
   zg::DevBuf counts, partial, agg, queue;
   uint64_t waves = 0;

@@ -70,6 +70,7 @@ struct LookupArgs {
   uint32_t n_colors;
   uint32_t nontemporal;       // 1: row loads bypass cache retention
   uint64_t sampling;          // modimizers: keep k-mers with mixsplit % sampling == 0 (0 = all)
+  SpacedRuns spaced;          // spaced seed applied to each k-mer (kSeedKmers; n = 0: none)
 };
 
 struct KernelConfig {
@@ -107,7 +108,18 @@ void launch_ocm_select(const uint64_t *codes, const uint64_t *invalid, const uin
                        hipStream_t s);
 void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_t *delta,
                         const uint64_t *block_off, uint64_t positions, uint32_t m,
-                        uint64_t *keys, uint64_t *key_pos, hipStream_t s);
+                        const SpacedRuns &spaced, uint64_t *keys, uint64_t *key_pos,
+                        hipStream_t s);
+// Homopolymer compression of packed reads: keep flags per 64 positions, then
+// (after an exclusive scan of counts into keep_off) the placement of kept
+// bases into zeroed codes / all-ones invalid.
+void launch_hpc_keep(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                     uint64_t n_reads, uint64_t positions, uint64_t *keep_bits,
+                     uint32_t *counts, hipStream_t s);
+void launch_hpc_place(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                      uint64_t n_reads, uint64_t positions, const uint64_t *keep_bits,
+                      const uint64_t *keep_off, uint64_t *codes, uint64_t *invalid,
+                      hipStream_t s);
 uint32_t ocm_block_positions();
 // Exclusive prefix sums (hipcub) with a caller-provided temporary buffer.
 size_t scan_temp_bytes_u64(uint64_t n);

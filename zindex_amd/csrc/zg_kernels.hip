@@ -134,6 +134,112 @@ void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *p
 }
 
 // ---------------------------------------------------------------------------
+// Homopolymer compression on the device (homopolymer_compress,
+// src/lib.rs:209-222). A position is kept unless it holds a valid base equal
+// to the base before it; invalid bytes and read separators are kept (they
+// break runs). Each read's kept bases are packed to the start of its own
+// region of the position space; the rest of the region stays invalid, so
+// every later kernel sees the compressed read followed by invalid positions.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int position_code(const uint8_t *seq, const uint64_t *read_off,
+                                             const uint64_t *pos_off, uint64_t r, uint64_t p) {
+  const uint64_t off = p - pos_off[r];
+  const uint64_t start = read_off[r], len = read_off[r + 1] - start;
+  return off < len ? encode_ascii(seq[start + off]) : -1;
+}
+
+// keep_bits[w]: the kept positions among 64w..64w+63; counts[w] their number.
+__global__ void hpc_keep_kernel(const uint8_t *seq, const uint64_t *read_off,
+                                const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+                                uint64_t *keep_bits, uint32_t *counts) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t n_words = (positions + 63) / 64;
+  if (w >= n_words) return;
+  const uint64_t p0 = w * 64;
+  uint64_t r = find_segment(pos_off, n_reads, p0 ? p0 - 1 : 0);
+  int prev = -1;
+  if (p0) prev = position_code(seq, read_off, pos_off, r, p0 - 1);
+  uint64_t keep = 0;
+  for (int i = 0; i < 64; i++) {
+    const uint64_t p = p0 + i;
+    if (p >= positions) break;
+    while (pos_off[r + 1] <= p) r++;
+    const int c = position_code(seq, read_off, pos_off, r, p);
+    if (c < 0 || c != prev) keep |= 1ULL << i;
+    prev = c;
+  }
+  keep_bits[w] = keep;
+  counts[w] = (uint32_t)__popcll(keep);
+}
+
+// Writes each kept valid base at its read's start plus the number of kept
+// positions before it in the read. codes must be zero and invalid all ones
+// beforehand. One thread per 64 positions; writes are merged per target word.
+__global__ void hpc_place_kernel(const uint8_t *seq, const uint64_t *read_off,
+                                 const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+                                 const uint64_t *keep_bits, const uint64_t *keep_off,
+                                 uint64_t *codes, uint64_t *invalid) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint64_t n_words = (positions + 63) / 64;
+  if (w >= n_words) return;
+  const uint64_t p0 = w * 64;
+  const uint64_t kb = keep_bits[w];
+  if (!kb) return;
+  uint64_t r = find_segment(pos_off, n_reads, p0);
+  auto rank = [&](uint64_t p) {
+    const uint64_t below = p & 63;
+    return keep_off[p >> 6] + __popcll(keep_bits[p >> 6] & (below ? (~0ULL >> (64 - below)) : 0));
+  };
+  uint64_t read_rank = rank(pos_off[r]);
+  uint64_t cw = ~0ULL, cbits = 0, iw = ~0ULL, ibits = 0;
+  for (uint64_t m = kb; m; m &= m - 1) {
+    const int i = __builtin_ctzll(m);
+    const uint64_t p = p0 + i;
+    if (pos_off[r + 1] <= p) {
+      while (pos_off[r + 1] <= p) r++;
+      read_rank = rank(pos_off[r]);
+    }
+    const int c = position_code(seq, read_off, pos_off, r, p);
+    if (c < 0) continue;
+    const uint64_t q = pos_off[r] + (keep_off[w] + __popcll(kb & (i ? (~0ULL >> (64 - i)) : 0)) -
+                                     read_rank);
+    if ((q >> 5) != cw) {
+      if (cbits) atomicOr((unsigned long long *)&codes[cw], (unsigned long long)cbits);
+      cw = q >> 5;
+      cbits = 0;
+    }
+    cbits |= (uint64_t)c << (2 * (q & 31));
+    if ((q >> 6) != iw) {
+      if (ibits) atomicAnd((unsigned long long *)&invalid[iw], (unsigned long long)~ibits);
+      iw = q >> 6;
+      ibits = 0;
+    }
+    ibits |= 1ULL << (q & 63);
+  }
+  if (cbits) atomicOr((unsigned long long *)&codes[cw], (unsigned long long)cbits);
+  if (ibits) atomicAnd((unsigned long long *)&invalid[iw], (unsigned long long)~ibits);
+}
+
+void launch_hpc_keep(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                     uint64_t n_reads, uint64_t positions, uint64_t *keep_bits,
+                     uint32_t *counts, hipStream_t s) {
+  uint64_t words = (positions + 63) / 64;
+  if (!words || !n_reads) return;
+  hpc_keep_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      seq, read_off, pos_off, n_reads, positions, keep_bits, counts);
+}
+
+void launch_hpc_place(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
+                      uint64_t n_reads, uint64_t positions, const uint64_t *keep_bits,
+                      const uint64_t *keep_off, uint64_t *codes, uint64_t *invalid,
+                      hipStream_t s) {
+  uint64_t words = (positions + 63) / 64;
+  if (!words || !n_reads) return;
+  hpc_place_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      seq, read_off, pos_off, n_reads, positions, keep_bits, keep_off, codes, invalid);
+}
+
+// ---------------------------------------------------------------------------
 // Valid k-mer windows per read (k <= 32): one thread per 64 window starts.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invalid, uint64_t w,
@@ -366,7 +472,8 @@ __global__ __launch_bounds__(256) void ocm_compact_kernel(const uint64_t *codes,
                                                           const uint8_t *delta,
                                                           const uint64_t *block_off,
                                                           uint64_t positions, uint32_t m,
-                                                          uint64_t *keys, uint64_t *key_win) {
+                                                          SpacedRuns spaced, uint64_t *keys,
+                                                          uint64_t *key_win) {
   __shared__ uint32_t wave_counts[4];
   const uint32_t tid = threadIdx.x;
   uint64_t S = (uint64_t)blockIdx.x * kOcmTile + tid;
@@ -381,7 +488,9 @@ __global__ __launch_bounds__(256) void ocm_compact_kernel(const uint64_t *codes,
   if (emit) {
     uint64_t at = block_off[blockIdx.x] + before + below;
     uint64_t pos = S + delta[S];
-    keys[at] = canonical_from_window(stream_bits(codes, 2 * pos), (int)m);
+    uint64_t key = canonical_from_window(stream_bits(codes, 2 * pos), (int)m);
+    if (spaced.n) key = spaced_canonical(key, spaced);  // masked after selection
+    keys[at] = key;
     key_win[at] = S;
   }
 }
@@ -402,11 +511,12 @@ void launch_ocm_select(const uint64_t *codes, const uint64_t *invalid, const uin
 
 void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_t *delta,
                         const uint64_t *block_off, uint64_t positions, uint32_t m,
-                        uint64_t *keys, uint64_t *key_pos, hipStream_t s) {
+                        const SpacedRuns &spaced, uint64_t *keys, uint64_t *key_pos,
+                        hipStream_t s) {
   uint64_t blocks = (positions + kOcmTile - 1) / kOcmTile;
   if (!blocks) return;
   ocm_compact_kernel<<<(uint32_t)blocks, 256, 0, s>>>(codes, flag, delta, block_off, positions,
-                                                      m, keys, key_pos);
+                                                      m, spaced, keys, key_pos);
 }
 
 // ---------------------------------------------------------------------------
@@ -666,6 +776,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
           uint64_t iv = stream_bits(a.invalid, idx);
           v = (iv & win_mask) == 0;
           key = canonical_from_window(stream_bits(a.codes, 2 * idx), (int)a.k);
+          if (a.spaced.n) key = spaced_canonical(key, a.spaced);
         }
       } else if (v) {
         key = a.keys[idx];

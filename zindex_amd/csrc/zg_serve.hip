@@ -101,6 +101,10 @@ static uint64_t slot_bytes(const zg_index *idx, const zg_server_config &c) {
     n += 2 * (pos + 64) + (blocks + 1) * 12 + 2 * (pos + 64) * 8 + (c.slot_reads + 1) * 8;
     n += scan_temp_bytes_u32_to_u64(blocks + 1);
   }
+  if (idx->m.homopolymer_compression) {
+    const uint64_t words = (pos + 63) / 64;
+    n += (words + 1) * 20 + scan_temp_bytes_u32_to_u64(words + 1);
+  }
   return n;
 }
 
@@ -134,6 +138,13 @@ static void init_slot(zg_index *idx, const zg_server_config &c, ServeSlot &sl) {
     b.key_win.alloc((pos + 64) * 8);
     b.key_off.alloc((c.slot_reads + 1) * 8);
     b.scan_temp.alloc(scan_temp_bytes_u32_to_u64(blocks + 1));
+  }
+  if (idx->m.homopolymer_compression) {
+    const uint64_t words = (pos + 63) / 64;
+    b.hpc_keep.alloc((words + 1) * 8);
+    b.hpc_counts.alloc((words + 1) * 4);
+    b.hpc_off.alloc((words + 1) * 8);
+    b.scan_temp.reserve(scan_temp_bytes_u32_to_u64(words + 1));
   }
   sl.h_roff.reserve(c.slot_reads + 1);
   sl.h_poff.reserve(c.slot_reads + 1);
