@@ -41,6 +41,11 @@
  *                        (src/anchors.rs:839-852); bit-run extraction
  *   zo_read_features     modimizer sampling-one cases (src/lib.rs:1089-1113),
  *                        nesting and density of sampled sets
+ *   zo_findere_sequence  known answers of src/findere.rs:466-490 (consecutive
+ *                        hits per colour, short records, N breaks), brute-force
+ *                        window intersection incl. reverse complements
+ *                        (src/findere.rs:493-529), brute-force k-window rule
+ *                        under the synthetic hit function (src/findere.rs:280-285)
  *   zo_query_reads       README known answers (970/970) on bundled data;
  *                        exact k-mer membership lower bound
  *                        (option_coverage_cli.rs:137-244)
@@ -702,6 +707,43 @@ size_t zo_read_features(const zo_index *idx, const uint8_t *seq, size_t len,
   return n;
 }
 
+void zo_findere_sequence(const uint8_t *seq, size_t len, int k, int z, uint32_t n_colors,
+                         zo_hits_fn lookup, void *ctx, uint64_t *scores, uint64_t *total,
+                         uint64_t *smer_queries) {
+  const int s = k - z, window = z + 1;
+  uint64_t *hits = malloc((n_colors ? n_colors : 1) * sizeof(uint64_t));
+  uint32_t *runs = malloc((n_colors ? n_colors : 1) * sizeof(uint32_t));
+  uint64_t *keys = malloc((len ? len : 1) * sizeof(uint64_t));
+  size_t a = 0;
+  while (a < len) {
+    if (zo_encode_base(seq[a]) < 0) { a++; continue; }
+    size_t b = a;
+    while (b < len && zo_encode_base(seq[b]) >= 0) b++;
+    if (b - a >= (size_t)k) {
+      memset(runs, 0, n_colors * sizeof(uint32_t));
+      *total += (uint64_t)(b - a - k + 1);
+      size_t n = zo_scan_kmers(seq + a, b - a, s, keys);
+      for (size_t i = 0; i < n; i++) {
+        memset(hits, 0, n_colors * sizeof(uint64_t));
+        lookup(ctx, keys[i], hits);
+        (*smer_queries)++;
+        for (uint32_t c = 0; c < n_colors; c++) {
+          runs[c] = hits[c] ? (runs[c] + 1 < (uint32_t)window ? runs[c] + 1 : (uint32_t)window) : 0;
+          if (runs[c] == (uint32_t)window) scores[c]++;
+        }
+      }
+    }
+    a = b;
+  }
+  free(hits);
+  free(runs);
+  free(keys);
+}
+
+static void index_hits(void *ctx, uint64_t key, uint64_t *hits) {
+  zo_lookup((const zo_index *)ctx, key, 1, hits);
+}
+
 static int cmp_u64(const void *a, const void *b) {
   uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
   return x < y ? -1 : x > y;
@@ -740,7 +782,7 @@ static void *query_worker(void *arg) {
   uint64_t *keys = NULL;
   uint64_t *scratch = job->agg ? calloc(idx->n_colors, sizeof(uint64_t)) : NULL;
   for (uint64_t r = job->begin; r < job->end; r++) {
-    size_t n;
+    size_t n = 0;
     if (job->keys) {
       n = job->key_off[r + 1] - job->key_off[r];
       if (n > cap) {
@@ -748,7 +790,7 @@ static void *query_worker(void *arg) {
         keys = realloc(keys, cap * sizeof(uint64_t));
       }
       memcpy(keys, job->keys + job->key_off[r], n * sizeof(uint64_t));
-    } else {
+    } else if (idx->mode != 3) {
       size_t len = job->read_off[r + 1] - job->read_off[r];
       if (len > cap) {
         cap = len;
@@ -756,8 +798,6 @@ static void *query_worker(void *arg) {
       }
       n = zo_read_features(idx, job->seq + job->read_off[r], len, keys);
     }
-    if (job->totals) job->totals[r] = n;
-    job->total += n;
     uint64_t *scores;
     if (job->agg) {
       memset(scratch, 0, idx->n_colors * sizeof(uint64_t));
@@ -766,7 +806,27 @@ static void *query_worker(void *arg) {
       scores = job->counts + r * (uint64_t)idx->n_colors;
       memset(scores, 0, idx->n_colors * sizeof(uint64_t));
     }
-    lookup_sorted(idx, keys, n, scores);
+    if (idx->mode == 3 && !job->keys) {
+      /* findere (storage.rs:3651-3700, 3770-3784): compressed first when the
+       * index says so, then the positional scorer over the index lookup */
+      const uint8_t *rs = job->seq + job->read_off[r];
+      size_t len = job->read_off[r + 1] - job->read_off[r];
+      uint8_t *compressed = NULL;
+      if (idx->hpc) {
+        compressed = malloc(len ? len : 1);
+        len = zo_homopolymer_compress(rs, len, compressed);
+        rs = compressed;
+      }
+      uint64_t kmers = 0, probes = 0;
+      zo_findere_sequence(rs, len, idx->k, (int)idx->findere_z, idx->n_colors, index_hits,
+                          (void *)idx, scores, &kmers, &probes);
+      free(compressed);
+      n = kmers;
+    } else {
+      lookup_sorted(idx, keys, n, scores);
+    }
+    if (job->totals) job->totals[r] = n;
+    job->total += n;
     if (job->agg)
       for (uint32_t c = 0; c < idx->n_colors; c++) job->agg[c] += scratch[c];
   }

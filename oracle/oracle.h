@@ -100,7 +100,18 @@ typedef struct zo_index {
   int hpc;                    /* homopolymer compression before seeding */
   uint32_t seed_width;        /* spaced seed width (0 = none) */
   uint64_t seed_mask;         /* 3 per retained base, first base most significant */
+  uint32_t findere_z;         /* mode 3: z (s-mers of k - z bases are indexed) */
 } zo_index;
+
+/* findere::Scorer::sequence (src/findere.rs:74-104): per run of valid bases
+ * of at least k, every canonical s-mer (s = k - z) is looked up through
+ * `lookup` (which fills hits[n_colors], zeroed before each call); a colour
+ * scores a k-mer when its z + 1 consecutive s-mers all hit. Adds to scores,
+ * *total (k-mer windows) and *smer_queries. */
+typedef void (*zo_hits_fn)(void *ctx, uint64_t key, uint64_t *hits);
+void zo_findere_sequence(const uint8_t *seq, size_t len, int k, int z, uint32_t n_colors,
+                         zo_hits_fn lookup, void *ctx, uint64_t *scores, uint64_t *total,
+                         uint64_t *smer_queries);
 
 /* homopolymer_compress (src/lib.rs:209-222); returns the output length. */
 size_t zo_homopolymer_compress(const uint8_t *seq, size_t len, uint8_t *out);

@@ -148,7 +148,48 @@ def test_build_with_compression_and_spaced_seeds(tmp_path, small_colors, mode, h
             assert int(scores[c]) == before + 1, (c, int(key))
 
 
+def _findere_smers(seq, k, z):
+    """Canonical (k - z)-mers of the runs of at least k valid bases
+    (src/lib.rs:632-640)."""
+    out = []
+    for run in re.split(rb"[^ACGTacgt]+", seq):
+        if len(run) >= k:
+            out += zr.scan_kmers(run, k - z).tolist()
+    return np.unique(np.array(out, np.uint64))
+
+
+@pytest.mark.parametrize("k,z,group", [(31, 10, 1), (41, 10, 2), (25, 0, 1)])
+def test_findere_build_indexes_smers_of_long_runs(tmp_path, small_colors, k, z, group):
+    idx = _build(small_colors, fingerprint_bits=8, hashes=4, index_mode="findere", k=k,
+                 findere_z=z, merge_group_size=group)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["index_mode"] == 3 and man["findere_z"] == z and man["k"] == k
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        keys = _findere_smers(seq, k, z)
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:1500]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+    info = tmp_path / "info.txt"
+    fd = os.open(info, os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+    try:
+        zindex_amd.info_command(str(out), out_fd=fd)
+    finally:
+        os.close(fd)
+    text = info.read_text()
+    assert f"index_mode\tfindere\n" in text
+    assert f"findere_z\t{z}\nindexed_k\t{k - z}\n" in text  # storage.rs:3982-3991
+
+
 @pytest.mark.parametrize("cfg,msg", [
+    (dict(index_mode=3, k=31, findere_z=31), "smaller than query k"),
+    (dict(index_mode=3, k=60, findere_z=10), "longer than 32"),
     (dict(seed_pattern="##_"), "symmetric"),
     (dict(seed_pattern="#x#"), "only"),
     (dict(seed_pattern="___"), "at least one"),
@@ -545,3 +586,19 @@ def test_cli_build_and_info(tmp_path):
     r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(out)],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "exists" in r.stderr
+    # seed options (src/bin/zorindex.rs:86-111)
+    for name, args, fields in [
+            ("fd", ["-i", "findere", "-z", "8", "-c"],
+             {"index_mode": 3, "findere_z": 8, "homopolymer_compression": True}),
+            ("sp", ["-i", "kmers", "-p", K31_PATTERN, "-d", "4"],
+             {"index_mode": 0, "seed_pattern": K31_PATTERN, "modimizer_sampling": 4}),
+            ("md", ["-i", "modimizers", "-d", "4"], {"index_mode": 2, "modimizer_sampling": 4})]:
+        o = tmp_path / name
+        r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(o)] + args,
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        man = json.loads((o / "manifest.json").read_text())
+        assert all(man[k] == v for k, v in fields.items()), (name, man)
+    r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(tmp_path / "bad"), "-p",
+                        "##_"], capture_output=True, text=True)
+    assert r.returncode != 0 and "symmetric" in r.stderr

@@ -244,6 +244,30 @@ def test_compressed_index_answers_compressed_and_expanded_queries(mode):
         assert int(totals[0]) == expected and int(counts[0][0]) == expected
 
 
+@pytest.mark.parametrize("bits,group,k,z,hpc,scale", [
+    (8, 1, 31, 10, False, 1.0), (16, 2, 41, 10, False, 1.0), (32, 1, 31, 10, False, 1.0),
+    (4, 3, 31, 5, True, 1.0), (8, 2, 25, 0, False, 0.5), (1, 1, 90, 60, False, 1.0)])
+def test_findere_parity(bits, group, k, z, hpc, scale):
+    colors = make_colors(160 + bits + k, 7, 30000)
+    reads = _with_runs(make_reads(161 + z, colors, 150, lo=20, hi=2500))
+    idx = build(colors, fingerprint_bits=bits, hashes=3, index_mode=3, k=k, findere_z=z,
+                merge_group_size=group, homopolymer_compression=int(hpc), slot_scale=scale)
+    if scale < 1:
+        assert idx.meta["abandoned_hashes"] > 0
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, _ = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.scan_kmers(zr.homopolymer_compress(r) if hpc else r, k - z)
+        assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
+    label = f"findere b{bits} g{group} k{k} z{z} hpc{hpc} scale{scale}"
+    assert check_parity(idx, oracle_for(idx), reads, label=label).sum() > 0
+    if group == 1 and not hpc:
+        idx.tuning = {"span": 256, "tiles_per_wave": 1, "prefetch": 1, "waves_per_cu": 0,
+                      "nontemporal": 0}
+        check_parity(idx, oracle_for(idx), reads, label=label + " span256")
+
+
 def test_kmer_seeds_match_oracle_scan(base_case):
     colors, reads, idx, orc = base_case
     buf, off = zindex_amd.pack_reads(reads)

@@ -217,6 +217,63 @@ def test_spaced_seeds_mask_plain_features(mode, k, m):
     assert sorted(rc) == sorted(got)
 
 
+def test_findere_consecutive_hits_in_one_colour():
+    # src/findere.rs:466-477: six s-mers; colour 0 has runs of 2 and 3
+    pos = [0]
+
+    def hits(_key):
+        h = [0, 0]
+        h[1 if pos[0] == 2 else 0] = 1
+        pos[0] += 1
+        return h
+    assert zr.findere_sequence(b"ACGTACGT", 5, 2, 2, hits) == ([1, 0], 4, 6)
+
+
+def test_findere_short_records_and_breaks():
+    # src/findere.rs:479-490
+    positive = lambda _key: [1]
+    for seq in (b"ACGT", b"ACGTNACGT"):
+        assert zr.findere_sequence(seq, 5, 2, 1, positive) == ([0], 0, 0)
+    assert zr.findere_sequence(b"ACGTANACGTA", 5, 2, 1, positive) == ([2], 2, 6)
+
+
+def test_findere_equals_window_intersection():
+    # src/findere.rs:493-529, reverse complement and a substitution included
+    reference = b"ACGTTAGCGGATCACGTAC"
+    queries = [reference, b"GTACGTGATCCGCTAACGT", b"AAACGTTAGCGGTTCACGTAC"]
+    for z in (0, 1, 3, 4):
+        k, s = 7, 7 - z
+        smers = set(zr.scan_kmers(reference, s).tolist())
+        for q in queries:
+            got, total, _ = zr.findere_sequence(q, k, z, 1, lambda key: [int(key in smers)])
+            expected = sum(all(x in smers for x in zr.scan_kmers(q[i:i + k], s).tolist())
+                           for i in range(len(q) - k + 1))
+            assert got == [expected] and total == len(q) - k + 1
+
+
+@pytest.mark.parametrize("k,z", [(31, 10), (41, 10), (5, 2), (31, 0)])
+def test_findere_brute_force_k_windows(k, z):
+    # the synthetic hit function of src/findere.rs:280-285 against the k-window
+    # rule: a window of k valid bases scores a colour when all its s-mers hit
+    synthetic = lambda key: [1, int(key % 127 != 0), int(key % 7 != 0)]
+    seq = bytearray(_sequence(1500, 23))
+    for at in (100, 101, 700, 1300):
+        seq[at] = ord("N")
+    seq = bytes(seq[:400]) + b"ACGT" + bytes(seq[400:])
+    s = k - z
+    exp, total = [0, 0, 0], 0
+    for i in range(len(seq) - k + 1):
+        w = seq[i:i + k]
+        if b"N" in w:
+            continue
+        total += 1
+        keys = zr.scan_kmers(w, s).tolist()
+        for c in range(3):
+            exp[c] += all(synthetic(x)[c] for x in keys)
+    got, got_total, _ = zr.findere_sequence(seq, k, z, 3, synthetic)
+    assert got == exp and got_total == total
+
+
 # src/lib.rs:1089-1113: with sampling 1 the modimizers are every canonical k-mer
 @pytest.mark.parametrize("seq,k,seed", [(b"ACGTACGT", 3, 1), (b"AAAACCCC", 4, 2),
                                         (b"TTTTGGGG", 5, 3), (b"GATTACA", 3, 4),

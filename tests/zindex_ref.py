@@ -42,7 +42,11 @@ class ZoIndex(C.Structure):
         ("n_colors", C.c_uint32), ("banks", C.POINTER(ZoBank)), ("n_banks", C.c_uint32),
         ("recovery", C.POINTER(ZoRecovery)), ("sampling", C.c_uint64),
         ("hpc", C.c_int), ("seed_width", C.c_uint32), ("seed_mask", C.c_uint64),
+        ("findere_z", C.c_uint32),
     ]
+
+
+HITS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64))
 
 
 def spaced_seed(pattern):
@@ -105,6 +109,9 @@ def oracle() -> C.CDLL:
         o.zo_spaced_project.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
         o.zo_spaced_canonical.restype = C.c_uint64
         o.zo_spaced_canonical.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+        o.zo_findere_sequence.restype = None
+        o.zo_findere_sequence.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint32,
+                                          HITS_FN, C.c_void_p, u64p, u64p, u64p]
         o.zo_read_features.restype = C.c_size_t
         o.zo_read_features.argtypes = [C.POINTER(ZoIndex), C.c_char_p, C.c_size_t, u64p]
         o.zo_lookup.restype = None
@@ -130,6 +137,20 @@ def scan_kmers(seq: bytes, k: int) -> np.ndarray:
     out = np.zeros(max(len(seq), 1), np.uint64)
     n = oracle().zo_scan_kmers(seq, len(seq), k, out.ctypes.data_as(u64p))
     return out[:n]
+
+
+def findere_sequence(seq: bytes, k: int, z: int, n_colors: int, hits_fn):
+    """findere::Scorer::sequence with hits_fn(key) -> list of per-colour hits;
+    returns (scores, total_kmers, smer_queries)."""
+    def cb(_ctx, key, hits):
+        for c, v in enumerate(hits_fn(key)):
+            hits[c] = v
+    fn = HITS_FN(cb)
+    scores = np.zeros(max(n_colors, 1), np.uint64)
+    total, probes = C.c_uint64(0), C.c_uint64(0)
+    oracle().zo_findere_sequence(seq, len(seq), k, z, n_colors, fn, None,
+                                 scores.ctypes.data_as(u64p), C.byref(total), C.byref(probes))
+    return scores[:n_colors].tolist(), total.value, probes.value
 
 
 def homopolymer_compress(seq: bytes) -> bytes:
@@ -421,7 +442,8 @@ class OracleIndex:
                          C.pointer(self._rec) if self._rec is not None else None,
                          manifest.get("modimizer_sampling") or 16,
                          int(bool(manifest.get("homopolymer_compression"))),
-                         *spaced_seed(manifest.get("seed_pattern")))
+                         *spaced_seed(manifest.get("seed_pattern")),
+                         manifest.get("findere_z", 10))
 
     @classmethod
     def from_dir(cls, path, recover: bool = True) -> "OracleIndex":

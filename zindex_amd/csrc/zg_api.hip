@@ -99,8 +99,11 @@ static void check_query_support(const Manifest &m) {
   } else if (m.index_mode == 1) {
     if (m.minimizer_scheme != 1)
       fail(ZG_ERR_UNSUPPORTED, "only open-closed mod-minimizer indexes are supported by the GPU query");
-  } else {
-    fail(ZG_ERR_UNSUPPORTED, "findere indexes are not supported by the GPU query");
+  } else if (m.index_mode == 3) {
+    if (m.k - m.findere_z > 32)
+      fail(ZG_ERR_UNSUPPORTED, "findere s-mers longer than 32 are not supported by the GPU query");
+    if (m.k > 255)
+      fail(ZG_ERR_UNSUPPORTED, "findere windows longer than 255 are not supported by the GPU query");
   }
 }
 
@@ -350,6 +353,8 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   if (any_streamed(idx)) idx->stream_data.alloc(largest);
   const uint32_t n_colors = (uint32_t)m.colors.size();
   std::vector<std::vector<uint32_t>> bank_color_col(nb);  // device columns
+  std::vector<std::vector<uint32_t>> fd_keys(nb);         // findere colour -> bit
+  idx->fd_bank_member_bit.assign(nb, 0);
   for (size_t bi = 0; bi < nb; bi++) {
     const BankMeta &meta = m.banks[bi];
     DeviceBank &db = idx->dbanks[bi];
@@ -396,6 +401,22 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       home = db.data.as<uint8_t>();
       put_bank(idx, bi, home, s);
     }
+    if (m.index_mode == 3) {
+      // findere bits: device column for singletons, then one bit per member
+      const uint32_t tile_bits = 64 * fields_per_lane(fb);
+      const uint32_t member_bit = ((G + tile_bits - 1) / tile_bits) * tile_bits;
+      std::vector<uint32_t> key(n_colors + 1, kNone);
+      for (uint32_t c = 0; c < G; c++) {
+        const GroupMeta &gm = meta.groups[perm[c]];
+        if (c < first_merged) key[gm.members[0]] = c;
+        else
+          for (size_t j = 0; j < gm.members.size(); j++)
+            key[gm.members[j]] = member_bit + mg_off[c - first_merged] + (uint32_t)j;
+      }
+      key[n_colors] = member_bit + (uint32_t)mg_members.size();  // bits in use
+      fd_keys[bi] = std::move(key);
+      idx->fd_bank_member_bit[bi] = member_bit;
+    }
     upload(db.col_color, col_color, s);
     upload(db.mg_off, mg_off, s);
     upload(db.mg_members, mg_members, s);
@@ -435,6 +456,34 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
     idx->dbanks[bi].view.colmap = (uint32_t)id;
   }
   std::vector<std::vector<uint32_t>>().swap(bank_color_col);
+  // findere regions: one per distinct colour -> bit map
+  idx->fd_regions.clear();
+  idx->fd_bank_region.assign(nb, 0);
+  idx->fd_row_words = 0;
+  if (m.index_mode == 3) {
+    std::vector<size_t> first;  // a bank of each region
+    std::vector<uint32_t> bit_color;
+    for (size_t bi = 0; bi < nb; bi++) {
+      size_t id = 0;
+      while (id < first.size() && fd_keys[first[id]] != fd_keys[bi]) id++;
+      if (id == first.size()) {
+        zg_index::FdRegion rg;
+        rg.bits = fd_keys[bi][n_colors];
+        rg.off = (uint32_t)idx->fd_row_words;
+        rg.color_at = bit_color.size();
+        idx->fd_row_words += (rg.bits + 31) / 32;
+        bit_color.resize(bit_color.size() + rg.bits, kNone);
+        for (uint32_t c = 0; c < n_colors; c++)
+          if (fd_keys[bi][c] != kNone) bit_color[rg.color_at + fd_keys[bi][c]] = c;
+        idx->fd_regions.push_back(rg);
+        first.push_back(bi);
+      }
+      idx->fd_bank_region[bi] = (uint32_t)id;
+    }
+    if (bit_color.empty()) bit_color.push_back(kNone);
+    upload(idx->fd_bit_color, bit_color, s);
+    check_hip(hipStreamSynchronize(s), "upload");
+  }
   // recovery directory (the query-side OriginMap, abandoned.rs:29-206)
   idx->rec_view = RecDev{};
   if (idx->recover && idx->have_rec && idx->rec.size()) {
@@ -793,6 +842,15 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   a.nontemporal = b->nontemporal;
   a.sampling = (idx->m.index_mode == 2 && !b->explicit_keys) ? idx->m.modimizer_sampling : 0;
   a.spaced = b->explicit_keys ? SpacedRuns{} : idx->spaced;
+  if (idx->m.index_mode == 3) {
+    // findere: s-mer windows, hits to rows; the scan after the lookups counts
+    a.k = (uint32_t)(idx->m.k - idx->m.findere_z);
+    a.records = 0;
+    a.fd_hits = b->fd_hits.as<uint32_t>();
+    a.fd_row_words = idx->fd_row_words;
+    a.fd_off = idx->fd_regions[idx->fd_bank_region[bank]].off;
+    a.fd_member_bit = idx->fd_bank_member_bit[bank];
+  }
   return a;
 }
 
@@ -803,6 +861,18 @@ void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
   if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
   else launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
   launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
+  if (idx->m.index_mode == 3)
+    launch_fill_u32(b->fd_hits.as<uint32_t>(), b->positions * idx->fd_row_words, 0, s);
+}
+
+// Findere: the run scan of every hit region into counts / partial sums.
+static void enqueue_findere_scan(zg_index *idx, zg_batch *b) {
+  for (const zg_index::FdRegion &rg : idx->fd_regions)
+    launch_findere_scan(b->fd_hits.as<uint32_t>(), idx->fd_row_words, rg.off, rg.bits,
+                        idx->fd_bit_color.as<uint32_t>() + rg.color_at, b->pos_off.as<uint64_t>(),
+                        b->n_reads, b->positions, (uint32_t)idx->m.findere_z, b->records ? 1 : 0,
+                        b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
+                        (uint32_t)idx->m.colors.size(), b->stream);
 }
 
 // The lookups: reset the work counters, one persistent launch per bank. This
@@ -830,6 +900,7 @@ void enqueue_lookups(zg_index *idx, zg_batch *b) {
       make_resident(idx, i, s);
       launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
     }
+  if (any && idx->m.index_mode == 3) enqueue_findere_scan(idx, b);
 }
 
 // Per-colour sums after the lookups.
@@ -888,7 +959,10 @@ void seed_batch(zg_index *idx, zg_batch *b) {
                 b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
   }
   launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
-  if (m.index_mode != 1) {  // k-mers, or modimizers (sampled k-mers)
+  if (m.index_mode == 3) b->fd_hits.reserve(std::max<uint64_t>(1, b->positions * idx->fd_row_words) * 4);
+  if (m.index_mode != 1) {
+    // k-mers, modimizers (sampled k-mers), or findere (its total is the
+    // number of valid k-mer windows, Scorer::sequence)
     launch_count_kmers(b->invalid.as<uint64_t>(), b->codes.as<uint64_t>(), m.seed,
                        m.index_mode == 2 ? m.modimizer_sampling : 0, b->pos_off.as<uint64_t>(),
                        b->n_reads, b->positions, (uint32_t)m.k, b->totals.as<uint64_t>(), s);
@@ -1060,6 +1134,8 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
     else zg::launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
     zg::launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
     zg::launch_fill_u32(queues.as<uint32_t>(), count * nb, 0, s);
+    if (idx->m.index_mode == 3)
+      zg::launch_fill_u32(b->fd_hits.as<uint32_t>(), b->positions * idx->fd_row_words, 0, s);
     check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
     for (size_t i : zg::bank_order(idx)) {
       zg::make_resident(idx, i, s);
@@ -1078,6 +1154,7 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
         zg::launch_lookup(a, c, s);
       }
     }
+    if (idx->m.index_mode == 3) zg::enqueue_findere_scan(idx, b);
     check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
     if (!b->records)
       zg::launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
@@ -1367,7 +1444,8 @@ extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_
     check_hip(hipMemcpyAsync(codes.data(), b->codes.p, codes.size() * 8, hipMemcpyDeviceToHost, s), "copy");
     check_hip(hipMemcpyAsync(inv.data(), b->invalid.p, inv.size() * 8, hipMemcpyDeviceToHost, s), "copy");
     check_hip(hipStreamSynchronize(s), "copy");
-    const uint32_t k = (uint32_t)idx->m.k;
+    // the looked-up windows: k-mers, or findere's s-mers
+    const uint32_t k = (uint32_t)(idx->m.index_mode == 3 ? idx->m.k - idx->m.findere_z : idx->m.k);
     auto bits = [](const std::vector<uint64_t> &w, uint64_t bit) {
       uint64_t i = bit >> 6, sh = bit & 63;
       return sh ? (w[i] >> sh) | (w[i + 1] << (64 - sh)) : w[i];

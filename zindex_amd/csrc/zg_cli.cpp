@@ -12,7 +12,8 @@ static int usage() {
   fprintf(stderr,
           "usage: zorindex <build|query|info|fusion> [options]\n"
           "  build -f FOF -o OUT [-k K] [-m M] [-H HASHES] [-b BITS] [-s SEED]\n"
-          "        [-i kmers|minimizers|modimizers] [--modimizer-sampling N] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
+          "        [-i kmers|minimizers|modimizers|findere] [-d MODIMIZER_SAMPLING] [-z FINDERE_Z]\n"
+          "        [-c] [-p SEED_PATTERN] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
           "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
           "        [-M subindex|all] [--hbm-budget BYTES] [--no-recovery]\n"
@@ -74,9 +75,21 @@ int main(int argc, char **argv) {
         if (v == "kmers") cfg.index_mode = 0;
         else if (v == "minimizers") cfg.index_mode = 1;
         else if (v == "modimizers") cfg.index_mode = 2;
+        else if (v == "findere") cfg.index_mode = 3;
         else { fprintf(stderr, "unsupported index mode %s\n", v.c_str()); return 2; }
-      } else if (o == "--modimizer-sampling") {
+      } else if (o == "-d" || o == "--modimizer-sampling") {
         cfg.modimizer_sampling = (uint32_t)atoi(value(i).c_str());
+      } else if (o == "-z" || o == "--findere-z") {
+        cfg.findere_z = (uint32_t)atoi(value(i).c_str());
+      } else if (o == "-c" || o == "--homopolymer-compression") {
+        cfg.homopolymer_compression = 1;
+      } else if (o == "-p" || o == "--seed-pattern") {
+        std::string v = value(i);
+        if (v.size() >= sizeof cfg.seed_pattern) {
+          fprintf(stderr, "seed pattern width must be in 1..=31\n");
+          return 2;
+        }
+        memcpy(cfg.seed_pattern, v.c_str(), v.size() + 1);
       } else return usage();
     }
     if (fof.empty() || out.empty()) return usage();

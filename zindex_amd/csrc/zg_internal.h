@@ -84,6 +84,16 @@ struct zg_index {
   hipEvent_t staging_ev[2] = {nullptr, nullptr};
   zg::DevBuf rec_dir, rec_post, rec_post_col, inner_table;
   zg::SpacedRuns spaced;  // the manifest's seed pattern (n = 0: none)
+  // findere: per-seed hit rows. Banks with the same colour -> bit map (the
+  // partitions of one size class) share a region of fd_row_words words.
+  struct FdRegion {
+    uint32_t off = 0, bits = 0;  // first word in a row, bits in use
+    uint64_t color_at = 0;       // first entry of its bit -> colour table
+  };
+  uint64_t fd_row_words = 0;
+  std::vector<FdRegion> fd_regions;
+  std::vector<uint32_t> fd_bank_region, fd_bank_member_bit;
+  zg::DevBuf fd_bit_color;
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();
@@ -103,6 +113,7 @@ struct zg_batch {
   zg::DevBuf flag, delta, block_counts, block_off, keys, key_win, key_off, scan_temp;
   // homopolymer compression: keep flags, counts and offsets per 64 positions
   zg::DevBuf hpc_keep, hpc_counts, hpc_off;
+  zg::DevBuf fd_hits;  // findere: positions x fd_row_words hit words
   // outputs
   zg::DevBuf counts, partial, agg, queue;
   uint64_t waves = 0;

@@ -71,6 +71,14 @@ struct LookupArgs {
   uint32_t nontemporal;       // 1: row loads bypass cache retention
   uint64_t sampling;          // modimizers: keep k-mers with mixsplit % sampling == 0 (0 = all)
   SpacedRuns spaced;          // spaced seed applied to each k-mer (kSeedKmers; n = 0: none)
+  // findere: instead of counting, each seed's hits go to its row of fd_hits
+  // (fd_row_words 32-bit words per seed index). This bank's region starts at
+  // word fd_off: bit = device column for singleton columns, then bit
+  // fd_member_bit + member slot (index into mg_members) for merged groups.
+  uint32_t *fd_hits;          // nullptr: count as usual
+  uint64_t fd_row_words;
+  uint32_t fd_off;
+  uint32_t fd_member_bit;
 };
 
 struct KernelConfig {
@@ -110,6 +118,15 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
                         const uint64_t *block_off, uint64_t positions, uint32_t m,
                         const SpacedRuns &spaced, uint64_t *keys, uint64_t *key_pos,
                         hipStream_t s);
+// Findere reconstruction over a region of hit rows (findere::Scorer,
+// src/findere.rs:74-104): colour bit_color[j] of region bit j scores a
+// position when that bit is set in the z + 1 rows ending there. Adds into
+// counts[r * n_colors + colour] (records) or partial[colour].
+void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
+                         uint32_t region_bits, const uint32_t *bit_color,
+                         const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+                         uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
+                         uint32_t n_colors, hipStream_t s);
 // Homopolymer compression of packed reads: keep flags per 64 positions, then
 // (after an exclusive scan of counts into keep_off) the placement of kept
 // bases into zeroed codes / all-ones invalid.

@@ -245,9 +245,14 @@ void launch_hpc_place(const uint8_t *seq, const uint64_t *read_off, const uint64
 __device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invalid, uint64_t w,
                                                           uint32_t k) {
   // bit i set iff window [64w+i, 64w+i+k) contains an invalid base
-  uint64_t lo = invalid[w], hi = invalid[w + 1];
+  // (k <= 255: the invalid words are padded by four)
   uint64_t m = 0;
-  for (uint32_t d = 0; d < k; d++) m |= d ? ((lo >> d) | (hi << (64 - d))) : lo;
+  if (k <= 64) {
+    uint64_t lo = invalid[w], hi = invalid[w + 1];
+    for (uint32_t d = 0; d < k; d++) m |= d ? ((lo >> d) | (hi << (64 - d))) : lo;
+  } else {
+    for (uint32_t d = 0; d < k; d++) m |= stream_bits(invalid, 64 * w + d);
+  }
   return m;
 }
 
@@ -762,6 +767,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
         }
       }
     };
+    auto fd_member = [&](uint64_t kidx, uint32_t slot) {
+      const uint32_t bit = a.fd_member_bit + slot;
+      atomicOr(a.fd_hits + kidx * a.fd_row_words + a.fd_off + (bit >> 5), 1u << (bit & 31));
+    };
     auto out_add = [&](uint32_t color) {
       if (a.records) atomicAdd(&a.counts[r * a.n_colors + color], 1u);
       else atomicAdd((unsigned long long *)&a.partial[wave_slot * a.n_colors + color], 1ULL);
@@ -924,7 +933,12 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
                     uint64_t bit = bk.mg_mask_bit[col - bk.first_merged] + bk.color_bit[o];
                     hit = xor_rows_bits(bk, kk[d], a.seed, bit, 1) & 1;
                   }
-                  if (!hit) out_add(o);
+                  if (!hit) {
+                    if (a.fd_hits)
+                      fd_member(kidx, bk.mg_off[col - bk.first_merged] + bk.color_bit[o]);
+                    else
+                      out_add(o);
+                  }
                 }
               }
             }
@@ -947,14 +961,32 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
               while (mask) {
                 int b = __builtin_ctzll(mask);
                 mask &= mask - 1;
-                out_add(bk.mg_members[lo + b]);
+                if (a.fd_hits) fd_member(kidx, lo + b);
+                else out_add(bk.mg_members[lo + b]);
               }
             }
           }
+          if (a.fd_hits) {
+            // this seed's singleton hits: F bits per lane at (tile * 64 + lane) * F
+            uint8_t *row = (uint8_t *)(a.fd_hits + kidx * a.fd_row_words + a.fd_off);
 #pragma unroll
-          for (int t = 0; t < T; t++)
+            for (int t = 0; t < T; t++) {
+              const uint32_t tile = tg * T + t;
+              if constexpr (F == 4) {
+                const uint32_t up = __shfl_down(m[t], 1);
+                if (tile < bk.n_tiles && !(lane & 1))
+                  row[(tile * 64 + lane) / 2] = (uint8_t)((m[t] & 0xFu) | (up & 0xFu) << 4);
+              } else if (tile < bk.n_tiles) {
+                if constexpr (F == 8) row[tile * 64 + lane] = (uint8_t)m[t];
+                else ((uint16_t *)row)[tile * 64 + lane] = (uint16_t)m[t];
+              }
+            }
+          } else {
 #pragma unroll
-            for (int f = 0; f < F; f++) acc[t][f] += (m[t] >> f) & 1u;
+            for (int t = 0; t < T; t++)
+#pragma unroll
+              for (int f = 0; f < F; f++) acc[t][f] += (m[t] >> f) & 1u;
+          }
         }
       }
     }
@@ -1028,6 +1060,65 @@ int lookup_blocks_per_cu(const KernelConfig &c) {
     case 16: return occupancy_t<16>(c);
     default: return occupancy_t<32>(c);
   }
+}
+
+// Findere reconstruction (findere::Scorer::sequence, src/findere.rs:74-104).
+// One lane per region bit (colour), one wave per (64 bits, span of
+// positions). Rows of invalid or unlooked-up s-mer windows are zero, so runs
+// break at ambiguous bases and read separators by themselves; a wave starts
+// z positions before its span to rebuild the run lengths it inherits.
+constexpr uint32_t kFindereSpan = 2048;
+
+__global__ __launch_bounds__(256) void findere_scan_kernel(
+    const uint32_t *hits, uint64_t row_words, uint32_t region_off, uint32_t region_bits,
+    const uint32_t *bit_color, const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+    uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t groups = (region_bits + 63) / 64;
+  const uint64_t spans = (positions + kFindereSpan - 1) / kFindereSpan;
+  if (wave >= groups * spans) return;
+  const uint64_t g = wave % groups, sp = wave / groups;
+  const uint32_t j = (uint32_t)(g * 64 + lane);
+  const uint32_t color = j < region_bits ? bit_color[j] : kNone;
+  const uint64_t p0 = sp * kFindereSpan;
+  const uint64_t p1 = p0 + kFindereSpan < positions ? p0 + kFindereSpan : positions;
+  const uint64_t start = p0 > z ? p0 - z : 0;
+  const uint32_t *col = hits + region_off + (j >> 5);
+  const uint32_t sh = j & 31;
+  const uint32_t window = z + 1;
+  uint64_t r = find_segment(pos_off, n_reads, p0);
+  uint64_t next = pos_off[r + 1];
+  uint32_t run = 0, cnt = 0;
+  for (uint64_t p = start; p < p1; p++) {
+    const uint32_t x = color != kNone ? (col[p * row_words] >> sh) & 1u : 0u;
+    run = x ? (run + 1 < window ? run + 1 : window) : 0;
+    if (p < p0) continue;
+    if (records && p >= next) {
+      if (cnt) atomicAdd(&counts[r * n_colors + color], cnt);
+      cnt = 0;
+      while (pos_off[r + 1] <= p) r++;
+      next = pos_off[r + 1];
+    }
+    cnt += run == window;
+  }
+  if (cnt) {
+    if (records) atomicAdd(&counts[r * n_colors + color], cnt);
+    else atomicAdd((unsigned long long *)&partial[color], (unsigned long long)cnt);
+  }
+}
+
+void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
+                         uint32_t region_bits, const uint32_t *bit_color,
+                         const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
+                         uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
+                         uint32_t n_colors, hipStream_t s) {
+  if (!positions || !n_reads || !region_bits) return;
+  const uint64_t waves =
+      ((region_bits + 63) / 64) * ((positions + kFindereSpan - 1) / kFindereSpan);
+  findere_scan_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+      hits, row_words, region_off, region_bits, bit_color, pos_off, n_reads, positions, z,
+      records, counts, partial, n_colors);
 }
 
 // Random-row gather probe: whole rows of ROW = CH * 64 * LB bytes at

@@ -105,6 +105,7 @@ static uint64_t slot_bytes(const zg_index *idx, const zg_server_config &c) {
     const uint64_t words = (pos + 63) / 64;
     n += (words + 1) * 20 + scan_temp_bytes_u32_to_u64(words + 1);
   }
+  if (idx->m.index_mode == 3) n += std::max<uint64_t>(1, pos * idx->fd_row_words) * 4;
   return n;
 }
 
@@ -139,6 +140,7 @@ static void init_slot(zg_index *idx, const zg_server_config &c, ServeSlot &sl) {
     b.key_off.alloc((c.slot_reads + 1) * 8);
     b.scan_temp.alloc(scan_temp_bytes_u32_to_u64(blocks + 1));
   }
+  if (idx->m.index_mode == 3) b.fd_hits.alloc(std::max<uint64_t>(1, pos * idx->fd_row_words) * 4);
   if (idx->m.homopolymer_compression) {
     const uint64_t words = (pos + 63) / 64;
     b.hpc_keep.alloc((words + 1) * 8);
