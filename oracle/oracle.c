@@ -46,6 +46,14 @@
  *                        window intersection incl. reverse complements
  *                        (src/findere.rs:493-529), brute-force k-window rule
  *                        under the synthetic hit function (src/findere.rs:280-285)
+ *   zo_strobe_hash       known answer strobe_hash(0x3332316f6c6c6568) =
+ *                        0x2d119fced5ee39ce (src/anchors.rs:823-838)
+ *   zo_link_strobes      checksums of the upstream strobealign iterator
+ *                        witnesses (src/anchors.rs:950-988); inclusive
+ *                        distances and self fallback (src/anchors.rs:1037-1062)
+ *   randstrobe features  break and compression equalities (src/anchors.rs:
+ *                        1013-1035); two-orientation multiset invariance
+ *                        (src/anchors.rs:868-908)
  *   zo_query_reads       README known answers (970/970) on bundled data;
  *                        exact k-mer membership lower bound
  *                        (option_coverage_cli.rs:137-244)
@@ -676,6 +684,90 @@ uint64_t zo_spaced_canonical(uint64_t encoded, uint32_t width, uint64_t mask) {
   return zo_canonical_encoded(zo_spaced_project(encoded, width, mask), weight);
 }
 
+static uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+uint64_t zo_strobe_hash(uint64_t value) {
+  const uint64_t p1 = 0x9E3779B185EBCA87ULL, p2 = 0xC2B2AE3D27D4EB4FULL,
+                 p3 = 0x165667B19E3779F9ULL, p4 = 0x85EBCA77C2B2AE63ULL,
+                 p5 = 0x27D4EB2F165667C5ULL;
+  uint64_t h = (p5 + 8) ^ (rotl(value * p2, 31) * p1);
+  h = rotl(h, 27) * p1 + p4;
+  h ^= h >> 33;
+  h *= p2;
+  h ^= h >> 29;
+  h *= p3;
+  return h ^ (h >> 32);
+}
+
+void zo_link_strobes(const uint64_t *pos, const uint64_t *hash, size_t n, uint64_t min_d,
+                     uint64_t max_d, uint64_t *keys) {
+  for (size_t i = 0; i < n; i++) {
+    size_t partner = i;
+    int best = 1 << 30;
+    for (size_t j = i + 1; j < n && pos[j] - pos[i] <= max_d; j++) {
+      if (pos[j] - pos[i] < min_d) continue;
+      int rank = __builtin_popcountll((hash[i] ^ hash[j]) & 255);
+      if (rank < best) {
+        best = rank;
+        partner = j;
+        if (rank == 0) break;
+      }
+    }
+    keys[i] = hash[i] / 2 + hash[partner] / 2;
+  }
+}
+
+static int cmp_pos_pair(const void *a, const void *b) {
+  const uint64_t *x = a, *y = b;
+  return x[0] < y[0] ? -1 : x[0] > y[0];
+}
+
+/* anchors::scan with randstrobemers (src/anchors.rs:284-351): per run of
+ * valid bases, canonical k-mers at every position or the selected minimizer
+ * positions (each once), hashed after the optional pattern, then linked. */
+static size_t strobe_features(const zo_index *idx, const uint8_t *seq, size_t len,
+                              uint64_t *out) {
+  size_t n_out = 0;
+  uint64_t *pos = malloc((len + 1) * sizeof(uint64_t));
+  uint64_t *key = malloc((len + 1) * sizeof(uint64_t));
+  uint64_t *pairs = malloc((len + 1) * 2 * sizeof(uint64_t));
+  size_t a = 0;
+  while (a < len) {
+    if (zo_encode_base(seq[a]) < 0) { a++; continue; }
+    size_t b = a;
+    while (b < len && zo_encode_base(seq[b]) >= 0) b++;
+    size_t n;
+    if (idx->mode == 1) {
+      n = zo_scan_ocm(seq + a, b - a, idx->k, idx->m, idx->seed, pos, key);
+      for (size_t i = 0; i < n; i++) { pairs[2 * i] = pos[i]; pairs[2 * i + 1] = key[i]; }
+      qsort(pairs, n, 2 * sizeof(uint64_t), cmp_pos_pair);
+      size_t u = 0;
+      for (size_t i = 0; i < n; i++)
+        if (u == 0 || pairs[2 * i] != pos[u - 1]) {
+          pos[u] = pairs[2 * i];
+          key[u] = pairs[2 * i + 1];
+          u++;
+        }
+      n = u;
+    } else {
+      n = zo_scan_kmers(seq + a, b - a, idx->k, key);
+      for (size_t i = 0; i < n; i++) pos[i] = i;
+    }
+    for (size_t i = 0; i < n; i++) {
+      uint64_t v = idx->seed_width ? zo_spaced_canonical(key[i], idx->seed_width, idx->seed_mask)
+                                   : key[i];
+      key[i] = zo_strobe_hash(v);
+    }
+    zo_link_strobes(pos, key, n, idx->strobe_min, idx->strobe_max, out + n_out);
+    n_out += n;
+    a = b;
+  }
+  free(pos);
+  free(key);
+  free(pairs);
+  return n_out;
+}
+
 size_t zo_read_features(const zo_index *idx, const uint8_t *seq, size_t len,
                         uint64_t *out) {
   uint8_t *compressed = NULL;
@@ -685,6 +777,11 @@ size_t zo_read_features(const zo_index *idx, const uint8_t *seq, size_t len,
     seq = compressed;
   }
   size_t n;
+  if (idx->strobe_max) {
+    n = strobe_features(idx, seq, len, out);
+    free(compressed);
+    return n;
+  }
   if (idx->mode == 1) {
     n = zo_scan_ocm(seq, len, idx->k, idx->m, idx->seed, NULL, out);
   } else {
@@ -822,6 +919,24 @@ static void *query_worker(void *arg) {
                           (void *)idx, scores, &kmers, &probes);
       free(compressed);
       n = kmers;
+    } else if (idx->strobe_max && !job->keys) {
+      /* randstrobemers: both query orientations, the larger count per colour
+       * (record_query.rs:649-685) */
+      lookup_sorted(idx, keys, n, scores);
+      size_t len = job->read_off[r + 1] - job->read_off[r];
+      const uint8_t *rs = job->seq + job->read_off[r];
+      uint8_t *rc = malloc(len ? len : 1);
+      for (size_t i = 0; i < len; i++) {
+        int c = zo_encode_base(rs[len - 1 - i]);
+        rc[i] = c < 0 ? rs[len - 1 - i] : (uint8_t)"TGCA"[c];
+      }
+      size_t n2 = zo_read_features(idx, rc, len, keys);
+      free(rc);
+      uint64_t *second = calloc(idx->n_colors ? idx->n_colors : 1, sizeof(uint64_t));
+      lookup_sorted(idx, keys, n2, second);
+      for (uint32_t c = 0; c < idx->n_colors; c++)
+        if (second[c] > scores[c]) scores[c] = second[c];
+      free(second);
     } else {
       lookup_sorted(idx, keys, n, scores);
     }

@@ -101,7 +101,19 @@ typedef struct zo_index {
   uint32_t seed_width;        /* spaced seed width (0 = none) */
   uint64_t seed_mask;         /* 3 per retained base, first base most significant */
   uint32_t findere_z;         /* mode 3: z (s-mers of k - z bases are indexed) */
+  uint64_t strobe_min;        /* randstrobemers: inclusive distance bounds */
+  uint64_t strobe_max;        /*   (strobe_max 0 = no randstrobemers) */
 } zo_index;
+
+/* strobe_hash (src/anchors.rs:134-151): XXH64 of one u64, seed 0. */
+uint64_t zo_strobe_hash(uint64_t value);
+
+/* Linker (src/anchors.rs:185-256) over n candidates of one run in position
+ * order: keys[i] = hash[i]/2 + hash[partner]/2, the partner being the later
+ * candidate at distance in [min_d, max_d] with the fewest differing low eight
+ * hash bits (first on ties), or i itself. */
+void zo_link_strobes(const uint64_t *pos, const uint64_t *hash, size_t n, uint64_t min_d,
+                     uint64_t max_d, uint64_t *keys);
 
 /* findere::Scorer::sequence (src/findere.rs:74-104): per run of valid bases
  * of at least k, every canonical s-mer (s = k - z) is looked up through

@@ -187,7 +187,41 @@ def test_findere_build_indexes_smers_of_long_runs(tmp_path, small_colors, k, z, 
     assert f"findere_z\t{z}\nindexed_k\t{k - z}\n" in text  # storage.rs:3982-3991
 
 
+@pytest.mark.parametrize("mode,pattern,bounds,hpc", [
+    (0, None, (0, 0), False), (0, K31_PATTERN, (10, 50), True), (1, None, (0, 0), False),
+    (1, M19_PATTERN, (5, 40), False)])
+def test_randstrobe_build(tmp_path, small_colors, mode, pattern, bounds, hpc):
+    cfg = dict(fingerprint_bits=16, hashes=3, index_mode=mode, randstrobemers=1,
+               strobe_min=bounds[0], strobe_max=bounds[1], homopolymer_compression=int(hpc))
+    if mode == 1:
+        cfg.update(k=31, minimizer_size=19)
+    if pattern:
+        cfg["seed_pattern"] = pattern
+    idx = _build(small_colors, **cfg)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    length = 19 if mode == 1 else 31
+    lo, hi = bounds if bounds[0] else (length, 3 * length)  # src/bin/zorindex.rs:469-485
+    assert man["randstrobemers"] == {"min_distance": lo, "max_distance": hi}
+    assert man["randstrobe_scheme"] == "strobealign-popcount8-symmetric64-downstream-v1"
+    with zindex_amd.Index.open(str(out)) as reopened:
+        assert reopened.meta["n_colors"] == len(small_colors)
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        keys = np.unique(zr.read_features(seq, man["k"], man["seed"], mode=mode, m=19,
+                                          hpc=hpc, pattern=pattern, strobes=(lo, hi)))
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:1500]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+
+
 @pytest.mark.parametrize("cfg,msg", [
+    (dict(randstrobemers=1, strobe_min=10, strobe_max=5), "must not exceed"),
     (dict(index_mode=3, k=31, findere_z=31), "smaller than query k"),
     (dict(index_mode=3, k=60, findere_z=10), "longer than 32"),
     (dict(seed_pattern="##_"), "symmetric"),
@@ -196,7 +230,6 @@ def test_findere_build_indexes_smers_of_long_runs(tmp_path, small_colors, k, z, 
     (dict(seed_pattern="#" * 21), "must match k"),
     (dict(index_mode=1, k=31, minimizer_size=21, seed_pattern="#" * 31), "minimizer size m"),
     (dict(index_mode=2, seed_pattern="#" * 31), "require --index-mode"),
-    (dict(randstrobemers=1), "randstrobemers"),
 ])
 def test_seed_option_validation(small_colors, cfg, msg):
     with pytest.raises(Exception, match=msg):
@@ -592,7 +625,9 @@ def test_cli_build_and_info(tmp_path):
              {"index_mode": 3, "findere_z": 8, "homopolymer_compression": True}),
             ("sp", ["-i", "kmers", "-p", K31_PATTERN, "-d", "4"],
              {"index_mode": 0, "seed_pattern": K31_PATTERN, "modimizer_sampling": 4}),
-            ("md", ["-i", "modimizers", "-d", "4"], {"index_mode": 2, "modimizer_sampling": 4})]:
+            ("md", ["-i", "modimizers", "-d", "4"], {"index_mode": 2, "modimizer_sampling": 4}),
+            ("rs", ["-i", "kmers", "-r", "-l", "7", "-u", "40"],
+             {"randstrobemers": {"min_distance": 7, "max_distance": 40}})]:
         o = tmp_path / name
         r = subprocess.run([str(cli), "build", "-f", str(fof), "-o", str(o)] + args,
                            capture_output=True, text=True)

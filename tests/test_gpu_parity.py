@@ -268,6 +268,39 @@ def test_findere_parity(bits, group, k, z, hpc, scale):
         check_parity(idx, oracle_for(idx), reads, label=label + " span256")
 
 
+def _revcomp(seq: bytes) -> bytes:
+    comp = {ord("A"): ord("T"), ord("C"): ord("G"), ord("G"): ord("C"), ord("T"): ord("A")}
+    return bytes(comp.get(ord(chr(c).upper()), c) for c in reversed(seq))
+
+
+@pytest.mark.parametrize("mode,pattern,bounds,hpc,bits", [
+    (0, None, (0, 0), False, 8), (0, K31_PATTERN, (10, 50), True, 16),
+    (1, None, (0, 0), False, 8), (1, M19_PATTERN, (5, 40), False, 4),
+    (0, None, (1, 1), False, 32)])
+def test_randstrobe_parity(mode, pattern, bounds, hpc, bits):
+    colors = make_colors(170 + mode + bits, 6, 30000)
+    reads = _with_runs(make_reads(171 + bits, colors, 150, lo=20, hi=2500))
+    idx = build(colors, fingerprint_bits=bits, hashes=3, index_mode=mode, k=31, minimizer_size=19,
+                randstrobemers=1, strobe_min=bounds[0], strobe_max=bounds[1],
+                homopolymer_compression=int(hpc), seed_pattern=pattern or "")
+    idx.load()
+    strobes = (idx.manifest()["randstrobemers"]["min_distance"],
+               idx.manifest()["randstrobemers"]["max_distance"])
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, _ = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.read_features(r, 31, 69, mode=mode, m=19, hpc=hpc, pattern=pattern,
+                               strobes=strobes)
+        assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
+    label = f"randstrobes mode{mode} {pattern} {strobes} hpc{hpc} b{bits}"
+    assert check_parity(idx, oracle_for(idx), reads, label=label).sum() > 0
+    # the count is the larger of the two orientations, so a read and its
+    # reverse complement score the same
+    pair = [reads[10], _revcomp(reads[10])]
+    t, c, _ = idx.query_reads(*zindex_amd.pack_reads(pair), records=True)
+    assert t[0] == t[1] and np.array_equal(c[0], c[1]) and c[0].sum() > 0
+
+
 def test_kmer_seeds_match_oracle_scan(base_case):
     colors, reads, idx, orc = base_case
     buf, off = zindex_amd.pack_reads(reads)

@@ -103,13 +103,15 @@ def test_server_concurrent_submitters_and_poll(serve_case):
         assert np.array_equal(got[2], exp[2]), i
 
 
-@pytest.mark.parametrize("mode", ["minimizers", "findere"])
+@pytest.mark.parametrize("mode", ["minimizers", "findere", "randstrobes"])
 def test_server_minimizers_mixed_equals_alone(mode):
     colors = make_colors(75, 6, 40000)
     reads = make_reads(76, colors, 240, lo=30, hi=3000)
-    cfg = dict(k=49, minimizer_size=31) if mode == "minimizers" else \
-        dict(k=31, findere_z=8, homopolymer_compression=1)
-    idx = build(colors, fingerprint_bits=16, hashes=3, index_mode=mode, **cfg)
+    cfg = {"minimizers": dict(index_mode=mode, k=49, minimizer_size=31),
+           "findere": dict(index_mode=mode, k=31, findere_z=8, homopolymer_compression=1),
+           "randstrobes": dict(index_mode="minimizers", k=31, minimizer_size=19,
+                               randstrobemers=1)}[mode]
+    idx = build(colors, fingerprint_bits=16, hashes=3, **cfg)
     idx.load()
     requests = split_requests(77, reads, 20)
     mixed, st = serve_all(idx, requests, slots=2, max_requests=5)

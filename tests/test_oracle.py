@@ -217,6 +217,60 @@ def test_spaced_seeds_mask_plain_features(mode, k, m):
     assert sorted(rc) == sorted(got)
 
 
+def test_strobe_hash_known_answer_and_pair_symmetry():
+    # src/anchors.rs:823-838
+    assert oracle().zo_strobe_hash(0x3332316f6c6c6568) == 0x2d119fced5ee39ce
+    h = lambda s: oracle().zo_strobe_hash(min(_encode(s), _encode(_rc(s))))
+    a, b = h(b"AC"), h(b"AG")
+    assert a == h(b"GT") and b == h(b"CT")  # canonical strobes
+    assert zr.link_strobes([0, 5], [a, b], 1, 10)[0] == a // 2 + b // 2
+    assert zr.link_strobes([0, 5], [b, a], 1, 10)[0] == a // 2 + b // 2
+
+
+@pytest.mark.parametrize("k,low,high,step,count,expected", [
+    (11, 4, 12, 1, 61, 11859818861789259928), (11, 2, 5, 3, 21, 7111390358099897965),
+    (21, 7, 15, 1, 51, 7250723734144174666)])
+def test_linker_matches_upstream_strobealign_witnesses(k, low, high, step, count, expected):
+    # src/anchors.rs:950-988: checksums of every key the upstream iterator emits
+    seq = b"ACGTTGCACTGATCGATGGCATACCTAGGCTAACGTCGATCTTACGGAATCGCTAGCTGACCATGGTTACG"
+    pos = list(range(0, len(seq) - k + 1, step))
+    hashes = [oracle().zo_strobe_hash(int(zr.scan_kmers(seq[p:p + k], k)[0])) for p in pos]
+    keys = zr.link_strobes(pos, hashes, low * step, high * step)
+    assert len(keys) == count
+    checksum = 0
+    for key in keys:
+        checksum = (checksum * 0x9e3779b185ebca87 + key) & MASK64
+    assert checksum == expected
+
+
+def test_randstrobe_distances_are_inclusive_with_self_fallback():
+    # src/anchors.rs:1037-1048
+    f = lambda s: zr.read_features(s, 3, 69, strobes=(5, 5))
+    assert len(f(b"AC")) == 0 and len(f(b"ACGTACG")) == 5 and len(f(b"ACGTACGT")) == 6
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_randstrobes_respect_breaks_compression_and_orientation(mode):
+    k = 9 if mode == 0 else 31
+    kw = dict(mode=mode, m=9, pattern="##__#__##", strobes=(9, 40))
+    seq = _sequence(300, 31)
+    # src/anchors.rs:1013-1035
+    once = zr.read_features(seq, k, 69, **kw).tolist()
+    assert zr.read_features(seq + b"NNN" + seq, k, 69, **kw).tolist() == once * 2
+    expanded = bytes(c for c in seq + b"NNN" + seq for _ in range(3))
+    assert zr.read_features(expanded, k, 69, hpc=True, **kw).tolist() == \
+        zr.read_features(zr.homopolymer_compress(expanded), k, 69, **kw).tolist()
+    # src/anchors.rs:868-908, 990-1011: one anchor per plain feature in either
+    # orientation; partner selection is directional
+    long = _sequence(1000, 37)
+    kw = dict(mode=mode, m=11, strobes=(15, 50))
+    k = 11 if mode == 0 else 29
+    fwd = zr.read_features(long, k, 69, **kw)
+    rev = zr.read_features(_rc(long), k, 69, **kw)
+    assert len(fwd) == len(rev) == len(zr.read_features(long, k, 69, mode=mode, m=11))
+    assert sorted(fwd.tolist()) != sorted(rev.tolist())
+
+
 def test_findere_consecutive_hits_in_one_colour():
     # src/findere.rs:466-477: six s-mers; colour 0 has runs of 2 and 3
     pos = [0]

@@ -42,7 +42,7 @@ class ZoIndex(C.Structure):
         ("n_colors", C.c_uint32), ("banks", C.POINTER(ZoBank)), ("n_banks", C.c_uint32),
         ("recovery", C.POINTER(ZoRecovery)), ("sampling", C.c_uint64),
         ("hpc", C.c_int), ("seed_width", C.c_uint32), ("seed_mask", C.c_uint64),
-        ("findere_z", C.c_uint32),
+        ("findere_z", C.c_uint32), ("strobe_min", C.c_uint64), ("strobe_max", C.c_uint64),
     ]
 
 
@@ -109,6 +109,10 @@ def oracle() -> C.CDLL:
         o.zo_spaced_project.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
         o.zo_spaced_canonical.restype = C.c_uint64
         o.zo_spaced_canonical.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64]
+        o.zo_strobe_hash.restype = C.c_uint64
+        o.zo_strobe_hash.argtypes = [C.c_uint64]
+        o.zo_link_strobes.restype = None
+        o.zo_link_strobes.argtypes = [u64p, u64p, C.c_size_t, C.c_uint64, C.c_uint64, u64p]
         o.zo_findere_sequence.restype = None
         o.zo_findere_sequence.argtypes = [C.c_char_p, C.c_size_t, C.c_int, C.c_int, C.c_uint32,
                                           HITS_FN, C.c_void_p, u64p, u64p, u64p]
@@ -159,12 +163,24 @@ def homopolymer_compress(seq: bytes) -> bytes:
     return out.raw[:n]
 
 
+def link_strobes(pos, hashes, lo: int, hi: int) -> list:
+    pos = np.ascontiguousarray(pos, np.uint64)
+    hashes = np.ascontiguousarray(hashes, np.uint64)
+    keys = np.zeros(max(len(pos), 1), np.uint64)
+    oracle().zo_link_strobes(pos.ctypes.data_as(u64p), hashes.ctypes.data_as(u64p), len(pos),
+                             lo, hi, keys.ctypes.data_as(u64p))
+    return keys[:len(pos)].tolist()
+
+
 def read_features(seq: bytes, k: int, seed: int, mode: int = 0, m: int = 0,
-                  sampling: int = 16, hpc: bool = False, pattern=None) -> np.ndarray:
-    """The query-side features of one read (k-mers, OCM or modimizers)."""
+                  sampling: int = 16, hpc: bool = False, pattern=None,
+                  strobes=None) -> np.ndarray:
+    """The query-side features of one read (k-mers, OCM, modimizers, with an
+    optional pattern; strobes=(min, max) links randstrobemers)."""
     width, mask = spaced_seed(pattern)
+    lo, hi = strobes or (0, 0)
     idx = ZoIndex(k=k, mode=mode, m=m, seed=seed, sampling=sampling, hpc=int(hpc),
-                  seed_width=width, seed_mask=mask)
+                  seed_width=width, seed_mask=mask, strobe_min=lo, strobe_max=hi)
     out = np.zeros(max(len(seq), 1), np.uint64)
     n = oracle().zo_read_features(C.byref(idx), seq, len(seq), out.ctypes.data_as(u64p))
     return out[:n]
@@ -443,7 +459,10 @@ class OracleIndex:
                          manifest.get("modimizer_sampling") or 16,
                          int(bool(manifest.get("homopolymer_compression"))),
                          *spaced_seed(manifest.get("seed_pattern")),
-                         manifest.get("findere_z", 10))
+                         manifest.get("findere_z", 10),
+                         *((manifest["randstrobemers"]["min_distance"],
+                            manifest["randstrobemers"]["max_distance"])
+                           if manifest.get("randstrobemers") else (0, 0)))
 
     @classmethod
     def from_dir(cls, path, recover: bool = True) -> "OracleIndex":

@@ -90,8 +90,6 @@ static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
 static void check_query_support(const Manifest &m) {
-  if (m.has_randstrobemers)
-    fail(ZG_ERR_UNSUPPORTED, "randstrobemers are not supported by the GPU query");
   if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32) fail(ZG_ERR_UNSUPPORTED, "k-mer indexes with k > 32 are not supported by the GPU query");
     if (m.index_mode == 2 && m.modimizer_sampling == 0)
@@ -842,6 +840,11 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   a.nontemporal = b->nontemporal;
   a.sampling = (idx->m.index_mode == 2 && !b->explicit_keys) ? idx->m.modimizer_sampling : 0;
   a.spaced = b->explicit_keys ? SpacedRuns{} : idx->spaced;
+  if (idx->m.has_randstrobemers) {
+    // both orientations: rows r (forward) and n + r (reverse complement)
+    a.records = 1;
+    a.n_reads = 2 * b->n_reads;
+  }
   if (idx->m.index_mode == 3) {
     // findere: s-mer windows, hits to rows; the scan after the lookups counts
     a.k = (uint32_t)(idx->m.k - idx->m.findere_z);
@@ -858,7 +861,8 @@ static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
 void enqueue_zero_outputs(zg_index *idx, zg_batch *b) {
   hipStream_t s = b->stream;
   const uint32_t C = (uint32_t)idx->m.colors.size();
-  if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
+  if (idx->m.has_randstrobemers) launch_fill_u32(b->counts.as<uint32_t>(), 2 * b->n_reads * C, 0, s);
+  else if (b->records) launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
   else launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
   launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
   if (idx->m.index_mode == 3)
@@ -901,14 +905,18 @@ void enqueue_lookups(zg_index *idx, zg_batch *b) {
       launch_lookup(lookup_args(idx, b, i), b->kcfg, s);
     }
   if (any && idx->m.index_mode == 3) enqueue_findere_scan(idx, b);
+  if (idx->m.has_randstrobemers)
+    launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * idx->m.colors.size(), s);
 }
 
 // Per-colour sums after the lookups.
 static void enqueue_reduce(zg_index *idx, zg_batch *b) {
   hipStream_t s = b->stream;
   const uint32_t C = (uint32_t)idx->m.colors.size();
-  if (!b->records) launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
-  else launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
+  if (!b->records && !idx->m.has_randstrobemers)
+    launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
+  else
+    launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
 }
 
 static void drop_graph(zg_batch *b) {
@@ -927,6 +935,74 @@ double elapsed_ms(hipEvent_t a, hipEvent_t b) {
     return -1.0;
   }
   return ms;
+}
+
+// Randstrobemer seeds (anchors::scan, src/anchors.rs:284-351) for both query
+// orientations (record_query.rs:659-685): explicit keys, forward ones first.
+static void seed_strobes(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
+  const Manifest &m = idx->m;
+  const uint64_t words = (b->positions + 63) / 64;
+  const uint32_t len = (uint32_t)(m.index_mode == 1 ? m.minimizer_size : m.k);
+  b->rs_cand.reserve((words + 1) * 8);
+  b->rs_counts.reserve((words + 1) * 4);
+  b->rs_off.reserve((words + 1) * 8);
+  b->rs_inv_off.reserve((words + 1) * 8);
+  const size_t tb = scan_temp_bytes_u32_to_u64(words + 1);
+  b->scan_temp.reserve(tb);
+  // run ids: invalid positions before each word
+  launch_popc_words(b->invalid.as<uint64_t>(), words, b->rs_counts.as<uint32_t>(), s);
+  launch_fill_u32(b->rs_counts.as<uint32_t>() + words, 1, 0, s);
+  launch_exclusive_scan_u32_to_u64(b->rs_counts.as<uint32_t>(), b->rs_inv_off.as<uint64_t>(),
+                                   words + 1, b->scan_temp.p, tb, s);
+  if (m.index_mode == 1) {
+    const uint64_t blocks = (b->positions + ocm_block_positions() - 1) / ocm_block_positions();
+    b->flag.reserve(b->positions + 64);
+    b->delta.reserve(b->positions + 64);
+    b->block_counts.reserve((blocks + 1) * sizeof(uint32_t));
+    launch_ocm_select(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), b->pos_off.as<uint64_t>(),
+                      b->n_reads, b->positions, (uint32_t)m.k, (uint32_t)m.minimizer_size, m.seed,
+                      idx->inner_table.as<uint64_t>(), b->flag.as<uint8_t>(),
+                      b->delta.as<uint8_t>(), b->block_counts.as<uint32_t>(), nullptr, s);
+    launch_fill_u64(b->rs_cand.as<uint64_t>(), words + 1, 0, s);
+    launch_strobe_mark_selected(b->flag.as<uint8_t>(), b->delta.as<uint8_t>(), b->positions,
+                                b->rs_cand.as<uint64_t>(), s);
+    launch_popc_words(b->rs_cand.as<uint64_t>(), words, b->rs_counts.as<uint32_t>(), s);
+  } else {
+    launch_strobe_kmer_candidates(b->invalid.as<uint64_t>(), b->positions, len,
+                                  b->rs_cand.as<uint64_t>(), b->rs_counts.as<uint32_t>(), s);
+  }
+  launch_fill_u32(b->rs_counts.as<uint32_t>() + words, 1, 0, s);
+  launch_exclusive_scan_u32_to_u64(b->rs_counts.as<uint32_t>(), b->rs_off.as<uint64_t>(), words + 1,
+                                   b->scan_temp.p, tb, s);
+  uint64_t total = 0;
+  check_hip(hipMemcpyAsync(&total, b->rs_off.as<uint64_t>() + words, 8, hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "seed");
+  b->key_win.reserve((total + 64) * 8);
+  b->rs_hash.reserve((total + 64) * 8);
+  b->rs_run.reserve((total + 64) * 8);
+  b->keys.reserve((2 * total + 64) * 8);
+  launch_strobe_compact(b->rs_cand.as<uint64_t>(), b->rs_off.as<uint64_t>(), b->invalid.as<uint64_t>(),
+                        b->rs_inv_off.as<uint64_t>(), b->codes.as<uint64_t>(), b->positions, len,
+                        idx->spaced, b->key_win.as<uint64_t>(), b->rs_hash.as<uint64_t>(),
+                        b->rs_run.as<uint64_t>(), s);
+  launch_strobe_link(b->key_win.as<uint64_t>(), b->rs_hash.as<uint64_t>(), b->rs_run.as<uint64_t>(),
+                     total, m.strobe_min, m.strobe_max, b->keys.as<uint64_t>(), s);
+  b->key_off.reserve((2 * b->n_reads + 1) * 8);
+  launch_segment_bounds(b->key_win.as<uint64_t>(), total, b->pos_off.as<uint64_t>(), b->n_reads,
+                        b->key_off.as<uint64_t>(), s);
+  launch_strobe_offsets(b->key_off.as<uint64_t>(), b->n_reads, total, s);
+  launch_diff(b->key_off.as<uint64_t>(), b->n_reads, b->totals.as<uint64_t>(), s);
+  check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+  b->h_seg_off.assign(2 * b->n_reads + 1, 0);
+  check_hip(hipMemcpyAsync(b->h_seg_off.data(), b->key_off.p, (2 * b->n_reads + 1) * 8,
+                           hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "seed");
+  b->n_keys = 2 * total;
+  b->rs_forward = total;
+  b->explicit_keys = true;
 }
 
 void seed_batch(zg_index *idx, zg_batch *b) {
@@ -959,6 +1035,11 @@ void seed_batch(zg_index *idx, zg_batch *b) {
                 b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
   }
   launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
+  if (m.has_randstrobemers) {
+    seed_strobes(idx, b);
+    b->last_seed_ms = elapsed_ms(b->ev[0], b->ev[1]);
+    return;
+  }
   if (m.index_mode == 3) b->fd_hits.reserve(std::max<uint64_t>(1, b->positions * idx->fd_row_words) * 4);
   if (m.index_mode != 1) {
     // k-mers, modimizers (sampled k-mers), or findere (its total is the
@@ -1045,7 +1126,8 @@ extern "C" int zg_batch_create(zg_index *idx, const uint8_t *seq, const uint64_t
     b->invalid.alloc(((b->positions + 63) / 64 + 4) * 8);
     b->totals.alloc((n_reads + 1) * 8);
     const uint32_t C = (uint32_t)idx->m.colors.size();
-    if (b->records) b->counts.alloc(std::max<uint64_t>(1, n_reads * C) * 4);
+    if (idx->m.has_randstrobemers) b->counts.alloc(std::max<uint64_t>(1, 2 * n_reads * C) * 4);
+    else if (b->records) b->counts.alloc(std::max<uint64_t>(1, n_reads * C) * 4);
     b->agg.alloc((uint64_t)C * 8 + 8);
     b->queue.alloc((idx->dbanks.size() + 1) * 4);
     b->h_seg_off = poff;
@@ -1128,18 +1210,23 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
     const uint32_t C = (uint32_t)idx->m.colors.size();
     const size_t nb = idx->dbanks.size();
     zg::DevBuf queues;
-    queues.alloc((count * nb + 1) * 4);
+    const bool strobes = idx->m.has_randstrobemers;
+    const uint64_t strands = strobes ? 2 : 1;
+    queues.alloc((strands * count * nb + 1) * 4);
     check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
-    if (b->records) zg::launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
+    if (strobes) zg::launch_fill_u32(b->counts.as<uint32_t>(), 2 * b->n_reads * C, 0, s);
+    else if (b->records) zg::launch_fill_u32(b->counts.as<uint32_t>(), b->n_reads * C, 0, s);
     else zg::launch_fill_u64(b->partial.as<uint64_t>(), b->waves * C, 0, s);
     zg::launch_fill_u64(b->agg.as<uint64_t>(), C, 0, s);
-    zg::launch_fill_u32(queues.as<uint32_t>(), count * nb, 0, s);
+    zg::launch_fill_u32(queues.as<uint32_t>(), strands * count * nb, 0, s);
     if (idx->m.index_mode == 3)
       zg::launch_fill_u32(b->fd_hits.as<uint32_t>(), b->positions * idx->fd_row_words, 0, s);
     check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
     for (size_t i : zg::bank_order(idx)) {
       zg::make_resident(idx, i, s);
-      for (uint64_t r = first; r < first + count; r++) {
+      for (uint64_t q = 0; q < strands * count; q++) {
+        // segment of read first + q (or, for randstrobes, its reverse complement)
+        const uint64_t r = q < count ? first + q : b->n_reads + first + (q - count);
         uint64_t lo = b->h_seg_off[r], hi = b->h_seg_off[r + 1];
         if (hi <= lo) continue;
         zg::LookupArgs a = zg::lookup_args(idx, b, i);
@@ -1147,7 +1234,7 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
         a.end = hi;
         uint64_t spans = (hi - lo + a.span - 1) / a.span;
         a.n_units = spans * a.n_tg;
-        a.queue = queues.as<uint32_t>() + (r - first) * nb + i;
+        a.queue = queues.as<uint32_t>() + q * nb + i;
         zg::KernelConfig c = b->kcfg;
         uint64_t want = (a.n_units + 3) / 4;
         c.blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, b->kcfg.blocks));
@@ -1155,8 +1242,9 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
       }
     }
     if (idx->m.index_mode == 3) zg::enqueue_findere_scan(idx, b);
+    if (strobes) zg::launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * C, s);
     check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
-    if (!b->records)
+    if (!b->records && !strobes)
       zg::launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
     else
       zg::launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
@@ -1417,6 +1505,26 @@ extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_
     if (rc) return rc;
     std::unique_ptr<zg_batch> hold(b);
     hipStream_t s = idx->stream;
+    if (idx->m.has_randstrobemers) {
+      // the forward keys, one per anchor, at the anchor's start
+      const uint64_t N = b->rs_forward;
+      std::vector<uint64_t> start(N), kk(N);
+      if (N) {
+        check_hip(hipMemcpyAsync(start.data(), b->key_win.p, N * 8, hipMemcpyDeviceToHost, s), "copy");
+        check_hip(hipMemcpyAsync(kk.data(), b->keys.p, N * 8, hipMemcpyDeviceToHost, s), "copy");
+      }
+      check_hip(hipStreamSynchronize(s), "copy");
+      for (uint64_t r = 0; r <= n_reads; r++) key_off[r] = b->h_seg_off[r];
+      if (N > cap) fail(ZG_ERR_INVALID, "seed buffer too small");
+      for (uint64_t r = 0; r < n_reads; r++) {
+        uint64_t p0 = (read_off[r] - read_off[0]) + r;
+        for (uint64_t i = key_off[r]; i < key_off[r + 1]; i++) {
+          if (keys) keys[i] = kk[i];
+          if (positions) positions[i] = start[i] - p0;
+        }
+      }
+      return 0;
+    }
     if (b->explicit_keys) {
       // key_win holds the window start; position = window start + delta
       std::vector<uint64_t> win(b->n_keys), kk(b->n_keys);

@@ -102,7 +102,7 @@ struct SlidingMin {
 // One all-ACGT run, the deque formulation of scan_run_scalar
 // (implementation.rs:349-414).
 static void host_ocm_run(const uint8_t *seq, size_t len, int k, int m, uint64_t seed,
-                         std::vector<uint64_t> &out) {
+                         std::vector<uint64_t> &out, std::vector<uint64_t> *positions = nullptr) {
   OcmParams P = ocm_parameters(k, m);
   const int w = P.w, t = P.t, s = P.s;
   const size_t inner_length = (size_t)(t - s + 1), anchor_length = (size_t)(k - t + 1);
@@ -140,6 +140,7 @@ static void host_ocm_run(const uint8_t *seq, size_t len, int k, int m, uint64_t 
     size_t pos = start + (chosen - start) % (size_t)w;
     if (!have_prev || prev != pos) {
       out.push_back(reps[pos % w]);
+      if (positions) positions->push_back(pos);
       prev = pos;
       have_prev = true;
     }
@@ -608,6 +609,81 @@ static SpacedRuns config_spaced(const zg_build_config &cfg) {
   return spaced_runs(width, mask);
 }
 
+// Linker (src/anchors.rs:185-256) over one run's candidates in position
+// order: every anchor emits one key, paired with the later candidate at a
+// distance in [min_d, max_d] of least rank (the earliest on ties), or with
+// itself when there is none.
+static void link_strobes(const std::vector<uint64_t> &pos, const std::vector<uint64_t> &hash,
+                         uint64_t min_d, uint64_t max_d, std::vector<uint64_t> &out) {
+  const size_t n = pos.size();
+  for (size_t i = 0; i < n; i++) {
+    uint64_t partner = hash[i];
+    uint32_t best = 64;
+    for (size_t j = i + 1; j < n; j++) {
+      const uint64_t d = pos[j] - pos[i];
+      if (d > max_d) break;
+      if (d < min_d) continue;
+      const uint32_t r = strobe_rank(hash[i], hash[j]);
+      if (r < best) {
+        best = r;
+        partner = hash[j];
+        if (r == 0) break;
+      }
+    }
+    out.push_back(strobe_pair(hash[i], partner));
+  }
+}
+
+static void strobe_bounds(const zg_build_config &cfg, uint64_t *lo, uint64_t *hi) {
+  const uint64_t length = cfg.index_mode == 1 ? cfg.minimizer_size : cfg.k;
+  *lo = cfg.strobe_min ? cfg.strobe_min : length;
+  *hi = cfg.strobe_max ? cfg.strobe_max : 3 * length;
+}
+
+// anchors::scan with randstrobemers (src/anchors.rs:284-351) on one record:
+// per run of valid bases, the canonical k-mers (or the selected minimizer
+// positions, each once) become strobes, linked downstream.
+static void extract_strobes(const zg_build_config &cfg, const SpacedRuns &spaced,
+                            const uint8_t *p, size_t len, std::vector<uint64_t> &keys) {
+  uint64_t lo = 0, hi = 0;
+  strobe_bounds(cfg, &lo, &hi);
+  std::vector<uint64_t> ckeys, cpos, chash;
+  for (size_t a = 0; a < len;) {
+    if (enc(p[a]) < 0) { a++; continue; }
+    size_t b = a;
+    while (b < len && enc(p[b]) >= 0) b++;
+    ckeys.clear();
+    cpos.clear();
+    if (cfg.index_mode == 1) {
+      if (b - a >= cfg.k) {
+        host_ocm_run(p + a, b - a, (int)cfg.k, (int)cfg.minimizer_size, cfg.seed, ckeys, &cpos);
+        std::vector<std::pair<uint64_t, uint64_t>> sel(cpos.size());
+        for (size_t i = 0; i < cpos.size(); i++) sel[i] = {cpos[i], ckeys[i]};
+        std::sort(sel.begin(), sel.end());
+        sel.erase(std::unique(sel.begin(), sel.end(),
+                              [](const std::pair<uint64_t, uint64_t> &x,
+                                 const std::pair<uint64_t, uint64_t> &y) { return x.first == y.first; }),
+                  sel.end());
+        cpos.resize(sel.size());
+        ckeys.resize(sel.size());
+        for (size_t i = 0; i < sel.size(); i++) {
+          cpos[i] = sel[i].first;
+          ckeys[i] = sel[i].second;
+        }
+      }
+    } else {
+      host_kmers(p + a, b - a, (int)cfg.k, ckeys);
+      cpos.resize(ckeys.size());
+      for (size_t i = 0; i < cpos.size(); i++) cpos[i] = i;
+    }
+    chash.resize(ckeys.size());
+    for (size_t i = 0; i < ckeys.size(); i++)
+      chash[i] = strobe_hash(spaced.n ? spaced_canonical(ckeys[i], spaced) : ckeys[i]);
+    link_strobes(cpos, chash, lo, hi, keys);
+    a = b;
+  }
+}
+
 // scan_index_features (src/lib.rs:573-642, anchors.rs:265-283) per record.
 static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64_t *off,
                     uint64_t n, std::vector<uint64_t> &keys) {
@@ -622,6 +698,10 @@ static void extract(const zg_build_config &cfg, const uint8_t *seq, const uint64
       p = compressed.data();
     }
     const size_t at = keys.size();
+    if (cfg.randstrobemers) {
+      extract_strobes(cfg, spaced, p, len, keys);
+      continue;
+    }
     if (cfg.index_mode == 1) {
       host_ocm(p, len, (int)cfg.k, (int)cfg.minimizer_size, cfg.seed, keys);
     } else if (cfg.index_mode == 3) {
@@ -754,7 +834,13 @@ static void validate_config(const zg_build_config &c) {
                    (c.index_mode == 1 ? "minimizer size m" : "k") + " (" +
                    std::to_string(length) + ")");
   }
-  ensure(!c.randstrobemers, "randstrobemers are not built here", -6);
+  if (c.randstrobemers) {
+    uint64_t lo = 0, hi = 0;
+    strobe_bounds(c, &lo, &hi);
+    // RandstrobeConfig::validate (src/anchors.rs:98-110)
+    ensure(lo > 0, "randstrobemer minimum distance must be greater than zero", -1);
+    ensure(lo <= hi, "randstrobemer minimum distance must not exceed maximum distance", -1);
+  }
   ensure(c.merge_group_size >= 1 && c.merge_group_size <= 64 - b, "invalid merge group size",
          -1);
   ensure(c.device <= 1, "device must be 0 (CPU peel) or 1 (GPU peel)", -1);
@@ -1086,6 +1172,11 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   m.modimizer_sampling = cfg.modimizer_sampling ? cfg.modimizer_sampling : 16;
   m.findere_z = cfg.findere_z;
   m.homopolymer_compression = cfg.homopolymer_compression != 0;
+  if (cfg.randstrobemers) {
+    m.has_randstrobemers = true;
+    strobe_bounds(cfg, &m.strobe_min, &m.strobe_max);
+    m.randstrobe_scheme = "strobealign-popcount8-symmetric64-downstream-v1";
+  }
   if (cfg.seed_pattern[0]) {
     m.has_seed_pattern = true;
     m.seed_pattern = cfg.seed_pattern;
@@ -1112,8 +1203,16 @@ static void finish_index(zg_builder *b, zg_index *idx, double build_seconds) {
   j.set("minimizer_scheme", Json::str("open-closed-mod"));
   j.set("homopolymer_compression", Json::boolean(m.homopolymer_compression));
   j.set("seed_pattern", cfg.seed_pattern[0] ? Json::str(cfg.seed_pattern) : Json::null());
-  j.set("randstrobemers", Json::null());
-  j.set("randstrobe_scheme", Json::null());
+  if (m.has_randstrobemers) {
+    Json r = Json::object();
+    r.set("min_distance", Json::u64(m.strobe_min));
+    r.set("max_distance", Json::u64(m.strobe_max));
+    j.set("randstrobemers", r);
+    j.set("randstrobe_scheme", Json::str(m.randstrobe_scheme));
+  } else {
+    j.set("randstrobemers", Json::null());
+    j.set("randstrobe_scheme", Json::null());
+  }
   j.set("modimizer_sampling", Json::u64(m.modimizer_sampling));
   j.set("findere_z", Json::u64(m.findere_z));
   j.set("sketch_size", Json::u64(128));

@@ -13,7 +13,8 @@ static int usage() {
           "usage: zorindex <build|query|info|fusion> [options]\n"
           "  build -f FOF -o OUT [-k K] [-m M] [-H HASHES] [-b BITS] [-s SEED]\n"
           "        [-i kmers|minimizers|modimizers|findere] [-d MODIMIZER_SAMPLING] [-z FINDERE_Z]\n"
-          "        [-c] [-p SEED_PATTERN] [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
+          "        [-c] [-p SEED_PATTERN] [-r [-l STROBE_MIN] [-u STROBE_MAX]]\n"
+          "        [-S SLOT_SCALE] [-L SEGMENT] [-X MAX_SUBINDEX]\n"
           "        [--merge-group-size N] [--drop-abandoned] [-t THREADS] [--gpu-peel]\n"
           "  query -i INDEX -q QUERY [-a MIN_RATIO] [-r] [-F tsv|compact] [-B BATCH]\n"
           "        [-M subindex|all] [--hbm-budget BYTES] [--no-recovery]\n"
@@ -83,6 +84,12 @@ int main(int argc, char **argv) {
         cfg.findere_z = (uint32_t)atoi(value(i).c_str());
       } else if (o == "-c" || o == "--homopolymer-compression") {
         cfg.homopolymer_compression = 1;
+      } else if (o == "-r" || o == "--randstrobemers") {
+        cfg.randstrobemers = 1;
+      } else if (o == "-l" || o == "--strobe-min") {
+        cfg.strobe_min = (uint32_t)atoi(value(i).c_str());
+      } else if (o == "-u" || o == "--strobe-max") {
+        cfg.strobe_max = (uint32_t)atoi(value(i).c_str());
       } else if (o == "-p" || o == "--seed-pattern") {
         std::string v = value(i);
         if (v.size() >= sizeof cfg.seed_pattern) {
@@ -93,6 +100,10 @@ int main(int argc, char **argv) {
       } else return usage();
     }
     if (fof.empty() || out.empty()) return usage();
+    if ((cfg.strobe_min || cfg.strobe_max) && !cfg.randstrobemers) {
+      fprintf(stderr, "--strobe-min and --strobe-max require --randstrobemers\n");
+      return 2;
+    }
     if (cfg.index_mode == 1 && !m_given) {
       fprintf(stderr, "--minimizer-size is required for minimizers\n");
       return 2;

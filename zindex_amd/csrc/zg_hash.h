@@ -144,6 +144,29 @@ inline SpacedRuns spaced_runs(uint32_t width, uint64_t mask) {
   return s;
 }
 
+// Randstrobe strobe hash (src/anchors.rs:134-151): XXH64 of one 8-byte
+// little-endian input with seed 0, per the published XXH64 specification.
+__host__ __device__ inline uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+__host__ __device__ inline uint64_t strobe_hash(uint64_t v) {
+  constexpr uint64_t P1 = 0x9E3779B185EBCA87ULL, P2 = 0xC2B2AE3D27D4EB4FULL,
+                     P3 = 0x165667B19E3779F9ULL, P4 = 0x85EBCA77C2B2AE63ULL,
+                     P5 = 0x27D4EB2F165667C5ULL;
+  uint64_t h = P5 + 8;
+  h ^= rotl64(v * P2, 31) * P1;
+  h = rotl64(h, 27) * P1 + P4;
+  h ^= h >> 33;
+  h *= P2;
+  h ^= h >> 29;
+  h *= P3;
+  return h ^ (h >> 32);
+}
+// The symmetric pair key h1/2 + h2/2 (src/anchors.rs:177-179).
+__host__ __device__ inline uint64_t strobe_pair(uint64_t a, uint64_t b) { return a / 2 + b / 2; }
+// Partner rank: popcount of the low eight bits of h1 ^ h2 (src/anchors.rs:228-248).
+__host__ __device__ inline uint32_t strobe_rank(uint64_t a, uint64_t b) {
+  return (uint32_t)__builtin_popcountll((a ^ b) & 255);
+}
+
 // Open-closed mod-minimizer parameters (implementation.rs:89-98).
 struct OcmParams {
   int w, t, s;

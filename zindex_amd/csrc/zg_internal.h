@@ -114,6 +114,11 @@ struct zg_batch {
   // homopolymer compression: keep flags, counts and offsets per 64 positions
   zg::DevBuf hpc_keep, hpc_counts, hpc_off;
   zg::DevBuf fd_hits;  // findere: positions x fd_row_words hit words
+  // randstrobemers: candidate bitmap, per-word counts and offsets, run-id
+  // offsets, candidate hashes and run ids (starts go to key_win); keys hold
+  // the forward keys then the reverse-complement keys (rs_forward each)
+  zg::DevBuf rs_cand, rs_counts, rs_off, rs_inv_off, rs_hash, rs_run;
+  uint64_t rs_forward = 0;
   // outputs
   zg::DevBuf counts, partial, agg, queue;
   uint64_t waves = 0;

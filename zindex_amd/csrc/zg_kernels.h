@@ -127,6 +127,24 @@ void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t regi
                          const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
                          uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
                          uint32_t n_colors, hipStream_t s);
+// Randstrobemers: candidate bitmaps (64 positions per word) with per-word
+// counts, their compaction into (start, strobe hash, run id) in position
+// order, and the linker for both query orientations (keys[i] forward,
+// keys[n + i] reverse complement).
+void launch_popc_words(const uint64_t *words, uint64_t n, uint32_t *counts, hipStream_t s);
+void launch_strobe_kmer_candidates(const uint64_t *invalid, uint64_t positions, uint32_t k,
+                                   uint64_t *cand, uint32_t *counts, hipStream_t s);
+void launch_strobe_mark_selected(const uint8_t *flag, const uint8_t *delta, uint64_t positions,
+                                 uint64_t *cand, hipStream_t s);
+void launch_strobe_compact(const uint64_t *cand, const uint64_t *cand_off, const uint64_t *invalid,
+                           const uint64_t *inv_off, const uint64_t *codes, uint64_t positions,
+                           uint32_t len, const SpacedRuns &spaced, uint64_t *cpos,
+                           uint64_t *chash, uint64_t *crun, hipStream_t s);
+void launch_strobe_link(const uint64_t *cpos, const uint64_t *chash, const uint64_t *crun,
+                        uint64_t n, uint64_t lo, uint64_t hi, uint64_t *keys, hipStream_t s);
+void launch_strobe_offsets(uint64_t *key_off, uint64_t n_reads, uint64_t total, hipStream_t s);
+// counts[i] = max(counts[i], counts[n + i]) for i < n (the two orientations).
+void launch_strand_max(uint32_t *counts, uint64_t n, hipStream_t s);
 // Homopolymer compression of packed reads: keep flags per 64 positions, then
 // (after an exclusive scan of counts into keep_off) the placement of kept
 // bases into zeroed codes / all-ones invalid.

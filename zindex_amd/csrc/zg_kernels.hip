@@ -525,6 +525,155 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
 }
 
 // ---------------------------------------------------------------------------
+// Randstrobemers (src/anchors.rs:136-351). Candidates are the valid k-mer
+// windows, or the selected minimizer starts (each once); both query
+// orientations are linked from the forward candidates, since the reverse
+// complement holds the same canonical strobes in mirrored order.
+// ---------------------------------------------------------------------------
+__global__ void popc_words_kernel(const uint64_t *words, uint64_t n, uint32_t *counts) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w < n) counts[w] = (uint32_t)__popcll(words[w]);
+}
+
+__global__ void strobe_kmer_candidates_kernel(const uint64_t *invalid, uint64_t n_words,
+                                              uint32_t k, uint64_t *cand, uint32_t *counts) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  const uint64_t v = ~invalid_window_starts(invalid, w, k);
+  cand[w] = v;
+  counts[w] = (uint32_t)__popcll(v);
+}
+
+__global__ void strobe_mark_selected_kernel(const uint8_t *flag, const uint8_t *delta,
+                                            uint64_t positions, uint64_t *cand) {
+  uint64_t S = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (S >= positions || !flag[S]) return;
+  const uint64_t p = S + delta[S];
+  atomicOr((unsigned long long *)&cand[p >> 6], (unsigned long long)(1ULL << (p & 63)));
+}
+
+// Candidate i in position order: start, strobe hash, run id (the number of
+// invalid positions before it; equal ids = same run of valid bases).
+__global__ void strobe_compact_kernel(const uint64_t *cand, const uint64_t *cand_off,
+                                      const uint64_t *invalid, const uint64_t *inv_off,
+                                      const uint64_t *codes, uint64_t n_words, uint32_t len,
+                                      SpacedRuns spaced, uint64_t *cpos, uint64_t *chash,
+                                      uint64_t *crun) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  const uint64_t c = cand[w], inv = invalid[w];
+  for (uint64_t m = c; m; m &= m - 1) {
+    const int i = __builtin_ctzll(m);
+    const uint64_t below = i ? (~0ULL >> (64 - i)) : 0;
+    const uint64_t at = cand_off[w] + __popcll(c & below);
+    const uint64_t p = w * 64 + i;
+    uint64_t key = canonical_from_window(stream_bits(codes, 2 * p), (int)len);
+    if (spaced.n) key = spaced_canonical(key, spaced);
+    cpos[at] = p;
+    chash[at] = strobe_hash(key);
+    crun[at] = inv_off[w] + __popcll(inv & below);
+  }
+}
+
+// keys[i]: the forward key of anchor i (partner: the later candidate of least
+// rank, first on ties); keys[n + i]: its key in the reverse-complement
+// orientation (partner: the earlier candidate of least rank, nearest on
+// ties). Partners lie at a distance in [lo, hi] in the same run; an anchor
+// without one pairs with itself.
+__global__ void strobe_link_kernel(const uint64_t *cpos, const uint64_t *chash,
+                                   const uint64_t *crun, uint64_t n, uint64_t lo, uint64_t hi,
+                                   uint64_t *keys) {
+  uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t p = cpos[i], h = chash[i], run = crun[i];
+  uint64_t partner = h;
+  uint32_t best = 64;
+  for (uint64_t j = i + 1; j < n; j++) {
+    const uint64_t d = cpos[j] - p;
+    if (d > hi || crun[j] != run) break;
+    if (d < lo) continue;
+    const uint32_t r = strobe_rank(h, chash[j]);
+    if (r < best) {
+      best = r;
+      partner = chash[j];
+      if (!r) break;
+    }
+  }
+  keys[i] = strobe_pair(h, partner);
+  partner = h;
+  best = 64;
+  for (uint64_t j = i; j-- > 0;) {
+    const uint64_t d = p - cpos[j];
+    if (d > hi || crun[j] != run) break;
+    if (d < lo) continue;
+    const uint32_t r = strobe_rank(h, chash[j]);
+    if (r < best) {
+      best = r;
+      partner = chash[j];
+      if (!r) break;
+    }
+  }
+  keys[n + i] = strobe_pair(h, partner);
+}
+
+// key_off[n + r] = total + key_off[r] for r in 1..n (key_off[n] == total).
+__global__ void strobe_offsets_kernel(uint64_t *key_off, uint64_t n_reads, uint64_t total) {
+  uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x + 1;
+  if (r <= n_reads) key_off[n_reads + r] = total + key_off[r];
+}
+
+// counts[r][c] = max over the two orientations (rows r and n + r).
+__global__ void strand_max_kernel(uint32_t *counts, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t a = counts[i], b = counts[n + i];
+    if (b > a) counts[i] = b;
+  }
+}
+
+void launch_popc_words(const uint64_t *words, uint64_t n, uint32_t *counts, hipStream_t s) {
+  if (!n) return;
+  popc_words_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(words, n, counts);
+}
+void launch_strobe_kmer_candidates(const uint64_t *invalid, uint64_t positions, uint32_t k,
+                                   uint64_t *cand, uint32_t *counts, hipStream_t s) {
+  const uint64_t words = (positions + 63) / 64;
+  if (!words) return;
+  strobe_kmer_candidates_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(invalid, words, k,
+                                                                                  cand, counts);
+}
+void launch_strobe_mark_selected(const uint8_t *flag, const uint8_t *delta, uint64_t positions,
+                                 uint64_t *cand, hipStream_t s) {
+  if (!positions) return;
+  strobe_mark_selected_kernel<<<(uint32_t)((positions + 255) / 256), 256, 0, s>>>(
+      flag, delta, positions, cand);
+}
+void launch_strobe_compact(const uint64_t *cand, const uint64_t *cand_off, const uint64_t *invalid,
+                           const uint64_t *inv_off, const uint64_t *codes, uint64_t positions,
+                           uint32_t len, const SpacedRuns &spaced, uint64_t *cpos,
+                           uint64_t *chash, uint64_t *crun, hipStream_t s) {
+  const uint64_t words = (positions + 63) / 64;
+  if (!words) return;
+  strobe_compact_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      cand, cand_off, invalid, inv_off, codes, words, len, spaced, cpos, chash, crun);
+}
+void launch_strobe_link(const uint64_t *cpos, const uint64_t *chash, const uint64_t *crun,
+                        uint64_t n, uint64_t lo, uint64_t hi, uint64_t *keys, hipStream_t s) {
+  if (!n) return;
+  strobe_link_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(cpos, chash, crun, n, lo, hi,
+                                                                   keys);
+}
+void launch_strobe_offsets(uint64_t *key_off, uint64_t n_reads, uint64_t total, hipStream_t s) {
+  if (!n_reads) return;
+  strobe_offsets_kernel<<<(uint32_t)((n_reads + 255) / 256), 256, 0, s>>>(key_off, n_reads,
+                                                                            total);
+}
+void launch_strand_max(uint32_t *counts, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  strand_max_kernel<<<grid_for(n, 256, 16384), 256, 0, s>>>(counts, n);
+}
+
+// ---------------------------------------------------------------------------
 // Scans (hipcub)
 // ---------------------------------------------------------------------------
 size_t scan_temp_bytes_u64(uint64_t n) {
