@@ -187,6 +187,26 @@ def test_findere_build_indexes_smers_of_long_runs(tmp_path, small_colors, k, z, 
     assert f"findere_z\t{z}\nindexed_k\t{k - z}\n" in text  # storage.rs:3982-3991
 
 
+@pytest.mark.parametrize("k,mode", [(33, 0), (49, 0), (64, 0), (65, 0), (255, 0), (45, 2)])
+def test_long_kmer_build(tmp_path, small_colors, k, mode):
+    idx = _build(small_colors, fingerprint_bits=8, hashes=4, k=k, index_mode=mode,
+                 modimizer_sampling=4)
+    out = tmp_path / "idx"
+    idx.write(str(out))
+    disk = zr.read_index_dir(out)
+    man = disk["manifest"]
+    assert man["k"] == k
+    o = zr.OracleIndex(man, disk["banks"], disk["recovery"])
+    for c, seq in enumerate(small_colors):
+        keys = np.unique(zr.read_features(seq, k, man["seed"], mode=mode, sampling=4))
+        assert man["colors"][c]["unique_kmers"] == len(keys)
+        scores = np.zeros(len(small_colors), np.uint64)
+        for key in keys[:1500]:
+            before = int(scores[c])
+            o.lookup(int(key), scores)
+            assert int(scores[c]) == before + 1, (c, int(key))
+
+
 @pytest.mark.parametrize("mode,pattern,bounds,hpc", [
     (0, None, (0, 0), False), (0, K31_PATTERN, (10, 50), True), (1, None, (0, 0), False),
     (1, M19_PATTERN, (5, 40), False)])

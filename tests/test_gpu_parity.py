@@ -301,6 +301,28 @@ def test_randstrobe_parity(mode, pattern, bounds, hpc, bits):
     assert t[0] == t[1] and np.array_equal(c[0], c[1]) and c[0].sum() > 0
 
 
+@pytest.mark.parametrize("k,mode,hpc", [(33, 0, False), (64, 0, False), (65, 0, True),
+                                         (255, 0, False), (511, 0, False), (45, 2, False)])
+def test_long_kmer_parity(k, mode, hpc):
+    # tests/long_kmer_cli.rs: k-mers longer than 32 hash both orientations
+    colors = make_colors(180 + k, 5, 20000)
+    reads = make_reads(181 + k, colors, 120, lo=20, hi=2500, error=0.002)
+    idx = build(colors, fingerprint_bits=8, hashes=4, k=k, index_mode=mode, modimizer_sampling=4,
+                homopolymer_compression=int(hpc))
+    idx.load()
+    buf, off = zindex_amd.pack_reads(reads)
+    key_off, keys, pos = idx.extract_seeds(buf, off)
+    for i, r in enumerate(reads):
+        exp = zr.read_features(r, k, 69, mode=mode, sampling=4, hpc=hpc)
+        assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
+    label = f"long k{k} mode{mode} hpc{hpc}"
+    assert check_parity(idx, oracle_for(idx), reads, label=label).sum() > 0
+    # a source and its reverse complement match every one of their k-mers
+    src = colors[0][1000:4000]
+    t, c, _ = idx.query_reads(*zindex_amd.pack_reads([src, _revcomp(src), b"ACGT"]), records=True)
+    assert t[0] == t[1] and c[0][0] == t[0] and c[1][0] == t[1] and t[2] == 0
+
+
 def test_kmer_seeds_match_oracle_scan(base_case):
     colors, reads, idx, orc = base_case
     buf, off = zindex_amd.pack_reads(reads)

@@ -91,7 +91,8 @@ static unsigned host_threads() { return zg_host_threads(); }
 // Query support of this library for an index (the modes BASELINE targets).
 static void check_query_support(const Manifest &m) {
   if (m.index_mode == 0 || m.index_mode == 2) {
-    if (m.k > 32) fail(ZG_ERR_UNSUPPORTED, "k-mer indexes with k > 32 are not supported by the GPU query");
+    if (m.k > 32 && m.has_randstrobemers)
+      fail(ZG_ERR_UNSUPPORTED, "randstrobemers of k-mers longer than 32 are not supported by the GPU query");
     if (m.index_mode == 2 && m.modimizer_sampling == 0)
       fail(ZG_ERR_FORMAT, "modimizer sampling must be positive");
   } else if (m.index_mode == 1) {
@@ -1005,11 +1006,53 @@ static void seed_strobes(zg_index *idx, zg_batch *b) {
   b->explicit_keys = true;
 }
 
+// Long k-mers (k > 32) as explicit keys: hashed per window, then compacted
+// in position order (modimizers keep only the sampled ones).
+static void seed_long_kmers(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
+  const Manifest &m = idx->m;
+  const uint64_t words = (b->positions + 63) / 64;
+  b->rs_cand.reserve((words + 1) * 8);
+  b->rs_counts.reserve((words + 1) * 4);
+  b->rs_off.reserve((words + 1) * 8);
+  b->rs_hash.reserve((b->positions + 64) * 8);
+  const size_t tb = scan_temp_bytes_u32_to_u64(words + 1);
+  b->scan_temp.reserve(tb);
+  launch_long_kmer_keys(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), b->positions,
+                        long_kmer_params((uint32_t)m.k), m.seed,
+                        m.index_mode == 2 ? m.modimizer_sampling : 0, b->rs_cand.as<uint64_t>(),
+                        b->rs_counts.as<uint32_t>(), b->rs_hash.as<uint64_t>(), s);
+  launch_fill_u32(b->rs_counts.as<uint32_t>() + words, 1, 0, s);
+  launch_exclusive_scan_u32_to_u64(b->rs_counts.as<uint32_t>(), b->rs_off.as<uint64_t>(), words + 1,
+                                   b->scan_temp.p, tb, s);
+  uint64_t total = 0;
+  check_hip(hipMemcpyAsync(&total, b->rs_off.as<uint64_t>() + words, 8, hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "seed");
+  b->keys.reserve((total + 64) * 8);
+  b->key_win.reserve((total + 64) * 8);
+  launch_compact_kept(b->rs_cand.as<uint64_t>(), b->rs_off.as<uint64_t>(), b->rs_hash.as<uint64_t>(),
+                      b->positions, b->keys.as<uint64_t>(), b->key_win.as<uint64_t>(), s);
+  b->key_off.reserve((b->n_reads + 1) * 8);
+  launch_segment_bounds(b->key_win.as<uint64_t>(), total, b->pos_off.as<uint64_t>(), b->n_reads,
+                        b->key_off.as<uint64_t>(), s);
+  launch_diff(b->key_off.as<uint64_t>(), b->n_reads, b->totals.as<uint64_t>(), s);
+  check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+  b->h_seg_off.assign(b->n_reads + 1, 0);
+  check_hip(hipMemcpyAsync(b->h_seg_off.data(), b->key_off.p, (b->n_reads + 1) * 8,
+                           hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  check_hip(hipStreamSynchronize(s), "seed");
+  b->n_keys = total;
+  b->explicit_keys = true;
+  b->keys_at_position = true;
+}
+
 void seed_batch(zg_index *idx, zg_batch *b) {
   hipStream_t s = b->stream;
   const Manifest &m = idx->m;
   check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
-  uint64_t cwords = (b->positions + 31) / 32 + 4, iwords = (b->positions + 63) / 64 + 4;
+  const uint64_t cwords = code_words(b->positions, m.k), iwords = invalid_words(b->positions, m.k);
   launch_fill_u64(b->codes.as<uint64_t>(), cwords, 0, s);
   launch_fill_u64(b->invalid.as<uint64_t>(), iwords, ~0ULL, s);
   if (m.homopolymer_compression) {
@@ -1035,8 +1078,14 @@ void seed_batch(zg_index *idx, zg_batch *b) {
                 b->n_reads, b->positions, b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), s);
   }
   launch_fill_u64(b->totals.as<uint64_t>(), b->n_reads, 0, s);
+  b->keys_at_position = false;
   if (m.has_randstrobemers) {
     seed_strobes(idx, b);
+    b->last_seed_ms = elapsed_ms(b->ev[0], b->ev[1]);
+    return;
+  }
+  if ((m.index_mode == 0 || m.index_mode == 2) && m.k > 32) {
+    seed_long_kmers(idx, b);
     b->last_seed_ms = elapsed_ms(b->ev[0], b->ev[1]);
     return;
   }
@@ -1122,8 +1171,8 @@ extern "C" int zg_batch_create(zg_index *idx, const uint8_t *seq, const uint64_t
                 "hipMemcpyAsync");
     upload(b->read_off, roff, s);
     upload(b->pos_off, poff, s);
-    b->codes.alloc(((b->positions + 31) / 32 + 4) * 8);
-    b->invalid.alloc(((b->positions + 63) / 64 + 4) * 8);
+    b->codes.alloc(code_words(b->positions, idx->m.k) * 8);
+    b->invalid.alloc(invalid_words(b->positions, idx->m.k) * 8);
     b->totals.alloc((n_reads + 1) * 8);
     const uint32_t C = (uint32_t)idx->m.colors.size();
     if (idx->m.has_randstrobemers) b->counts.alloc(std::max<uint64_t>(1, 2 * n_reads * C) * 4);
@@ -1528,11 +1577,12 @@ extern "C" int zg_extract_seeds(zg_index *idx, const uint8_t *seq, const uint64_
     if (b->explicit_keys) {
       // key_win holds the window start; position = window start + delta
       std::vector<uint64_t> win(b->n_keys), kk(b->n_keys);
-      std::vector<uint8_t> delta(b->positions);
+      std::vector<uint8_t> delta(b->positions, 0);  // long k-mers: keys sit at their start
       if (b->n_keys) {
         check_hip(hipMemcpyAsync(win.data(), b->key_win.p, b->n_keys * 8, hipMemcpyDeviceToHost, s), "copy");
         check_hip(hipMemcpyAsync(kk.data(), b->keys.p, b->n_keys * 8, hipMemcpyDeviceToHost, s), "copy");
-        check_hip(hipMemcpyAsync(delta.data(), b->delta.p, b->positions, hipMemcpyDeviceToHost, s), "copy");
+        if (!b->keys_at_position)
+          check_hip(hipMemcpyAsync(delta.data(), b->delta.p, b->positions, hipMemcpyDeviceToHost, s), "copy");
       }
       check_hip(hipStreamSynchronize(s), "copy");
       for (uint64_t r = 0; r <= n_reads; r++) key_off[r] = b->h_seg_off[r];

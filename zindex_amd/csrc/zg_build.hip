@@ -44,9 +44,42 @@ static inline int enc(uint8_t c) {
   }
 }
 
-// scan_canonical_kmers, src/lib.rs:645-681 (k <= 32).
+// scan_long_kmers (src/lib.rs:683-730): rolling polynomial hashes of both
+// orientations, the smaller one mixed with a k-dependent domain.
+static void host_long_kmers(const uint8_t *seq, size_t len, int k, std::vector<uint64_t> &out) {
+  const LongKmerParams lp = long_kmer_params((uint32_t)k);
+  const uint64_t B = 0x9e3779b185ebca87ULL;
+  uint64_t f = 0, r = 0, weight = 1;
+  size_t seen = 0;
+  for (size_t i = 0; i < len; i++) {
+    int c = enc(seq[i]);
+    if (c < 0) {
+      f = r = 0;
+      weight = 1;
+      seen = 0;
+      continue;
+    }
+    if (seen < (size_t)k) {
+      f = f * B + (uint64_t)c + 1;
+      r += (uint64_t)(4 - c) * weight;
+      weight *= B;
+      seen++;
+    } else {
+      const uint64_t o = (uint64_t)enc(seq[i - k]);
+      f = (f - (o + 1) * lp.power) * B + (uint64_t)c + 1;
+      r = (r - (4 - o)) * lp.inverse + (uint64_t)(4 - c) * lp.power;
+    }
+    if (seen == (size_t)k) out.push_back(splitmix64((f < r ? f : r) ^ lp.domain));
+  }
+}
+
+// scan_canonical_kmers, src/lib.rs:645-681.
 static void host_kmers(const uint8_t *seq, size_t len, int k, std::vector<uint64_t> &out) {
   if (k <= 0 || len < (size_t)k) return;
+  if (k > 32) {
+    host_long_kmers(seq, len, k, out);
+    return;
+  }
   uint64_t mask = k == 32 ? ~0ULL : ((1ULL << (2 * k)) - 1);
   int shift = 2 * (k - 1);
   uint64_t f = 0, r = 0;
@@ -805,7 +838,9 @@ static void validate_config(const zg_build_config &c) {
   if (c.index_mode == 0 || c.index_mode == 2) {
     ensure(c.index_mode == 0 || c.modimizer_sampling >= 1, "modimizer sampling must be positive",
            -1);
-    ensure(c.k >= 1 && c.k <= 32, "k must be in 1..=32 for k-mer indexes in this build", -6);
+    ensure(c.k >= 1, "--k must be positive", -1);
+    ensure(c.k <= 32 || !c.randstrobemers,
+           "randstrobemers of k-mers longer than 32 are not built here", -6);
   } else if (c.index_mode == 1) {
     ensure(c.k >= 1 && c.k <= 255 && c.k % 2 == 1 && c.minimizer_size >= 1 &&
                c.minimizer_size <= 31 && c.minimizer_size <= c.k,

@@ -119,6 +119,7 @@ struct zg_batch {
   // the forward keys then the reverse-complement keys (rs_forward each)
   zg::DevBuf rs_cand, rs_counts, rs_off, rs_inv_off, rs_hash, rs_run;
   uint64_t rs_forward = 0;
+  bool keys_at_position = false;  // explicit keys sit at their window start (long k-mers)
   // outputs
   zg::DevBuf counts, partial, agg, queue;
   uint64_t waves = 0;

@@ -127,6 +127,31 @@ void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t regi
                          const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
                          uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
                          uint32_t n_colors, hipStream_t s);
+// Padding, in words, after the invalid-base bits of `positions` positions so
+// that a window of `window` bases starting at any position stays in bounds.
+inline uint64_t invalid_words(uint64_t positions, uint64_t window) {
+  return (positions + 63) / 64 + 4 + (window + 63) / 64;
+}
+inline uint64_t code_words(uint64_t positions, uint64_t window) {
+  return (positions + 31) / 32 + 4 + (window + 31) / 32;
+}
+// Long k-mers (k > 32): rolling polynomial hashes of both orientations
+// (scan_long_kmers, src/lib.rs:683-730).
+struct LongKmerParams {
+  uint64_t power;    // base^(k-1)
+  uint64_t inverse;  // base^-1 mod 2^64
+  uint64_t domain;   // splitmix64(k ^ 0x6b6d65722d763031)
+  uint32_t k;
+};
+LongKmerParams long_kmer_params(uint32_t k);
+// Per 64 positions: keep bits of the valid windows (modimizers: sampled
+// ones), their count, and the key of each kept window at pos_key[p].
+void launch_long_kmer_keys(const uint64_t *codes, const uint64_t *invalid, uint64_t positions,
+                           const LongKmerParams &lp, uint64_t seed, uint64_t sampling,
+                           uint64_t *keep, uint32_t *counts, uint64_t *pos_key, hipStream_t s);
+// keys[off[w] + rank] = pos_key[p], key_win[...] = p for every kept bit p.
+void launch_compact_kept(const uint64_t *keep, const uint64_t *off, const uint64_t *pos_key,
+                         uint64_t positions, uint64_t *keys, uint64_t *key_win, hipStream_t s);
 // Randstrobemers: candidate bitmaps (64 positions per word) with per-word
 // counts, their compaction into (start, strobe hash, run id) in position
 // order, and the linker for both query orientations (keys[i] forward,

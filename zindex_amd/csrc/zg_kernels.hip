@@ -244,8 +244,8 @@ void launch_hpc_place(const uint8_t *seq, const uint64_t *read_off, const uint64
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invalid, uint64_t w,
                                                           uint32_t k) {
-  // bit i set iff window [64w+i, 64w+i+k) contains an invalid base
-  // (k <= 255: the invalid words are padded by four)
+  // bit i set iff window [64w+i, 64w+i+k) contains an invalid base (the
+  // invalid words are padded for the window length, invalid_words)
   uint64_t m = 0;
   if (k <= 64) {
     uint64_t lo = invalid[w], hi = invalid[w + 1];
@@ -522,6 +522,107 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
   if (!blocks) return;
   ocm_compact_kernel<<<(uint32_t)blocks, 256, 0, s>>>(codes, flag, delta, block_off, positions,
                                                       m, spaced, keys, key_pos);
+}
+
+// ---------------------------------------------------------------------------
+// Long k-mers (k > 32, scan_long_kmers, src/lib.rs:683-730). Forward hash
+// sum (c_i + 1) B^(k-1-i), reverse-complement hash sum (4 - c_i) B^i, both
+// mod 2^64; the key is splitmix64(min ^ domain). One thread per 64 window
+// starts: the first valid window of a streak is hashed directly in O(k), the
+// next ones roll in O(1).
+// ---------------------------------------------------------------------------
+constexpr uint64_t kKmerHashBase = 0x9e3779b185ebca87ULL;
+
+LongKmerParams long_kmer_params(uint32_t k) {
+  LongKmerParams lp{};
+  uint64_t inv = kKmerHashBase;
+  for (int i = 0; i < 6; i++) inv *= 2 - kKmerHashBase * inv;  // Newton steps
+  uint64_t power = 1, b = kKmerHashBase;
+  for (uint32_t e = k - 1; e; e >>= 1) {
+    if (e & 1) power *= b;
+    b *= b;
+  }
+  lp.power = power;
+  lp.inverse = inv;
+  lp.domain = splitmix64((uint64_t)k ^ 0x6b6d65722d763031ULL);
+  lp.k = k;
+  return lp;
+}
+
+__device__ __forceinline__ uint64_t code_at(const uint64_t *codes, uint64_t p) {
+  return (codes[p >> 5] >> (2 * (p & 31))) & 3;
+}
+
+__global__ void long_kmer_keys_kernel(const uint64_t *codes, const uint64_t *invalid,
+                                      uint64_t n_words, LongKmerParams lp, uint64_t seed,
+                                      uint64_t sampling, uint64_t *keep, uint32_t *counts,
+                                      uint64_t *pos_key) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  const uint64_t valid = ~invalid_window_starts(invalid, w, lp.k);
+  uint64_t kept = 0, fwd = 0, rev = 0;
+  bool have = false;
+  for (int i = 0; i < 64; i++) {
+    if (!((valid >> i) & 1)) {
+      have = false;
+      continue;
+    }
+    const uint64_t p = w * 64 + i;
+    if (!have) {
+      fwd = 0;
+      rev = 0;
+      uint64_t weight = 1;
+      for (uint32_t j = 0; j < lp.k; j++) {
+        const uint64_t c = code_at(codes, p + j);
+        fwd = fwd * kKmerHashBase + c + 1;
+        rev += (4 - c) * weight;
+        weight *= kKmerHashBase;
+      }
+      have = true;
+    } else {
+      const uint64_t out = code_at(codes, p - 1), in = code_at(codes, p + lp.k - 1);
+      fwd = (fwd - (out + 1) * lp.power) * kKmerHashBase + in + 1;
+      rev = (rev - (4 - out)) * lp.inverse + (4 - in) * lp.power;
+    }
+    const uint64_t key = splitmix64((fwd < rev ? fwd : rev) ^ lp.domain);
+    if (sampling && mixsplit(key, seed) % sampling) continue;  // not a modimizer
+    kept |= 1ULL << i;
+    pos_key[p] = key;
+  }
+  keep[w] = kept;
+  counts[w] = (uint32_t)__popcll(kept);
+}
+
+__global__ void compact_kept_kernel(const uint64_t *keep, const uint64_t *off,
+                                    const uint64_t *pos_key, uint64_t n_words, uint64_t *keys,
+                                    uint64_t *key_win) {
+  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (w >= n_words) return;
+  const uint64_t kb = keep[w];
+  uint64_t at = off[w];
+  for (uint64_t m = kb; m; m &= m - 1) {
+    const uint64_t p = w * 64 + __builtin_ctzll(m);
+    keys[at] = pos_key[p];
+    key_win[at] = p;
+    at++;
+  }
+}
+
+void launch_long_kmer_keys(const uint64_t *codes, const uint64_t *invalid, uint64_t positions,
+                           const LongKmerParams &lp, uint64_t seed, uint64_t sampling,
+                           uint64_t *keep, uint32_t *counts, uint64_t *pos_key, hipStream_t s) {
+  const uint64_t words = (positions + 63) / 64;
+  if (!words) return;
+  long_kmer_keys_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      codes, invalid, words, lp, seed, sampling, keep, counts, pos_key);
+}
+
+void launch_compact_kept(const uint64_t *keep, const uint64_t *off, const uint64_t *pos_key,
+                         uint64_t positions, uint64_t *keys, uint64_t *key_win, hipStream_t s) {
+  const uint64_t words = (positions + 63) / 64;
+  if (!words) return;
+  compact_kept_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(keep, off, pos_key, words,
+                                                                      keys, key_win);
 }
 
 // ---------------------------------------------------------------------------

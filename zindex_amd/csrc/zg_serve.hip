@@ -91,7 +91,7 @@ static uint64_t slot_bytes(const zg_index *idx, const zg_server_config &c) {
   const uint64_t C = idx->m.colors.size();
   const uint64_t pos = c.slot_bases + c.slot_reads;
   uint64_t n = (c.slot_bases + 64) + 2 * (c.slot_reads + 1) * 8;  // seq, offsets
-  n += ((pos + 31) / 32 + 4) * 8 + ((pos + 63) / 64 + 4) * 8;     // codes, invalid bits
+  n += (code_words(pos, idx->m.k) + invalid_words(pos, idx->m.k)) * 8;  // codes, invalid bits
   n += (c.slot_reads + 1) * 8;                                    // totals
   const uint64_t strands = idx->m.has_randstrobemers ? 2 : 1;     // both orientations
   n += std::max<uint64_t>(1, strands * c.slot_reads * C) * 4;     // counters
@@ -107,6 +107,11 @@ static uint64_t slot_bytes(const zg_index *idx, const zg_server_config &c) {
     n += (words + 1) * 20 + scan_temp_bytes_u32_to_u64(words + 1);
   }
   if (idx->m.index_mode == 3) n += std::max<uint64_t>(1, pos * idx->fd_row_words) * 4;
+  if ((idx->m.index_mode == 0 || idx->m.index_mode == 2) && idx->m.k > 32) {
+    const uint64_t words = (pos + 63) / 64;
+    n += (words + 1) * 20 + (pos + 64) * 8 * 3 + (c.slot_reads + 1) * 8;
+    n += scan_temp_bytes_u32_to_u64(words + 1);
+  }
   if (idx->m.has_randstrobemers) {
     const uint64_t words = (pos + 63) / 64;
     n += (words + 1) * 28 + (pos + 64) * 8 * 5 + (2 * c.slot_reads + 1) * 8;
@@ -127,8 +132,8 @@ static void init_slot(zg_index *idx, const zg_server_config &c, ServeSlot &sl) {
   b.seq.alloc(c.slot_bases + 64);
   b.read_off.alloc((c.slot_reads + 1) * 8);
   b.pos_off.alloc((c.slot_reads + 1) * 8);
-  b.codes.alloc(((pos + 31) / 32 + 4) * 8);
-  b.invalid.alloc(((pos + 63) / 64 + 4) * 8);
+  b.codes.alloc(code_words(pos, idx->m.k) * 8);
+  b.invalid.alloc(invalid_words(pos, idx->m.k) * 8);
   b.totals.alloc((c.slot_reads + 1) * 8);
   b.counts.alloc(std::max<uint64_t>(1, (idx->m.has_randstrobemers ? 2 : 1) * c.slot_reads * C) * 4);
   b.agg.alloc(C * 8 + 8);
@@ -147,6 +152,18 @@ static void init_slot(zg_index *idx, const zg_server_config &c, ServeSlot &sl) {
     b.scan_temp.alloc(scan_temp_bytes_u32_to_u64(blocks + 1));
   }
   if (idx->m.index_mode == 3) b.fd_hits.alloc(std::max<uint64_t>(1, pos * idx->fd_row_words) * 4);
+  if ((idx->m.index_mode == 0 || idx->m.index_mode == 2) && idx->m.k > 32) {
+    // long k-mers: per-position keys, compacted
+    const uint64_t words = (pos + 63) / 64;
+    b.rs_cand.reserve((words + 1) * 8);
+    b.rs_counts.reserve((words + 1) * 4);
+    b.rs_off.reserve((words + 1) * 8);
+    b.rs_hash.reserve((pos + 64) * 8);
+    b.keys.reserve((pos + 64) * 8);
+    b.key_win.reserve((pos + 64) * 8);
+    b.key_off.reserve((c.slot_reads + 1) * 8);
+    b.scan_temp.reserve(scan_temp_bytes_u32_to_u64(words + 1));
+  }
   if (idx->m.has_randstrobemers) {
     // candidates (at most one per position) and both orientations' keys
     const uint64_t words = (pos + 63) / 64;
