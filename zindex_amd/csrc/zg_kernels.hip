@@ -935,11 +935,31 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   (void)kmask;
   const uint64_t win_mask = a.k >= 64 ? ~0ULL : ((1ULL << a.k) - 1);
 
-  uint32_t acc[T][F];
+  // Field counters: per key, the F hit bits of a lane are added as byte
+  // counters, four fields per word (bit i of a nibble spread to byte i by one
+  // multiply), and spilled into the 32-bit counters every 255 keys and at
+  // every flush.
+  constexpr int Q = F / 4;
+  uint32_t acc[T][F], acc8[T][Q];
 #pragma unroll
-  for (int t = 0; t < T; t++)
+  for (int t = 0; t < T; t++) {
 #pragma unroll
     for (int f = 0; f < F; f++) acc[t][f] = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) acc8[t][q] = 0;
+  }
+  uint32_t pending = 0;  // keys in acc8 (wave-uniform, below 255)
+  auto spill = [&]() {
+#pragma unroll
+    for (int t = 0; t < T; t++)
+#pragma unroll
+      for (int q = 0; q < Q; q++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[t][4 * q + i] += (acc8[t][q] >> (8 * i)) & 0xFFu;
+        acc8[t][q] = 0;
+      }
+    pending = 0;
+  };
   int64_t held_tg = -1;     // aggregate mode: tile group whose sums acc holds
   uint64_t held_keys = 0;
 
@@ -950,6 +970,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   uint32_t *rec_bits = rec_bits_all + (threadIdx.x >> 6) * T * 64;
 
   auto flush_aggregate = [&]() {
+    spill();
 #pragma unroll
     for (int t = 0; t < T; t++) {
 #pragma unroll
@@ -1005,6 +1026,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
       seg_end = a.seg_off[r + 1] < i1 ? a.seg_off[r + 1] : i1;
     }
     auto flush_read = [&]() {
+      spill();
 #pragma unroll
       for (int t = 0; t < T; t++) {
 #pragma unroll
@@ -1235,7 +1257,9 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
 #pragma unroll
             for (int t = 0; t < T; t++)
 #pragma unroll
-              for (int f = 0; f < F; f++) acc[t][f] += (m[t] >> f) & 1u;
+              for (int q = 0; q < Q; q++)
+                acc8[t][q] += (((m[t] >> (4 * q)) & 0xFu) * 0x00204081u) & 0x01010101u;
+            if (++pending == 255) spill();
           }
         }
       }
