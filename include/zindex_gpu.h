@@ -247,7 +247,8 @@ typedef struct zg_batch zg_batch;
 typedef struct zg_tuning {
   uint32_t span;           /* seed positions per work unit (multiple of 64) */
   uint32_t tiles_per_wave; /* row tiles (64 lanes x field bytes) per wave */
-  uint32_t prefetch;       /* seeds whose rows are in flight per wave: 1,2,4 */
+  uint32_t prefetch;       /* seeds whose rows are in flight per wave: 1, 2,
+                              4; also 8, 16 for fingerprints of <= 4 bits */
   uint32_t waves_per_cu;   /* persistent waves per compute unit (0 = max) */
   uint32_t nontemporal;    /* 1: row gathers use non-temporal loads */
 } zg_tuning;

@@ -256,6 +256,24 @@ def test_seed_option_validation(small_colors, cfg, msg):
         _build(small_colors[:1], **cfg)
 
 
+def test_tuning_validation(tmp_path, small_colors):
+    for bits, ok in ((4, True), (8, False)):
+        idx = _build(small_colors[:2], fingerprint_bits=bits)
+        out = tmp_path / f"idx{bits}"
+        idx.write(str(out))
+        with zindex_amd.Index.open(str(out)) as h:
+            t = {"span": 1024, "tiles_per_wave": 1, "prefetch": 16, "waves_per_cu": 0,
+                 "nontemporal": 0}
+            if ok:
+                h.tuning = t
+                assert h.tuning["prefetch"] == 16
+            else:
+                with pytest.raises(zindex_amd.ZindexError):
+                    h.tuning = t
+            with pytest.raises(zindex_amd.ZindexError):
+                h.tuning = dict(t, prefetch=3)
+
+
 def test_modimizer_sampling_zero_is_rejected(small_colors):
     with pytest.raises(Exception, match="sampling"):
         _build(small_colors[:1], index_mode=2, modimizer_sampling=0)

@@ -141,6 +141,20 @@ def test_many_tiles_and_tile_grouping():
         check_parity(idx, orc, reads, label=f"tiles{tiles} d{prefetch} nt{nt}")
 
 
+@pytest.mark.parametrize("bits,group,scale", [(1, 1, 1.0), (2, 3, 0.5), (4, 1, 0.5)])
+def test_deep_prefetch_narrow_rows(bits, group, scale):
+    colors = make_colors(75 + bits, 8, 20000)
+    reads = make_reads(76 + bits, colors, 150)
+    idx = build(colors, fingerprint_bits=bits, hashes=3, merge_group_size=group, slot_scale=scale)
+    idx.load()
+    orc = oracle_for(idx)
+    for depth in (8, 16):
+        for span in (64, 1024):
+            idx.tuning = {"span": span, "tiles_per_wave": 1, "prefetch": depth,
+                          "waves_per_cu": 0, "nontemporal": 0}
+            check_parity(idx, orc, reads, label=f"b{bits} g{group} d{depth} span{span}")
+
+
 def test_minimizer_positions_and_counts():
     colors = make_colors(80, 5, 30000)
     reads = make_reads(81, colors, 120, lo=30, hi=3000, error=0.05)

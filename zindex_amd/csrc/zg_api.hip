@@ -749,8 +749,10 @@ extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
            ZG_ERR_INVALID);
     ensure(t->tiles_per_wave == 1 || t->tiles_per_wave == 2, "tiles_per_wave must be 1 or 2",
            ZG_ERR_INVALID);
-    ensure(t->prefetch == 1 || t->prefetch == 2 || t->prefetch == 4,
-           "prefetch must be 1, 2 or 4", ZG_ERR_INVALID);
+    ensure(t->prefetch == 1 || t->prefetch == 2 || t->prefetch == 4 ||
+               ((t->prefetch == 8 || t->prefetch == 16) && idx->m.fingerprint_bits <= 4),
+           "prefetch must be 1, 2 or 4 (8 or 16 for fingerprints of at most 4 bits)",
+           ZG_ERR_INVALID);
     idx->tuning.span = t->span;
     idx->tuning.tiles_per_wave = t->tiles_per_wave;
     idx->tuning.prefetch = t->prefetch;
@@ -1384,9 +1386,11 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
     if (zg::max_tiles(idx) >= 2) tiles.push_back(2);
     int base_per_cu = 0;
     const bool wide = idx->m.fingerprint_bits >= 8;  // 16-byte lane loads
+    std::vector<uint32_t> depths = {1, 2, 4};
+    if (idx->m.fingerprint_bits <= 4) depths.insert(depths.end(), {8, 16});
     for (uint32_t span : {1024u, 4096u, 16384u})
       for (uint32_t t : tiles)
-        for (uint32_t d : {1u, 2u, 4u})
+        for (uint32_t d : depths)
           for (uint32_t frac : {1u, 2u})
             for (uint32_t nt : {0u, 1u}) {
             if (nt && !wide) continue;

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <type_traits>
+
 #include "zg_kernels.h"
 
 namespace zg {
@@ -1126,27 +1128,36 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
         }
         for (uint32_t h0 = 0; h0 < H; h0 += HB) {
           uint32_t raw[D][HB][T][NW];
+          // One copy of the loads per alignment, so the choice is made once
+          // per block of rows. The load condition is wave-uniform: lanes past
+          // the last column read padding (a bank buffer ends with 4 KiB of
+          // zeros), and their fields are masked out below.
+          auto issue = [&](auto al) {
+            constexpr bool AL = decltype(al)::value;
 #pragma unroll
-          for (int d = 0; d < D; d++) {
+            for (int d = 0; d < D; d++) {
 #pragma unroll
-            for (int hh = 0; hh < HB; hh++) {
-              uint32_t i = h0 + hh;
-              uint64_t hv = first[d] + (uint64_t)i * step[d];
-              uint64_t slot = slot_at(base[d], hv, i, bk.geo);
-              const uint8_t *row = bk.data + slot * bk.row_bytes;
+              for (int hh = 0; hh < HB; hh++) {
+                const uint32_t i = h0 + hh;
+                const uint64_t hv = first[d] + (uint64_t)i * step[d];
+                const uint64_t slot = slot_at(base[d], hv, i, bk.geo);
+                const uint8_t *row = bk.data + slot * bk.row_bytes;
 #pragma unroll
-              for (int t = 0; t < T; t++) {
-                if (jj[d] >= 0 && i < H && fmask[t]) {
-                  uint32_t tile = tg * T + t;
-                  load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, aligned,
-                               a.nontemporal != 0, raw[d][hh][t]);
-                } else {
+                for (int t = 0; t < T; t++) {
+                  const uint32_t tile = tg * T + t;
+                  if (jj[d] >= 0 && i < H && tile < bk.n_tiles) {
+                    load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, AL,
+                                 a.nontemporal != 0, raw[d][hh][t]);
+                  } else {
 #pragma unroll
-                  for (int w = 0; w < NW; w++) raw[d][hh][t][w] = 0;
+                    for (int w = 0; w < NW; w++) raw[d][hh][t][w] = 0;
+                  }
                 }
               }
             }
-          }
+          };
+          if (aligned) issue(std::true_type{});
+          else issue(std::false_type{});
 #pragma unroll
           for (int d = 0; d < D; d++)
 #pragma unroll
@@ -1279,8 +1290,14 @@ static void launch_one(const LookupArgs &a, uint32_t blocks, hipStream_t s) {
   lookup_kernel<B, T, D><<<blocks, 256, 0, s>>>(a);
 }
 
+// Deeper prefetch (8, 16 seeds in flight) only for narrow lanes (b <= 4):
+// small rows need more loads in flight per wave to cover HBM latency.
 template <int B, int T>
 static void launch_d(const LookupArgs &a, const KernelConfig &c, hipStream_t s) {
+  if constexpr (B <= 4) {
+    if (c.prefetch == 8) return launch_one<B, T, 8>(a, c.blocks, s);
+    if (c.prefetch == 16) return launch_one<B, T, 16>(a, c.blocks, s);
+  }
   switch (c.prefetch) {
     case 1: launch_one<B, T, 1>(a, c.blocks, s); break;
     case 4: launch_one<B, T, 4>(a, c.blocks, s); break;
@@ -1316,6 +1333,10 @@ static int occupancy_one() {
 
 template <int B, int T>
 static int occupancy_d(uint32_t d) {
+  if constexpr (B <= 4) {
+    if (d == 8) return occupancy_one<B, T, 8>();
+    if (d == 16) return occupancy_one<B, T, 16>();
+  }
   return d == 1 ? occupancy_one<B, T, 1>() : d == 4 ? occupancy_one<B, T, 4>()
                                                     : occupancy_one<B, T, 2>();
 }
