@@ -262,19 +262,46 @@ static void stream_frames(zg_index *idx, const BankFrames &bf, const BankMeta &m
   }
 }
 
+// Device rows start on 16-byte boundaries, the widest lane load: a bank whose
+// row length is not a multiple of 16 is stored with a padded pitch.
+static uint64_t row_pitch(const BankMeta &m) { return (m.row_bytes() + 15) / 16 * 16; }
+static uint64_t device_bytes(const BankMeta &m) { return m.slots * row_pitch(m); }
+
+// Rows of rb bytes copied to a pitch of `pitch` bytes, the padding zeroed.
+static std::vector<uint8_t> pitched_rows(const uint8_t *src, uint64_t slots, uint64_t rb,
+                                         uint64_t pitch) {
+  std::vector<uint8_t> out(slots * pitch);
+  parallel_ranges(slots, 1 << 16, [&](uint64_t a, uint64_t b) {
+    for (uint64_t r = a; r < b; r++) {
+      memcpy(out.data() + r * pitch, src + r * rb, rb);
+      memset(out.data() + r * pitch + rb, 0, pitch - rb);
+    }
+  });
+  return out;
+}
+
 // Bank bi's payload in the device layout into dst (plus a zeroed pad), from
 // the host copy when the index has one, else from the bank file.
 static void put_bank(zg_index *idx, size_t bi, uint8_t *dst, hipStream_t s) {
   const BankMeta &meta = idx->m.banks[bi];
-  const uint64_t bytes = meta.payload_bytes();
+  const uint64_t bytes = meta.payload_bytes(), pitch = row_pitch(meta);
   const bool host = bi < idx->payload.size() && idx->payload[bi].size() == bytes;
-  if (!idx->as_is[bi]) {
+  if (!idx->as_is[bi] || pitch != meta.row_bytes()) {
+    // rows rewritten on the host: legacy layout and/or padded pitch
     std::vector<uint8_t> tmp;
     if (!host) tmp = load_bank_payload(idx->dir, meta, (int)host_threads());
-    std::vector<uint8_t> rows = plane_rows(meta, host ? idx->payload[bi] : tmp, idx->perm[bi],
-                                           (uint32_t)idx->m.fingerprint_bits);
-    staged_copy(idx, dst, rows.data(), bytes, s);
-    check_hip(hipStreamSynchronize(s), "upload");  // rows end here
+    const std::vector<uint8_t> &src = host ? idx->payload[bi] : tmp;
+    std::vector<uint8_t> rows;
+    if (!idx->as_is[bi])
+      rows = plane_rows(meta, src, idx->perm[bi], (uint32_t)idx->m.fingerprint_bits);
+    const uint8_t *plane = idx->as_is[bi] ? src.data() : rows.data();
+    if (pitch != meta.row_bytes()) {
+      std::vector<uint8_t> padded = pitched_rows(plane, meta.slots, meta.row_bytes(), pitch);
+      staged_copy(idx, dst, padded.data(), padded.size(), s);
+    } else {
+      staged_copy(idx, dst, plane, bytes, s);
+    }
+    check_hip(hipStreamSynchronize(s), "upload");  // host rows end here
   } else if (host) {
     staged_copy(idx, dst, idx->payload[bi].data(), bytes, s);
   } else {
@@ -287,7 +314,7 @@ static void put_bank(zg_index *idx, size_t bi, uint8_t *dst, hipStream_t s) {
       stream_frames(idx, bf, meta, dst, s);
     }
   }
-  check_hip(hipMemsetAsync(dst + bytes, 0, kBankPad, s), "hipMemsetAsync");
+  check_hip(hipMemsetAsync(dst + device_bytes(meta), 0, kBankPad, s), "hipMemsetAsync");
 }
 
 bool any_streamed(const zg_index *idx) {
@@ -321,8 +348,8 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   // residency plan
   uint64_t payload_total = 0, largest = 0;
   for (auto &b : m.banks) {
-    payload_total += b.payload_bytes() + kBankPad;
-    largest = std::max(largest, b.payload_bytes() + kBankPad);
+    payload_total += device_bytes(b) + kBankPad;
+    largest = std::max(largest, device_bytes(b) + kBankPad);
   }
   uint64_t rec_bytes = 0;
   if (idx->recover && idx->have_rec) rec_bytes = idx->rec.size() * 28 + idx->rec.origins.size() * 8;
@@ -337,7 +364,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
                               std::to_string(budget));
     uint64_t left = budget - rec_bytes - largest;  // beside one streaming buffer
     for (size_t bi = 0; bi < nb; bi++) {
-      const uint64_t b = m.banks[bi].payload_bytes() + kBankPad;
+      const uint64_t b = device_bytes(m.banks[bi]) + kBankPad;
       if (b <= left) left -= b;
       else idx->streamed[bi] = true;
     }
@@ -396,7 +423,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
     if (idx->streamed[bi]) {
       home = idx->stream_data.as<uint8_t>();
     } else {
-      db.data.alloc(meta.payload_bytes() + kBankPad);
+      db.data.alloc(device_bytes(meta) + kBankPad);
       home = db.data.as<uint8_t>();
       put_bank(idx, bi, home, s);
     }
@@ -425,7 +452,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
     upload(db.color_bit, color_bit, s);
     BankDev &v = db.view;
     v.data = home;
-    v.row_bytes = meta.row_bytes();
+    v.row_bytes = row_pitch(meta);
     v.plane_bytes = ((uint64_t)G * fb + 7) / 8;
     v.groups = G;
     v.first_merged = first_merged;
@@ -1442,7 +1469,7 @@ extern "C" int zg_gather_probe(zg_index *idx, uint32_t bank, uint32_t row_bytes,
     ensure(bank < idx->dbanks.size(), "bank out of range", ZG_ERR_INVALID);
     make_resident(idx, bank, idx->stream);
     const DeviceBank &db = idx->dbanks[bank];
-    const uint64_t bytes = idx->m.banks[bank].payload_bytes();
+    const uint64_t bytes = device_bytes(idx->m.banks[bank]);
     ensure(n_rows > 0 && bytes >= row_bytes, "empty probe", ZG_ERR_INVALID);
     hipStream_t s = idx->stream;
     DevBuf sink;

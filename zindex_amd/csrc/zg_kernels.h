@@ -14,7 +14,7 @@ constexpr uint32_t kNone = 0xFFFFFFFFu;
 // One bank resident in HBM, in the plane layout (fingerprint_first).
 struct BankDev {
   const uint8_t *data;        // slots * row_bytes (+ padding)
-  uint64_t row_bytes;
+  uint64_t row_bytes;         // device row pitch: the row length rounded up to 16 bytes
   uint64_t plane_bytes;       // ceil(G * b / 8)
   uint32_t groups;            // G
   uint32_t first_merged;      // columns >= this are multi-member groups

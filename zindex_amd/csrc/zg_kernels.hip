@@ -809,49 +809,42 @@ template <int B>
 struct FieldCfg {
   static constexpr int VB = B == 1 ? 2 : B == 2 ? 4 : B == 4 ? 8 : 16;  // bytes per lane
   static constexpr int F = VB * 8 / B;                                   // fields per lane
-  static constexpr int NW = VB >= 4 ? VB / 4 : 1;                        // dwords per lane
+  static constexpr int NW = VB >= 4 ? VB / 4 : 1;                        // words per lane
+  // 16-bit words for 2-byte lanes: the loaded value needs no widening, so no
+  // wait is placed right after a load
+  using Word = std::conditional_t<VB == 2, uint16_t, uint32_t>;
 };
 
-// VB bytes at p (any alignment when !aligned; over-reads stay in padding).
+// VB bytes at p, which is VB-aligned: device rows start on 16-byte
+// boundaries (the host pads the row pitch), lanes read VB-byte slices of them.
 template <int B>
-__device__ __forceinline__ void load_lane(const uint8_t *p, bool aligned, bool nontemporal,
-                                          uint32_t (&x)[FieldCfg<B>::NW]) {
+__device__ __forceinline__ void load_lane(const uint8_t *p, bool nontemporal,
+                                          typename FieldCfg<B>::Word (&x)[FieldCfg<B>::NW]) {
   constexpr int VB = FieldCfg<B>::VB;
-  constexpr int NW = FieldCfg<B>::NW;
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-  if (aligned) {
-    if constexpr (VB == 16) {
-      if (nontemporal) {
-        // rows are read once per seed: stream them past the caches
-        u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-      } else {
-        uint4 v = *(const uint4 *)p;
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-      }
-    } else if constexpr (VB == 8) {
-      uint2 v = *(const uint2 *)p;
-      x[0] = v.x; x[1] = v.y;
-    } else if constexpr (VB == 4) {
-      x[0] = *(const uint32_t *)p;
+  if constexpr (VB == 16) {
+    if (nontemporal) {
+      // rows are read once per seed: stream them past the caches
+      u32x4 v = __builtin_nontemporal_load((const u32x4 *)p);
+      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
     } else {
-      x[0] = *(const uint16_t *)p;
+      uint4 v = *(const uint4 *)p;
+      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
     }
-    return;
+  } else if constexpr (VB == 8) {
+    uint2 v = *(const uint2 *)p;
+    x[0] = v.x; x[1] = v.y;
+  } else if constexpr (VB == 4) {
+    x[0] = *(const uint32_t *)p;
+  } else {
+    (void)nontemporal;
+    x[0] = *(const uint16_t *)p;
   }
-  const uint32_t *a = (const uint32_t *)((uintptr_t)p & ~(uintptr_t)3);
-  uint32_t sh = (uint32_t)((uintptr_t)p & 3);
-  uint32_t d[NW + 1];
-#pragma unroll
-  for (int i = 0; i <= NW; i++) d[i] = a[i];
-#pragma unroll
-  for (int i = 0; i < NW; i++) x[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
-  if constexpr (VB == 2) x[0] &= 0xFFFFu;
 }
 
 // Bit f of the result: field f of the lane's bytes equals fp.
 template <int B>
-__device__ __forceinline__ uint32_t field_hits(const uint32_t (&x)[FieldCfg<B>::NW],
+__device__ __forceinline__ uint32_t field_hits(const typename FieldCfg<B>::Word (&x)[FieldCfg<B>::NW],
                                                uint32_t fp) {
   uint32_t m = 0;
   if constexpr (B == 8) {
@@ -923,6 +916,7 @@ __device__ uint64_t xor_rows_bits(const BankDev &bk, uint64_t key, uint64_t seed
 template <int B, int T, int D>
 __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   using C = FieldCfg<B>;
+  using LaneWord = typename C::Word;
   constexpr int F = C::F;
   constexpr int NW = C::NW;
   constexpr int VB = C::VB;
@@ -930,12 +924,13 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave_slot = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const BankDev &bk = a.bank;
-  const bool aligned = (bk.row_bytes % VB) == 0;
   const uint32_t tile_fields = 64u * F;
   const uint32_t H = bk.hashes;
   const uint64_t kmask = a.k >= 32 ? ~0ULL : ((1ULL << (2 * a.k)) - 1);
   (void)kmask;
   const uint64_t win_mask = a.k >= 64 ? ~0ULL : ((1ULL << a.k) - 1);
+  const uint64_t len_mask = bk.geo.length - 1;
+  const uint64_t seg_stride = bk.geo.length * bk.row_bytes;
 
   // Field counters: per key, the F hit bits of a lane are added as byte
   // counters, four fields per word (bit i of a nibble spread to byte i by one
@@ -1069,7 +1064,8 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
       // PositionHash::new for this lane's key, broadcast per key below
       const uint64_t lane_first = mixsplit(key, a.seed);
       const uint64_t lane_step = splitmix64(lane_first);
-      const uint64_t lane_base = slot_base(lane_first, bk.geo);
+      // byte offset of the key's first segment; rows i follow at i segments
+      const uint64_t lane_rowb = slot_base(lane_first, bk.geo) * bk.row_bytes;
       if (a.sampling && lane_first % a.sampling) v = false;  // not a modimizer
       uint32_t rec_start = 0, rec_len = 0;
       if (a.rec.enabled && v) {
@@ -1106,57 +1102,59 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
           }
         }
         // Issue the row loads of all D keys before consuming any.
-        uint32_t x[D][T][NW];
+        LaneWord x[D][T][NW];
 #pragma unroll
         for (int d = 0; d < D; d++)
 #pragma unroll
           for (int t = 0; t < T; t++)
 #pragma unroll
             for (int w = 0; w < NW; w++) x[d][t][w] = 0;
-        uint64_t first[D], step[D], base[D];
+        // PositionHash per key, advanced one row at a time: h_i = first + i*step,
+        // row i at (base + i*len + ((h_i ^ h_i >> 33) & (len - 1))) * row_bytes
+        uint64_t hv[D], step[D], rowb[D];
         uint32_t rl[D], rs[D], pre[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
           int l = jj[d] < 0 ? 0 : jj[d];
-          first[d] = readlane64(lane_first, l);
+          hv[d] = readlane64(lane_first, l);
           step[d] = readlane64(lane_step, l);
-          base[d] = readlane64(lane_base, l);
+          rowb[d] = readlane64(lane_rowb, l);
           // the first 64 recovery postings of the key travel with its rows
           rl[d] = jj[d] < 0 ? 0 : readlane32(rec_len, l);
           rs[d] = readlane32(rec_start, l);
           pre[d] = lane < rl[d] ? a.rec.post_col[rs[d] + lane] : kNone;
         }
         for (uint32_t h0 = 0; h0 < H; h0 += HB) {
-          uint32_t raw[D][HB][T][NW];
-          // One copy of the loads per alignment, so the choice is made once
-          // per block of rows. The load condition is wave-uniform: lanes past
-          // the last column read padding (a bank buffer ends with 4 KiB of
-          // zeros), and their fields are masked out below.
-          auto issue = [&](auto al) {
-            constexpr bool AL = decltype(al)::value;
+          LaneWord raw[D][HB][T][NW];
+          // The load condition is wave-uniform: lanes past the last column
+          // read row padding (a bank buffer ends with 4 KiB of zeros), and
+          // their fields are masked out below. Non-temporal or plain loads are
+          // chosen once per block of rows.
+          auto issue = [&](auto nt_tag) {
+            constexpr bool NT = decltype(nt_tag)::value;
 #pragma unroll
             for (int d = 0; d < D; d++) {
 #pragma unroll
               for (int hh = 0; hh < HB; hh++) {
-                const uint32_t i = h0 + hh;
-                const uint64_t hv = first[d] + (uint64_t)i * step[d];
-                const uint64_t slot = slot_at(base[d], hv, i, bk.geo);
-                const uint8_t *row = bk.data + slot * bk.row_bytes;
+                const bool live = jj[d] >= 0 && h0 + hh < H;
+                const uint8_t *row =
+                    bk.data + rowb[d] + ((hv[d] ^ (hv[d] >> 33)) & len_mask) * bk.row_bytes;
 #pragma unroll
                 for (int t = 0; t < T; t++) {
                   const uint32_t tile = tg * T + t;
-                  if (jj[d] >= 0 && i < H && tile < bk.n_tiles) {
-                    load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, AL,
-                                 a.nontemporal != 0, raw[d][hh][t]);
+                  if (live && tile < bk.n_tiles) {
+                    load_lane<B>(row + (uint64_t)tile * 64 * VB + lane * VB, NT, raw[d][hh][t]);
                   } else {
 #pragma unroll
                     for (int w = 0; w < NW; w++) raw[d][hh][t][w] = 0;
                   }
                 }
+                hv[d] += step[d];
+                rowb[d] += seg_stride;
               }
             }
           };
-          if (aligned) issue(std::true_type{});
+          if (VB == 16 && a.nontemporal) issue(std::true_type{});
           else issue(std::false_type{});
 #pragma unroll
           for (int d = 0; d < D; d++)
