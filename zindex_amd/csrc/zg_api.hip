@@ -828,6 +828,16 @@ void configure(zg_index *idx, zg_batch *b) {
   b->kcfg = c;
   b->span = t.span;
   b->nontemporal = t.nontemporal;
+  // A small batch halves the span until there are at least two work units
+  // per wave on the bank with the fewest tile groups, so a short request
+  // still spreads over the whole chip.
+  uint64_t min_tg = ~0ULL;
+  for (const DeviceBank &db : idx->dbanks)
+    min_tg = std::min<uint64_t>(min_tg, (db.view.n_tiles + c.tiles_per_wave - 1) / c.tiles_per_wave);
+  if (min_tg == ~0ULL) min_tg = 1;
+  const uint64_t seeds = b->explicit_keys ? b->n_keys : b->positions;
+  const uint64_t waves = (uint64_t)c.blocks * 4;
+  while (b->span > 256 && (seeds + b->span - 1) / b->span * min_tg < 2 * waves) b->span /= 2;
 }
 
 static void ensure_partials(zg_batch *b) {
