@@ -58,7 +58,7 @@ def parse_args():
     p.add_argument("--tuning", default="",
                    help="span,tiles_per_wave,prefetch,waves_per_cu: fixed launch shape "
                         "(skips the autotuner)")
-    p.add_argument("--tune-reads", type=int, default=20000)
+    p.add_argument("--tune-reads", type=int, default=100000)
     p.add_argument("--plain-reads", type=int, default=2000)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--cpu-threads", type=int, default=0)

@@ -155,6 +155,21 @@ def test_deep_prefetch_narrow_rows(bits, group, scale):
             check_parity(idx, orc, reads, label=f"b{bits} g{group} d{depth} span{span}")
 
 
+@pytest.mark.parametrize("n_colors,bits,group,mode", [(37, 4, 3, 0), (13, 8, 1, 1), (70, 16, 2, 3),
+                                                     (5, 32, 1, 0), (99, 2, 4, 2)])
+def test_padded_row_pitch(n_colors, bits, group, mode):
+    # row lengths that are not multiples of 16 bytes are padded in HBM
+    colors = make_colors(190 + n_colors, n_colors, 6000)
+    reads = _with_runs(make_reads(191 + n_colors, colors, 120, lo=20, hi=1500))
+    cfg = dict(fingerprint_bits=bits, hashes=3, merge_group_size=group, slot_scale=0.7,
+               index_mode=mode, k=31, minimizer_size=19, modimizer_sampling=3, findere_z=6)
+    idx = build(colors, **cfg)
+    rb = [(b["row_bits"] + 7) // 8 for b in idx.manifest()["banks"]]
+    assert any(r % 16 for r in rb), rb
+    idx.load()
+    check_parity(idx, oracle_for(idx), reads, label=f"pitch c{n_colors} b{bits} g{group} m{mode}")
+
+
 def test_minimizer_positions_and_counts():
     colors = make_colors(80, 5, 30000)
     reads = make_reads(81, colors, 120, lo=30, hi=3000, error=0.05)
