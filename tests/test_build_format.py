@@ -256,6 +256,43 @@ def test_seed_option_validation(small_colors, cfg, msg):
         _build(small_colors[:1], **cfg)
 
 
+def test_compressed_inputs_build_the_same_index(tmp_path, small_colors):
+    """src/lib.rs:560-571: inputs may be gzip, zstd, xz or bzip2 (here also as
+    concatenated members); they index exactly as the plain files do."""
+    import bz2
+    import gzip
+    import lzma
+    cli = ROOT / "zindex_amd" / "zorindex"
+    fa = [b">r%d\n%s\n" % (i, s[:20000]) for i, s in enumerate(small_colors[:4])]
+    enc = [lambda b: gzip.compress(b[:500]) + gzip.compress(b[500:]),
+           lambda b: zr.zstd_compress(b),
+           lambda b: lzma.compress(b[:700]) + lzma.compress(b[700:]),
+           lambda b: bz2.compress(b[:900]) + bz2.compress(b[900:])]
+    plain, packed = [], []
+    for i, (data, f) in enumerate(zip(fa, enc)):
+        (tmp_path / f"p{i}.fa").write_bytes(data)
+        (tmp_path / f"z{i}.fa.x").write_bytes(f(data))
+        plain.append(f"c{i}\t{tmp_path / f'p{i}.fa'}\n")
+        packed.append(f"c{i}\t{tmp_path / f'z{i}.fa.x'}\n")
+    (tmp_path / "plain.fof").write_text("".join(plain))
+    (tmp_path / "packed.fof").write_text("".join(packed))
+    for fof, out in (("plain.fof", "ip"), ("packed.fof", "iz")):
+        r = subprocess.run([str(cli), "build", "-f", str(tmp_path / fof), "-o", str(tmp_path / out)],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    a, b = zr.read_index_dir(tmp_path / "ip"), zr.read_index_dir(tmp_path / "iz")
+    assert [c["unique_kmers"] for c in a["manifest"]["colors"]] == \
+        [c["unique_kmers"] for c in b["manifest"]["colors"]]
+    for x, y in zip(a["banks"], b["banks"]):
+        assert np.array_equal(x, y)
+    bad = tmp_path / "bad.fa.xz"
+    bad.write_bytes(lzma.compress(fa[0])[:-9])
+    (tmp_path / "bad.fof").write_text(f"c\t{bad}\n")
+    r = subprocess.run([str(cli), "build", "-f", str(tmp_path / "bad.fof"), "-o",
+                        str(tmp_path / "ib")], capture_output=True, text=True)
+    assert r.returncode != 0 and "xz" in r.stderr
+
+
 def test_tuning_validation(tmp_path, small_colors):
     for bits, ok in ((4, True), (8, False)):
         idx = _build(small_colors[:2], fingerprint_bits=bits)

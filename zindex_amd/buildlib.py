@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 CPP_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
-LINK_LIBS = ["-lz", "-l:libzstd.so.1", "-lpthread"]
+LINK_LIBS = ["-lz", "-l:libzstd.so.1", "-l:liblzma.so.5", "-l:libbz2.so.1", "-lpthread"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -3,9 +3,12 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "zg_format.h"
+#include "zg_xzbz2.h"
 
 namespace zg {
 
@@ -47,10 +50,56 @@ std::string read_maybe_compressed(const std::string &path) {
   }
   if (raw.size() >= 4 && p[0] == 0x28 && p[1] == 0xb5 && p[2] == 0x2f && p[3] == 0xfd)
     return zstd_decompress_all(raw);
-  if (raw.size() >= 6 && memcmp(p, "\xfd" "7zXZ", 5) == 0)
-    fail(-6, "xz input is not supported by this build: " + path);
-  if (raw.size() >= 3 && memcmp(p, "BZh", 3) == 0)
-    fail(-6, "bzip2 input is not supported by this build: " + path);
+  if (raw.size() >= 6 && memcmp(p, "\xfd" "7zXZ", 5) == 0) {
+    // xz, concatenated streams allowed
+    zg_lzma_stream z;
+    memset(&z, 0, sizeof z);
+    if (lzma_stream_decoder(&z, UINT64_MAX, kLzmaConcatenated) != kLzmaOk)
+      fail(-2, "xz: init failed");
+    z.next_in = p;
+    z.avail_in = raw.size();
+    std::string out;
+    std::vector<uint8_t> buf(1 << 20);
+    while (true) {
+      z.next_out = buf.data();
+      z.avail_out = buf.size();
+      const int rc = lzma_code(&z, kLzmaFinish);
+      out.append((const char *)buf.data(), buf.size() - z.avail_out);
+      if (rc == kLzmaStreamEnd) break;
+      if (rc != kLzmaOk) {
+        lzma_end(&z);
+        fail(-2, "xz: corrupt or truncated stream in " + path);
+      }
+    }
+    lzma_end(&z);
+    return out;
+  }
+  if (raw.size() >= 4 && memcmp(p, "BZh", 3) == 0 && p[3] >= '1' && p[3] <= '9') {
+    // bzip2, possibly several concatenated streams
+    std::string out;
+    std::vector<char> buf(1 << 20);
+    size_t at = 0;
+    while (at < raw.size()) {
+      zg_bz_stream z;
+      memset(&z, 0, sizeof z);
+      if (BZ2_bzDecompressInit(&z, 0, 0) != kBzOk) fail(-2, "bzip2: init failed");
+      z.next_in = (char *)raw.data() + at;
+      z.avail_in = (unsigned)std::min<size_t>(raw.size() - at, 1u << 30);
+      int rc = kBzOk;
+      while (rc == kBzOk) {
+        z.next_out = buf.data();
+        z.avail_out = (unsigned)buf.size();
+        rc = BZ2_bzDecompress(&z);
+        out.append(buf.data(), buf.size() - z.avail_out);
+        if (rc == kBzOk && z.avail_in == 0 && z.avail_out != 0) rc = -100;  // truncated
+      }
+      const size_t used = (size_t)(z.next_in - (raw.data() + at));
+      BZ2_bzDecompressEnd(&z);
+      if (rc != kBzStreamEnd) fail(-2, "bzip2: corrupt or truncated stream in " + path);
+      at += used;
+    }
+    return out;
+  }
   return raw;
 }
 

@@ -15,7 +15,8 @@ struct Records {
   size_t size() const { return off.size() - 1; }
 };
 
-// Decompresses a file by its magic bytes (gzip, zstd) or returns it as is.
+// Decompresses a file by its magic bytes (gzip, zstd, xz, bzip2; concatenated
+// members or streams allowed) or returns it as is.
 std::string read_maybe_compressed(const std::string &path);
 
 // Record reader of records.rs:150-275 (strict: ids, FASTQ quality checks,
