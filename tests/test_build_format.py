@@ -627,6 +627,47 @@ def test_info_command_runs_on_cpu(readme_index, tmp_path):
     assert text.count("bank-") >= 1
 
 
+# the key lines of info_command in order (storage.rs:3914-4060), for a
+# non-fused, non-findere index without anchors, segment length or limits
+INFO_KEYS = [
+    "collection_status", "indexed_documents", "index", "k", "hashes", "seed", "fingerprint_bits",
+    "index_mode", "technical_layout", "colors", "groups", "banks", "size_classes",
+    "max_subindex_size", "largest_subindex_bytes", "hash_partitions", "input_documents",
+    "skipped_documents", "memory_guard_split_groups", "minimizer_size", "modimizer_sampling",
+    "minimizer_scheme", "homopolymer_compression", "seed_pattern", "randstrobemers",
+    "merge_planner", "planning_pair_evaluations", "planning_possible_pair_evaluations",
+    "merge_reverts_enabled", "construction_cell_storage", "construction_abandoned_storage",
+    "segment_length", "overflow_groups", "merge_column_order", "column_order_method",
+    "abandoned_recovery",
+    "abandoned_hash_bits", "abandoned_keys", "abandoned_unique_hashes",
+    "abandoned_dataset_associations", "abandoned_map_ram_bytes", "logical_bits", "padding_bits",
+    "stack_bytes", "stack_stored_bytes", "fallback_bytes", "structure_bytes",
+    "singleton_logical_bits", "predicted_logical_bits", "planned_groups", "rejected_groups",
+    "exact_group_verifications", "sketch_seconds", "plan_seconds", "build_seconds",
+    "metadata_bytes", "uncompressed_index_bytes", "disk_bytes", "min_subindex_documents",
+    "promoted_groups", "max_subindexes"]
+
+
+def test_info_lines_follow_the_reference_order(readme_index, tmp_path):
+    out = tmp_path / "info.txt"
+    fd = os.open(out, os.O_WRONLY | os.O_CREAT | os.O_TRUNC)
+    try:
+        zindex_amd.info_command(str(readme_index), out_fd=fd)
+    finally:
+        os.close(fd)
+    lines = out.read_text().splitlines()
+    head = [l.split("\t")[0] for l in lines[:lines.index(
+        "bank\tslots\tgroups\trow_bits\tpadding_bits_per_row\tfallback_keys\tdocuments")]]
+    expected = [k for k in INFO_KEYS if k in head]
+    assert head == expected
+    man = json.loads((readme_index / "manifest.json").read_text())
+    optional = {"segment_length", "max_subindexes", "column_order_method"}
+    assert set(INFO_KEYS) - set(head) <= optional
+    disk = sum(p.stat().st_size for p in readme_index.iterdir())
+    assert f"disk_bytes\t{disk}" in lines
+    assert f"input_documents\t{len(man['colors'])}" in lines
+
+
 def _tamper(src: Path, dst: Path, edit):
     shutil.copytree(src, dst)
     man = json.loads((dst / "manifest.json").read_text())

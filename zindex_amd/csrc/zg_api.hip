@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <dirent.h>
 #include <mutex>
 #include <set>
 #include <thread>
@@ -1919,10 +1920,20 @@ extern "C" int zg_info_command(const char *dir, int out_fd) {
   try {
     ensure(dir, "null argument", ZG_ERR_INVALID);
     Manifest m = load_manifest(dir);
-    // validate payloads one at a time (storage.rs:3905-3913)
-    for (auto &b : m.banks) (void)load_bank_payload(dir, b, (int)host_threads());
     RecoveryMap rec;
-    if (m.abandoned_recovery) rec = read_recovery_map(dir, m);
+    if (m.version == 1) {
+      // version 1 loads whole, its fallback keys becoming the recovery map
+      // (storage.rs:3903-3904)
+      zg_index v1;
+      v1.m = m;
+      v1.dir = dir;
+      load_v1(&v1);
+      rec = std::move(v1.rec);
+    } else {
+      // validate payloads one at a time (storage.rs:3905-3913)
+      for (auto &b : m.banks) (void)load_bank_payload(dir, b, (int)host_threads());
+      if (m.abandoned_recovery) rec = read_recovery_map(dir, m);
+    }
     uint64_t groups = 0, logical = 0, stack = 0, abandoned_keys = 0;
     for (auto &b : m.banks) {
       groups += b.groups.size();
@@ -1935,14 +1946,63 @@ extern "C" int zg_info_command(const char *dir, int out_fd) {
       largest = std::max(largest, b.payload_bytes());
       partitions += b.routed;
     }
+    // fallback and metadata bytes (storage.rs:3939-3955)
     uint64_t fallback = m.abandoned_hashes * 8 + m.abandoned_origins * 4;
+    if (m.version == 1) {
+      fallback = 0;
+      for (auto &b : m.banks)
+        for (auto &g : b.groups) fallback += g.fallback_keys * 16;
+    }
+    const std::string manifest_path = path_join(dir, "manifest.json");
+    struct stat st {};
+    const uint64_t manifest_bytes = stat(manifest_path.c_str(), &st) == 0 ? (uint64_t)st.st_size : 0;
+    const uint64_t metadata_bytes = manifest_bytes + m.banks.size() * 40 +
+                                    (m.version == 1 ? groups * 8 : m.has_abandoned_file ? 8 : 0);
+    uint64_t disk_bytes = 0;
+    if (DIR *d = opendir(dir)) {
+      while (struct dirent *e = readdir(d)) {
+        const std::string n = e->d_name;
+        if (n == "." || n == "..") continue;
+        if (stat(path_join(dir, n).c_str(), &st) == 0) disk_bytes += (uint64_t)st.st_size;
+      }
+      closedir(d);
+    }
+    // optional manifest fields with the reference's serde defaults
+    auto num = [&](const char *k, const char *dflt) -> std::string {
+      const Json *v = m.json.get(k);
+      return v && v->kind == Json::Number ? v->text : dflt;
+    };
+    auto secs = [&](const char *k) {
+      const Json *v = m.json.get(k);
+      char buf[64];
+      snprintf(buf, sizeof buf, "%.6f", v && v->kind == Json::Number ? strtod(v->text.c_str(), nullptr) : 0.0);
+      return std::string(buf);
+    };
+    auto str = [&](const char *k, const char *dflt) -> std::string {
+      const Json *v = m.json.get(k);
+      return v && v->kind == Json::String ? v->text : dflt;
+    };
+    auto flag = [&](const char *k, bool dflt) {
+      const Json *v = m.json.get(k);
+      return (v && v->kind == Json::Bool ? v->b : dflt) ? "true" : "false";
+    };
+    const std::string cells = str("construction_cell_storage", "in-memory-u64");
+    const bool copied = cells == "copied-compressed-banks";
     std::string s;
     auto line = [&](const std::string &k, const std::string &v) { s += k + "\t" + v + "\n"; };
     size_t docs = std::max<size_t>(m.colors.size(), m.json.get("input_document_count")
                                                         ? strtoull(m.json.get("input_document_count")->text.c_str(), nullptr, 10)
                                                         : 0);
-    line("collection_status", docs == m.colors.size() ? "complete" : "incomplete");
+    const Json *skipped = m.json.get("skipped_documents");
+    const size_t n_skipped = skipped && skipped->kind == Json::Array ? skipped->items.size() : 0;
+    // info_command (storage.rs:3901-4090), line for line
+    line("collection_status", n_skipped == 0 && docs == m.colors.size() ? "complete" : "incomplete");
     line("indexed_documents", std::to_string(m.colors.size()));
+    if (copied) {
+      std::set<uint64_t> comps;
+      for (auto &b : m.banks) comps.insert(b.component);
+      line("components", std::to_string(comps.size()));
+    }
     line("index", dir);
     line("k", std::to_string(m.k));
     line("hashes", std::to_string(m.hashes));
@@ -1958,6 +2018,8 @@ extern "C" int zg_info_command(const char *dir, int out_fd) {
     line("largest_subindex_bytes", std::to_string(largest));
     line("hash_partitions", std::to_string(partitions));
     line("input_documents", std::to_string(docs));
+    line("skipped_documents", std::to_string(n_skipped));
+    line("memory_guard_split_groups", num("memory_guard_split_groups", "0"));
     if (m.index_mode == 3) {
       line("findere_z", std::to_string(m.findere_z));
       line("indexed_k", std::to_string(m.k - m.findere_z));
@@ -1973,18 +2035,50 @@ extern "C" int zg_info_command(const char *dir, int out_fd) {
       line("strobe_min", std::to_string(m.strobe_min));
       line("strobe_max", std::to_string(m.strobe_max));
     }
-    line("merge_planner", m.merge_planner_label());
+    line("merge_planner", copied ? "preserved" : m.merge_planner_label());
+    line("planning_pair_evaluations", num("planning_pair_evaluations", "0"));
+    line("planning_possible_pair_evaluations", num("planning_possible_pair_evaluations", "0"));
+    line("merge_reverts_enabled", flag("merge_reverts_enabled", true));
+    line("construction_cell_storage", cells);
+    if (const Json *v = m.json.get("merge_memory_budget_bytes"); v && v->kind == Json::Number)
+      line("merge_memory_budget_bytes", v->text);
+    line("construction_abandoned_storage", str("construction_abandoned_storage", "legacy-memory"));
     if (m.has_segment_length) line("segment_length", std::to_string(m.segment_length));
+    line("overflow_groups", num("overflow_groups", "0"));
+    line("merge_column_order", copied ? "preserved" : str("column_order", "width"));
+    if (const Json *v = m.json.get("column_order_method"); v && v->kind == Json::String)
+      line("column_order_method", v->text);
     line("abandoned_recovery", m.abandoned_recovery ? "true" : "false");
     line("abandoned_hash_bits", "32");
     line("abandoned_keys", std::to_string(abandoned_keys));
     line("abandoned_unique_hashes", std::to_string(rec.size()));
     line("abandoned_dataset_associations", std::to_string(rec.origins.size()));
+    // this library's recovery directory (open addressing, 8 B entries) and
+    // postings; the reference prints its own OriginMap's heap bytes here
+    uint64_t cap = 4;
+    while (cap < (4 * rec.size() + 2) / 3) cap <<= 1;
+    line("abandoned_map_ram_bytes",
+         std::to_string(rec.size() ? cap * 8 + (rec.size() + rec.origins.size()) * 8 : 0));
     line("logical_bits", std::to_string(logical));
     line("padding_bits", std::to_string(stack * 8 - logical));
     line("stack_bytes", std::to_string(stack));
+    line("stack_stored_bytes", std::to_string(stack));
     line("fallback_bytes", std::to_string(fallback));
     line("structure_bytes", std::to_string(stack + fallback));
+    line("singleton_logical_bits", num("singleton_logical_bits", "0"));
+    line("predicted_logical_bits", num("predicted_logical_bits", "0"));
+    line("planned_groups", num("planned_groups", "0"));
+    line("rejected_groups", num("rejected_groups", "0"));
+    line("exact_group_verifications", num("exact_group_verifications", "0"));
+    line("sketch_seconds", secs("sketch_seconds"));
+    line("plan_seconds", secs("plan_seconds"));
+    line("build_seconds", secs("build_seconds"));
+    line("metadata_bytes", std::to_string(metadata_bytes));
+    line("uncompressed_index_bytes", std::to_string(stack + fallback + metadata_bytes));
+    line("disk_bytes", std::to_string(disk_bytes));
+    line("min_subindex_documents", num("min_subindex_documents", "1"));
+    line("promoted_groups", num("promoted_groups", "0"));
+    if (m.has_max_subindexes) line("max_subindexes", std::to_string(m.max_subindexes));
     s += "bank\tslots\tgroups\trow_bits\tpadding_bits_per_row\tfallback_keys\tdocuments\n";
     for (auto &b : m.banks) {
       uint64_t fk = 0, docs_b = 0;
