@@ -103,14 +103,18 @@ def test_server_concurrent_submitters_and_poll(serve_case):
         assert np.array_equal(got[2], exp[2]), i
 
 
-@pytest.mark.parametrize("mode", ["minimizers", "findere", "randstrobes"])
-def test_server_minimizers_mixed_equals_alone(mode):
+@pytest.mark.parametrize("mode", ["minimizers", "findere", "randstrobes", "long_kmers",
+                                  "spaced_hpc"])
+def test_server_seed_modes_mixed_equals_alone(mode):
     colors = make_colors(75, 6, 40000)
     reads = make_reads(76, colors, 240, lo=30, hi=3000)
     cfg = {"minimizers": dict(index_mode=mode, k=49, minimizer_size=31),
            "findere": dict(index_mode=mode, k=31, findere_z=8, homopolymer_compression=1),
            "randstrobes": dict(index_mode="minimizers", k=31, minimizer_size=19,
-                               randstrobemers=1)}[mode]
+                               randstrobemers=1),
+           "long_kmers": dict(index_mode="kmers", k=45),
+           "spaced_hpc": dict(index_mode="kmers", k=31, homopolymer_compression=1,
+                              seed_pattern="###_##_#_##_#_###_#_##_#_##_###")}[mode]
     idx = build(colors, fingerprint_bits=16, hashes=3, **cfg)
     idx.load()
     requests = split_requests(77, reads, 20)
