@@ -56,8 +56,8 @@ def parse_args():
     p.add_argument("--aggregate", action="store_true", help="aggregate counts only")
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--tuning", default="",
-                   help="span,tiles_per_wave,prefetch,waves_per_cu: fixed launch shape "
-                        "(skips the autotuner)")
+                   help="span,tiles_per_wave,prefetch,waves_per_cu[,nontemporal[,sweep]]: "
+                        "fixed launch shape (skips the autotuner)")
     p.add_argument("--tune-reads", type=int, default=100000)
     p.add_argument("--plain-reads", type=int, default=2000)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -546,7 +546,8 @@ def main() -> int:
         span, tiles, prefetch, waves = vals[:4]
         nontemporal = vals[4] if len(vals) > 4 else 0
         idx.tuning = {"span": span, "tiles_per_wave": tiles, "prefetch": prefetch,
-                      "waves_per_cu": waves, "nontemporal": nontemporal}
+                      "waves_per_cu": waves, "nontemporal": nontemporal,
+                      "sweep": vals[5] if len(vals) > 5 else 0}
         tuning = idx.tuning
     elif not args.no_autotune:
         n_t = min(args.tune_reads, args.reads)
@@ -576,16 +577,30 @@ def main() -> int:
 
     # roofline of the lookup kernel: algorithmic bytes per launch / launch time
     positions = int(off[-1]) + args.reads
-    if args.mode == "kmers":  # packed bases and invalid bits read by the lookup
+    lk_ms = float(np.mean(lookup_ms))
+    sweep = bool(st["tuning"]["sweep"])
+    if sweep:
+        # sorted sweep: one lookup launch per chunk of reads over the chunk's
+        # distinct seeds (filter rows + the 8-byte first hash of each seed)
+        launches = max(1, st["sweep_launches"])
+        per_lookup = st["algorithmic_bytes"] // max(1, seeds)
+        algo = st["distinct_seeds"] * (per_lookup + 8) // launches
+        kernel_ms = st["sweep_lookup_ms"] / launches
+    elif args.mode == "kmers":  # packed bases and invalid bits read by the lookup
         algo = st["algorithmic_bytes"] + positions * 3 // 8
+        kernel_ms = lk_ms
     else:  # explicit minimizer keys
         algo = st["algorithmic_bytes"] + seeds * 8
-    lk_ms = float(np.mean(lookup_ms))
-    achieved = algo / (lk_ms / 1000) / 1e9
+        kernel_ms = lk_ms
+    achieved = algo / (kernel_ms / 1000) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "lookup_kernel", "launch_ms": round(lk_ms, 3),
+                "kernel": "lookup_kernel", "launch_ms": round(kernel_ms, 3),
                 "algorithmic_bytes_per_launch": algo}
+    if sweep:
+        roofline.update({"sorted_sweep": True, "launches_per_step": launches,
+                         "seeds_per_step": seeds, "distinct_seeds_per_step": st["distinct_seeds"],
+                         "sweep_stage_ms": round(lk_ms, 3)})
     # the ceiling of the row gathers alone: random whole rows of the bank's row
     # size, the same bytes per launch, no other work
     ceiling = gather_ceiling(idx, algo)

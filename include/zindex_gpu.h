@@ -251,6 +251,10 @@ typedef struct zg_tuning {
                               4; also 8, 16 for fingerprints of <= 4 bits */
   uint32_t waves_per_cu;   /* persistent waves per compute unit (0 = max) */
   uint32_t nontemporal;    /* 1: row gathers use non-temporal loads */
+  uint32_t sweep;          /* 1: seeds of a single-bank k-mer index are
+                              looked up in PositionHash segment order (a
+                              radix sort per chunk of reads), each seed's hits
+                              into a hit row, the rows summed per read */
 } zg_tuning;
 
 typedef struct zg_batch_stats {
@@ -263,6 +267,10 @@ typedef struct zg_batch_stats {
   double last_seed_ms;       /* HIP-event time of packing + seed extraction */
   double last_total_ms;      /* HIP-event time of the whole last run */
   zg_tuning tuning;
+  uint64_t distinct_seeds;   /* seeds the last sorted-sweep run looked up (its
+                                distinct seeds per chunk); 0 for other runs */
+  uint64_t sweep_launches;   /* its lookup launches (one per chunk of reads) */
+  double sweep_lookup_ms;    /* their summed HIP-event time */
 } zg_batch_stats;
 
 /* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base

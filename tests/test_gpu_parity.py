@@ -170,6 +170,30 @@ def test_padded_row_pitch(n_colors, bits, group, mode):
     check_parity(idx, oracle_for(idx), reads, label=f"pitch c{n_colors} b{bits} g{group} m{mode}")
 
 
+SWEEP = {"span": 1024, "tiles_per_wave": 1, "prefetch": 2, "waves_per_cu": 0,
+         "nontemporal": 0, "sweep": 1}
+
+
+@pytest.mark.parametrize("bits,group,scale,mode,hpc,pattern", [
+    (8, 1, 1.0, 0, False, None), (8, 3, 0.5, 0, False, None), (4, 2, 0.5, 2, False, None),
+    (16, 1, 1.0, 0, True, "###_##_#_##_#_###_#_##_#_##_###"), (1, 4, 0.6, 0, False, None)])
+def test_sorted_sweep_parity(bits, group, scale, mode, hpc, pattern, monkeypatch):
+    # seeds looked up in PositionHash segment order, hits summed per read
+    colors = make_colors(200 + bits + group, 9, 20000)
+    reads = _with_runs(make_reads(201 + bits, colors, 200, lo=20, hi=2500))
+    idx = build(colors, fingerprint_bits=bits, hashes=4, merge_group_size=group,
+                slot_scale=scale, index_mode=mode, modimizer_sampling=3,
+                homopolymer_compression=int(hpc), seed_pattern=pattern or "")
+    idx.load()
+    orc = oracle_for(idx)
+    idx.tuning = SWEEP
+    assert idx.tuning["sweep"] == 1
+    check_parity(idx, orc, reads, label=f"sweep b{bits} g{group} m{mode}")
+    # several chunks of whole reads
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "5000")
+    check_parity(idx, orc, reads, label=f"sweep chunks b{bits} g{group} m{mode}")
+
+
 def test_minimizer_positions_and_counts():
     colors = make_colors(80, 5, 30000)
     reads = make_reads(81, colors, 120, lo=30, hi=3000, error=0.05)

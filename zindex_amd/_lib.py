@@ -60,7 +60,7 @@ class BuildConfig(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("span", C.c_uint32), ("tiles_per_wave", C.c_uint32),
                 ("prefetch", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("nontemporal", C.c_uint32)]
+                ("nontemporal", C.c_uint32), ("sweep", C.c_uint32)]
 
 
 class BatchStats(C.Structure):
@@ -68,7 +68,8 @@ class BatchStats(C.Structure):
         ("n_reads", C.c_uint64), ("bases", C.c_uint64), ("seeds", C.c_uint64),
         ("lookups", C.c_uint64), ("algorithmic_bytes", C.c_uint64),
         ("last_lookup_ms", C.c_double), ("last_seed_ms", C.c_double),
-        ("last_total_ms", C.c_double), ("tuning", Tuning),
+        ("last_total_ms", C.c_double), ("tuning", Tuning), ("distinct_seeds", C.c_uint64),
+        ("sweep_launches", C.c_uint64), ("sweep_lookup_ms", C.c_double),
     ]
 
 

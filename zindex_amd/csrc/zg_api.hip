@@ -90,6 +90,13 @@ namespace zg {
 static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
+// Indexes the sorted sweep serves: one bank (no size classes, partitions or
+// components) of k-mer or modimizer seeds of at most 32 bases.
+static bool sweep_layout(const Manifest &m) {
+  return m.banks.size() == 1 && !m.banks[0].routed && (m.index_mode == 0 || m.index_mode == 2) &&
+         m.k <= 32 && !m.has_randstrobemers;
+}
+
 static void check_query_support(const Manifest &m) {
   if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32 && m.has_randstrobemers)
@@ -428,8 +435,8 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       home = db.data.as<uint8_t>();
       put_bank(idx, bi, home, s);
     }
-    if (m.index_mode == 3) {
-      // findere bits: device column for singletons, then one bit per member
+    if (m.index_mode == 3 || sweep_layout(m)) {
+      // hit-row bits: device column for singletons, then one bit per member
       const uint32_t tile_bits = 64 * fields_per_lane(fb);
       const uint32_t member_bit = ((G + tile_bits - 1) / tile_bits) * tile_bits;
       std::vector<uint32_t> key(n_colors + 1, kNone);
@@ -487,7 +494,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   idx->fd_regions.clear();
   idx->fd_bank_region.assign(nb, 0);
   idx->fd_row_words = 0;
-  if (m.index_mode == 3) {
+  if (m.index_mode == 3 || sweep_layout(m)) {
     std::vector<size_t> first;  // a bank of each region
     std::vector<uint32_t> bit_color;
     for (size_t bi = 0; bi < nb; bi++) {
@@ -786,6 +793,7 @@ extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
     idx->tuning.prefetch = t->prefetch;
     idx->tuning.waves_per_cu = t->waves_per_cu;
     idx->tuning.nontemporal = t->nontemporal ? 1 : 0;
+    idx->tuning.sweep = t->sweep ? 1 : 0;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -802,6 +810,7 @@ extern "C" int zg_index_get_tuning(const zg_index *idx, zg_tuning *t) {
   t->prefetch = idx->tuning.prefetch;
   t->waves_per_cu = idx->tuning.waves_per_cu;
   t->nontemporal = idx->tuning.nontemporal;
+  t->sweep = idx->tuning.sweep;
   return 0;
 }
 
@@ -958,6 +967,133 @@ static void enqueue_reduce(zg_index *idx, zg_batch *b) {
     launch_reduce_partials(b->partial.as<uint64_t>(), b->waves, C, b->agg.as<uint64_t>(), s);
   else
     launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
+}
+
+// Hit-matrix bytes of one sorted-sweep chunk (the reads of a chunk take at
+// most this many positions, or one read alone when it is longer).
+constexpr uint64_t kSweepHitBytes = 8ULL << 30;
+
+double elapsed_ms(hipEvent_t a, hipEvent_t b);
+
+static bool sweep_on(const zg_index *idx, const zg_batch *b) {
+  return idx->tuning.sweep && sweep_layout(idx->m) && !b->explicit_keys && !any_streamed(idx) &&
+         !idx->fd_regions.empty();
+}
+
+// The lookups of a batch as a sorted sweep over its distinct seeds. Per
+// chunk of whole reads: the valid windows as (first hash, position) pairs, a
+// radix sort on the hash's high half (PositionHash order), one lookup per
+// run of equal hashes writing its hit row, and the sum of each read's rows
+// into its counts. A seed whose entries interleave with another seed of the
+// same high half forms several runs and is looked up once per run, which
+// changes no count.
+static void run_sweep(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
+  const Manifest &m = idx->m;
+  const zg_index::FdRegion &rg = idx->fd_regions[0];
+  const uint64_t row_words = idx->fd_row_words;
+  uint64_t cap = std::max<uint64_t>(1 << 20, kSweepHitBytes / (row_words * 4));
+  if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // chunk size override
+    cap = std::max<uint64_t>(64, strtoull(e, nullptr, 10));
+  b->sweep_distinct = 0;
+  b->sweep_chunks = 0;
+  b->sweep_lookup_ms = 0;
+  bool timing = false;  // a lookup bracketed by ev[4], ev[5] awaits its time
+  const std::vector<uint64_t> &poff = b->h_seg_off;  // position offsets of k-mer batches
+  // chunks of whole reads, then every buffer at the largest chunk's size
+  std::vector<uint64_t> cuts{0};
+  uint64_t most = 0;
+  while (cuts.back() < b->n_reads) {
+    const uint64_t r0 = cuts.back();
+    uint64_t r1 = r0 + 1;
+    while (r1 < b->n_reads && poff[r1 + 1] - poff[r0] <= cap) r1++;
+    most = std::max(most, poff[r1] - poff[r0]);
+    cuts.push_back(r1);
+  }
+  const uint64_t most_words = (most + 63) / 64;
+  b->rs_counts.reserve((most_words + 1) * 4);
+  b->rs_off.reserve((most_words + 1) * 8);
+  b->scan_temp.reserve(std::max(scan_temp_bytes_u32_to_u64(most_words + 1),
+                                scan_temp_bytes_u32_to_u64(most + 1)));
+  b->sw_hash.reserve(most * 4 + 4);
+  b->sw_pos.reserve(most * 8 + 8);
+  b->sw_hash2.reserve(most * 4 + 4);
+  b->sw_pos2.reserve(most * 8 + 8);
+  b->sw_flag.reserve(most * 4 + 4);
+  b->sw_off.reserve(most * 8 + 8);
+  b->sw_rep.reserve(most * 8 + 8);
+  b->sw_slot.reserve(most * 4 + 4);
+  b->sw_temp.reserve(sort_pairs_temp_bytes(most));
+  b->fd_hits.reserve(most * row_words * 4 + 4);
+  for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
+    const uint64_t P0 = poff[cuts[ci]], P1 = poff[cuts[ci + 1]], n_pos = P1 - P0;
+    const uint64_t words = (n_pos + 63) / 64;
+    const size_t tb = scan_temp_bytes_u32_to_u64(words + 1);
+    const uint64_t sampling = m.index_mode == 2 ? m.modimizer_sampling : 0;
+    launch_sweep_prep(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), P0, P1, (uint32_t)m.k,
+                      idx->spaced, m.seed, sampling, b->rs_counts.as<uint32_t>(), nullptr,
+                      nullptr, nullptr, s);
+    launch_fill_u32(b->rs_counts.as<uint32_t>() + words, 1, 0, s);
+    launch_exclusive_scan_u32_to_u64(b->rs_counts.as<uint32_t>(), b->rs_off.as<uint64_t>(),
+                                     words + 1, b->scan_temp.p, tb, s);
+    uint64_t n = 0;
+    check_hip(hipMemcpyAsync(&n, b->rs_off.as<uint64_t>() + words, 8, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "sweep");
+    if (timing) b->sweep_lookup_ms += elapsed_ms(b->ev[4], b->ev[5]);
+    timing = false;
+    if (!n) continue;
+    const size_t st = sort_pairs_temp_bytes(n);
+    b->sw_temp.reserve(st);
+    launch_sweep_prep(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), P0, P1, (uint32_t)m.k,
+                      idx->spaced, m.seed, sampling, nullptr, b->rs_off.as<uint64_t>(),
+                      b->sw_hash.as<uint32_t>(), b->sw_pos.as<uint64_t>(), s);
+    launch_sort_pairs(b->sw_temp.p, st, b->sw_hash.as<uint32_t>(), b->sw_hash2.as<uint32_t>(),
+                      b->sw_pos.as<uint64_t>(), b->sw_pos2.as<uint64_t>(), n, s);
+    launch_sweep_heads(b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
+                       b->sw_flag.as<uint32_t>(), s);
+    launch_exclusive_scan_u32_to_u64(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(), n + 1,
+                                     b->scan_temp.p, scan_temp_bytes_u32_to_u64(n + 1), s);
+    launch_fill_u32(b->sw_slot.as<uint32_t>(), n_pos, kNone, s);
+    launch_sweep_assign(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(),
+                        b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
+                        b->sw_rep.as<uint64_t>(), b->sw_slot.as<uint32_t>(), s);
+    uint64_t distinct = 0;
+    check_hip(hipMemcpyAsync(&distinct, b->sw_off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s),
+              "hipMemcpyAsync");
+    check_hip(hipStreamSynchronize(s), "sweep");
+    b->sweep_distinct += distinct;
+    // every seed's singleton bits are stored whole; member bits are ORed in
+    if (rg.bits > idx->fd_bank_member_bit[0])
+      launch_fill_u32(b->fd_hits.as<uint32_t>(), distinct * row_words, 0, s);
+    LookupArgs a = lookup_args(idx, b, 0);
+    a.source = kSeedSorted;
+    a.sorted = b->sw_rep.as<uint64_t>();
+    a.begin = 0;
+    a.end = distinct;
+    a.n_units = (distinct + a.span - 1) / a.span * a.n_tg;
+    a.records = 0;
+    a.sampling = 0;  // the list holds sampled windows only
+    a.fd_hits = b->fd_hits.as<uint32_t>();
+    a.fd_row_words = row_words;
+    a.fd_off = rg.off;
+    a.fd_member_bit = idx->fd_bank_member_bit[0];
+    launch_fill_u32(a.queue, 1, 0, s);
+    check_hip(hipEventRecord(b->ev[4], s), "hipEventRecord");
+    launch_lookup(a, b->kcfg, s);
+    check_hip(hipEventRecord(b->ev[5], s), "hipEventRecord");
+    timing = true;
+    b->sweep_chunks++;
+    launch_sweep_sum(b->fd_hits.as<uint32_t>(), row_words, rg.off, rg.bits,
+                     idx->fd_bit_color.as<uint32_t>() + rg.color_at, b->sw_slot.as<uint32_t>(),
+                     b->pos_off.as<uint64_t>(), b->n_reads, P0, P1, b->records ? 1 : 0,
+                     b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
+                     (uint32_t)m.colors.size(), s);
+  }
+  if (timing) {
+    check_hip(hipEventSynchronize(b->ev[5]), "hipEventSynchronize");
+    b->sweep_lookup_ms += elapsed_ms(b->ev[4], b->ev[5]);
+  }
 }
 
 static void drop_graph(zg_batch *b) {
@@ -1244,6 +1380,20 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
   try {
     ensure(idx && b && b->idx == idx, "invalid batch", ZG_ERR_INVALID);
     hipStream_t s = b->stream;
+    if (zg::sweep_on(idx, b)) {
+      zg::drop_graph(b);
+      check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+      zg::enqueue_zero_outputs(idx, b);
+      check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
+      zg::run_sweep(idx, b);
+      check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
+      zg::enqueue_reduce(idx, b);
+      check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
+      check_hip(hipStreamSynchronize(s), "lookup");
+      b->last_lookup_ms = zg::elapsed_ms(b->ev[1], b->ev[2]);
+      b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
+      return 0;
+    }
     // banks streamed per query cannot sit in a graph: launch directly
     if (any_streamed(idx)) zg::drop_graph(b);
     if (!b->graph_exec && !b->graph_failed && !any_streamed(idx)) {
@@ -1404,6 +1554,10 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->tuning.prefetch = b->kcfg.prefetch;
     o->tuning.waves_per_cu = b->kcfg.blocks * 4 / std::max(1u, idx->cu_count);
     o->tuning.nontemporal = b->nontemporal;
+    o->tuning.sweep = zg::sweep_on(idx, b) ? 1 : 0;
+    o->distinct_seeds = o->tuning.sweep ? b->sweep_distinct : 0;
+    o->sweep_launches = o->tuning.sweep ? b->sweep_chunks : 0;
+    o->sweep_lookup_ms = o->tuning.sweep ? b->sweep_lookup_ms : 0;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -1426,14 +1580,20 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
     const bool wide = idx->m.fingerprint_bits >= 8;  // 16-byte lane loads
     std::vector<uint32_t> depths = {1, 2, 4};
     if (idx->m.fingerprint_bits <= 4) depths.insert(depths.end(), {8, 16});
+    // the sorted sweep, where the index and batch allow it, with shorter spans
+    // (neighbouring waves then share rows)
+    const bool can_sweep = zg::sweep_layout(idx->m) && !b->explicit_keys && !any_streamed(idx);
+    for (uint32_t sw : {0u, 1u})
     for (uint32_t span : {1024u, 4096u, 16384u})
       for (uint32_t t : tiles)
         for (uint32_t d : depths)
           for (uint32_t frac : {1u, 2u})
             for (uint32_t nt : {0u, 1u}) {
             if (nt && !wide) continue;
+            if (sw && !can_sweep) continue;
             zg::Tuning cand;
-            cand.span = span;
+            cand.sweep = sw;
+            cand.span = sw ? span / 16 : span;
             cand.tiles_per_wave = t;
             cand.prefetch = d;
             cand.nontemporal = nt;
@@ -1452,9 +1612,10 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
               ms = std::min(ms, b->last_lookup_ms);
             }
             if (getenv("ZG_AUTOTUNE_TRACE"))
-              fprintf(stderr, "autotune span=%u tiles=%u prefetch=%u waves/cu=%u nt=%u: %.3f ms\n",
-                      cand.span, cand.tiles_per_wave, cand.prefetch, cand.waves_per_cu,
-                      cand.nontemporal, ms);
+              fprintf(stderr,
+                      "autotune sweep=%u span=%u tiles=%u prefetch=%u waves/cu=%u nt=%u: %.3f ms\n",
+                      cand.sweep, cand.span, cand.tiles_per_wave, cand.prefetch,
+                      cand.waves_per_cu, cand.nontemporal, ms);
             if (ms < best_ms) {
               best_ms = ms;
               best = cand;

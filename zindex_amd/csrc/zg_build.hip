@@ -196,19 +196,6 @@ static void host_ocm(const uint8_t *seq, size_t len, int k, int m, uint64_t seed
 // bijection, h is PositionHash's `first` (storage.rs:3458-3465), and sorting
 // by h both deduplicates keys and orders them by first segment (the segment
 // is mulhi(h, count), monotone in h), which is the locality the peel wants.
-constexpr uint64_t inverse_odd(uint64_t a) {
-  uint64_t x = a;
-  for (int i = 0; i < 6; i++) x *= 2 - a * x;
-  return x;
-}
-static inline uint64_t unsplitmix64(uint64_t x) {
-  x = x ^ (x >> 31) ^ (x >> 62);
-  x *= inverse_odd(0x94D049BB133111EBULL);
-  x = x ^ (x >> 27) ^ (x >> 54);
-  x *= inverse_odd(0xBF58476D1CE4E5B9ULL);
-  x = x ^ (x >> 30) ^ (x >> 60);
-  return x - 0x9E3779B97F4A7C15ULL;
-}
 static inline uint64_t key_of(uint64_t h, uint64_t seed) { return unsplitmix64(h) - seed; }
 
 // keys -> sorted distinct h values: one MSD pass on the top 16 bits, then

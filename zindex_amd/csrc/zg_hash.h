@@ -21,6 +21,26 @@ __host__ __device__ inline uint64_t mixsplit(uint64_t key, uint64_t seed) {
   return splitmix64(key + seed);
 }
 
+// The inverse of an odd number modulo 2^64 (Newton steps from a, which is
+// its own inverse modulo 8; each step doubles the correct low bits).
+__host__ __device__ constexpr uint64_t inverse_odd(uint64_t a) {
+  uint64_t x = a;
+  for (int i = 0; i < 5; i++) x *= 2 - a * x;
+  return x;
+}
+
+// splitmix64 run backwards: each xorshift and odd multiply is a bijection of
+// 64-bit words, undone in reverse order. mixsplit(key, seed) = h gives
+// key = unsplitmix64(h) - seed.
+__host__ __device__ inline uint64_t unsplitmix64(uint64_t z) {
+  z ^= (z >> 31) ^ (z >> 62);
+  z *= inverse_odd(0x94D049BB133111EBULL);
+  z ^= (z >> 27) ^ (z >> 54);
+  z *= inverse_odd(0xBF58476D1CE4E5B9ULL);
+  z ^= (z >> 30) ^ (z >> 60);
+  return z - 0x9E3779B97F4A7C15ULL;
+}
+
 // storage.rs:3214-3216
 __host__ __device__ inline uint64_t fingerprint(uint64_t key, uint64_t seed, uint32_t bits) {
   return splitmix64(key ^ seed ^ 0x8E2D4F6A9B13C5D7ULL) & ((1ULL << bits) - 1);

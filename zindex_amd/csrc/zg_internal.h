@@ -53,6 +53,7 @@ struct Tuning {
   uint32_t prefetch = 2;
   uint32_t waves_per_cu = 0;  // 0 = occupancy limit
   uint32_t nontemporal = 0;   // non-temporal row loads
+  uint32_t sweep = 0;         // seeds in segment order (sorted sweep)
 };
 
 }  // namespace zg
@@ -114,6 +115,13 @@ struct zg_batch {
   // homopolymer compression: keep flags, counts and offsets per 64 positions
   zg::DevBuf hpc_keep, hpc_counts, hpc_off;
   zg::DevBuf fd_hits;  // findere: positions x fd_row_words hit words
+  // sorted sweep: per-chunk (hash high half, hash low half | position) pairs
+  // before and after the sort, head flags and their scan, distinct seeds'
+  // first hashes, seed ids
+  zg::DevBuf sw_hash, sw_pos, sw_hash2, sw_pos2, sw_temp, sw_flag, sw_off, sw_rep, sw_slot;
+  uint64_t sweep_distinct = 0;  // distinct seeds looked up by the last sweep
+  uint64_t sweep_chunks = 0;    // its lookup launches (one per chunk)
+  double sweep_lookup_ms = 0;   // their summed HIP-event time
   // randstrobemers: candidate bitmap, per-word counts and offsets, run-id
   // offsets, candidate hashes and run ids (starts go to key_win); keys hold
   // the forward keys then the reverse-complement keys (rs_forward each)
@@ -131,7 +139,8 @@ struct zg_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_failed = false;         // capture refused: launch directly
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // run stages; ev[4], ev[5] bracket each sorted-sweep lookup launch
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double last_lookup_ms = 0, last_seed_ms = 0, last_total_ms = 0;
   ~zg_batch();
 };

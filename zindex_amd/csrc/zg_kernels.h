@@ -11,6 +11,9 @@ namespace zg {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// Throws with the runtime's message when e is not hipSuccess.
+void check_hip(hipError_t e, const char *what);
+
 // One bank resident in HBM, in the plane layout (fingerprint_first).
 struct BankDev {
   const uint8_t *data;        // slots * row_bytes (+ padding)
@@ -44,7 +47,10 @@ struct RecDev {
   uint32_t enabled;
 };
 
-enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1 };
+// kSeedSorted: seeds are the distinct k-mer windows of a chunk in
+// PositionHash order, listed as their first hash mixsplit(key, seed)
+// (`sorted`); seed i's hits go to hit row i.
+enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1, kSeedSorted = 2 };
 
 struct LookupArgs {
   BankDev bank;
@@ -79,6 +85,8 @@ struct LookupArgs {
   uint64_t fd_row_words;
   uint32_t fd_off;
   uint32_t fd_member_bit;
+  // kSeedSorted: first hashes of the listed seeds
+  const uint64_t *sorted;
 };
 
 struct KernelConfig {
@@ -122,11 +130,40 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
 // src/findere.rs:74-104): colour bit_color[j] of region bit j scores a
 // position when that bit is set in the z + 1 rows ending there. Adds into
 // counts[r * n_colors + colour] (records) or partial[colour].
+// Hit rows are indexed by position - p_base (z = 0 counts every hit).
 void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
                          uint32_t region_bits, const uint32_t *bit_color,
                          const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
                          uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
-                         uint32_t n_colors, hipStream_t s);
+                         uint32_t n_colors, hipStream_t s, uint64_t p_base = 0);
+// Sorted sweep: the valid k-mer windows of positions [p0, p1) (modimizers:
+// the sampled ones), keyed by their first PositionHash word h: hi = h >> 32,
+// lo_pos = (h << 32) | (position - p0). Two passes: per-64-position counts,
+// then the entries at the scanned offsets.
+void launch_sweep_prep(const uint64_t *codes, const uint64_t *invalid, uint64_t p0, uint64_t p1,
+                       uint32_t k, const SpacedRuns &spaced, uint64_t seed, uint64_t sampling,
+                       uint32_t *counts, const uint64_t *off, uint32_t *hi, uint64_t *lo_pos,
+                       hipStream_t s);
+// Radix sort of (u32 key, u64 value) pairs on the whole key.
+size_t sort_pairs_temp_bytes(uint64_t n);
+void launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                       const uint64_t *vals_in, uint64_t *vals_out, uint64_t n, hipStream_t s);
+// Distinct seeds of the sorted list: head flags (n + 1 entries), then from
+// their exclusive scan each head's first hash first[u] and every position's
+// seed id slot[position - p0] = u.
+void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
+                        hipStream_t s);
+void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
+                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *slot,
+                         hipStream_t s);
+// Sum of the hit rows (indexed by seed id) of each read's positions in
+// [p0, p1) into counts[r * n_colors + colour] (records) or partial[colour],
+// colour bit_color[j] for region bit j.
+void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
+                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *slot,
+                      const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t p1,
+                      uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors,
+                      hipStream_t s);
 // Padding, in words, after the invalid-base bits of `positions` positions so
 // that a window of `window` bases starting at any position stays in bounds.
 inline uint64_t invalid_words(uint64_t positions, uint64_t window) {

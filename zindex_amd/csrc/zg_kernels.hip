@@ -527,6 +527,210 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
 }
 
 // ---------------------------------------------------------------------------
+// Sorted sweep: seeds in PositionHash segment order, so the waves working at
+// the same time read neighbouring rows of the bank (shared through L2 and
+// the Infinity Cache) instead of random ones.
+// ---------------------------------------------------------------------------
+__global__ void sweep_prep_kernel(const uint64_t *codes, const uint64_t *invalid, uint64_t p0,
+                                  uint64_t p1, uint32_t k, SpacedRuns spaced, uint64_t seed,
+                                  uint64_t sampling, uint32_t *counts, const uint64_t *off,
+                                  uint32_t *hi, uint64_t *lo_pos) {
+  const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;  // word of [p0, p1)
+  const uint64_t words = (p1 - p0 + 63) / 64;
+  if (w >= words) return;
+  // windows of the word: positions p0 + 64w + i, i < 64, below p1
+  const uint64_t q = p0 + 64 * w;
+  uint64_t bad = 0;
+  for (uint32_t d = 0; d < k; d++) bad |= stream_bits(invalid, q + d);
+  uint64_t valid = ~bad;
+  if (p1 - q < 64) valid &= (1ULL << (p1 - q)) - 1;
+  if (!off && !sampling) {  // every valid window is listed
+    counts[w] = (uint32_t)__builtin_popcountll(valid);
+    return;
+  }
+  uint64_t at = off ? off[w] : 0;
+  uint32_t n = 0;
+  for (uint64_t m = valid; m; m &= m - 1) {
+    const int i = __builtin_ctzll(m);
+    uint64_t key = canonical_from_window(stream_bits(codes, 2 * (q + i)), (int)k);
+    if (spaced.n) key = spaced_canonical(key, spaced);
+    const uint64_t first = mixsplit(key, seed);
+    if (sampling && first % sampling) continue;  // not a modimizer
+    if (off) {
+      hi[at] = (uint32_t)(first >> 32);
+      lo_pos[at] = (first << 32) | (q + i - p0);
+      at++;
+    }
+    n++;
+  }
+  if (!off) counts[w] = n;
+}
+
+void launch_sweep_prep(const uint64_t *codes, const uint64_t *invalid, uint64_t p0, uint64_t p1,
+                       uint32_t k, const SpacedRuns &spaced, uint64_t seed, uint64_t sampling,
+                       uint32_t *counts, const uint64_t *off, uint32_t *hi, uint64_t *lo_pos,
+                       hipStream_t s) {
+  const uint64_t words = (p1 - p0 + 63) / 64;
+  if (!words) return;
+  sweep_prep_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
+      codes, invalid, p0, p1, k, spaced, seed, sampling, counts, off, hi, lo_pos);
+}
+
+size_t sort_pairs_temp_bytes(uint64_t n) {
+  size_t bytes = 0;
+  check_hip(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
+                                               (uint32_t *)nullptr, (const uint64_t *)nullptr,
+                                               (uint64_t *)nullptr, (int64_t)n),
+            "radix sort size");
+  return bytes;
+}
+
+void launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                       const uint64_t *vals_in, uint64_t *vals_out, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  check_hip(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in,
+                                               vals_out, (int64_t)n, 0, 32, s),
+            "radix sort");
+}
+
+// Distinct seeds of a sorted sweep list: entry i heads a run when its first
+// hash (high half hi, low half in lo_pos's high word) differs from entry
+// i - 1's (flags[n] = 0 closes the scan input).
+__global__ void sweep_heads_kernel(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n,
+                                   uint32_t *flags) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i > n) return;
+  flags[i] = i < n && (i == 0 || hi[i] != hi[i - 1] || (lo_pos[i] >> 32) != (lo_pos[i - 1] >> 32))
+                 ? 1u
+                 : 0u;
+}
+
+// Entry i belongs to distinct seed u = off[i] + flags[i] - 1 (off = exclusive
+// scan of flags): a head stores the seed's first hash, every entry's position
+// maps to u.
+__global__ void sweep_assign_kernel(const uint32_t *flags, const uint64_t *off,
+                                    const uint32_t *hi, const uint64_t *lo_pos, uint64_t n,
+                                    uint64_t *first, uint32_t *slot) {
+  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t u = (uint32_t)(off[i] + flags[i] - 1);
+  const uint64_t lp = lo_pos[i];
+  if (flags[i]) first[u] = (uint64_t)hi[i] << 32 | lp >> 32;
+  slot[(uint32_t)lp] = u;
+}
+
+void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
+                        hipStream_t s) {
+  sweep_heads_kernel<<<(uint32_t)((n + 1 + 255) / 256), 256, 0, s>>>(hi, lo_pos, n, flags);
+}
+
+void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
+                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *slot,
+                         hipStream_t s) {
+  if (!n) return;
+  sweep_assign_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(flags, off, hi, lo_pos, n,
+                                                                    first, slot);
+}
+
+// Per-read sums of the hit rows of positions [p0, p1): position p adds row
+// slot[p - p0] (kNone: no valid window). One wave per span of positions and
+// group of 1024 region bits; lane l holds bits 16l..16l+15 of the group as
+// byte counters (bit 4q+b in byte b of acc[q]), spilled every 255 rows and
+// added to the read's counts (records) or partial[colour] at read ends.
+constexpr uint32_t kSweepSumSpan = 4096;
+
+__device__ __forceinline__ uint32_t spread_nibble(uint32_t x) {
+  // bit i of x (i < 4) to bit 8i: the products' bits i + 7s never collide
+  return (x * 0x00204081u) & 0x01010101u;
+}
+
+__global__ __launch_bounds__(256) void sweep_sum_kernel(
+    const uint32_t *hits, uint64_t row_words, uint32_t region_off, uint32_t region_bits,
+    const uint32_t *bit_color, const uint32_t *slot, const uint64_t *pos_off, uint64_t n_reads,
+    uint64_t p0, uint64_t p1, uint32_t records, uint32_t *counts, uint64_t *partial,
+    uint32_t n_colors) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint64_t groups = (region_bits + 1023) / 1024;
+  const uint64_t spans = (p1 - p0 + kSweepSumSpan - 1) / kSweepSumSpan;
+  if (wave >= groups * spans) return;
+  const uint64_t g = wave % groups, sp = wave / groups;
+  const uint32_t j0 = (uint32_t)(g * 1024) + lane * 16;
+  const bool live = j0 < region_bits;
+  // the lane's 16 bits of row u: halfword u * 2 * row_words + j0 / 16
+  const uint16_t *col = (const uint16_t *)(hits + region_off) + j0 / 16;
+  const uint64_t q0 = p0 + sp * kSweepSumSpan;
+  const uint64_t q1 = q0 + kSweepSumSpan < p1 ? q0 + kSweepSumSpan : p1;
+  uint64_t r = find_segment(pos_off, n_reads, q0);
+  uint64_t next = pos_off[r + 1];
+  uint32_t acc[4] = {0, 0, 0, 0}, tot[16], fill = 0;
+#pragma unroll
+  for (int b = 0; b < 16; b++) tot[b] = 0;
+  auto spill = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) tot[4 * q + b] += (acc[q] >> (8 * b)) & 0xFFu;
+      acc[q] = 0;
+    }
+    fill = 0;
+  };
+  auto flush = [&]() {
+    spill();
+#pragma unroll
+    for (int b = 0; b < 16; b++) {
+      if (tot[b] && j0 + b < region_bits) {
+        const uint32_t color = bit_color[j0 + b];
+        if (color != kNone) {
+          if (records) atomicAdd(&counts[r * n_colors + color], tot[b]);
+          else atomicAdd((unsigned long long *)&partial[color], (unsigned long long)tot[b]);
+        }
+      }
+      tot[b] = 0;
+    }
+  };
+  for (uint64_t q = q0; q < q1; q += 64) {
+    const uint32_t s = q + lane < q1 ? slot[q + lane - p0] : kNone;
+    const uint32_t m = q1 - q < 64 ? (uint32_t)(q1 - q) : 64u;
+    for (uint32_t jb = 0; jb < m; jb += 8) {
+      uint32_t x[8];
+#pragma unroll
+      for (int d = 0; d < 8; d++) {
+        const uint32_t u = readlane32(s, (int)(jb + d));
+        x[d] = live && u != kNone ? col[(uint64_t)u * 2 * row_words] : 0u;
+      }
+#pragma unroll
+      for (int d = 0; d < 8; d++) {
+        const uint64_t p = q + jb + d;
+        if (p >= q1) break;
+        if (records && p >= next) {
+          flush();
+          while (pos_off[r + 1] <= p) r++;
+          next = pos_off[r + 1];
+        }
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) acc[qq] += spread_nibble((x[d] >> (4 * qq)) & 0xFu);
+        if (++fill == 255) spill();
+      }
+    }
+  }
+  flush();
+}
+
+void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
+                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *slot,
+                      const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t p1,
+                      uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors,
+                      hipStream_t s) {
+  if (p1 <= p0 || !n_reads || !region_bits) return;
+  const uint64_t waves = ((region_bits + 1023) / 1024) *
+                         ((p1 - p0 + kSweepSumSpan - 1) / kSweepSumSpan);
+  sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+      hits, row_words, region_off, region_bits, bit_color, slot, pos_off, n_reads, p0, p1,
+      records, counts, partial, n_colors);
+}
+
+// ---------------------------------------------------------------------------
 // Long k-mers (k > 32, scan_long_kmers, src/lib.rs:683-730). Forward hash
 // sum (c_i + 1) B^(k-1-i), reverse-complement hash sum (4 - c_i) B^i, both
 // mod 2^64; the key is splitmix64(min ^ domain). One thread per 64 window
@@ -781,25 +985,28 @@ void launch_strand_max(uint32_t *counts, uint64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 size_t scan_temp_bytes_u64(uint64_t n) {
   size_t bytes = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t *)nullptr,
-                                   (uint64_t *)nullptr, (int64_t)n);
+  check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t *)nullptr,
+                                             (uint64_t *)nullptr, (int64_t)n),
+            "scan size");
   return bytes;
 }
 void launch_exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *temp,
                                size_t temp_bytes, hipStream_t s) {
-  hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int64_t)n, s);
+  check_hip(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int64_t)n, s), "scan");
 }
 size_t scan_temp_bytes_u32_to_u64(uint64_t n) {
   size_t bytes = 0;
-  hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, (const uint32_t *)nullptr,
-                                    (uint64_t *)nullptr, hipcub::Sum(), (uint64_t)0,
-                                    (int64_t)n);
+  check_hip(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, (const uint32_t *)nullptr,
+                                              (uint64_t *)nullptr, hipcub::Sum(), (uint64_t)0,
+                                              (int64_t)n),
+            "scan size");
   return bytes;
 }
 void launch_exclusive_scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_t n,
                                       void *temp, size_t temp_bytes, hipStream_t s) {
-  hipcub::DeviceScan::ExclusiveScan(temp, temp_bytes, in, out, hipcub::Sum(), (uint64_t)0,
-                                    (int64_t)n, s);
+  check_hip(hipcub::DeviceScan::ExclusiveScan(temp, temp_bytes, in, out, hipcub::Sum(),
+                                              (uint64_t)0, (int64_t)n, s),
+            "scan");
 }
 
 // ---------------------------------------------------------------------------
@@ -1055,6 +1262,10 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
           v = (iv & win_mask) == 0;
           key = canonical_from_window(stream_bits(a.codes, 2 * idx), (int)a.k);
           if (a.spaced.n) key = spaced_canonical(key, a.spaced);
+        }
+      } else if (a.source == kSeedSorted) {
+        if (v) {
+          key = unsplitmix64(a.sorted[idx]) - a.seed;  // mixsplit(key, seed) was listed
         }
       } else if (v) {
         key = a.keys[idx];
@@ -1365,18 +1576,20 @@ constexpr uint32_t kFindereSpan = 2048;
 __global__ __launch_bounds__(256) void findere_scan_kernel(
     const uint32_t *hits, uint64_t row_words, uint32_t region_off, uint32_t region_bits,
     const uint32_t *bit_color, const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
-    uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
+    uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors,
+    uint64_t p_base) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint64_t groups = (region_bits + 63) / 64;
-  const uint64_t spans = (positions + kFindereSpan - 1) / kFindereSpan;
+  const uint64_t spans = (positions - p_base + kFindereSpan - 1) / kFindereSpan;
   if (wave >= groups * spans) return;
   const uint64_t g = wave % groups, sp = wave / groups;
   const uint32_t j = (uint32_t)(g * 64 + lane);
   const uint32_t color = j < region_bits ? bit_color[j] : kNone;
-  const uint64_t p0 = sp * kFindereSpan;
+  // positions [p_base, positions); hit rows are indexed by position - p_base
+  const uint64_t p0 = p_base + sp * kFindereSpan;
   const uint64_t p1 = p0 + kFindereSpan < positions ? p0 + kFindereSpan : positions;
-  const uint64_t start = p0 > z ? p0 - z : 0;
+  const uint64_t start = p0 > p_base + z ? p0 - z : p_base;
   const uint32_t *col = hits + region_off + (j >> 5);
   const uint32_t sh = j & 31;
   const uint32_t window = z + 1;
@@ -1384,7 +1597,7 @@ __global__ __launch_bounds__(256) void findere_scan_kernel(
   uint64_t next = pos_off[r + 1];
   uint32_t run = 0, cnt = 0;
   for (uint64_t p = start; p < p1; p++) {
-    const uint32_t x = color != kNone ? (col[p * row_words] >> sh) & 1u : 0u;
+    const uint32_t x = color != kNone ? (col[(p - p_base) * row_words] >> sh) & 1u : 0u;
     run = x ? (run + 1 < window ? run + 1 : window) : 0;
     if (p < p0) continue;
     if (records && p >= next) {
@@ -1405,13 +1618,13 @@ void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t regi
                          uint32_t region_bits, const uint32_t *bit_color,
                          const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
                          uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
-                         uint32_t n_colors, hipStream_t s) {
-  if (!positions || !n_reads || !region_bits) return;
+                         uint32_t n_colors, hipStream_t s, uint64_t p_base) {
+  if (positions <= p_base || !n_reads || !region_bits) return;
   const uint64_t waves =
-      ((region_bits + 63) / 64) * ((positions + kFindereSpan - 1) / kFindereSpan);
+      ((region_bits + 63) / 64) * ((positions - p_base + kFindereSpan - 1) / kFindereSpan);
   findere_scan_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
       hits, row_words, region_off, region_bits, bit_color, pos_off, n_reads, positions, z,
-      records, counts, partial, n_colors);
+      records, counts, partial, n_colors, p_base);
 }
 
 // Random-row gather probe: whole rows of ROW = CH * 64 * LB bytes at
