@@ -625,6 +625,9 @@ def main() -> int:
         if rec.get("workload") == workload and rec.get("algorithmic_bytes") == algo:
             roofline["traffic"] = rec["traffic_bytes"]
             roofline["traffic_source"] = str(tf.relative_to(ROOT))
+            # the measured HBM bytes at this run's launch time
+            roofline["traffic_gbs"] = round(rec["traffic_bytes"] / (kernel_ms / 1000) / 1e9, 1)
+            roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
 
     plain = None
     if args.plain_reads:

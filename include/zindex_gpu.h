@@ -252,9 +252,10 @@ typedef struct zg_tuning {
   uint32_t waves_per_cu;   /* persistent waves per compute unit (0 = max) */
   uint32_t nontemporal;    /* 1: row gathers use non-temporal loads */
   uint32_t sweep;          /* 1: seeds of a single-bank k-mer index are
-                              looked up in PositionHash segment order (a
-                              radix sort per chunk of reads), each seed's hits
-                              into a hit row, the rows summed per read */
+                              looked up once per distinct seed in PositionHash
+                              order (a radix sort per chunk of reads), each
+                              seed's hits into a hit row, the rows summed per
+                              read; same results as 0 */
 } zg_tuning;
 
 typedef struct zg_batch_stats {
