@@ -972,6 +972,7 @@ static void enqueue_reduce(zg_index *idx, zg_batch *b) {
 // Hit-matrix bytes of one sorted-sweep chunk (the reads of a chunk take at
 // most this many positions, or one read alone when it is longer).
 constexpr uint64_t kSweepHitBytes = 8ULL << 30;
+constexpr uint32_t kSweepSlices = 8;  // gfx950 XCDs
 
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
 
@@ -1025,6 +1026,7 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
   b->sw_slot.reserve(most * 4 + 4);
   b->sw_temp.reserve(sort_pairs_temp_bytes(most));
   b->fd_hits.reserve(most * row_words * 4 + 4);
+  b->sw_queue.reserve(32 * kSweepSlices * 4);
   for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
     const uint64_t P0 = poff[cuts[ci]], P1 = poff[cuts[ci + 1]], n_pos = P1 - P0;
     const uint64_t words = (n_pos + 63) / 64;
@@ -1078,7 +1080,10 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
     a.fd_row_words = row_words;
     a.fd_off = rg.off;
     a.fd_member_bit = idx->fd_bank_member_bit[0];
-    launch_fill_u32(a.queue, 1, 0, s);
+    // one contiguous range of the sorted list per XCD
+    a.queue = b->sw_queue.as<uint32_t>();
+    a.slices = b->kcfg.blocks >= kSweepSlices ? kSweepSlices : 1;
+    launch_fill_u32(a.queue, 32 * kSweepSlices, 0, s);
     check_hip(hipEventRecord(b->ev[4], s), "hipEventRecord");
     launch_lookup(a, b->kcfg, s);
     check_hip(hipEventRecord(b->ev[5], s), "hipEventRecord");

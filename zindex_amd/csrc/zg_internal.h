@@ -119,6 +119,7 @@ struct zg_batch {
   // before and after the sort, head flags and their scan, distinct seeds'
   // first hashes, seed ids
   zg::DevBuf sw_hash, sw_pos, sw_hash2, sw_pos2, sw_temp, sw_flag, sw_off, sw_rep, sw_slot;
+  zg::DevBuf sw_queue;  // per-slice work counters of the sorted lookup
   uint64_t sweep_distinct = 0;  // distinct seeds looked up by the last sweep
   uint64_t sweep_chunks = 0;    // its lookup launches (one per chunk)
   double sweep_lookup_ms = 0;   // their summed HIP-event time

@@ -67,7 +67,10 @@ struct LookupArgs {
   uint64_t n_units;           // spans * tile groups
   uint32_t span;              // seed indices per work unit (multiple of 64)
   uint32_t n_tg;              // tile groups per span
-  uint32_t *queue;            // work counter, zero before launch
+  uint32_t *queue;            // work counter(s), zero before launch
+  uint32_t slices;            // > 1: units split into that many ranges, one per
+                              // block residue, counters at queue[32 * slice];
+                              // at most the grid's block count
   uint64_t seed;
   uint32_t bits;
   uint32_t records;           // 1: per-read counts; 0: per-wave partial sums

@@ -1189,11 +1189,19 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
     }
   };
 
+  // Work slices: block x takes the units of slice x % slices, a contiguous
+  // range, from that slice's own counter (128 B apart). Blocks are dealt to
+  // the 8 XCDs in turn, so with 8 slices each XCD's L2 serves one range of a
+  // sorted seed list, where neighbouring seeds share rows.
+  const uint32_t n_slices = a.slices > 1 ? a.slices : 1;
+  const uint32_t slice = blockIdx.x % n_slices;
+  const uint32_t u_lo = (uint32_t)((uint64_t)a.n_units * slice / n_slices);
+  const uint32_t u_hi = (uint32_t)((uint64_t)a.n_units * (slice + 1) / n_slices);
   while (true) {
     uint32_t unit = 0;
-    if (lane == 0) unit = atomicAdd(a.queue, 1u);
-    unit = readlane32(unit, 0);
-    if (unit >= a.n_units) break;
+    if (lane == 0) unit = atomicAdd(a.queue + 32 * slice, 1u);
+    unit = u_lo + readlane32(unit, 0);
+    if (unit >= u_hi) break;
     const uint64_t sp = unit / a.n_tg;
     const uint32_t tg = unit % a.n_tg;
     const uint64_t i0 = a.begin + sp * (uint64_t)a.span;
