@@ -622,7 +622,10 @@ def main() -> int:
     # workload (scaled by a copy kernel of known size), when recorded for it
     for tf in sorted((ROOT / "profiles").glob("*_traffic.json")):
         rec = json.loads(tf.read_text())
-        if rec.get("workload") == workload and rec.get("algorithmic_bytes") == algo:
+        tuned = ",".join(str(st["tuning"][k]) for k in ("span", "tiles_per_wave", "prefetch",
+                                                          "waves_per_cu", "nontemporal", "sweep"))
+        if (rec.get("workload") == workload and rec.get("algorithmic_bytes") == algo
+                and tuned.startswith(rec.get("tuning", ""))):
             roofline["traffic"] = rec["traffic_bytes"]
             roofline["traffic_source"] = str(tf.relative_to(ROOT))
             # the measured HBM bytes at this run's launch time
