@@ -994,8 +994,21 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
   const zg_index::FdRegion &rg = idx->fd_regions[0];
   const uint64_t row_words = idx->fd_row_words;
   uint64_t cap = std::max<uint64_t>(1 << 20, kSweepHitBytes / (row_words * 4));
-  if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // chunk size override
-    cap = std::max<uint64_t>(64, strtoull(e, nullptr, 10));
+  // a chunk's buffers (a hit row and ~64 B of list entries per position) take at
+  // most 3/4 of the device memory that is free or already held by them
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    uint64_t held = b->fd_hits.bytes + b->sw_temp.bytes + b->scan_temp.bytes;
+    for (const DevBuf *d : {&b->sw_hash, &b->sw_pos, &b->sw_hash2, &b->sw_pos2, &b->sw_flag,
+                            &b->sw_off, &b->sw_rep, &b->sw_slot})
+      held += d->bytes;
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, (free_b + held) / 4 * 3 /
+                                                             (row_words * 4 + 64)));
+  } else {
+    (void)hipGetLastError();
+  }
+  if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // smaller chunks
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, strtoull(e, nullptr, 10)));
   b->sweep_distinct = 0;
   b->sweep_chunks = 0;
   b->sweep_lookup_ms = 0;
