@@ -971,7 +971,7 @@ static void enqueue_reduce(zg_index *idx, zg_batch *b) {
 
 // Hit-matrix bytes of one sorted-sweep chunk (the reads of a chunk take at
 // most this many positions, or one read alone when it is longer).
-constexpr uint64_t kSweepHitBytes = 8ULL << 30;
+constexpr uint64_t kSweepHitBytes = 64ULL << 30;
 constexpr uint32_t kSweepSlices = 8;  // gfx950 XCDs
 
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
