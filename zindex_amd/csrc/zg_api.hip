@@ -1037,7 +1037,7 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
   b->sw_off.reserve(most * 8 + 8);
   b->sw_rep.reserve(most * 8 + 8);
   b->sw_slot.reserve(most * 4 + 4);
-  b->sw_temp.reserve(sort_pairs_temp_bytes(most));
+  b->sw_temp.reserve(std::max(sort_pairs_temp_bytes(most), sort_pairs_u32_temp_bytes(most)));
   b->fd_hits.reserve(most * row_words * 4 + 4);
   b->sw_queue.reserve(32 * kSweepSlices * 4);
   for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
@@ -1069,10 +1069,18 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
                        b->sw_flag.as<uint32_t>(), s);
     launch_exclusive_scan_u32_to_u64(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(), n + 1,
                                      b->scan_temp.p, scan_temp_bytes_u32_to_u64(n + 1), s);
-    launch_fill_u32(b->sw_slot.as<uint32_t>(), n_pos, kNone, s);
+    // (position, seed id) per entry, sorted back to position order: sequential
+    // traffic where a scatter by position would write 4 bytes at random
     launch_sweep_assign(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(),
                         b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
-                        b->sw_rep.as<uint64_t>(), b->sw_slot.as<uint32_t>(), s);
+                        b->sw_rep.as<uint64_t>(), b->sw_hash.as<uint32_t>(),
+                        b->sw_slot.as<uint32_t>(), s);
+    int pos_bits = 1;
+    while (pos_bits < 32 && (1ULL << pos_bits) < n_pos) pos_bits++;
+    const size_t st2 = sort_pairs_u32_temp_bytes(n);
+    b->sw_temp.reserve(st2);
+    launch_sort_pairs_u32(b->sw_temp.p, st2, b->sw_hash.as<uint32_t>(), b->sw_hash2.as<uint32_t>(),
+                          b->sw_slot.as<uint32_t>(), b->sw_flag.as<uint32_t>(), n, pos_bits, s);
     uint64_t distinct = 0;
     check_hip(hipMemcpyAsync(&distinct, b->sw_off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s),
               "hipMemcpyAsync");
@@ -1103,9 +1111,9 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
     timing = true;
     b->sweep_chunks++;
     launch_sweep_sum(b->fd_hits.as<uint32_t>(), row_words, rg.off, rg.bits,
-                     idx->fd_bit_color.as<uint32_t>() + rg.color_at, b->sw_slot.as<uint32_t>(),
-                     b->pos_off.as<uint64_t>(), b->n_reads, P0, P1, b->records ? 1 : 0,
-                     b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
+                     idx->fd_bit_color.as<uint32_t>() + rg.color_at, b->sw_hash2.as<uint32_t>(),
+                     b->sw_flag.as<uint32_t>(), n, b->pos_off.as<uint64_t>(), b->n_reads, P0,
+                     b->records ? 1 : 0, b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
                      (uint32_t)m.colors.size(), s);
   }
   if (timing) {

@@ -115,9 +115,10 @@ struct zg_batch {
   // homopolymer compression: keep flags, counts and offsets per 64 positions
   zg::DevBuf hpc_keep, hpc_counts, hpc_off;
   zg::DevBuf fd_hits;  // findere: positions x fd_row_words hit words
-  // sorted sweep: per-chunk (hash high half, hash low half | position) pairs
+  // sorted sweep, per chunk: (hash high half, hash low half | position) pairs
   // before and after the sort, head flags and their scan, distinct seeds'
-  // first hashes, seed ids
+  // first hashes, seed ids; the first buffers are reused for the (position,
+  // seed id) pairs and their sort back to position order
   zg::DevBuf sw_hash, sw_pos, sw_hash2, sw_pos2, sw_temp, sw_flag, sw_off, sw_rep, sw_slot;
   zg::DevBuf sw_queue;  // per-slice work counters of the sorted lookup
   uint64_t sweep_distinct = 0;  // distinct seeds looked up by the last sweep

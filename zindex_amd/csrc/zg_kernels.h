@@ -152,21 +152,26 @@ size_t sort_pairs_temp_bytes(uint64_t n);
 void launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
                        const uint64_t *vals_in, uint64_t *vals_out, uint64_t n, hipStream_t s);
 // Distinct seeds of the sorted list: head flags (n + 1 entries), then from
-// their exclusive scan each head's first hash first[u] and every position's
-// seed id slot[position - p0] = u.
+// their exclusive scan each head's first hash first[u], and for every entry
+// its position - p0 and seed id u (pos[i], ids[i], in list order).
 void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
                         hipStream_t s);
 void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
-                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *slot,
-                         hipStream_t s);
-// Sum of the hit rows (indexed by seed id) of each read's positions in
-// [p0, p1) into counts[r * n_colors + colour] (records) or partial[colour],
-// colour bit_color[j] for region bit j.
+                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *pos,
+                         uint32_t *ids, hipStream_t s);
+// Radix sort of (u32 key, u32 value) pairs on the key's low `bits` bits.
+size_t sort_pairs_u32_temp_bytes(uint64_t n);
+void launch_sort_pairs_u32(void *temp, size_t temp_bytes, const uint32_t *keys_in,
+                           uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                           uint64_t n, int bits, hipStream_t s);
+// Sum of the hit rows ids[e] of a chunk's n entries (pos[e] = position - p0,
+// ascending) into counts[r * n_colors + colour] of the read r holding the
+// position (records) or partial[colour]; colour bit_color[j] for region bit j.
 void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
-                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *slot,
-                      const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t p1,
-                      uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors,
-                      hipStream_t s);
+                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *pos,
+                      const uint32_t *ids, uint64_t n, const uint64_t *pos_off, uint64_t n_reads,
+                      uint64_t p0, uint32_t records, uint32_t *counts, uint64_t *partial,
+                      uint32_t n_colors, hipStream_t s);
 // Padding, in words, after the invalid-base bits of `positions` positions so
 // that a window of `window` bases starting at any position stays in bounds.
 inline uint64_t invalid_words(uint64_t positions, uint64_t window) {

@@ -606,17 +606,18 @@ __global__ void sweep_heads_kernel(const uint32_t *hi, const uint64_t *lo_pos, u
 }
 
 // Entry i belongs to distinct seed u = off[i] + flags[i] - 1 (off = exclusive
-// scan of flags): a head stores the seed's first hash, every entry's position
-// maps to u.
+// scan of flags): a head stores the seed's first hash, and every entry gives
+// (its position, u) for the sort back to position order.
 __global__ void sweep_assign_kernel(const uint32_t *flags, const uint64_t *off,
                                     const uint32_t *hi, const uint64_t *lo_pos, uint64_t n,
-                                    uint64_t *first, uint32_t *slot) {
+                                    uint64_t *first, uint32_t *pos, uint32_t *ids) {
   const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t u = (uint32_t)(off[i] + flags[i] - 1);
   const uint64_t lp = lo_pos[i];
   if (flags[i]) first[u] = (uint64_t)hi[i] << 32 | lp >> 32;
-  slot[(uint32_t)lp] = u;
+  pos[i] = (uint32_t)lp;
+  ids[i] = u;
 }
 
 void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
@@ -625,18 +626,37 @@ void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, 
 }
 
 void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
-                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *slot,
-                         hipStream_t s) {
+                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *pos,
+                         uint32_t *ids, hipStream_t s) {
   if (!n) return;
   sweep_assign_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(flags, off, hi, lo_pos, n,
-                                                                    first, slot);
+                                                                    first, pos, ids);
 }
 
-// Per-read sums of the hit rows of positions [p0, p1): position p adds row
-// slot[p - p0] (kNone: no valid window). One wave per span of positions and
-// group of 1024 region bits; lane l holds bits 16l..16l+15 of the group as
-// byte counters (bit 4q+b in byte b of acc[q]), spilled every 255 rows and
-// added to the read's counts (records) or partial[colour] at read ends.
+size_t sort_pairs_u32_temp_bytes(uint64_t n) {
+  size_t bytes = 0;
+  check_hip(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
+                                               (uint32_t *)nullptr, (const uint32_t *)nullptr,
+                                               (uint32_t *)nullptr, (int64_t)n),
+            "radix sort size");
+  return bytes;
+}
+
+void launch_sort_pairs_u32(void *temp, size_t temp_bytes, const uint32_t *keys_in,
+                           uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
+                           uint64_t n, int bits, hipStream_t s) {
+  if (!n) return;
+  check_hip(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in,
+                                               vals_out, (int64_t)n, 0, bits, s),
+            "radix sort");
+}
+
+// Per-read sums of the hit rows of a chunk's seeds: entry e (in position
+// order) adds row ids[e] to the read holding position p0 + pos[e]. One wave
+// per span of entries and group of 1024 region bits; lane l holds bits
+// 16l..16l+15 of the group as byte counters (bit 4q+b in byte b of acc[q]),
+// spilled every 255 rows and added to the read's counts (records) or
+// partial[colour] at read ends.
 constexpr uint32_t kSweepSumSpan = 4096;
 
 __device__ __forceinline__ uint32_t spread_nibble(uint32_t x) {
@@ -646,22 +666,22 @@ __device__ __forceinline__ uint32_t spread_nibble(uint32_t x) {
 
 __global__ __launch_bounds__(256) void sweep_sum_kernel(
     const uint32_t *hits, uint64_t row_words, uint32_t region_off, uint32_t region_bits,
-    const uint32_t *bit_color, const uint32_t *slot, const uint64_t *pos_off, uint64_t n_reads,
-    uint64_t p0, uint64_t p1, uint32_t records, uint32_t *counts, uint64_t *partial,
-    uint32_t n_colors) {
+    const uint32_t *bit_color, const uint32_t *pos, const uint32_t *ids, uint64_t n,
+    const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint32_t records, uint32_t *counts,
+    uint64_t *partial, uint32_t n_colors) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint64_t groups = (region_bits + 1023) / 1024;
-  const uint64_t spans = (p1 - p0 + kSweepSumSpan - 1) / kSweepSumSpan;
+  const uint64_t spans = (n + kSweepSumSpan - 1) / kSweepSumSpan;
   if (wave >= groups * spans) return;
   const uint64_t g = wave % groups, sp = wave / groups;
   const uint32_t j0 = (uint32_t)(g * 1024) + lane * 16;
   const bool live = j0 < region_bits;
   // the lane's 16 bits of row u: halfword u * 2 * row_words + j0 / 16
   const uint16_t *col = (const uint16_t *)(hits + region_off) + j0 / 16;
-  const uint64_t q0 = p0 + sp * kSweepSumSpan;
-  const uint64_t q1 = q0 + kSweepSumSpan < p1 ? q0 + kSweepSumSpan : p1;
-  uint64_t r = find_segment(pos_off, n_reads, q0);
+  const uint64_t e0 = sp * kSweepSumSpan;
+  const uint64_t e1 = e0 + kSweepSumSpan < n ? e0 + kSweepSumSpan : n;
+  uint64_t r = find_segment(pos_off, n_reads, p0 + pos[e0]);
   uint64_t next = pos_off[r + 1];
   uint32_t acc[4] = {0, 0, 0, 0}, tot[16], fill = 0;
 #pragma unroll
@@ -689,20 +709,22 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
       tot[b] = 0;
     }
   };
-  for (uint64_t q = q0; q < q1; q += 64) {
-    const uint32_t s = q + lane < q1 ? slot[q + lane - p0] : kNone;
-    const uint32_t m = q1 - q < 64 ? (uint32_t)(q1 - q) : 64u;
+  for (uint64_t e = e0; e < e1; e += 64) {
+    const bool in = e + lane < e1;
+    const uint32_t my_id = in ? ids[e + lane] : kNone;
+    const uint32_t my_pos = in ? pos[e + lane] : 0u;
+    const uint32_t m = e1 - e < 64 ? (uint32_t)(e1 - e) : 64u;
     for (uint32_t jb = 0; jb < m; jb += 8) {
       uint32_t x[8];
 #pragma unroll
       for (int d = 0; d < 8; d++) {
-        const uint32_t u = readlane32(s, (int)(jb + d));
+        const uint32_t u = readlane32(my_id, (int)(jb + d));
         x[d] = live && u != kNone ? col[(uint64_t)u * 2 * row_words] : 0u;
       }
 #pragma unroll
       for (int d = 0; d < 8; d++) {
-        const uint64_t p = q + jb + d;
-        if (p >= q1) break;
+        if (jb + d >= m) break;
+        const uint64_t p = p0 + readlane32(my_pos, (int)(jb + d));
         if (records && p >= next) {
           flush();
           while (pos_off[r + 1] <= p) r++;
@@ -718,15 +740,14 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
 }
 
 void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
-                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *slot,
-                      const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t p1,
-                      uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors,
-                      hipStream_t s) {
-  if (p1 <= p0 || !n_reads || !region_bits) return;
-  const uint64_t waves = ((region_bits + 1023) / 1024) *
-                         ((p1 - p0 + kSweepSumSpan - 1) / kSweepSumSpan);
+                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *pos,
+                      const uint32_t *ids, uint64_t n, const uint64_t *pos_off, uint64_t n_reads,
+                      uint64_t p0, uint32_t records, uint32_t *counts, uint64_t *partial,
+                      uint32_t n_colors, hipStream_t s) {
+  if (!n || !n_reads || !region_bits) return;
+  const uint64_t waves = ((region_bits + 1023) / 1024) * ((n + kSweepSumSpan - 1) / kSweepSumSpan);
   sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
-      hits, row_words, region_off, region_bits, bit_color, slot, pos_off, n_reads, p0, p1,
+      hits, row_words, region_off, region_bits, bit_color, pos, ids, n, pos_off, n_reads, p0,
       records, counts, partial, n_colors);
 }
 
