@@ -29,6 +29,40 @@ def test_library_exports_every_declared_symbol():
     assert b"gfx950" in zindex_amd._lib.lib().zg_version()
 
 
+def test_first_hash_inverse(tmp_path):
+    """The sorted lookup lists each seed as its first PositionHash word
+    h = mixsplit(key, seed) (storage.rs:3458-3465) and recovers the key as
+    unsplitmix64(h) - seed (zg_hash.h). splitmix64 is a bijection of 64-bit
+    words, so the round trip holds both ways for any key and seed, and the
+    multipliers' inverses are exact modulo 2^64."""
+    from zindex_amd.buildlib import HIPCC
+    src = tmp_path / "inverse.cpp"
+    src.write_text(r"""
+#include "zg_hash.h"
+#include <cstdio>
+#include <random>
+int main() {
+  std::mt19937_64 rng(11);
+  long bad = 0;
+  for (int i = 0; i < 200000; i++) {
+    const uint64_t k = rng(), s = rng();
+    bad += zg::unsplitmix64(zg::mixsplit(k, s)) - s != k;
+    bad += zg::mixsplit(zg::unsplitmix64(k) - s, s) != k;
+  }
+  for (uint64_t k : {0ULL, 1ULL, ~0ULL}) bad += zg::unsplitmix64(zg::splitmix64(k)) != k;
+  bad += zg::inverse_odd(0x94D049BB133111EBULL) * 0x94D049BB133111EBULL != 1;
+  bad += zg::inverse_odd(0xBF58476D1CE4E5B9ULL) * 0xBF58476D1CE4E5B9ULL != 1;
+  printf("%ld\n", bad);
+  return bad != 0;
+}
+""")
+    exe = tmp_path / "inverse"
+    subprocess.run([HIPCC, "-O1", "-I", str(ROOT / "zindex_amd" / "csrc"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr
+
+
 def _random_genome(rng, n):
     return synth.random_genome(rng, n).tobytes()
 
