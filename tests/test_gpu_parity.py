@@ -405,6 +405,15 @@ def test_plain_path_graph_replays_and_autotune_agree(base_case):
         assert np.array_equal(counts.astype(np.uint64), exp_counts), f"autotuned {chosen}"
         st = b.stats()
         assert st["seeds"] == int(exp_totals.sum()) and st["last_lookup_ms"] > 0
+        # the batch alternates between the sorted sweep and the read-ordered lookup
+        # (whose runs free the sweep's buffers)
+        tuned = idx.tuning
+        for sweep in (1, 0, 1, 0):
+            idx.tuning = {**tuned, "sweep": sweep}
+            b.run()
+            _, counts, _ = b.results()
+            assert np.array_equal(counts.astype(np.uint64), exp_counts), f"sweep {sweep}"
+        idx.tuning = tuned
 
 
 def test_aggregate_batch_and_stats(base_case):

@@ -1122,6 +1122,16 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
   }
 }
 
+// Frees the sorted sweep's chunk buffers when a batch runs the read-ordered
+// lookup (the autotuner may have tried the sweep on it). Findere keeps its
+// hit matrix.
+static void release_sweep(const zg_index *idx, zg_batch *b) {
+  for (DevBuf *d : {&b->sw_hash, &b->sw_pos, &b->sw_hash2, &b->sw_pos2, &b->sw_temp,
+                    &b->sw_flag, &b->sw_off, &b->sw_rep, &b->sw_slot, &b->sw_queue})
+    d->reset();
+  if (idx->m.index_mode != 3) b->fd_hits.reset();
+}
+
 static void drop_graph(zg_batch *b) {
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1420,6 +1430,7 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
       b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
       return 0;
     }
+    zg::release_sweep(idx, b);
     // banks streamed per query cannot sit in a graph: launch directly
     if (any_streamed(idx)) zg::drop_graph(b);
     if (!b->graph_exec && !b->graph_failed && !any_streamed(idx)) {
