@@ -1058,8 +1058,11 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
     if (timing) b->sweep_lookup_ms += elapsed_ms(b->ev[4], b->ev[5]);
     timing = false;
     if (!n) continue;
-    const size_t st = sort_pairs_temp_bytes(n);
-    b->sw_temp.reserve(st);
+    // the stream is idle here, so growing a buffer cannot free one in use
+    const size_t st = sort_pairs_temp_bytes(n), st2 = sort_pairs_u32_temp_bytes(n);
+    const size_t tb2 = scan_temp_bytes_u32_to_u64(n + 1);
+    b->sw_temp.reserve(std::max(st, st2));
+    b->scan_temp.reserve(tb2);
     launch_sweep_prep(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), P0, P1, (uint32_t)m.k,
                       idx->spaced, m.seed, sampling, nullptr, b->rs_off.as<uint64_t>(),
                       b->sw_hash.as<uint32_t>(), b->sw_pos.as<uint64_t>(), s);
@@ -1068,7 +1071,7 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
     launch_sweep_heads(b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
                        b->sw_flag.as<uint32_t>(), s);
     launch_exclusive_scan_u32_to_u64(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(), n + 1,
-                                     b->scan_temp.p, scan_temp_bytes_u32_to_u64(n + 1), s);
+                                     b->scan_temp.p, tb2, s);
     // (position, seed id) per entry, sorted back to position order: sequential
     // traffic where a scatter by position would write 4 bytes at random
     launch_sweep_assign(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(),
@@ -1077,8 +1080,6 @@ static void run_sweep(zg_index *idx, zg_batch *b) {
                         b->sw_slot.as<uint32_t>(), s);
     int pos_bits = 1;
     while (pos_bits < 32 && (1ULL << pos_bits) < n_pos) pos_bits++;
-    const size_t st2 = sort_pairs_u32_temp_bytes(n);
-    b->sw_temp.reserve(st2);
     launch_sort_pairs_u32(b->sw_temp.p, st2, b->sw_hash.as<uint32_t>(), b->sw_hash2.as<uint32_t>(),
                           b->sw_slot.as<uint32_t>(), b->sw_flag.as<uint32_t>(), n, pos_bits, s);
     uint64_t distinct = 0;
