@@ -504,8 +504,8 @@ def serve_main(args, zindex_amd, synth) -> int:
 def main() -> int:
     args = parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "the query runs on one GPU; --gpus must be 1 (sharding "
+                                   "banks or reads over several GPUs is designed, not built)"}))
         return 2
     import zindex_amd
     from zindex_amd import synth
