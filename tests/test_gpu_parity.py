@@ -194,6 +194,19 @@ def test_sorted_sweep_parity(bits, group, scale, mode, hpc, pattern, monkeypatch
     check_parity(idx, orc, reads, label=f"sweep chunks b{bits} g{group} m{mode}")
 
 
+def test_sweep_pieces_against_cpu():
+    # the sweep's sort, run heads, distinct-seed ids, sort back to position
+    # order and per-read row sums, each against its CPU restatement
+    exe = ROOT / "tests" / "sweep_pieces"
+    assert exe.exists(), "tests/sweep_pieces is built by __graft_entry__.build()"
+    say("sweep pieces: 100000 entries over 120000 positions")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    say(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "sort mismatches 0" in r.stdout and "id mismatches 0" in r.stdout
+    assert "count mismatches 0, total mismatches 0" in r.stdout
+
+
 def test_minimizer_positions_and_counts():
     colors = make_colors(80, 5, 30000)
     reads = make_reads(81, colors, 120, lo=30, hi=3000, error=0.05)
