@@ -1,0 +1,102 @@
+"""GPU parity at the shapes the benchmark configurations run: recovery keys
+with more postings than one wave holds, the sorted sweep at the batched
+query's launch shape over 1024+ colours (one chunk and many), and the small
+plain-method configuration at its stated size. Every case is bit-exact
+against the CPU oracle on per-read totals, per-read per-colour counts and the
+aggregate."""
+import numpy as np
+import pytest
+
+import zindex_ref as zr
+
+zindex_amd = pytest.importorskip("zindex_amd")
+from zindex_amd import synth  # noqa: E402
+from test_gpu_parity import build, check_parity, make_colors, make_reads, oracle_for, say  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+# the batched query's autotuned shape: sorted sweep, 64-seed spans, one tile,
+# one seed in flight, 20 waves per CU
+SWEEP_BENCH = {"span": 64, "tiles_per_wave": 1, "prefetch": 1, "waves_per_cu": 20,
+               "nontemporal": 0, "sweep": 1}
+READ_ORDER = {"span": 16384, "tiles_per_wave": 1, "prefetch": 2, "waves_per_cu": 0,
+              "nontemporal": 1, "sweep": 0}
+
+
+def postings_of_query_keys(idx, reads, k=31):
+    """Largest posting count among the recovery entries the reads' k-mers hit."""
+    hashes, offs, _ = idx.recovery()
+    count = dict(zip(hashes.tolist(), np.diff(offs).tolist()))
+    seed = idx.manifest()["seed"]
+    o = zr.oracle()
+    best = 0
+    for r in reads:
+        for key in zr.scan_kmers(r, k).tolist():
+            best = max(best, count.get(o.zo_abandoned_hash(key, seed), 0))
+    return best
+
+
+@pytest.mark.parametrize("group", [1, 4])
+def test_recovery_keys_with_many_postings(group, monkeypatch):
+    # 160 near-identical colours at load 2 (slot_scale 0.5): k-mers shared by
+    # every colour stay in the 2-core of every column, so their recovery
+    # entries list well over 64 colours and the lookup walks their postings
+    # in several passes of 64 lanes (storage.rs:2771-2787, 2916-2924)
+    colors = make_colors(300 + group, 160, 5000, rates=(0.0, 0.003))
+    reads = make_reads(301 + group, colors, 120, lo=20, hi=2000)
+    idx = build(colors, fingerprint_bits=8, hashes=4, slot_scale=0.5, merge_group_size=group)
+    most = postings_of_query_keys(idx, reads)
+    say(f"group {group}: largest posting list hit by the reads: {most}")
+    assert most > 128
+    idx.load(recover=True)
+    orc = oracle_for(idx, True)
+    for name, tuning in (("read order", READ_ORDER), ("sweep", SWEEP_BENCH)):
+        idx.tuning = tuning
+        check_parity(idx, orc, reads, True, label=f"postings>128 g{group} {name}")
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "20000")
+    check_parity(idx, orc, reads, True, label=f"postings>128 g{group} sweep chunks")
+
+
+@pytest.mark.parametrize("n_colors", [1024, 1100])
+def test_sorted_sweep_at_bench_shape(n_colors, monkeypatch):
+    # 1024 singleton colours (one 1 KiB tile per row) and 1100 (two tiles)
+    rng = np.random.default_rng(400 + n_colors)
+    root = synth.random_genome(rng, 4000)
+    colors = [synth.substitute(rng, root, float(rng.uniform(0.005, 0.05))).tobytes()
+              for _ in range(n_colors)]
+    reads = make_reads(401, colors, 300, lo=30, hi=3000)
+    idx = build(colors, fingerprint_bits=8, hashes=4)
+    idx.load()
+    orc = oracle_for(idx)
+    idx.tuning = SWEEP_BENCH
+    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours")
+    # many chunks: each a few reads, chunk starts far from position 0
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "7000")
+    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours, chunks")
+
+
+def test_plain_method_small_at_stated_size():
+    # 10 colours x 2 Mbp: colour 0 uniform i.i.d. ACGT, colours 1-9 from it at
+    # U[1%, 20%] substitutions; k=31 H=4 b=8. Query: 64 fragments of 1 kbp
+    # from random colours and offsets, half reverse-complemented, U[0, 5%]
+    # substitutions each
+    rng = np.random.default_rng(2024)
+    root = synth.random_genome(rng, 2_000_000)
+    colors = [root] + [synth.substitute(rng, root, float(rng.uniform(0.01, 0.20)))
+                       for _ in range(9)]
+    reads = []
+    for _ in range(64):
+        c = colors[int(rng.integers(0, 10))]
+        start = int(rng.integers(0, len(c) - 1000 + 1))
+        piece = synth.substitute(rng, c[start:start + 1000], float(rng.uniform(0.0, 0.05)))
+        if rng.random() < 0.5:
+            piece = synth.reverse_complement(piece)
+        reads.append(piece.tobytes())
+    idx = build([c.tobytes() for c in colors], fingerprint_bits=8, hashes=4)
+    idx.load()
+    orc = oracle_for(idx)
+    for name, tuning in (("default", idx.tuning), ("read order", READ_ORDER),
+                         ("sweep", SWEEP_BENCH)):
+        idx.tuning = tuning
+        counts = check_parity(idx, orc, reads, label=f"10 x 2 Mbp, 64 x 1 kbp, {name}")
+    assert counts.sum() > 0
