@@ -275,12 +275,18 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
     payloads = [idx.bank_payload(i, copy=False) for i in range(len(man["banks"]))]
     orc = zr.OracleIndex(man, payloads, idx.recovery())
     threads = args.cpu_threads or zr.cpu_count()
-    # parity on the first reads (per-read per-colour counts, bit-exact)
-    n_par = min(args.parity_reads, len(off) - 1)
-    exp_totals, exp_counts = orc.query_reads(buf[: int(off[n_par])], off[: n_par + 1], threads)
+    # parity (per-read per-colour counts, bit-exact) on reads drawn across the
+    # whole batch, so every chunk of a chunked run is checked: evenly spaced
+    # reads plus the longest ones
+    sel = parity_sample(off, args.parity_reads)
+    lens = np.diff(off.astype(np.int64))[sel]
+    soff = np.zeros(len(sel) + 1, np.uint64)
+    soff[1:] = np.cumsum(lens)
+    sbuf = np.concatenate([buf[int(off[i]):int(off[i + 1])] for i in sel]) if len(sel) else buf[:0]
+    exp_totals, exp_counts = orc.query_reads(sbuf, soff, threads)
     parity = None
     if gpu_counts is not None:
-        parity = bool(np.array_equal(gpu_counts[:n_par].astype(np.uint64), exp_counts))
+        parity = bool(np.array_equal(gpu_counts[sel].astype(np.uint64), exp_counts))
     # calibrate then time the lookup phase on a sample taking ~cpu_seconds
     def keys_for(n):
         parts, koff = [], np.zeros(n + 1, np.uint64)
@@ -315,7 +321,29 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
         "sample": f"lookup phase of the CPU oracle (oracle/oracle.c, sorted keys with "
                   f"duplicate folding, 64-bit XOR screens) over the first {n} reads "
                   f"({nk} k-mers) of the same batch and index, {dt:.1f}s, keys pre-extracted",
-    }, parity, n_par
+    }, parity, sel
+
+
+def parity_stride(off, n_reads):
+    return max(1, (len(off) - 1) // max(1, n_reads - 20))
+
+
+def parity_sample(off, n_reads):
+    """Indices of the parity reads: every stride-th read from 0 (about
+    n_reads - 20 of them) plus the 20 longest, sorted."""
+    n = len(off) - 1
+    if n <= 0 or n_reads <= 0:
+        return np.zeros(0, np.int64)
+    lens = np.diff(off.astype(np.int64))
+    return np.unique(np.concatenate([np.arange(0, n, parity_stride(off, n_reads)),
+                                     np.argsort(lens, kind="stable")[-20:]]))
+
+
+def sample_record(sel, off, n_reads, parity):
+    stride = parity_stride(off, n_reads)
+    longest = sorted(int(i) for i in sel if i % stride)
+    return {"reads": int(len(sel)), "bit_exact": parity,
+            "indices": {"every": stride, "from": 0, "plus": longest}}
 
 
 def gather_ceiling(idx, algo_bytes):
@@ -649,11 +677,11 @@ def main() -> int:
                  "plain_equals_batched": same}
         log(f"plain path: {plain}")
 
-    cpu, parity, n_par = (None, None, 0)
+    cpu, parity, sel = (None, None, np.zeros(0, np.int64))
     if not args.no_cpu:
-        cpu, parity, n_par = cpu_baseline(args, idx, buf, off, counts)
+        cpu, parity, sel = cpu_baseline(args, idx, buf, off, counts)
         log(f"cpu baseline: {cpu['value']:.3e} k-mers/s on {cpu['cores']} threads; "
-            f"parity on {n_par} reads: {parity}")
+            f"parity on {len(sel)} reads across the batch: {parity}")
 
     out = {
         "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
@@ -682,7 +710,8 @@ def main() -> int:
         "plain_path": plain,
         "speedup_vs_plain": (plain["batched_kmers_per_s_same_reads"] / plain["plain_kmers_per_s"])
         if plain else None,
-        "parity_sample": {"reads": n_par, "bit_exact": parity} if parity is not None else None,
+        "parity_sample": sample_record(sel, off, args.parity_reads, parity)
+        if parity is not None else None,
         "load": load,
     }
     print(json.dumps(out))
