@@ -612,7 +612,7 @@ def main() -> int:
         # distinct seeds (filter rows + the 8-byte first hash of each seed)
         launches = max(1, st["sweep_launches"])
         per_lookup = st["algorithmic_bytes"] // max(1, seeds)
-        algo = st["distinct_seeds"] * (per_lookup + 8) // launches
+        algo = seeds * (per_lookup + 8) // launches
         kernel_ms = st["sweep_lookup_ms"] / launches
     elif args.mode == "kmers":  # packed bases and invalid bits read by the lookup
         algo = st["algorithmic_bytes"] + positions * 3 // 8
@@ -627,7 +627,7 @@ def main() -> int:
                 "algorithmic_bytes_per_launch": algo}
     if sweep:
         roofline.update({"sorted_sweep": True, "launches_per_step": launches,
-                         "seeds_per_step": seeds, "distinct_seeds_per_step": st["distinct_seeds"],
+                         "seeds_per_step": seeds, "folded_per_step": st["sweep_folded"],
                          "sweep_stage_ms": round(lk_ms, 3)})
     # the ceiling of the row gathers alone: random whole rows of the bank's row
     # size, the same bytes per launch, no other work

@@ -252,9 +252,10 @@ typedef struct zg_tuning {
   uint32_t waves_per_cu;   /* persistent waves per compute unit (0 = max) */
   uint32_t nontemporal;    /* 1: row gathers use non-temporal loads */
   uint32_t sweep;          /* 1: seeds of a single-bank k-mer index are
-                              looked up once per distinct seed in PositionHash
-                              order (a radix sort per chunk of reads), each
-                              seed's hits into a hit row, the rows summed per
+                              looked up in PositionHash segment order (a
+                              radix sort per chunk of reads) from segment
+                              windows staged in LDS, each seed's hits into a
+                              hit row at its position, the rows summed per
                               read; same results as 0 */
 } zg_tuning;
 
@@ -268,10 +269,13 @@ typedef struct zg_batch_stats {
   double last_seed_ms;       /* HIP-event time of packing + seed extraction */
   double last_total_ms;      /* HIP-event time of the whole last run */
   zg_tuning tuning;
-  uint64_t distinct_seeds;   /* seeds the last sorted-sweep run looked up (its
-                                distinct seeds per chunk); 0 for other runs */
-  uint64_t sweep_launches;   /* its lookup launches (one per chunk of reads) */
-  double sweep_lookup_ms;    /* their summed HIP-event time */
+  uint64_t sweep_folded;     /* last sorted-sweep run: seed occurrences answered
+                                from the occurrence before them in bank order
+                                (equal keys, the per-read fold); 0 otherwise */
+  uint64_t sweep_launches;   /* its chunks (one staged lookup launch each) */
+  double sweep_lookup_ms;    /* HIP-event time of its staged lookups */
+  double sweep_sort_ms;      /* ... of its radix passes and segment bounds */
+  double sweep_sum_ms;       /* ... of its per-read sums */
 } zg_batch_stats;
 
 /* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base

@@ -65,18 +65,40 @@ def oracle_for(idx, recover=True):
     return zr.OracleIndex(disk["manifest"], disk["banks"], disk["recovery"], recover)
 
 
-def check_parity(idx, orc, reads, recover=True, label=""):
+def check_parity(idx, orc, reads, recover=True, label="", sweep=True):
+    """Records and aggregate against the oracle with the index's tuning and,
+    with sweep=True, again as the sorted sweep (where the index allows it)."""
     buf, off = zindex_amd.pack_reads(reads)
     say(f"parity {label}: {len(reads)} reads, {int(off[-1])} bases")
     exp_totals, exp_counts = orc.query_reads(buf, off, threads=zr.cpu_count())
-    totals, counts, agg = idx.query_reads(buf, off, records=True, recover=recover)
-    assert np.array_equal(totals, exp_totals), label
-    bad = np.argwhere(counts.astype(np.uint64) != exp_counts)
-    assert len(bad) == 0, f"{label}: {len(bad)} mismatches, first {bad[:5].tolist()}"
-    assert np.array_equal(agg, exp_counts.sum(axis=0)), label
-    t2, _, agg2 = idx.query_reads(buf, off, records=False, recover=recover)
-    assert np.array_equal(t2, exp_totals) and np.array_equal(agg2, exp_counts.sum(axis=0)), label
+    saved = idx.tuning
+    tunings = [("", saved)]
+    if sweep and not saved.get("sweep"):
+        tunings.append((" [sweep]", dict(saved, sweep=1)))
+    for tag, tuning in tunings:
+        idx.tuning = tuning
+        totals, counts, agg = idx.query_reads(buf, off, records=True, recover=recover)
+        assert np.array_equal(totals, exp_totals), label + tag
+        bad = np.argwhere(counts.astype(np.uint64) != exp_counts)
+        assert len(bad) == 0, f"{label}{tag}: {len(bad)} mismatches, first {bad[:5].tolist()}"
+        assert np.array_equal(agg, exp_counts.sum(axis=0)), label + tag
+        t2, _, agg2 = idx.query_reads(buf, off, records=False, recover=recover)
+        assert np.array_equal(t2, exp_totals) and np.array_equal(agg2, exp_counts.sum(axis=0)), \
+            label + tag
+    idx.tuning = saved
     return exp_counts
+
+
+def assert_sweep_runs(idx, reads):
+    """The batch runs as the sorted sweep (not the read-ordered lookup)."""
+    buf, off = zindex_amd.pack_reads(reads)
+    saved = idx.tuning
+    idx.tuning = dict(saved, sweep=1)
+    with idx.batch(buf, off, records=True) as b:
+        b.run()
+        st = b.stats()
+    idx.tuning = saved
+    assert st["tuning"]["sweep"] == 1 and st["sweep_launches"] >= 1, st
 
 
 @pytest.fixture(scope="module")
@@ -174,6 +196,7 @@ SWEEP = {"span": 1024, "tiles_per_wave": 1, "prefetch": 2, "waves_per_cu": 0,
          "nontemporal": 0, "sweep": 1}
 
 
+
 @pytest.mark.parametrize("bits,group,scale,mode,hpc,pattern", [
     (8, 1, 1.0, 0, False, None), (8, 3, 0.5, 0, False, None), (4, 2, 0.5, 2, False, None),
     (16, 1, 1.0, 0, True, "###_##_#_##_#_###_#_##_#_##_###"), (1, 4, 0.6, 0, False, None)])
@@ -187,7 +210,7 @@ def test_sorted_sweep_parity(bits, group, scale, mode, hpc, pattern, monkeypatch
     idx.load()
     orc = oracle_for(idx)
     idx.tuning = SWEEP
-    assert idx.tuning["sweep"] == 1
+    assert_sweep_runs(idx, reads)
     check_parity(idx, orc, reads, label=f"sweep b{bits} g{group} m{mode}")
     # several chunks of whole reads
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "5000")
@@ -195,16 +218,15 @@ def test_sorted_sweep_parity(bits, group, scale, mode, hpc, pattern, monkeypatch
 
 
 def test_sweep_pieces_against_cpu():
-    # the sweep's sort, run heads, distinct-seed ids, sort back to position
-    # order and per-read row sums, each against its CPU restatement
+    # the sweep's scans, stable radix passes by first segment, segment starts
+    # and per-read row sums, each against its CPU restatement
     exe = ROOT / "tests" / "sweep_pieces"
     assert exe.exists(), "tests/sweep_pieces is built by __graft_entry__.build()"
-    say("sweep pieces: 100000 entries over 120000 positions")
+    say("sweep pieces: scans, radix passes, segment starts, row sums")
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     say(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "sort mismatches 0" in r.stdout and "id mismatches 0" in r.stdout
-    assert "count mismatches 0, total mismatches 0" in r.stdout
+    assert "all pieces match" in r.stdout
 
 
 def test_minimizer_positions_and_counts():
@@ -219,6 +241,7 @@ def test_minimizer_positions_and_counts():
         a, b = int(key_off[i]), int(key_off[i + 1])
         assert pos[a:b].tolist() == exp_pos.tolist(), i
         assert keys[a:b].tolist() == exp_keys.tolist(), i
+    assert_sweep_runs(idx, reads)
     check_parity(idx, oracle_for(idx), reads, label="ocm k49 m31 b16 h3")
 
 
@@ -359,6 +382,7 @@ def test_randstrobe_parity(mode, pattern, bounds, hpc, bits):
                                strobes=strobes)
         assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
     label = f"randstrobes mode{mode} {pattern} {strobes} hpc{hpc} b{bits}"
+    assert_sweep_runs(idx, reads)
     assert check_parity(idx, oracle_for(idx), reads, label=label).sum() > 0
     # the count is the larger of the two orientations, so a read and its
     # reverse complement score the same
@@ -382,6 +406,7 @@ def test_long_kmer_parity(k, mode, hpc):
         exp = zr.read_features(r, k, 69, mode=mode, sampling=4, hpc=hpc)
         assert keys[int(key_off[i]):int(key_off[i + 1])].tolist() == exp.tolist(), i
     label = f"long k{k} mode{mode} hpc{hpc}"
+    assert_sweep_runs(idx, reads)
     assert check_parity(idx, oracle_for(idx), reads, label=label).sum() > 0
     # a source and its reverse complement match every one of their k-mers
     src = colors[0][1000:4000]
@@ -534,6 +559,7 @@ def test_hash_partitioned_index(group, cap, device):
     assert man["version"] == 9 and idx.meta["n_banks"] > 1
     assert all(b.get("routing") for b in man["banks"])
     idx.load()
+    assert_sweep_runs(idx, reads)
     check_parity(idx, oracle_for(idx), reads, label=f"partitioned g{group} cap {cap}")
 
 
@@ -667,6 +693,7 @@ def test_fused_index_components(tmp_path):
     with zindex_amd.Index.open(str(out)) as z:
         assert z.meta["version"] == 10
         z.load()
+        assert_sweep_runs(z, reads)
         check_parity(z, orc, reads, label="fused")
         t, c, _ = z.query_reads(reads, records=True)
         assert np.array_equal(c, np.concatenate([a[1] for a in alone], axis=1))

@@ -11,7 +11,8 @@ import zindex_ref as zr
 
 zindex_amd = pytest.importorskip("zindex_amd")
 from zindex_amd import synth  # noqa: E402
-from test_gpu_parity import build, check_parity, make_colors, make_reads, oracle_for, say  # noqa: E402
+from test_gpu_parity import (assert_sweep_runs, build, check_parity, make_colors,  # noqa: E402
+                             make_reads, oracle_for, say)
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +54,7 @@ def test_recovery_keys_with_many_postings(group, monkeypatch):
     for name, tuning in (("read order", READ_ORDER), ("sweep", SWEEP_BENCH)):
         idx.tuning = tuning
         check_parity(idx, orc, reads, True, label=f"postings>128 g{group} {name}")
+    assert_sweep_runs(idx, reads)
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "20000")
     check_parity(idx, orc, reads, True, label=f"postings>128 g{group} sweep chunks")
 
@@ -69,6 +71,7 @@ def test_sorted_sweep_at_bench_shape(n_colors, monkeypatch):
     idx.load()
     orc = oracle_for(idx)
     idx.tuning = SWEEP_BENCH
+    assert_sweep_runs(idx, reads)
     check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours")
     # many chunks: each a few reads, chunk starts far from position 0
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "7000")

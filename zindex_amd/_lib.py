@@ -68,8 +68,9 @@ class BatchStats(C.Structure):
         ("n_reads", C.c_uint64), ("bases", C.c_uint64), ("seeds", C.c_uint64),
         ("lookups", C.c_uint64), ("algorithmic_bytes", C.c_uint64),
         ("last_lookup_ms", C.c_double), ("last_seed_ms", C.c_double),
-        ("last_total_ms", C.c_double), ("tuning", Tuning), ("distinct_seeds", C.c_uint64),
+        ("last_total_ms", C.c_double), ("tuning", Tuning), ("sweep_folded", C.c_uint64),
         ("sweep_launches", C.c_uint64), ("sweep_lookup_ms", C.c_double),
+        ("sweep_sort_ms", C.c_double), ("sweep_sum_ms", C.c_double),
     ]
 
 

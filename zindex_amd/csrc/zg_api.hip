@@ -90,13 +90,6 @@ namespace zg {
 static unsigned host_threads() { return zg_host_threads(); }
 
 // Query support of this library for an index (the modes BASELINE targets).
-// Indexes the sorted sweep serves: one bank (no size classes, partitions or
-// components) of k-mer or modimizer seeds of at most 32 bases.
-static bool sweep_layout(const Manifest &m) {
-  return m.banks.size() == 1 && !m.banks[0].routed && (m.index_mode == 0 || m.index_mode == 2) &&
-         m.k <= 32 && !m.has_randstrobemers;
-}
-
 static void check_query_support(const Manifest &m) {
   if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32 && m.has_randstrobemers)
@@ -435,7 +428,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       home = db.data.as<uint8_t>();
       put_bank(idx, bi, home, s);
     }
-    if (m.index_mode == 3 || sweep_layout(m)) {
+    if (m.index_mode == 3) {
       // hit-row bits: device column for singletons, then one bit per member
       const uint32_t tile_bits = 64 * fields_per_lane(fb);
       const uint32_t member_bit = ((G + tile_bits - 1) / tile_bits) * tile_bits;
@@ -494,7 +487,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
   idx->fd_regions.clear();
   idx->fd_bank_region.assign(nb, 0);
   idx->fd_row_words = 0;
-  if (m.index_mode == 3 || sweep_layout(m)) {
+  if (m.index_mode == 3) {
     std::vector<size_t> first;  // a bank of each region
     std::vector<uint32_t> bit_color;
     for (size_t bi = 0; bi < nb; bi++) {
@@ -603,6 +596,7 @@ zg_batch::~zg_batch() {
   if (graph) (void)hipGraphDestroy(graph);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : sw_ev) (void)hipEventDestroy(e);
 }
 
 extern "C" const char *zg_last_error(void) { return g_error.c_str(); }
@@ -969,166 +963,17 @@ static void enqueue_reduce(zg_index *idx, zg_batch *b) {
     launch_colsum(b->counts.as<uint32_t>(), b->n_reads, C, b->agg.as<uint64_t>(), s);
 }
 
-// Hit-matrix bytes of one sorted-sweep chunk (the reads of a chunk take at
-// most this many positions, or one read alone when it is longer).
-constexpr uint64_t kSweepHitBytes = 64ULL << 30;
-constexpr uint32_t kSweepSlices = 8;  // gfx950 XCDs
-
-double elapsed_ms(hipEvent_t a, hipEvent_t b);
-
 static bool sweep_on(const zg_index *idx, const zg_batch *b) {
-  return idx->tuning.sweep && sweep_layout(idx->m) && !b->explicit_keys && !any_streamed(idx) &&
-         !idx->fd_regions.empty();
-}
-
-// The lookups of a batch as a sorted sweep over its distinct seeds. Per
-// chunk of whole reads: the valid windows as (first hash, position) pairs, a
-// radix sort on the hash's high half (PositionHash order), one lookup per
-// run of equal hashes writing its hit row, and the sum of each read's rows
-// into its counts. A seed whose entries interleave with another seed of the
-// same high half forms several runs and is looked up once per run, which
-// changes no count.
-static void run_sweep(zg_index *idx, zg_batch *b) {
-  hipStream_t s = b->stream;
-  const Manifest &m = idx->m;
-  const zg_index::FdRegion &rg = idx->fd_regions[0];
-  const uint64_t row_words = idx->fd_row_words;
-  uint64_t cap = std::max<uint64_t>(1 << 20, kSweepHitBytes / (row_words * 4));
-  // a chunk's buffers (a hit row and ~64 B of list entries per position) take at
-  // most 3/4 of the device memory that is free or already held by them
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-    uint64_t held = b->fd_hits.bytes + b->sw_temp.bytes + b->scan_temp.bytes;
-    for (const DevBuf *d : {&b->sw_hash, &b->sw_pos, &b->sw_hash2, &b->sw_pos2, &b->sw_flag,
-                            &b->sw_off, &b->sw_rep, &b->sw_slot})
-      held += d->bytes;
-    cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, (free_b + held) / 4 * 3 /
-                                                             (row_words * 4 + 64)));
-  } else {
-    (void)hipGetLastError();
-  }
-  if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // smaller chunks
-    cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, strtoull(e, nullptr, 10)));
-  b->sweep_distinct = 0;
-  b->sweep_chunks = 0;
-  b->sweep_lookup_ms = 0;
-  bool timing = false;  // a lookup bracketed by ev[4], ev[5] awaits its time
-  const std::vector<uint64_t> &poff = b->h_seg_off;  // position offsets of k-mer batches
-  // chunks of whole reads, then every buffer at the largest chunk's size
-  std::vector<uint64_t> cuts{0};
-  uint64_t most = 0;
-  while (cuts.back() < b->n_reads) {
-    const uint64_t r0 = cuts.back();
-    uint64_t r1 = r0 + 1;
-    while (r1 < b->n_reads && poff[r1 + 1] - poff[r0] <= cap) r1++;
-    most = std::max(most, poff[r1] - poff[r0]);
-    cuts.push_back(r1);
-  }
-  const uint64_t most_words = (most + 63) / 64;
-  b->rs_counts.reserve((most_words + 1) * 4);
-  b->rs_off.reserve((most_words + 1) * 8);
-  b->scan_temp.reserve(std::max(scan_temp_bytes_u32_to_u64(most_words + 1),
-                                scan_temp_bytes_u32_to_u64(most + 1)));
-  b->sw_hash.reserve(most * 4 + 4);
-  b->sw_pos.reserve(most * 8 + 8);
-  b->sw_hash2.reserve(most * 4 + 4);
-  b->sw_pos2.reserve(most * 8 + 8);
-  b->sw_flag.reserve(most * 4 + 4);
-  b->sw_off.reserve(most * 8 + 8);
-  b->sw_rep.reserve(most * 8 + 8);
-  b->sw_slot.reserve(most * 4 + 4);
-  b->sw_temp.reserve(std::max(sort_pairs_temp_bytes(most), sort_pairs_u32_temp_bytes(most)));
-  b->fd_hits.reserve(most * row_words * 4 + 4);
-  b->sw_queue.reserve(32 * kSweepSlices * 4);
-  for (size_t ci = 0; ci + 1 < cuts.size(); ci++) {
-    const uint64_t P0 = poff[cuts[ci]], P1 = poff[cuts[ci + 1]], n_pos = P1 - P0;
-    const uint64_t words = (n_pos + 63) / 64;
-    const size_t tb = scan_temp_bytes_u32_to_u64(words + 1);
-    const uint64_t sampling = m.index_mode == 2 ? m.modimizer_sampling : 0;
-    launch_sweep_prep(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), P0, P1, (uint32_t)m.k,
-                      idx->spaced, m.seed, sampling, b->rs_counts.as<uint32_t>(), nullptr,
-                      nullptr, nullptr, s);
-    launch_fill_u32(b->rs_counts.as<uint32_t>() + words, 1, 0, s);
-    launch_exclusive_scan_u32_to_u64(b->rs_counts.as<uint32_t>(), b->rs_off.as<uint64_t>(),
-                                     words + 1, b->scan_temp.p, tb, s);
-    uint64_t n = 0;
-    check_hip(hipMemcpyAsync(&n, b->rs_off.as<uint64_t>() + words, 8, hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync");
-    check_hip(hipStreamSynchronize(s), "sweep");
-    if (timing) b->sweep_lookup_ms += elapsed_ms(b->ev[4], b->ev[5]);
-    timing = false;
-    if (!n) continue;
-    // the stream is idle here, so growing a buffer cannot free one in use
-    const size_t st = sort_pairs_temp_bytes(n), st2 = sort_pairs_u32_temp_bytes(n);
-    const size_t tb2 = scan_temp_bytes_u32_to_u64(n + 1);
-    b->sw_temp.reserve(std::max(st, st2));
-    b->scan_temp.reserve(tb2);
-    launch_sweep_prep(b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), P0, P1, (uint32_t)m.k,
-                      idx->spaced, m.seed, sampling, nullptr, b->rs_off.as<uint64_t>(),
-                      b->sw_hash.as<uint32_t>(), b->sw_pos.as<uint64_t>(), s);
-    launch_sort_pairs(b->sw_temp.p, st, b->sw_hash.as<uint32_t>(), b->sw_hash2.as<uint32_t>(),
-                      b->sw_pos.as<uint64_t>(), b->sw_pos2.as<uint64_t>(), n, s);
-    launch_sweep_heads(b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
-                       b->sw_flag.as<uint32_t>(), s);
-    launch_exclusive_scan_u32_to_u64(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(), n + 1,
-                                     b->scan_temp.p, tb2, s);
-    // (position, seed id) per entry, sorted back to position order: sequential
-    // traffic where a scatter by position would write 4 bytes at random
-    launch_sweep_assign(b->sw_flag.as<uint32_t>(), b->sw_off.as<uint64_t>(),
-                        b->sw_hash2.as<uint32_t>(), b->sw_pos2.as<uint64_t>(), n,
-                        b->sw_rep.as<uint64_t>(), b->sw_hash.as<uint32_t>(),
-                        b->sw_slot.as<uint32_t>(), s);
-    int pos_bits = 1;
-    while (pos_bits < 32 && (1ULL << pos_bits) < n_pos) pos_bits++;
-    launch_sort_pairs_u32(b->sw_temp.p, st2, b->sw_hash.as<uint32_t>(), b->sw_hash2.as<uint32_t>(),
-                          b->sw_slot.as<uint32_t>(), b->sw_flag.as<uint32_t>(), n, pos_bits, s);
-    uint64_t distinct = 0;
-    check_hip(hipMemcpyAsync(&distinct, b->sw_off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost, s),
-              "hipMemcpyAsync");
-    check_hip(hipStreamSynchronize(s), "sweep");
-    b->sweep_distinct += distinct;
-    // every seed's singleton bits are stored whole; member bits are ORed in
-    if (rg.bits > idx->fd_bank_member_bit[0])
-      launch_fill_u32(b->fd_hits.as<uint32_t>(), distinct * row_words, 0, s);
-    LookupArgs a = lookup_args(idx, b, 0);
-    a.source = kSeedSorted;
-    a.sorted = b->sw_rep.as<uint64_t>();
-    a.begin = 0;
-    a.end = distinct;
-    a.n_units = (distinct + a.span - 1) / a.span * a.n_tg;
-    a.records = 0;
-    a.sampling = 0;  // the list holds sampled windows only
-    a.fd_hits = b->fd_hits.as<uint32_t>();
-    a.fd_row_words = row_words;
-    a.fd_off = rg.off;
-    a.fd_member_bit = idx->fd_bank_member_bit[0];
-    // one contiguous range of the sorted list per XCD
-    a.queue = b->sw_queue.as<uint32_t>();
-    a.slices = b->kcfg.blocks >= kSweepSlices ? kSweepSlices : 1;
-    launch_fill_u32(a.queue, 32 * kSweepSlices, 0, s);
-    check_hip(hipEventRecord(b->ev[4], s), "hipEventRecord");
-    launch_lookup(a, b->kcfg, s);
-    check_hip(hipEventRecord(b->ev[5], s), "hipEventRecord");
-    timing = true;
-    b->sweep_chunks++;
-    launch_sweep_sum(b->fd_hits.as<uint32_t>(), row_words, rg.off, rg.bits,
-                     idx->fd_bit_color.as<uint32_t>() + rg.color_at, b->sw_hash2.as<uint32_t>(),
-                     b->sw_flag.as<uint32_t>(), n, b->pos_off.as<uint64_t>(), b->n_reads, P0,
-                     b->records ? 1 : 0, b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
-                     (uint32_t)m.colors.size(), s);
-  }
-  if (timing) {
-    check_hip(hipEventSynchronize(b->ev[5]), "hipEventSynchronize");
-    b->sweep_lookup_ms += elapsed_ms(b->ev[4], b->ev[5]);
-  }
+  (void)b;
+  return idx->tuning.sweep && !any_streamed(idx) && sweep_layout_ok(idx);
 }
 
 // Frees the sorted sweep's chunk buffers when a batch runs the read-ordered
 // lookup (the autotuner may have tried the sweep on it). Findere keeps its
 // hit matrix.
 static void release_sweep(const zg_index *idx, zg_batch *b) {
-  for (DevBuf *d : {&b->sw_hash, &b->sw_pos, &b->sw_hash2, &b->sw_pos2, &b->sw_temp,
-                    &b->sw_flag, &b->sw_off, &b->sw_rep, &b->sw_slot, &b->sw_queue})
+  for (DevBuf *d : {&b->sw_h[0], &b->sw_h[1], &b->sw_p[0], &b->sw_p[1], &b->sw_listed,
+                    &b->sw_hist, &b->sw_off, &b->sw_start, &b->sw_queue})
     d->reset();
   if (idx->m.index_mode != 3) b->fd_hits.reset();
 }
@@ -1422,13 +1267,15 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
       check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
       zg::enqueue_zero_outputs(idx, b);
       check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
-      zg::run_sweep(idx, b);
+      zg::sweep_run(idx, b);
       check_hip(hipEventRecord(b->ev[2], s), "hipEventRecord");
       zg::enqueue_reduce(idx, b);
       check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
       check_hip(hipStreamSynchronize(s), "lookup");
       b->last_lookup_ms = zg::elapsed_ms(b->ev[1], b->ev[2]);
       b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
+      zg::sweep_times(b, b->sweep_sort_ms, b->sweep_lookup_ms, b->sweep_sum_ms);
+      b->sweep_folded = zg::sweep_folded(b);
       return 0;
     }
     zg::release_sweep(idx, b);
@@ -1593,9 +1440,11 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->tuning.waves_per_cu = b->kcfg.blocks * 4 / std::max(1u, idx->cu_count);
     o->tuning.nontemporal = b->nontemporal;
     o->tuning.sweep = zg::sweep_on(idx, b) ? 1 : 0;
-    o->distinct_seeds = o->tuning.sweep ? b->sweep_distinct : 0;
+    o->sweep_folded = o->tuning.sweep ? b->sweep_folded : 0;
     o->sweep_launches = o->tuning.sweep ? b->sweep_chunks : 0;
     o->sweep_lookup_ms = o->tuning.sweep ? b->sweep_lookup_ms : 0;
+    o->sweep_sort_ms = o->tuning.sweep ? b->sweep_sort_ms : 0;
+    o->sweep_sum_ms = o->tuning.sweep ? b->sweep_sum_ms : 0;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -1620,7 +1469,7 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
     if (idx->m.fingerprint_bits <= 4) depths.insert(depths.end(), {8, 16});
     // the sorted sweep, where the index and batch allow it, with shorter spans
     // (neighbouring waves then share rows)
-    const bool can_sweep = zg::sweep_layout(idx->m) && !b->explicit_keys && !any_streamed(idx);
+    const bool can_sweep = !any_streamed(idx) && zg::sweep_layout_ok(idx);
     for (uint32_t sw : {0u, 1u})
     for (uint32_t span : {1024u, 4096u, 16384u})
       for (uint32_t t : tiles)
@@ -1628,10 +1477,12 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
           for (uint32_t frac : {1u, 2u})
             for (uint32_t nt : {0u, 1u}) {
             if (nt && !wide) continue;
-            if (sw && !can_sweep) continue;
+            // the sweep's staged lookup has one launch shape of its own
+            if (sw && (!can_sweep || span != 1024u || t != 1u || d != 1u || frac != 1u || nt))
+              continue;
             zg::Tuning cand;
             cand.sweep = sw;
-            cand.span = sw ? span / 16 : span;
+            cand.span = span;
             cand.tiles_per_wave = t;
             cand.prefetch = d;
             cand.nontemporal = nt;

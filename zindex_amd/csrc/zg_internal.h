@@ -41,6 +41,19 @@ struct DevBuf {
 
 void check_hip(hipError_t e, const char *what);
 
+// A size class of the sorted sweep: the banks (partitions) sharing one
+// geometry and one colour -> column map. One radix sort per chunk serves
+// them all; their hits go to one region of the hit row.
+struct SweepClass {
+  std::vector<uint32_t> banks;
+  uint64_t count = 0;            // first segments of the geometry
+  uint32_t slice_bytes = 0;      // LDS row stride (plane bytes per slice, <= 512)
+  uint32_t n_slices = 0;
+  uint32_t col_word0 = 0;        // first word of its column bits in a row
+  uint32_t member_bit0 = 0;      // first bit of its member bits in a row
+  uint32_t members = 0;
+};
+
 struct DeviceBank {
   DevBuf data, col_color, mg_off, mg_members, mg_mask_bit, color_col, color_bit;
   BankDev view{};
@@ -95,6 +108,11 @@ struct zg_index {
   std::vector<FdRegion> fd_regions;
   std::vector<uint32_t> fd_bank_region, fd_bank_member_bit;
   zg::DevBuf fd_bit_color;
+  // sorted sweep: size classes, hit-row words (column regions, then member
+  // bits) and the colour of every row bit
+  std::vector<zg::SweepClass> sw_classes;
+  uint32_t sw_row_words = 0, sw_col_words = 0, sw_members = 0;
+  zg::DevBuf sw_bit_color;
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();
@@ -115,15 +133,16 @@ struct zg_batch {
   // homopolymer compression: keep flags, counts and offsets per 64 positions
   zg::DevBuf hpc_keep, hpc_counts, hpc_off;
   zg::DevBuf fd_hits;  // findere: positions x fd_row_words hit words
-  // sorted sweep, per chunk: (hash high half, hash low half | position) pairs
-  // before and after the sort, head flags and their scan, distinct seeds'
-  // first hashes, seed ids; the first buffers are reused for the (position,
-  // seed id) pairs and their sort back to position order
-  zg::DevBuf sw_hash, sw_pos, sw_hash2, sw_pos2, sw_temp, sw_flag, sw_off, sw_rep, sw_slot;
-  zg::DevBuf sw_queue;  // per-slice work counters of the sorted lookup
-  uint64_t sweep_distinct = 0;  // distinct seeds looked up by the last sweep
-  uint64_t sweep_chunks = 0;    // its lookup launches (one per chunk)
-  double sweep_lookup_ms = 0;   // their summed HIP-event time
+  // sorted sweep (zg_sweep.hip), per chunk: entries (first hash, position)
+  // of the two radix passes, the valid-window bitmap, tile histograms and
+  // their scan, segment starts, per-XCD work counters (+ the fold count);
+  // hit rows go to fd_hits
+  zg::DevBuf sw_h[2], sw_p[2], sw_listed, sw_hist, sw_off, sw_start, sw_queue;
+  std::vector<hipEvent_t> sw_ev;  // per chunk: sort, lookup and sum bounds
+  uint64_t sweep_chunks = 0;      // chunks of the last sweep
+  uint32_t sweep_classes = 0;     // its size classes (a sort and lookups each per chunk)
+  double sweep_lookup_ms = 0, sweep_sort_ms = 0, sweep_sum_ms = 0;  // their phase times
+  uint64_t sweep_folded = 0;      // entries answered from the entry before them
   // randstrobemers: candidate bitmap, per-word counts and offsets, run-id
   // offsets, candidate hashes and run ids (starts go to key_win); keys hold
   // the forward keys then the reverse-complement keys (rs_forward each)
@@ -164,6 +183,13 @@ void make_resident(zg_index *idx, size_t i, hipStream_t s);
 bool any_streamed(const zg_index *idx);
 // Bank payload in host memory, decoded from the bank file on first use.
 const std::vector<uint8_t> &host_payload(zg_index *idx, size_t bank);
+// The sorted sweep (zg_sweep.hip): whether an index's layout allows it, the
+// lookups of a batch enqueued with no host synchronization, and the phase
+// times and fold count of the last run (after the stream has drained).
+bool sweep_layout_ok(const zg_index *idx);
+void sweep_run(zg_index *idx, zg_batch *b);
+void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &sum_ms);
+uint64_t sweep_folded(zg_batch *b);
 // Batch pipeline pieces, all enqueued on b->stream (zg_api.hip).
 void configure(zg_index *idx, zg_batch *b);
 void seed_batch(zg_index *idx, zg_batch *b);

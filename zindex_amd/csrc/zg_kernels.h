@@ -47,10 +47,7 @@ struct RecDev {
   uint32_t enabled;
 };
 
-// kSeedSorted: seeds are the distinct k-mer windows of a chunk in
-// PositionHash order, listed as their first hash mixsplit(key, seed)
-// (`sorted`); seed i's hits go to hit row i.
-enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1, kSeedSorted = 2 };
+enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1 };
 
 struct LookupArgs {
   BankDev bank;
@@ -69,8 +66,8 @@ struct LookupArgs {
   uint32_t n_tg;              // tile groups per span
   uint32_t *queue;            // work counter(s), zero before launch
   uint32_t slices;            // > 1: units split into that many ranges, one per
-                              // block residue, counters at queue[32 * slice];
-                              // at most the grid's block count
+                              // block residue (one per XCD), counters at
+                              // queue[32 * slice]; at most the grid's block count
   uint64_t seed;
   uint32_t bits;
   uint32_t records;           // 1: per-read counts; 0: per-wave partial sums
@@ -88,8 +85,6 @@ struct LookupArgs {
   uint64_t fd_row_words;
   uint32_t fd_off;
   uint32_t fd_member_bit;
-  // kSeedSorted: first hashes of the listed seeds
-  const uint64_t *sorted;
 };
 
 struct KernelConfig {
@@ -139,39 +134,6 @@ void launch_findere_scan(const uint32_t *hits, uint64_t row_words, uint32_t regi
                          const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
                          uint32_t z, uint32_t records, uint32_t *counts, uint64_t *partial,
                          uint32_t n_colors, hipStream_t s, uint64_t p_base = 0);
-// Sorted sweep: the valid k-mer windows of positions [p0, p1) (modimizers:
-// the sampled ones), keyed by their first PositionHash word h: hi = h >> 32,
-// lo_pos = (h << 32) | (position - p0). Two passes: per-64-position counts,
-// then the entries at the scanned offsets.
-void launch_sweep_prep(const uint64_t *codes, const uint64_t *invalid, uint64_t p0, uint64_t p1,
-                       uint32_t k, const SpacedRuns &spaced, uint64_t seed, uint64_t sampling,
-                       uint32_t *counts, const uint64_t *off, uint32_t *hi, uint64_t *lo_pos,
-                       hipStream_t s);
-// Radix sort of (u32 key, u64 value) pairs on the whole key.
-size_t sort_pairs_temp_bytes(uint64_t n);
-void launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
-                       const uint64_t *vals_in, uint64_t *vals_out, uint64_t n, hipStream_t s);
-// Distinct seeds of the sorted list: head flags (n + 1 entries), then from
-// their exclusive scan each head's first hash first[u], and for every entry
-// its position - p0 and seed id u (pos[i], ids[i], in list order).
-void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
-                        hipStream_t s);
-void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
-                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *pos,
-                         uint32_t *ids, hipStream_t s);
-// Radix sort of (u32 key, u32 value) pairs on the key's low `bits` bits.
-size_t sort_pairs_u32_temp_bytes(uint64_t n);
-void launch_sort_pairs_u32(void *temp, size_t temp_bytes, const uint32_t *keys_in,
-                           uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
-                           uint64_t n, int bits, hipStream_t s);
-// Sum of the hit rows ids[e] of a chunk's n entries (pos[e] = position - p0,
-// ascending) into counts[r * n_colors + colour] of the read r holding the
-// position (records) or partial[colour]; colour bit_color[j] for region bit j.
-void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
-                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *pos,
-                      const uint32_t *ids, uint64_t n, const uint64_t *pos_off, uint64_t n_reads,
-                      uint64_t p0, uint32_t records, uint32_t *counts, uint64_t *partial,
-                      uint32_t n_colors, hipStream_t s);
 // Padding, in words, after the invalid-base bits of `positions` positions so
 // that a window of `window` bases starting at any position stays in bounds.
 inline uint64_t invalid_words(uint64_t positions, uint64_t window) {
@@ -226,7 +188,10 @@ void launch_hpc_place(const uint8_t *seq, const uint64_t *read_off, const uint64
                       const uint64_t *keep_off, uint64_t *codes, uint64_t *invalid,
                       hipStream_t s);
 uint32_t ocm_block_positions();
-// Exclusive prefix sums (hipcub) with a caller-provided temporary buffer.
+// Exclusive prefix sums (zg_scan.hip): block sums of 4096-element tiles,
+// scanned recursively, then each tile scanned from its block offset. The
+// caller provides the temporary buffer (the block-sum levels); in may equal
+// out for the u64 form.
 size_t scan_temp_bytes_u64(uint64_t n);
 void launch_exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *temp,
                                size_t temp_bytes, hipStream_t s);

@@ -12,10 +12,10 @@
 // rows, so one seed costs H coalesced 64*VB-byte reads per tile, XOR and
 // compare stay in registers and per-read colour counters live in VGPRs.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <type_traits>
 
+#include "zg_device.h"
 #include "zg_kernels.h"
 
 namespace zg {
@@ -30,36 +30,6 @@ __device__ __forceinline__ int encode_ascii(uint8_t c) {
     case 'T': case 't': return 3;
     default: return -1;
   }
-}
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-  uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ uint32_t readlane32(uint32_t v, int l) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
-}
-
-// 64 bits of a bit/pair stream starting at bit `bit` (words padded by one).
-__device__ __forceinline__ uint64_t stream_bits(const uint64_t *words, uint64_t bit) {
-  uint64_t w = bit >> 6;
-  uint32_t sh = (uint32_t)(bit & 63);
-  uint64_t lo = words[w];
-  if (!sh) return lo;
-  return (lo >> sh) | (words[w + 1] << (64 - sh));
-}
-
-// Largest r in [0, n) with off[r] <= x (off is non-decreasing, off[0] <= x).
-__device__ __forceinline__ uint64_t find_segment(const uint64_t *off, uint64_t n, uint64_t x) {
-  uint64_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= x) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
 }
 
 }  // namespace
@@ -527,231 +497,6 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
 }
 
 // ---------------------------------------------------------------------------
-// Sorted sweep: seeds in PositionHash segment order, so the waves working at
-// the same time read neighbouring rows of the bank (shared through L2 and
-// the Infinity Cache) instead of random ones.
-// ---------------------------------------------------------------------------
-__global__ void sweep_prep_kernel(const uint64_t *codes, const uint64_t *invalid, uint64_t p0,
-                                  uint64_t p1, uint32_t k, SpacedRuns spaced, uint64_t seed,
-                                  uint64_t sampling, uint32_t *counts, const uint64_t *off,
-                                  uint32_t *hi, uint64_t *lo_pos) {
-  const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;  // word of [p0, p1)
-  const uint64_t words = (p1 - p0 + 63) / 64;
-  if (w >= words) return;
-  // windows of the word: positions p0 + 64w + i, i < 64, below p1
-  const uint64_t q = p0 + 64 * w;
-  uint64_t bad = 0;
-  for (uint32_t d = 0; d < k; d++) bad |= stream_bits(invalid, q + d);
-  uint64_t valid = ~bad;
-  if (p1 - q < 64) valid &= (1ULL << (p1 - q)) - 1;
-  if (!off && !sampling) {  // every valid window is listed
-    counts[w] = (uint32_t)__builtin_popcountll(valid);
-    return;
-  }
-  uint64_t at = off ? off[w] : 0;
-  uint32_t n = 0;
-  for (uint64_t m = valid; m; m &= m - 1) {
-    const int i = __builtin_ctzll(m);
-    uint64_t key = canonical_from_window(stream_bits(codes, 2 * (q + i)), (int)k);
-    if (spaced.n) key = spaced_canonical(key, spaced);
-    const uint64_t first = mixsplit(key, seed);
-    if (sampling && first % sampling) continue;  // not a modimizer
-    if (off) {
-      hi[at] = (uint32_t)(first >> 32);
-      lo_pos[at] = (first << 32) | (q + i - p0);
-      at++;
-    }
-    n++;
-  }
-  if (!off) counts[w] = n;
-}
-
-void launch_sweep_prep(const uint64_t *codes, const uint64_t *invalid, uint64_t p0, uint64_t p1,
-                       uint32_t k, const SpacedRuns &spaced, uint64_t seed, uint64_t sampling,
-                       uint32_t *counts, const uint64_t *off, uint32_t *hi, uint64_t *lo_pos,
-                       hipStream_t s) {
-  const uint64_t words = (p1 - p0 + 63) / 64;
-  if (!words) return;
-  sweep_prep_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(
-      codes, invalid, p0, p1, k, spaced, seed, sampling, counts, off, hi, lo_pos);
-}
-
-size_t sort_pairs_temp_bytes(uint64_t n) {
-  size_t bytes = 0;
-  check_hip(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
-                                               (uint32_t *)nullptr, (const uint64_t *)nullptr,
-                                               (uint64_t *)nullptr, (int64_t)n),
-            "radix sort size");
-  return bytes;
-}
-
-void launch_sort_pairs(void *temp, size_t temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
-                       const uint64_t *vals_in, uint64_t *vals_out, uint64_t n, hipStream_t s) {
-  if (!n) return;
-  check_hip(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in,
-                                               vals_out, (int64_t)n, 0, 32, s),
-            "radix sort");
-}
-
-// Distinct seeds of a sorted sweep list: entry i heads a run when its first
-// hash (high half hi, low half in lo_pos's high word) differs from entry
-// i - 1's (flags[n] = 0 closes the scan input).
-__global__ void sweep_heads_kernel(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n,
-                                   uint32_t *flags) {
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i > n) return;
-  flags[i] = i < n && (i == 0 || hi[i] != hi[i - 1] || (lo_pos[i] >> 32) != (lo_pos[i - 1] >> 32))
-                 ? 1u
-                 : 0u;
-}
-
-// Entry i belongs to distinct seed u = off[i] + flags[i] - 1 (off = exclusive
-// scan of flags): a head stores the seed's first hash, and every entry gives
-// (its position, u) for the sort back to position order.
-__global__ void sweep_assign_kernel(const uint32_t *flags, const uint64_t *off,
-                                    const uint32_t *hi, const uint64_t *lo_pos, uint64_t n,
-                                    uint64_t *first, uint32_t *pos, uint32_t *ids) {
-  const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t u = (uint32_t)(off[i] + flags[i] - 1);
-  const uint64_t lp = lo_pos[i];
-  if (flags[i]) first[u] = (uint64_t)hi[i] << 32 | lp >> 32;
-  pos[i] = (uint32_t)lp;
-  ids[i] = u;
-}
-
-void launch_sweep_heads(const uint32_t *hi, const uint64_t *lo_pos, uint64_t n, uint32_t *flags,
-                        hipStream_t s) {
-  sweep_heads_kernel<<<(uint32_t)((n + 1 + 255) / 256), 256, 0, s>>>(hi, lo_pos, n, flags);
-}
-
-void launch_sweep_assign(const uint32_t *flags, const uint64_t *off, const uint32_t *hi,
-                         const uint64_t *lo_pos, uint64_t n, uint64_t *first, uint32_t *pos,
-                         uint32_t *ids, hipStream_t s) {
-  if (!n) return;
-  sweep_assign_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, s>>>(flags, off, hi, lo_pos, n,
-                                                                    first, pos, ids);
-}
-
-size_t sort_pairs_u32_temp_bytes(uint64_t n) {
-  size_t bytes = 0;
-  check_hip(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t *)nullptr,
-                                               (uint32_t *)nullptr, (const uint32_t *)nullptr,
-                                               (uint32_t *)nullptr, (int64_t)n),
-            "radix sort size");
-  return bytes;
-}
-
-void launch_sort_pairs_u32(void *temp, size_t temp_bytes, const uint32_t *keys_in,
-                           uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
-                           uint64_t n, int bits, hipStream_t s) {
-  if (!n) return;
-  check_hip(hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_in, keys_out, vals_in,
-                                               vals_out, (int64_t)n, 0, bits, s),
-            "radix sort");
-}
-
-// Per-read sums of the hit rows of a chunk's seeds: entry e (in position
-// order) adds row ids[e] to the read holding position p0 + pos[e]. One wave
-// per span of entries and group of 1024 region bits; lane l holds bits
-// 16l..16l+15 of the group as byte counters (bit 4q+b in byte b of acc[q]),
-// spilled every 255 rows and added to the read's counts (records) or
-// partial[colour] at read ends.
-constexpr uint32_t kSweepSumSpan = 4096;
-
-__device__ __forceinline__ uint32_t spread_nibble(uint32_t x) {
-  // bit i of x (i < 4) to bit 8i: the products' bits i + 7s never collide
-  return (x * 0x00204081u) & 0x01010101u;
-}
-
-__global__ __launch_bounds__(256) void sweep_sum_kernel(
-    const uint32_t *hits, uint64_t row_words, uint32_t region_off, uint32_t region_bits,
-    const uint32_t *bit_color, const uint32_t *pos, const uint32_t *ids, uint64_t n,
-    const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint32_t records, uint32_t *counts,
-    uint64_t *partial, uint32_t n_colors) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint64_t groups = (region_bits + 1023) / 1024;
-  const uint64_t spans = (n + kSweepSumSpan - 1) / kSweepSumSpan;
-  if (wave >= groups * spans) return;
-  const uint64_t g = wave % groups, sp = wave / groups;
-  const uint32_t j0 = (uint32_t)(g * 1024) + lane * 16;
-  const bool live = j0 < region_bits;
-  // the lane's 16 bits of row u: halfword u * 2 * row_words + j0 / 16
-  const uint16_t *col = (const uint16_t *)(hits + region_off) + j0 / 16;
-  const uint64_t e0 = sp * kSweepSumSpan;
-  const uint64_t e1 = e0 + kSweepSumSpan < n ? e0 + kSweepSumSpan : n;
-  uint64_t r = find_segment(pos_off, n_reads, p0 + pos[e0]);
-  uint64_t next = pos_off[r + 1];
-  uint32_t acc[4] = {0, 0, 0, 0}, tot[16], fill = 0;
-#pragma unroll
-  for (int b = 0; b < 16; b++) tot[b] = 0;
-  auto spill = [&]() {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-#pragma unroll
-      for (int b = 0; b < 4; b++) tot[4 * q + b] += (acc[q] >> (8 * b)) & 0xFFu;
-      acc[q] = 0;
-    }
-    fill = 0;
-  };
-  auto flush = [&]() {
-    spill();
-#pragma unroll
-    for (int b = 0; b < 16; b++) {
-      if (tot[b] && j0 + b < region_bits) {
-        const uint32_t color = bit_color[j0 + b];
-        if (color != kNone) {
-          if (records) atomicAdd(&counts[r * n_colors + color], tot[b]);
-          else atomicAdd((unsigned long long *)&partial[color], (unsigned long long)tot[b]);
-        }
-      }
-      tot[b] = 0;
-    }
-  };
-  for (uint64_t e = e0; e < e1; e += 64) {
-    const bool in = e + lane < e1;
-    const uint32_t my_id = in ? ids[e + lane] : kNone;
-    const uint32_t my_pos = in ? pos[e + lane] : 0u;
-    const uint32_t m = e1 - e < 64 ? (uint32_t)(e1 - e) : 64u;
-    for (uint32_t jb = 0; jb < m; jb += 8) {
-      uint32_t x[8];
-#pragma unroll
-      for (int d = 0; d < 8; d++) {
-        const uint32_t u = readlane32(my_id, (int)(jb + d));
-        x[d] = live && u != kNone ? col[(uint64_t)u * 2 * row_words] : 0u;
-      }
-#pragma unroll
-      for (int d = 0; d < 8; d++) {
-        if (jb + d >= m) break;
-        const uint64_t p = p0 + readlane32(my_pos, (int)(jb + d));
-        if (records && p >= next) {
-          flush();
-          while (pos_off[r + 1] <= p) r++;
-          next = pos_off[r + 1];
-        }
-#pragma unroll
-        for (int qq = 0; qq < 4; qq++) acc[qq] += spread_nibble((x[d] >> (4 * qq)) & 0xFu);
-        if (++fill == 255) spill();
-      }
-    }
-  }
-  flush();
-}
-
-void launch_sweep_sum(const uint32_t *hits, uint64_t row_words, uint32_t region_off,
-                      uint32_t region_bits, const uint32_t *bit_color, const uint32_t *pos,
-                      const uint32_t *ids, uint64_t n, const uint64_t *pos_off, uint64_t n_reads,
-                      uint64_t p0, uint32_t records, uint32_t *counts, uint64_t *partial,
-                      uint32_t n_colors, hipStream_t s) {
-  if (!n || !n_reads || !region_bits) return;
-  const uint64_t waves = ((region_bits + 1023) / 1024) * ((n + kSweepSumSpan - 1) / kSweepSumSpan);
-  sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
-      hits, row_words, region_off, region_bits, bit_color, pos, ids, n, pos_off, n_reads, p0,
-      records, counts, partial, n_colors);
-}
-
-// ---------------------------------------------------------------------------
 // Long k-mers (k > 32, scan_long_kmers, src/lib.rs:683-730). Forward hash
 // sum (c_i + 1) B^(k-1-i), reverse-complement hash sum (4 - c_i) B^i, both
 // mod 2^64; the key is splitmix64(min ^ domain). One thread per 64 window
@@ -1002,35 +747,6 @@ void launch_strand_max(uint32_t *counts, uint64_t n, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Scans (hipcub)
-// ---------------------------------------------------------------------------
-size_t scan_temp_bytes_u64(uint64_t n) {
-  size_t bytes = 0;
-  check_hip(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint64_t *)nullptr,
-                                             (uint64_t *)nullptr, (int64_t)n),
-            "scan size");
-  return bytes;
-}
-void launch_exclusive_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, void *temp,
-                               size_t temp_bytes, hipStream_t s) {
-  check_hip(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, (int64_t)n, s), "scan");
-}
-size_t scan_temp_bytes_u32_to_u64(uint64_t n) {
-  size_t bytes = 0;
-  check_hip(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, (const uint32_t *)nullptr,
-                                              (uint64_t *)nullptr, hipcub::Sum(), (uint64_t)0,
-                                              (int64_t)n),
-            "scan size");
-  return bytes;
-}
-void launch_exclusive_scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_t n,
-                                      void *temp, size_t temp_bytes, hipStream_t s) {
-  check_hip(hipcub::DeviceScan::ExclusiveScan(temp, temp_bytes, in, out, hipcub::Sum(),
-                                              (uint64_t)0, (int64_t)n, s),
-            "scan");
-}
-
-// ---------------------------------------------------------------------------
 // The lookup (storage.rs:2761-2925 on the plane layout)
 // ---------------------------------------------------------------------------
 template <int B>
@@ -1110,35 +826,6 @@ __device__ __forceinline__ uint32_t field_hits(const typename FieldCfg<B>::Word 
     m = (fp ? x[0] : ~x[0]) & 0xFFFFu;
   }
   return m;
-}
-
-// get_bits (storage.rs:3525-3538) on one row, byte loads (rare paths only).
-__device__ uint64_t row_bits(const uint8_t *row, uint64_t offset, uint32_t width) {
-  uint64_t byte = offset >> 3;
-  uint32_t shift = (uint32_t)(offset & 7);
-  uint32_t count = (shift + width + 7) >> 3;
-  uint64_t v = 0;
-  uint32_t first = count < 8 ? count : 8;
-  for (uint32_t i = 0; i < first; i++) v |= (uint64_t)row[byte + i] << (8 * i);
-  v >>= shift;
-  if (count == 9) v |= (uint64_t)row[byte + 8] << (64 - shift);
-  if (width < 64) v &= (1ULL << width) - 1;
-  return v;
-}
-
-// XOR over the key's H rows of get_bits(row, offset, width).
-__device__ uint64_t xor_rows_bits(const BankDev &bk, uint64_t key, uint64_t seed,
-                                  uint64_t offset, uint32_t width) {
-  uint64_t first = mixsplit(key, seed);
-  uint64_t step = splitmix64(first);
-  uint64_t base = slot_base(first, bk.geo);
-  uint64_t h = first, v = 0;
-  for (uint32_t i = 0; i < bk.hashes; i++) {
-    uint64_t slot = slot_at(base, h, i, bk.geo);
-    v ^= row_bits(bk.data + slot * bk.row_bytes, offset, width);
-    h += step;
-  }
-  return v;
 }
 
 template <int B, int T, int D>
@@ -1291,10 +978,6 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupArgs a) {
           v = (iv & win_mask) == 0;
           key = canonical_from_window(stream_bits(a.codes, 2 * idx), (int)a.k);
           if (a.spaced.n) key = spaced_canonical(key, a.spaced);
-        }
-      } else if (a.source == kSeedSorted) {
-        if (v) {
-          key = unsplitmix64(a.sorted[idx]) - a.seed;  // mixsplit(key, seed) was listed
         }
       } else if (v) {
         key = a.keys[idx];

@@ -1,0 +1,896 @@
+// zg_sweep.hip -- the batched query in bank order (the sorted sweep) on gfx950.
+//
+// A k-mer's H rows lie in H consecutive segments of L rows starting at its
+// first segment mulhi(h, count), h = mixsplit(key, seed) (PositionHash,
+// storage.rs:3452-3481). Per chunk of whole reads:
+//   1. the chunk's valid windows become entries (h, position), put in first-
+//      segment order by a stable LSD radix sort on the segment index (2
+//      passes of 9 bits for 2^17 segments; tile histograms, a scan, and a
+//      stable scatter per pass: no inter-block spinning);
+//   2. bucket starts per segment;
+//   3. the staged lookup: a block takes one segment and one 512-byte column
+//      slice of the fingerprint plane, copies the H segments x L rows of that
+//      slice into LDS (128 KiB at H=4, L=64), and looks up every entry of the
+//      segment from LDS (XOR of H rows, b-bit compare), writing the hit bits
+//      of its slice into the entry's hit row, indexed by position;
+//   4. the per-read sum walks the hit rows in position order (sequential
+//      reads) with bit-sliced counters and adds them to the read's counts.
+// Each row is read from HBM about once per chunk instead of once per seed,
+// the hit rows are written and read once, and nothing waits on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "zg_device.h"
+#include "zg_internal.h"
+
+namespace zg {
+
+namespace {
+
+constexpr uint32_t kSortThreads = 256;               // 4 waves
+constexpr uint32_t kSortRounds = 64;                 // rounds of 64 entries per wave
+constexpr uint64_t kSortWaveItems = 64 * kSortRounds;
+constexpr uint64_t kSortTile = 4 * kSortWaveItems;   // 16384 entries per tile
+constexpr uint32_t kMaxDigitBits = 9;
+constexpr uint32_t kMaxBins = 1u << kMaxDigitBits;
+constexpr uint32_t kStageThreads = 1024;             // 16 waves, one block per CU
+constexpr uint32_t kStageWaves = kStageThreads / 64;
+constexpr uint32_t kStageWindow = 128 * 1024;        // LDS rows of one segment window
+constexpr uint32_t kSliceBytes = 512;                // plane bytes per slice (8 per lane)
+constexpr uint64_t kSweepHitBytes = 64ULL << 30;     // hit rows of one chunk, at most
+constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
+constexpr uint32_t kXcds = 8;
+
+// The seeds of positions [p0, p1): canonical k-mers (k <= 32), optionally
+// through a spaced seed, keyed by their first PositionHash word; modimizers
+// keep the sampled ones (scan_modimizers, src/lib.rs:796-806).
+struct PosSource {
+  const uint64_t *codes, *invalid;
+  uint64_t p0, n;  // chunk start, positions
+  uint32_t k;
+  uint64_t seed, sampling;
+  SpacedRuns spaced;
+  __device__ bool get(uint64_t i, uint64_t &h, uint32_t &pos) const {
+    if (i >= n) return false;
+    const uint64_t p = p0 + i;
+    const uint64_t win = k >= 64 ? ~0ULL : (1ULL << k) - 1;
+    if (stream_bits(invalid, p) & win) return false;
+    uint64_t key = canonical_from_window(stream_bits(codes, 2 * p), (int)k);
+    if (spaced.n) key = spaced_canonical(key, spaced);
+    h = mixsplit(key, seed);
+    if (sampling && h % sampling) return false;
+    pos = (uint32_t)i;
+    return true;
+  }
+};
+
+// Explicit seed keys (minimizers, long k-mers, randstrobes) k0 .. k0 + n - 1.
+struct KeySource {
+  const uint64_t *keys;
+  uint64_t k0, n;
+  uint64_t seed;
+  __device__ bool get(uint64_t i, uint64_t &h, uint32_t &pos) const {
+    if (i >= n) return false;
+    h = mixsplit(keys[k0 + i], seed);
+    pos = (uint32_t)i;
+    return true;
+  }
+};
+
+// Entries of an earlier pass; the count is read from device memory.
+struct EntSource {
+  const uint64_t *h;
+  const uint32_t *p;
+  const uint64_t *n_ptr;
+  uint64_t n;
+  __device__ bool get(uint64_t i, uint64_t &hv, uint32_t &pos) const {
+    if (i >= n) return false;
+    hv = h[i];
+    pos = p[i];
+    return true;
+  }
+};
+
+__device__ __forceinline__ void resolve(PosSource &) {}
+__device__ __forceinline__ void resolve(KeySource &) {}
+__device__ __forceinline__ void resolve(EntSource &s) { s.n = *s.n_ptr; }
+
+// Tile t, wave w, round r, lane l holds item t*16384 + w*4096 + r*64 + l:
+// the tile order is the input order, which keeps the sort stable.
+__device__ __forceinline__ uint64_t sort_item(uint64_t tile, uint32_t wave, uint32_t r,
+                                              uint32_t lane) {
+  return tile * kSortTile + wave * kSortWaveItems + (uint64_t)r * 64 + lane;
+}
+
+template <class Src>
+__global__ __launch_bounds__(256) void radix_hist_kernel(Src src, uint64_t count, uint32_t shift,
+                                                         uint32_t dbits, uint64_t n_tiles,
+                                                         uint32_t *hist, uint64_t *listed) {
+  __shared__ uint32_t bins[kMaxBins];
+  resolve(src);
+  const uint32_t nb = 1u << dbits, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < nb; i += kSortThreads) bins[i] = 0;
+  __syncthreads();
+  const uint64_t tile = blockIdx.x;
+  for (uint32_t r = 0; r < kSortRounds; r++) {
+    const uint64_t i = sort_item(tile, wave, r, lane);
+    uint64_t h = 0;
+    uint32_t p = 0;
+    const bool v = src.get(i, h, p);
+    if (listed) {  // valid windows of the round's 64 positions
+      const uint64_t m = __ballot(v);
+      if (lane == 0 && i < src.n) listed[i >> 6] = m;
+    }
+    if (v) atomicAdd(&bins[(mulhi64(h, count) >> shift) & (nb - 1)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < nb; d += kSortThreads) hist[(uint64_t)d * n_tiles + tile] = bins[d];
+  if (tile == 0 && threadIdx.x == 0) hist[(uint64_t)nb * n_tiles] = 0;  // closes the scan input
+}
+
+// Stable scatter: each wave's items go, in order, after the items of the
+// earlier waves of the tile and of the earlier tiles with the same digit.
+template <class Src>
+__global__ __launch_bounds__(256) void radix_scatter_kernel(Src src, uint64_t count, uint32_t shift,
+                                                            uint32_t dbits, uint64_t n_tiles,
+                                                            const uint64_t *off, uint64_t *out_h,
+                                                            uint32_t *out_p) {
+  __shared__ uint32_t base[4][kMaxBins];
+  resolve(src);
+  const uint32_t nb = 1u << dbits, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = lane; i < nb; i += 64) base[wave][i] = 0;
+  __syncthreads();
+  const uint64_t tile = blockIdx.x;
+  for (uint32_t r = 0; r < kSortRounds; r++) {
+    uint64_t h = 0;
+    uint32_t p = 0;
+    if (src.get(sort_item(tile, wave, r, lane), h, p))
+      atomicAdd(&base[wave][(mulhi64(h, count) >> shift) & (nb - 1)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < nb; d += kSortThreads) {
+    uint32_t b = (uint32_t)off[(uint64_t)d * n_tiles + tile];
+    for (int w = 0; w < 4; w++) {
+      const uint32_t c = base[w][d];
+      base[w][d] = b;
+      b += c;
+    }
+  }
+  __syncthreads();
+  const uint64_t below = lanemask_lt(lane);
+  for (uint32_t r = 0; r < kSortRounds; r++) {
+    uint64_t h = 0;
+    uint32_t p = 0;
+    const bool v = src.get(sort_item(tile, wave, r, lane), h, p);
+    const uint32_t d = v ? (uint32_t)((mulhi64(h, count) >> shift) & (nb - 1)) : 0u;
+    // lanes holding the same digit: one ballot per digit bit
+    uint64_t same = __ballot(v);
+    for (uint32_t bit = 0; bit < dbits; bit++) {
+      const uint64_t ones = __ballot((d >> bit) & 1);
+      same &= ((d >> bit) & 1) ? ones : ~ones;
+    }
+    if (v) {
+      const uint32_t at = base[wave][d] + (uint32_t)__popcll(same & below);
+      out_h[at] = h;
+      out_p[at] = p;
+      if (!(same & below)) base[wave][d] += (uint32_t)__popcll(same);  // the group's first lane
+    }
+  }
+}
+
+// start[s] = first entry of segment s (start[count] = n).
+__global__ void bucket_bounds_kernel(const uint64_t *h, const uint64_t *n_ptr, uint64_t count,
+                                     uint64_t max_n, uint32_t *start) {
+  const uint64_t n = *n_ptr;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= max_n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i > n) break;
+    const int64_t sp = i ? (int64_t)mulhi64(h[i - 1], count) : -1;
+    const int64_t sc = i < n ? (int64_t)mulhi64(h[i], count) : (int64_t)count;
+    for (int64_t s = sp + 1; s <= sc; s++) start[s] = (uint32_t)i;
+  }
+}
+
+// Member words of every hit row to zero (merged groups OR their bits in).
+__global__ void zero_members_kernel(uint32_t *hits, uint64_t n_pos, uint32_t row_words,
+                                    uint32_t first, uint32_t words) {
+  const uint64_t total = n_pos * words;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    hits[(i / words) * row_words + first + i % words] = 0;
+}
+
+// Bit f of the result: field f of the lane's 8 bytes (w0 low, w1 high) equals
+// fp. Fields are b bits, little endian (the plane layout, storage.rs:1721-1754).
+template <int B>
+__device__ __forceinline__ uint64_t slice_hits(uint32_t w0, uint32_t w1, uint32_t fp) {
+  if constexpr (B == 8) {
+    auto four = [&](uint32_t w) {
+      const uint32_t v = w ^ (fp * 0x01010101u);
+      uint32_t z = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // zero bytes
+      z >>= 7;                             // fields at bits 0, 8, 16, 24
+      z = (z | (z >> 7)) & 0x00030003u;    // 0, 1, 16, 17
+      return (z | (z >> 14)) & 0xFu;       // 0..3
+    };
+    return four(w0) | (four(w1) << 4);
+  } else if constexpr (B == 16) {
+    auto two = [&](uint32_t w) {
+      return (uint32_t)((w & 0xFFFFu) == fp) | ((uint32_t)((w >> 16) == fp) << 1);
+    };
+    return two(w0) | (two(w1) << 2);
+  } else if constexpr (B == 32) {
+    return (uint64_t)(w0 == fp) | ((uint64_t)(w1 == fp) << 1);
+  } else if constexpr (B == 4) {
+    auto eight = [&](uint32_t w) {
+      const uint32_t v = w ^ (fp * 0x11111111u);
+      uint32_t z = (~(((v & 0x77777777u) + 0x77777777u) | v) & 0x88888888u) >> 3;  // 0,4,..,28
+      z = (z | (z >> 3)) & 0x03030303u;
+      z = (z | (z >> 6)) & 0x000F000Fu;
+      return (z | (z >> 12)) & 0xFFu;
+    };
+    return eight(w0) | (eight(w1) << 8);
+  } else if constexpr (B == 2) {
+    auto sixteen = [&](uint32_t w) {
+      const uint32_t v = w ^ (fp * 0x55555555u);
+      uint32_t z = ~(v | (v >> 1)) & 0x55555555u;  // even bits
+      z = (z | (z >> 1)) & 0x33333333u;
+      z = (z | (z >> 2)) & 0x0F0F0F0Fu;
+      z = (z | (z >> 4)) & 0x00FF00FFu;
+      return (z | (z >> 8)) & 0xFFFFu;
+    };
+    return (uint64_t)sixteen(w0) | ((uint64_t)sixteen(w1) << 16);
+  } else {
+    const uint64_t x = (uint64_t)w0 | ((uint64_t)w1 << 32);
+    return fp ? x : ~x;
+  }
+}
+
+__device__ __forceinline__ uint64_t low_bits(int64_t n, int F) {
+  if (n <= 0) return 0;
+  if (n >= F) return F >= 64 ? ~0ULL : ((1ULL << F) - 1);
+  return (1ULL << n) - 1;
+}
+
+}  // namespace
+
+struct StageArgs {
+  BankDev bank;
+  RecDev rec;
+  const uint64_t *ent_h;
+  const uint32_t *ent_p;
+  const uint32_t *start;  // [count + 1]
+  uint64_t seed;
+  uint32_t slice_bytes;   // LDS row stride: plane bytes per slice (<= 512)
+  uint32_t n_slices;
+  uint32_t *hits;         // entries' rows: positions (or keys) x row_words
+  uint32_t row_words;
+  uint32_t col_word0;     // first word of this bank's column bits in a row
+  uint32_t member_bit0;   // first bit of this bank's member bits in a row
+  uint32_t *queue;        // per-XCD unit counters at stride 32
+  uint32_t xcds;
+  unsigned long long *folded;  // entries answered from the entry before them
+};
+
+namespace {
+
+template <int B>
+__global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
+  constexpr int F = 64 / B;          // fields per lane (8 bytes)
+  constexpr uint32_t OUT = 8 * F;    // hit bytes per slice (64 lanes x F bits)
+  __shared__ __attribute__((aligned(16))) uint8_t win[kStageWindow];
+  __shared__ unsigned long long rec_bits_all[kStageWaves * 64];
+  __shared__ uint32_t unit_sh;
+  const BankDev &bk = a.bank;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long *rec_bits = rec_bits_all + wave * 64;
+  const uint32_t H = bk.hashes;
+  const uint64_t L = bk.geo.length, count = bk.geo.count;
+  const uint32_t SB = a.slice_bytes;
+  const uint32_t lane_off = lane * 8 < SB ? lane * 8 : SB - 8;  // lanes past the plane are masked
+  const uint32_t x = blockIdx.x % a.xcds;
+  const uint64_t s_lo = count * x / a.xcds, s_hi = count * (x + 1) / a.xcds;
+  const uint64_t units = (s_hi - s_lo) * a.n_slices;
+  uint64_t folded = 0;
+  while (true) {
+    __syncthreads();  // the previous unit's window is no longer read
+    if (threadIdx.x == 0) unit_sh = atomicAdd(a.queue + 32 * x, 1u);
+    __syncthreads();
+    const uint64_t u = unit_sh;
+    if (u >= units) break;
+    const uint64_t s = s_lo + u / a.n_slices;
+    const uint32_t q = (uint32_t)(u % a.n_slices);
+    const uint32_t e0 = a.start[s], e1 = a.start[s + 1];
+    if (e0 == e1) continue;
+    // the window: rows s*L .. (s+H)*L - 1, bytes [q*SB, (q+1)*SB) of each
+    {
+      const uint8_t *src = bk.data + s * L * bk.row_bytes + (uint64_t)q * SB;
+      const uint32_t per_row = SB / 16, chunks = H * (uint32_t)L * per_row;
+      for (uint32_t c = threadIdx.x; c < chunks; c += kStageThreads) {
+        const uint32_t row = c / per_row, col = c % per_row;
+        *(uint4 *)(win + row * SB + col * 16) =
+            *(const uint4 *)(src + (uint64_t)row * bk.row_bytes + col * 16);
+      }
+    }
+    __syncthreads();
+    // this lane's fields: columns c0 .. c0 + F - 1
+    const int64_t c0 = (int64_t)q * SB * 8 / B + (int64_t)lane * F;
+    const uint64_t fmask = low_bits((int64_t)bk.groups - c0, F);
+    const uint64_t mmask = fmask & ~low_bits((int64_t)bk.first_merged - c0, F);
+    const uint32_t slice_c0 = q * SB * 8 / B, slice_c1 = slice_c0 + 64 * F;
+    const bool mmask_any = __ballot(mmask != 0) != 0;  // merged groups in this slice
+    for (uint32_t b0 = e0 + wave * 64; b0 < e1; b0 += kStageWaves * 64) {
+      const uint32_t e = b0 + lane;
+      const bool v = e < e1;
+      const uint32_t nv = e1 - b0 < 64 ? e1 - b0 : 64;
+      const uint64_t h = v ? a.ent_h[e] : 0;
+      const uint32_t p = v ? a.ent_p[e] : 0;
+      const uint64_t key = unsplitmix64(h) - a.seed;  // mixsplit(key, seed) == h
+      const uint32_t fp = (uint32_t)fingerprint(key, a.seed, B);
+      const uint64_t step = splitmix64(h);
+      uint64_t offs = 0, hv = h;
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+        if (i < (int)H) {
+          offs |= ((hv ^ (hv >> 33)) & (L - 1)) << (8 * i);
+          hv += step;
+        }
+      uint32_t rec_start = 0, rec_len = 0;
+      if (a.rec.enabled && v) {
+        const uint32_t ah = abandoned_hash(key, a.seed);
+        const uint64_t cap_mask = (1ULL << a.rec.cap_log2) - 1;
+        uint64_t slot = a.rec.cap_log2 ? (uint64_t)(ah >> (32 - a.rec.cap_log2)) : 0;
+        while (true) {
+          const uint64_t de = a.rec.dir[slot];
+          const uint32_t pi = (uint32_t)de;
+          if (pi == kNone) break;
+          if ((uint32_t)(de >> 32) == ah) {
+            rec_len = a.rec.post[pi];
+            rec_start = pi + 1;
+            break;
+          }
+          slot = (slot + 1) & cap_mask;
+        }
+      }
+      // a partitioned bank answers the keys routed to it (storage.rs:99-101)
+      const uint64_t own_mask =
+          __ballot(v && (bk.partitions <= 1 || partition_id(key, a.seed, bk.partitions) == bk.partition));
+      // an entry whose seed equals the entry before it reuses its hits
+      const uint64_t prev_h = shfl_up64(h, 1);
+      const uint64_t same_mask = __ballot(v && lane > 0 && prev_h == h) & own_mask & (own_mask << 1);
+      folded += __popcll(same_mask);
+      uint64_t m_seed = 0;
+      for (uint32_t j = 0; j < nv; j++) {
+        if (!((own_mask >> j) & 1)) continue;
+        const uint32_t pj = readlane32(p, (int)j);
+        if (!((same_mask >> j) & 1)) {
+          const uint64_t oj = readlane64(offs, (int)j);
+          const uint32_t fpj = readlane32(fp, (int)j);
+          uint32_t w0 = 0, w1 = 0;
+#pragma unroll
+          for (int i = 0; i < 8; i++)
+            if (i < (int)H) {
+              const uint32_t row = (uint32_t)(i * L + ((oj >> (8 * i)) & 0xFF));
+              const uint2 rv = *(const uint2 *)(win + row * SB + lane_off);
+              w0 ^= rv.x;
+              w1 ^= rv.y;
+            }
+          m_seed = slice_hits<B>(w0, w1, fpj) & fmask;
+        }
+        uint64_t m = m_seed;
+        uint32_t *row_out = a.hits + (uint64_t)pj * a.row_words;
+        // recovery: a recovered colour counts as a hit (storage.rs:2771-2787,
+        // 2916-2924); postings spread over the lanes, 64 per pass
+        const uint32_t rl = readlane32(rec_len, (int)j);
+        if (rl) {
+          const uint32_t rs = readlane32(rec_start, (int)j);
+          rec_bits[lane] = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t q0 = 0; q0 < rl; q0 += 64) {
+            const uint32_t qq = q0 + lane;
+            const uint32_t pe = qq < rl ? a.rec.post_col[rs + qq] : kNone;
+            const uint32_t col = pe & 0xFFFFFFu;
+            const bool mine = pe != kNone && (pe >> 24) == bk.colmap && col >= slice_c0 &&
+                              col < slice_c1 && col < bk.groups;
+            if (mine && col < bk.first_merged)
+              atomicOr(&rec_bits[(col - slice_c0) / F], 1ULL << ((col - slice_c0) % F));
+            if (mine && col >= bk.first_merged) {
+              const uint32_t o = a.rec.post[rs + qq];
+              const uint32_t bit =
+                  a.member_bit0 + bk.mg_off[col - bk.first_merged] + bk.color_bit[o];
+              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          m |= rec_bits[lane] & fmask;
+        }
+        // merged groups: decode the XORed membership mask (get_bits on the
+        // H rows) and set each member's bit
+        const uint64_t kj = mmask_any ? readlane64(key, (int)j) : 0;
+        uint64_t mh = m_seed & mmask;
+        if (mh) {
+          while (mh) {
+            const int f = __builtin_ctzll(mh);
+            mh &= mh - 1;
+            const uint32_t g = (uint32_t)(c0 + f) - bk.first_merged;
+            const uint32_t lo = bk.mg_off[g], width = bk.mg_off[g + 1] - lo;
+            uint64_t mask = xor_rows_bits(bk, kj, a.seed, bk.mg_mask_bit[g], width);
+            while (mask) {
+              const uint32_t bit = a.member_bit0 + lo + __builtin_ctzll(mask);
+              mask &= mask - 1;
+              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
+            }
+          }
+        }
+        m &= ~mmask;  // a merged group's column bit is not a colour
+        uint8_t *out = (uint8_t *)(row_out + a.col_word0) + (uint64_t)q * OUT;
+        if constexpr (F == 64) ((uint64_t *)out)[lane] = m;
+        else if constexpr (F == 32) ((uint32_t *)out)[lane] = (uint32_t)m;
+        else if constexpr (F == 16) ((uint16_t *)out)[lane] = (uint16_t)m;
+        else if constexpr (F == 8) out[lane] = (uint8_t)m;
+        else if constexpr (F == 4) {
+          const uint32_t up = __shfl_down((uint32_t)m, 1);
+          if (!(lane & 1)) out[lane >> 1] = (uint8_t)((m & 0xFu) | ((up & 0xFu) << 4));
+        } else {  // F == 2: four lanes per byte
+          uint32_t bits2 = (uint32_t)m & 3u;
+          bits2 |= (__shfl_down(bits2, 1) & 3u) << 2;
+          bits2 |= (__shfl_down(bits2, 2) & 15u) << 4;
+          if (!(lane & 3)) out[lane >> 2] = (uint8_t)bits2;
+        }
+      }
+    }
+  }
+  if (folded && lane == 0) atomicAdd(a.folded, (unsigned long long)folded);
+}
+
+// Bit-sliced counters: plane j holds bit j of each of the 32 counts.
+__device__ __forceinline__ void sum_spill(uint32_t (&pl)[8], uint32_t (&tot)[32]) {
+#pragma clang loop unroll(full)
+  for (int b = 0; b < 32; b++) {
+    uint32_t c = 0;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 8; j++) c |= ((pl[j] >> b) & 1u) << j;
+    tot[b] += c;
+  }
+#pragma clang loop unroll(full)
+  for (int j = 0; j < 8; j++) pl[j] = 0;
+}
+
+// Adds the 32 counts of a lane's word to its read (row: counts + r * n_colors)
+// or to the colour sums; color[b] is the colour of bit b of the word.
+__device__ __forceinline__ void sum_flush(uint32_t (&pl)[8], uint32_t (&tot)[32], bool live,
+                                          const uint32_t *color, uint32_t records, uint32_t *row,
+                                          uint64_t *partial) {
+  sum_spill(pl, tot);
+  if (live) {
+#pragma clang loop unroll(full)
+    for (int b = 0; b < 32; b++) {
+      if (tot[b]) {
+        const uint32_t c = color[b];
+        if (c != kNone) {
+          if (records) atomicAdd(&row[c], tot[b]);
+          else atomicAdd((unsigned long long *)&partial[c], (unsigned long long)tot[b]);
+        }
+      }
+    }
+  }
+#pragma clang loop unroll(full)
+  for (int b = 0; b < 32; b++) tot[b] = 0;
+}
+
+// Per-read sums of the hit rows, in position order. A wave takes a span of
+// positions and a group of 64 row words; lane l reads word (l % LP) of
+// position (l / LP) + PPI*i (LP lanes per position, PPI = 64 / LP positions
+// per step; listed null: every index holds an entry) and adds its 32 bits to bit-sliced counters (plane j holds bit j
+// of each bit's count), spilled to per-bit counters every 255 steps and
+// added to the read's counts at its end (records) or to partial[colour].
+__global__ __launch_bounds__(256) void sweep_sum_kernel(
+    const uint32_t *hits, uint32_t row_words, uint32_t lp_log2, const uint32_t *word_color,
+    const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t n_pos,
+    uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t groups = (row_words + 63) / 64;
+  const uint64_t spans = (n_pos + kSumSpan - 1) / kSumSpan;
+  if (wave >= groups * spans) return;
+  const uint32_t g = (uint32_t)(wave % groups);
+  const uint64_t sp = wave / groups;
+  const uint32_t LP = 1u << lp_log2, PPI = 64 / LP;
+  const uint32_t slot = lane >> lp_log2;
+  const uint32_t w = g * 64 + (lane & (LP - 1));
+  const bool live = (lane & (LP - 1)) < 64 && w < row_words;
+  const uint64_t q0 = sp * kSumSpan, q1 = q0 + kSumSpan < n_pos ? q0 + kSumSpan : n_pos;
+  uint64_t first = q0 + slot < q1 ? q0 + slot : q1 - 1;
+  uint64_t r = records ? find_segment(pos_off, n_reads, p0 + first) : 0;
+  uint64_t next = records ? pos_off[r + 1] - p0 : ~0ULL;  // relative end of read r
+  uint32_t pl[8], tot[32];
+#pragma clang loop unroll(full)
+  for (int j = 0; j < 8; j++) pl[j] = 0;
+#pragma clang loop unroll(full)
+  for (int b = 0; b < 32; b++) tot[b] = 0;
+  uint32_t fill = 0;
+  for (uint64_t pp = q0; pp < q1; pp += PPI) {
+    const uint64_t p = pp + slot;
+    uint32_t xw = 0;
+    if (p < q1) {
+      if (records && p >= next) {
+        sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
+        while (pos_off[r + 1] - p0 <= p) r++;
+        next = pos_off[r + 1] - p0;
+      }
+      if (live && (!listed || ((listed[p >> 6] >> (p & 63)) & 1))) xw = hits[p * row_words + w];
+    }
+    uint32_t c = xw;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 8; j++) {
+      const uint32_t t = pl[j] & c;
+      pl[j] ^= c;
+      c = t;
+    }
+    if (++fill == 255) {
+      sum_spill(pl, tot);
+      fill = 0;
+    }
+  }
+  sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
+}
+
+uint32_t bits_for(uint64_t v) {  // bits to hold values 0..v
+  uint32_t b = 1;
+  while (b < 64 && (v >> b)) b++;
+  return b;
+}
+
+}  // namespace
+
+static uint32_t slice_bytes_for(const BankDev &bk) {
+  return (uint32_t)std::min<uint64_t>(kSliceBytes, (bk.plane_bytes + 63) / 64 * 64);
+}
+
+// Indexes the sweep serves: every seed mode but findere, every bank's
+// segment window of one 512-byte slice within the LDS window
+// (H * L * slice <= 128 KiB, L <= 256, H <= 8), and banks resident.
+bool sweep_layout_ok(const zg_index *idx) {
+  if (idx->m.index_mode == 3 || idx->dbanks.empty()) return false;
+  for (const DeviceBank &db : idx->dbanks) {
+    const BankDev &bk = db.view;
+    if (bk.geo.length > 256 || bk.hashes > 8 || bk.geo.count < 1 || bk.groups == 0) return false;
+    if ((uint64_t)bk.hashes * bk.geo.length * slice_bytes_for(bk) > kStageWindow) return false;
+  }
+  return true;
+}
+
+template <int B>
+static void launch_stage(const StageArgs &a, uint32_t blocks, hipStream_t s) {
+  stage_lookup_kernel<B><<<blocks, kStageThreads, 0, s>>>(a);
+}
+
+static std::vector<uint32_t> fetch_u32(const uint32_t *p, uint64_t n) {
+  std::vector<uint32_t> v(std::max<uint64_t>(1, n));
+  if (n) check_hip(hipMemcpy(v.data(), p, n * 4, hipMemcpyDeviceToHost), "sweep tables");
+  return v;
+}
+
+// The per-index tables of the sweep (built on first use, outside any
+// capture): size classes, the hit-row layout and the colour of every row bit.
+// Row layout: each class's column region (n_slices x 512/b bytes, column c
+// of the class at bit c of its region), then every class's member bits.
+static void ensure_sweep_tables(zg_index *idx) {
+  if (idx->sw_row_words) return;
+  const uint32_t B = (uint32_t)idx->m.fingerprint_bits;
+  std::vector<SweepClass> classes;
+  for (uint32_t bi = 0; bi < idx->dbanks.size(); bi++) {
+    const BankDev &bk = idx->dbanks[bi].view;
+    bool placed = false;
+    for (SweepClass &c : classes) {
+      const BankDev &o = idx->dbanks[c.banks[0]].view;
+      if (o.colmap == bk.colmap && o.geo.count == bk.geo.count && o.geo.length == bk.geo.length &&
+          o.row_bytes == bk.row_bytes && o.groups == bk.groups) {
+        c.banks.push_back(bi);
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) {
+      SweepClass c;
+      c.banks.push_back(bi);
+      c.count = bk.geo.count;
+      c.slice_bytes = slice_bytes_for(bk);
+      c.n_slices = (uint32_t)((bk.plane_bytes + c.slice_bytes - 1) / c.slice_bytes);
+      classes.push_back(c);
+    }
+  }
+  uint32_t col_words = 0, members = 0;
+  for (SweepClass &c : classes) {
+    const BankDev &bk = idx->dbanks[c.banks[0]].view;
+    c.col_word0 = col_words;
+    col_words += c.n_slices * (kSliceBytes / B) / 4;  // 64 lanes x 64/b bits per slice
+    c.members = bk.groups > bk.first_merged
+                    ? fetch_u32(bk.mg_off, bk.groups - bk.first_merged + 1)[bk.groups - bk.first_merged]
+                    : 0;
+    c.member_bit0 = members;  // relative to the member region until the columns are placed
+    members += c.members;
+  }
+  const uint32_t row_words = col_words + (members + 31) / 32;
+  std::vector<uint32_t> color(row_words * 32ULL, kNone);
+  for (SweepClass &c : classes) {
+    const BankDev &bk = idx->dbanks[c.banks[0]].view;
+    c.member_bit0 += col_words * 32;
+    const std::vector<uint32_t> col_color = fetch_u32(bk.col_color, bk.groups);
+    for (uint32_t col = 0; col < bk.first_merged; col++)
+      color[c.col_word0 * 32ULL + col] = col_color[col];
+    const std::vector<uint32_t> mem = fetch_u32(bk.mg_members, c.members);
+    for (uint32_t i = 0; i < c.members; i++) color[c.member_bit0 + (uint64_t)i] = mem[i];
+  }
+  idx->sw_bit_color.alloc(color.size() * 4);
+  check_hip(hipMemcpy(idx->sw_bit_color.p, color.data(), color.size() * 4, hipMemcpyHostToDevice),
+            "sweep tables");
+  idx->sw_classes = classes;
+  idx->sw_col_words = col_words;
+  idx->sw_members = members;
+  idx->sw_row_words = row_words;
+}
+
+// The sort of one class: LSD passes on the first segment, then the segment
+// starts. Returns the buffer pair holding the sorted entries.
+template <class Src>
+static int sort_class(zg_batch *b, Src first_pass, uint64_t count, uint64_t n_items,
+                      uint64_t *listed, hipStream_t s) {
+  const uint32_t seg_bits = bits_for(count - 1);
+  const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const uint32_t dbits = (seg_bits + passes - 1) / passes, nb = 1u << dbits;
+  const uint64_t tiles = std::max<uint64_t>(1, (n_items + kSortTile - 1) / kSortTile);
+  const uint64_t hn = (uint64_t)nb * tiles + 1;
+  const size_t tb = scan_temp_bytes_u32_to_u64(hn);
+  const uint64_t *n_ptr = b->sw_off.as<uint64_t>() + (hn - 1);  // the scan's total
+  int cur = 0;
+  for (uint32_t pass = 0; pass < passes; pass++) {
+    const uint32_t shift = pass * dbits;
+    EntSource es{b->sw_h[cur].as<uint64_t>(), b->sw_p[cur].as<uint32_t>(), n_ptr, 0};
+    if (pass == 0)
+      radix_hist_kernel<Src><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          first_pass, count, shift, dbits, tiles, b->sw_hist.as<uint32_t>(), listed);
+    else
+      radix_hist_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          es, count, shift, dbits, tiles, b->sw_hist.as<uint32_t>(), nullptr);
+    launch_exclusive_scan_u32_to_u64(b->sw_hist.as<uint32_t>(), b->sw_off.as<uint64_t>(), hn,
+                                     b->scan_temp.p, tb, s);
+    if (pass == 0) {
+      radix_scatter_kernel<Src><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          first_pass, count, shift, dbits, tiles, b->sw_off.as<uint64_t>(),
+          b->sw_h[0].as<uint64_t>(), b->sw_p[0].as<uint32_t>());
+      cur = 0;
+    } else {
+      radix_scatter_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+          es, count, shift, dbits, tiles, b->sw_off.as<uint64_t>(), b->sw_h[1 - cur].as<uint64_t>(),
+          b->sw_p[1 - cur].as<uint32_t>());
+      cur = 1 - cur;
+    }
+  }
+  bucket_bounds_kernel<<<(uint32_t)std::min<uint64_t>(65536, n_items / 256 + 2), 256, 0, s>>>(
+      b->sw_h[cur].as<uint64_t>(), n_ptr, count, n_items, b->sw_start.as<uint32_t>());
+  return cur;
+}
+
+// The lookups of a batch as the sweep, every chunk enqueued on b->stream with
+// no host synchronization (sizes come from the host's read offsets). Entries
+// are the valid windows (positions) or the explicit keys of a chunk of whole
+// reads; per chunk, events bracket each class's sort and lookups and the sum.
+void sweep_run(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
+  const Manifest &m = idx->m;
+  ensure_sweep_tables(idx);
+  const uint32_t row_words = idx->sw_row_words;
+  const uint32_t B = (uint32_t)m.fingerprint_bits;
+  const bool keys = b->explicit_keys;
+  const bool strobes = m.has_randstrobemers;
+  // rows of counts: reads, or both orientations of each read for randstrobes
+  const uint64_t n_rows = strobes ? 2 * b->n_reads : b->n_reads;
+  // chunk size: hit rows of at most 64 GiB, entries below 2^32, and the
+  // chunk's buffers (hit row + 24 B of entries per item) within 3/4 of the
+  // device memory free or already held by them
+  uint64_t cap = std::min<uint64_t>((1ULL << 32) - 64, kSweepHitBytes / (row_words * 4ULL));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    uint64_t held = 0;
+    for (const DevBuf *d : {&b->fd_hits, &b->sw_h[0], &b->sw_h[1], &b->sw_p[0], &b->sw_p[1],
+                            &b->sw_listed, &b->sw_hist, &b->sw_off, &b->scan_temp})
+      held += d->bytes;
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(1 << 16, (free_b + held) / 4 * 3 /
+                                                                   (row_words * 4ULL + 32)));
+  } else {
+    (void)hipGetLastError();
+  }
+  if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // smaller chunks
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, strtoull(e, nullptr, 10)));
+  const std::vector<uint64_t> &off = b->h_seg_off;  // row starts in the item space
+  std::vector<uint64_t> cuts{0};
+  uint64_t most = 0;
+  while (cuts.back() < n_rows) {
+    const uint64_t r0 = cuts.back();
+    uint64_t r1 = r0 + 1;
+    while (r1 < n_rows && off[r1 + 1] - off[r0] <= cap) r1++;
+    ensure(off[r1] - off[r0] < (1ULL << 32) - 64,
+           "a read is too long for the sorted sweep (2^32 seeds per chunk)", ZG_ERR_INVALID);
+    most = std::max(most, off[r1] - off[r0]);
+    cuts.push_back(r1);
+  }
+  uint64_t max_count = 1, max_hist = 1;
+  for (const SweepClass &c : idx->sw_classes) {
+    max_count = std::max(max_count, c.count);
+    const uint32_t seg_bits = bits_for(c.count - 1);
+    const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const uint32_t dbits = (seg_bits + passes - 1) / passes;
+    max_hist = std::max<uint64_t>(
+        max_hist, ((uint64_t)1 << dbits) * std::max<uint64_t>(1, (most + kSortTile - 1) / kSortTile) + 1);
+  }
+  for (int i = 0; i < 2; i++) {
+    b->sw_h[i].reserve(most * 8 + 64);
+    b->sw_p[i].reserve(most * 4 + 64);
+  }
+  if (!keys) b->sw_listed.reserve(((most + 63) / 64 + 1) * 8);
+  b->sw_hist.reserve(max_hist * 4);
+  b->sw_off.reserve(max_hist * 8);
+  b->scan_temp.reserve(scan_temp_bytes_u32_to_u64(max_hist));
+  b->sw_start.reserve((max_count + 1) * 4);
+  b->sw_queue.reserve(32 * kXcds * 4 + 64);
+  b->fd_hits.reserve(std::max<uint64_t>(1, most * row_words) * 4);
+  const uint32_t n_chunks = (uint32_t)cuts.size() - 1;
+  const size_t ev_per_chunk = 2 + 2 * idx->sw_classes.size();
+  while (b->sw_ev.size() < ev_per_chunk * n_chunks) {
+    hipEvent_t e;
+    check_hip(hipEventCreate(&e), "hipEventCreate");
+    b->sw_ev.push_back(e);
+  }
+  unsigned long long *folded = (unsigned long long *)(b->sw_queue.as<uint32_t>() + 32 * kXcds);
+  launch_fill_u64((uint64_t *)folded, 1, 0, s);
+  StageArgs sa{};
+  sa.rec = idx->rec_view;
+  if (b->flags & ZG_QUERY_NO_RECOVERY) sa.rec.enabled = 0;
+  sa.seed = m.seed;
+  sa.row_words = row_words;
+  sa.hits = b->fd_hits.as<uint32_t>();
+  sa.start = b->sw_start.as<uint32_t>();
+  sa.queue = b->sw_queue.as<uint32_t>();
+  sa.xcds = idx->cu_count >= kXcds ? kXcds : 1;
+  sa.folded = folded;
+  const uint32_t stage_blocks = std::max<uint32_t>(sa.xcds, idx->cu_count);
+  // sum: lanes per item, the row's words rounded up to a power of two (at most 64)
+  uint32_t lp_bits = 0;
+  while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
+  b->sweep_chunks = n_chunks;
+  b->sweep_classes = (uint32_t)idx->sw_classes.size();
+  for (uint32_t ci = 0; ci < n_chunks; ci++) {
+    const uint64_t I0 = off[cuts[ci]], n_items = off[cuts[ci + 1]] - I0;
+    hipEvent_t *ev = &b->sw_ev[ev_per_chunk * ci];
+    check_hip(hipEventRecord(ev[0], s), "hipEventRecord");
+    if (idx->sw_members)
+      zero_members_kernel<<<16384, 256, 0, s>>>(b->fd_hits.as<uint32_t>(), n_items, row_words,
+                                                 idx->sw_col_words, row_words - idx->sw_col_words);
+    for (size_t k = 0; k < idx->sw_classes.size(); k++) {
+      const SweepClass &c = idx->sw_classes[k];
+      int cur;
+      if (keys) {
+        KeySource ks{b->keys.as<uint64_t>(), I0, n_items, m.seed};
+        cur = sort_class(b, ks, c.count, n_items, nullptr, s);
+      } else {
+        PosSource ps{b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), I0, n_items, (uint32_t)m.k,
+                     m.seed, m.index_mode == 2 ? m.modimizer_sampling : 0, idx->spaced};
+        cur = sort_class(b, ps, c.count, n_items, k == 0 ? b->sw_listed.as<uint64_t>() : nullptr, s);
+      }
+      check_hip(hipEventRecord(ev[1 + 2 * k], s), "hipEventRecord");
+      sa.ent_h = b->sw_h[cur].as<uint64_t>();
+      sa.ent_p = b->sw_p[cur].as<uint32_t>();
+      sa.slice_bytes = c.slice_bytes;
+      sa.n_slices = c.n_slices;
+      sa.col_word0 = c.col_word0;
+      sa.member_bit0 = c.member_bit0;
+      for (uint32_t bi : c.banks) {
+        sa.bank = idx->dbanks[bi].view;
+        launch_fill_u32(sa.queue, 32 * kXcds, 0, s);
+        switch (B) {
+          case 1: launch_stage<1>(sa, stage_blocks, s); break;
+          case 2: launch_stage<2>(sa, stage_blocks, s); break;
+          case 4: launch_stage<4>(sa, stage_blocks, s); break;
+          case 8: launch_stage<8>(sa, stage_blocks, s); break;
+          case 16: launch_stage<16>(sa, stage_blocks, s); break;
+          default: launch_stage<32>(sa, stage_blocks, s); break;
+        }
+      }
+      check_hip(hipEventRecord(ev[2 + 2 * k], s), "hipEventRecord");
+    }
+    const uint64_t groups = (row_words + 63) / 64;
+    const uint64_t waves = groups * ((n_items + kSumSpan - 1) / kSumSpan);
+    if (waves)
+      sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+          b->fd_hits.as<uint32_t>(), row_words, lp_bits, idx->sw_bit_color.as<uint32_t>(),
+          keys ? nullptr : b->sw_listed.as<uint64_t>(), keys ? b->key_off.as<uint64_t>() : b->pos_off.as<uint64_t>(), n_rows, I0,
+          n_items,
+          (b->records || strobes) ? 1 : 0, b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
+          (uint32_t)m.colors.size());
+    check_hip(hipEventRecord(ev[ev_per_chunk - 1], s), "hipEventRecord");
+  }
+  if (strobes) launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * m.colors.size(), s);
+}
+
+// Phase times of the last sweep (its events have completed).
+void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &sum_ms) {
+  sort_ms = lookup_ms = sum_ms = 0;
+  const size_t per = 2 + 2 * (size_t)b->sweep_classes;
+  for (uint64_t ci = 0; ci < b->sweep_chunks; ci++) {
+    hipEvent_t *ev = &b->sw_ev[per * ci];
+    for (uint32_t k = 0; k < b->sweep_classes; k++) {
+      sort_ms += elapsed_ms(ev[2 * k], ev[1 + 2 * k]);
+      lookup_ms += elapsed_ms(ev[1 + 2 * k], ev[2 + 2 * k]);
+    }
+    sum_ms += elapsed_ms(ev[per - 2], ev[per - 1]);
+  }
+}
+
+uint64_t sweep_folded(zg_batch *b) {
+  unsigned long long v = 0;
+  if (b->sw_queue.p)
+    check_hip(hipMemcpy(&v, b->sw_queue.as<uint32_t>() + 32 * kXcds, 8, hipMemcpyDeviceToHost),
+              "hipMemcpy");
+  return v;
+}
+
+// Building blocks run alone, for the GPU check against their CPU
+// restatement (tests/sweep_pieces.cpp). Entries (h[i], p[i]), i < *n_dev,
+// sorted stably by their first segment mulhi(h, count); returns the buffer
+// pair holding the result (0: h1/p1, 1: h2/p2).
+int sweep_sort_entries(const uint64_t *h, const uint32_t *p, const uint64_t *n_dev, uint64_t n_max,
+                       uint64_t count, uint64_t *h1, uint32_t *p1, uint64_t *h2, uint32_t *p2,
+                       hipStream_t s) {
+  const uint32_t seg_bits = bits_for(count - 1);
+  const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+  const uint32_t dbits = (seg_bits + passes - 1) / passes, nb = 1u << dbits;
+  const uint64_t tiles = std::max<uint64_t>(1, (n_max + kSortTile - 1) / kSortTile);
+  const uint64_t hn = (uint64_t)nb * tiles + 1;
+  DevBuf hist, off, temp, nbuf;
+  hist.alloc(hn * 4);
+  off.alloc(hn * 8);
+  const size_t tb = scan_temp_bytes_u32_to_u64(hn);
+  temp.alloc(tb);
+  uint64_t *hs[2] = {h1, h2};
+  uint32_t *ps[2] = {p1, p2};
+  EntSource es{h, p, n_dev, 0};
+  int cur = -1;
+  for (uint32_t pass = 0; pass < passes; pass++) {
+    radix_hist_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+        es, count, pass * dbits, dbits, tiles, hist.as<uint32_t>(), nullptr);
+    launch_exclusive_scan_u32_to_u64(hist.as<uint32_t>(), off.as<uint64_t>(), hn, temp.p, tb, s);
+    const int nxt = cur < 0 ? 0 : 1 - cur;
+    radix_scatter_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
+        es, count, pass * dbits, dbits, tiles, off.as<uint64_t>(), hs[nxt], ps[nxt]);
+    es = EntSource{hs[nxt], ps[nxt], n_dev, 0};
+    cur = nxt;
+  }
+  check_hip(hipStreamSynchronize(s), "sweep sort");
+  return cur;
+}
+
+void sweep_bounds(const uint64_t *h, const uint64_t *n_dev, uint64_t n_max, uint64_t count,
+                  uint32_t *start, hipStream_t s) {
+  bucket_bounds_kernel<<<(uint32_t)std::min<uint64_t>(65536, n_max / 256 + 2), 256, 0, s>>>(
+      h, n_dev, count, n_max, start);
+}
+
+void sweep_sum_rows(const uint32_t *hits, uint32_t row_words, const uint32_t *word_color,
+                    const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0,
+                    uint64_t n_pos, uint32_t records, uint32_t *counts, uint64_t *partial,
+                    uint32_t n_colors, hipStream_t s) {
+  uint32_t lp_bits = 0;
+  while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
+  const uint64_t waves = (uint64_t)((row_words + 63) / 64) * ((n_pos + kSumSpan - 1) / kSumSpan);
+  if (waves)
+    sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+        hits, row_words, lp_bits, word_color, listed, pos_off, n_reads, p0, n_pos, records, counts,
+        partial, n_colors);
+}
+
+}  // namespace zg
