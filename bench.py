@@ -602,6 +602,11 @@ def main() -> int:
     value = seeds * args.steps / elapsed
     totals, counts, agg = batch.results()
     log(f"{seeds} k-mers per step, {ms_per_step:.1f} ms/step, lookup {np.mean(lookup_ms):.1f} ms")
+    if st["tuning"]["sweep"]:
+        log(f"sweep phases (last step): sort {st['sweep_sort_ms']:.1f} ms, staged lookup "
+            f"{st['sweep_lookup_ms']:.1f} ms, sum {st['sweep_sum_ms']:.1f} ms, "
+            f"{st['sweep_launches']} chunks, {st['sweep_folded']} folded; seeding "
+            f"{st['last_seed_ms']:.1f} ms")
 
     # roofline of the lookup kernel: algorithmic bytes per launch / launch time
     positions = int(off[-1]) + args.reads

@@ -41,6 +41,13 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t *off, uint64_t n
   return lo;
 }
 
+// Lane src's value of v, src per lane (through the LDS crossbar).
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src, 64);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src, 64);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane) {
   return lane ? (~0ULL >> (64 - lane)) : 0ULL;
 }

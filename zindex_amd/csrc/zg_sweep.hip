@@ -4,8 +4,8 @@
 // first segment mulhi(h, count), h = mixsplit(key, seed) (PositionHash,
 // storage.rs:3452-3481). Per chunk of whole reads:
 //   1. the chunk's valid windows become entries (h, position), put in first-
-//      segment order by a stable LSD radix sort on the segment index (2
-//      passes of 9 bits for 2^17 segments; tile histograms, a scan, and a
+//      segment order by a stable LSD radix sort on the segment index (3
+//      passes of 6 bits for 2^17 segments; tile histograms, a scan, and a
 //      stable scatter per pass: no inter-block spinning);
 //   2. bucket starts per segment;
 //   3. the staged lookup: a block takes one segment and one 512-byte column
@@ -34,12 +34,16 @@ constexpr uint32_t kSortThreads = 256;               // 4 waves
 constexpr uint32_t kSortRounds = 64;                 // rounds of 64 entries per wave
 constexpr uint64_t kSortWaveItems = 64 * kSortRounds;
 constexpr uint64_t kSortTile = 4 * kSortWaveItems;   // 16384 entries per tile
-constexpr uint32_t kMaxDigitBits = 9;
+// Digit bits per radix pass: 64 buckets per tile keep each bucket's run of
+// writes in few cache lines, which L2 combines into whole-line writes (512
+// buckets left most writes as partial lines: 0.5 TB/s per scatter)
+constexpr uint32_t kMaxDigitBits = 6;
 constexpr uint32_t kMaxBins = 1u << kMaxDigitBits;
 constexpr uint32_t kStageThreads = 1024;             // 16 waves, one block per CU
 constexpr uint32_t kStageWaves = kStageThreads / 64;
 constexpr uint32_t kStageWindow = 128 * 1024;        // LDS rows of one segment window
 constexpr uint32_t kSliceBytes = 512;                // plane bytes per slice (8 per lane)
+constexpr uint32_t kStageOut = 512;                  // staged hit bytes per wave
 constexpr uint64_t kSweepHitBytes = 64ULL << 30;     // hit rows of one chunk, at most
 constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
 constexpr uint32_t kXcds = 8;
@@ -203,19 +207,35 @@ __global__ void zero_members_kernel(uint32_t *hits, uint64_t n_pos, uint32_t row
     hits[(i / words) * row_words + first + i % words] = 0;
 }
 
-// Bit f of the result: field f of the lane's 8 bytes (w0 low, w1 high) equals
-// fp. Fields are b bits, little endian (the plane layout, storage.rs:1721-1754).
+// Hit bits of the lane's 8 plane bytes (w0 low, w1 high), fields of b bits
+// little endian (the plane layout, storage.rs:1721-1754): field f equal to
+// fp sets bit field_bit<B>(f). At b = 8 the zero bytes of the two words are
+// gathered by one multiply, which leaves the fields in the order
+// f4 f0 f5 f1 f6 f2 f7 f3; every other width keeps field order.
+template <int B>
+__host__ __device__ __forceinline__ uint32_t field_bit(uint32_t f) {
+  if constexpr (B == 8) return f < 4 ? 2 * f + 1 : 2 * (f - 4);
+  else return f;
+}
+template <int B>
+__device__ __forceinline__ uint32_t bit_field(uint32_t bit) {
+  if constexpr (B == 8) return (bit & 1) ? (bit >> 1) : 4 + (bit >> 1);
+  else return bit;
+}
+
 template <int B>
 __device__ __forceinline__ uint64_t slice_hits(uint32_t w0, uint32_t w1, uint32_t fp) {
   if constexpr (B == 8) {
-    auto four = [&](uint32_t w) {
-      const uint32_t v = w ^ (fp * 0x01010101u);
-      uint32_t z = ~(((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;  // zero bytes
-      z >>= 7;                             // fields at bits 0, 8, 16, 24
-      z = (z | (z >> 7)) & 0x00030003u;    // 0, 1, 16, 17
-      return (z | (z >> 14)) & 0xFu;       // 0..3
-    };
-    return four(w0) | (four(w1) << 4);
+    const uint32_t fpb = fp * 0x01010101u;
+    const uint32_t v0 = w0 ^ fpb, v1 = w1 ^ fpb;
+    // 0x80 in each zero byte
+    const uint32_t z0 = ~(((v0 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v0) & 0x80808080u;
+    const uint32_t z1 = ~(((v1 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v1) & 0x80808080u;
+    // byte i's bits at 8i+6 (word 1, field 4+i) and 8i+7 (word 0, field i);
+    // y * (2^18 + 2^12 + 2^6 + 1) moves them to bits 24+2i and 25+2i, and
+    // no two partial products overlap, so nothing carries into bits 24..31
+    const uint32_t y = z0 | (z1 >> 1);
+    return __umulhi(y, 0x04104100u) & 0xFFu;  // (y * M) >> 24, bits 24..31
   } else if constexpr (B == 16) {
     auto two = [&](uint32_t w) {
       return (uint32_t)((w & 0xFFFFu) == fp) | ((uint32_t)((w >> 16) == fp) << 1);
@@ -248,10 +268,19 @@ __device__ __forceinline__ uint64_t slice_hits(uint32_t w0, uint32_t w1, uint32_
   }
 }
 
-__device__ __forceinline__ uint64_t low_bits(int64_t n, int F) {
+// Bits of the fields f < n (n clamped to [0, F]) in field_bit order.
+template <int B>
+__device__ __forceinline__ uint64_t fields_below(int64_t n) {
+  constexpr int F = 64 / B;
   if (n <= 0) return 0;
   if (n >= F) return F >= 64 ? ~0ULL : ((1ULL << F) - 1);
-  return (1ULL << n) - 1;
+  if constexpr (B == 8) {
+    uint64_t m = 0;
+    for (int f = 0; f < n; f++) m |= 1ULL << field_bit<8>((uint32_t)f);
+    return m;
+  } else {
+    return (1ULL << n) - 1;
+  }
 }
 
 }  // namespace
@@ -276,23 +305,30 @@ struct StageArgs {
 
 namespace {
 
-template <int B>
+// HT: the hash count when known at compile time (2, 3, 4), 0 for any H <= 8.
+template <int B, int HT>
 __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
   constexpr int F = 64 / B;          // fields per lane (8 bytes)
   constexpr uint32_t OUT = 8 * F;    // hit bytes per slice (64 lanes x F bits)
+  constexpr int HMAX = HT ? HT : 8;
   __shared__ __attribute__((aligned(16))) uint8_t win[kStageWindow];
   __shared__ unsigned long long rec_bits_all[kStageWaves * 64];
+  // each wave's hit bytes of G entries, written out together
+  constexpr uint32_t G = OUT >= kStageOut ? 1 : (kStageOut / OUT < 8 ? kStageOut / OUT : 8);
+  __shared__ __attribute__((aligned(16))) uint8_t out_all[kStageWaves * kStageOut];
   __shared__ uint32_t unit_sh;
   const BankDev &bk = a.bank;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint8_t *staged = out_all + wave * kStageOut;
   unsigned long long *rec_bits = rec_bits_all + wave * 64;
-  const uint32_t H = bk.hashes;
+  const uint32_t H = HT ? (uint32_t)HT : bk.hashes;
   const uint64_t L = bk.geo.length, count = bk.geo.count;
   const uint32_t SB = a.slice_bytes;
   const uint32_t lane_off = lane * 8 < SB ? lane * 8 : SB - 8;  // lanes past the plane are masked
   const uint32_t x = blockIdx.x % a.xcds;
   const uint64_t s_lo = count * x / a.xcds, s_hi = count * (x + 1) / a.xcds;
   const uint64_t units = (s_hi - s_lo) * a.n_slices;
+  const uint64_t seed = a.seed;
   uint64_t folded = 0;
   while (true) {
     __syncthreads();  // the previous unit's window is no longer read
@@ -317,29 +353,42 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
     __syncthreads();
     // this lane's fields: columns c0 .. c0 + F - 1
     const int64_t c0 = (int64_t)q * SB * 8 / B + (int64_t)lane * F;
-    const uint64_t fmask = low_bits((int64_t)bk.groups - c0, F);
-    const uint64_t mmask = fmask & ~low_bits((int64_t)bk.first_merged - c0, F);
+    const uint64_t fmask = fields_below<B>((int64_t)bk.groups - c0);
+    const uint64_t mmask = fmask & ~fields_below<B>((int64_t)bk.first_merged - c0);
+    const bool all_fields = __ballot(fmask != fields_below<B>(F)) == 0;
+    const bool any_merged = __ballot(mmask != 0) != 0;  // merged groups in this slice
     const uint32_t slice_c0 = q * SB * 8 / B, slice_c1 = slice_c0 + 64 * F;
-    const bool mmask_any = __ballot(mmask != 0) != 0;  // merged groups in this slice
     for (uint32_t b0 = e0 + wave * 64; b0 < e1; b0 += kStageWaves * 64) {
       const uint32_t e = b0 + lane;
       const bool v = e < e1;
-      const uint32_t nv = e1 - b0 < 64 ? e1 - b0 : 64;
       const uint64_t h = v ? a.ent_h[e] : 0;
       const uint32_t p = v ? a.ent_p[e] : 0;
-      const uint64_t key = unsplitmix64(h) - a.seed;  // mixsplit(key, seed) == h
-      const uint32_t fp = (uint32_t)fingerprint(key, a.seed, B);
+      const uint64_t key = unsplitmix64(h) - seed;  // mixsplit(key, seed) == h
+      const uint32_t fp = (uint32_t)fingerprint(key, seed, B);
       const uint64_t step = splitmix64(h);
-      uint64_t offs = 0, hv = h;
+      // window rows of the H hashes, i*L + ((h_i ^ h_i >> 33) & (L - 1)), as
+      // byte offsets in the window in 64-byte units (SB is a multiple of 64,
+      // the window at most 128 KiB), 16 bits each: hashes 0..3 in offs,
+      // 4..7 in offs_hi
+      const uint32_t sb64 = SB / 64;
+      uint64_t offs = 0, offs_hi = 0, hv = h;
 #pragma unroll
-      for (int i = 0; i < 8; i++)
-        if (i < (int)H) {
-          offs |= ((hv ^ (hv >> 33)) & (L - 1)) << (8 * i);
+      for (int i = 0; i < (HMAX < 4 ? HMAX : 4); i++)
+        if (HT || i < (int)H) {
+          offs |= (uint64_t)((i * L + ((hv ^ (hv >> 33)) & (L - 1))) * sb64) << (16 * i);
           hv += step;
         }
+      if (!HT && H > 4)
+        for (uint32_t i = 4; i < H && i < 8; i++) {
+          offs_hi |= (uint64_t)((i * L + ((hv ^ (hv >> 33)) & (L - 1))) * sb64) << (16 * (i - 4));
+          hv += step;
+        }
+      // row offsets and the fingerprint are computed here, before the entry
+      // loop, so no load is outstanding when the loop reads them
+      const uint64_t prow = (uint64_t)p * a.row_words;
       uint32_t rec_start = 0, rec_len = 0;
       if (a.rec.enabled && v) {
-        const uint32_t ah = abandoned_hash(key, a.seed);
+        const uint32_t ah = abandoned_hash(key, seed);
         const uint64_t cap_mask = (1ULL << a.rec.cap_log2) - 1;
         uint64_t slot = a.rec.cap_log2 ? (uint64_t)(ah >> (32 - a.rec.cap_log2)) : 0;
         while (true) {
@@ -354,80 +403,79 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
           slot = (slot + 1) & cap_mask;
         }
       }
+      const uint64_t rec_mask = __ballot(rec_len != 0);
       // a partitioned bank answers the keys routed to it (storage.rs:99-101)
       const uint64_t own_mask =
-          __ballot(v && (bk.partitions <= 1 || partition_id(key, a.seed, bk.partitions) == bk.partition));
+          __ballot(v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition));
       // an entry whose seed equals the entry before it reuses its hits
       const uint64_t prev_h = shfl_up64(h, 1);
       const uint64_t same_mask = __ballot(v && lane > 0 && prev_h == h) & own_mask & (own_mask << 1);
       folded += __popcll(same_mask);
       uint64_t m_seed = 0;
-      for (uint32_t j = 0; j < nv; j++) {
-        if (!((own_mask >> j) & 1)) continue;
-        const uint32_t pj = readlane32(p, (int)j);
-        if (!((same_mask >> j) & 1)) {
-          const uint64_t oj = readlane64(offs, (int)j);
-          const uint32_t fpj = readlane32(fp, (int)j);
-          uint32_t w0 = 0, w1 = 0;
+      // staged entries (their lanes j) and the write-out: lane l stores 16
+      // bytes of entry l / (OUT / 16), so a row's OUT bytes are one access
+      uint64_t staged_js = 0;  // lane j of the entry in slot t: bits 8t .. 8t+5
+      uint32_t n_staged = 0;
+      auto flush_staged = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        constexpr uint32_t per = OUT / 16;  // lanes per entry (OUT >= 16)
+        const uint32_t r = lane / per;
+        const uint32_t jr = r < G ? (uint32_t)(staged_js >> (8 * r)) & 63u : 0u;
+        const uint64_t pr = shfl64(prow, jr);
+        if (r < n_staged) {
+          uint8_t *dst = (uint8_t *)(a.hits + pr + a.col_word0) + (uint64_t)q * OUT;
+          const uint32_t c = (lane % per) * 16;
+          *(uint4 *)(dst + c) = *(const uint4 *)(staged + r * OUT + c);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        staged_js = 0;
+      };
+      const uint8_t *lane_win = win + lane_off;
+      // The H rows of entry j XORed and compared in this slice (from LDS).
+      auto seed_hits = [&](uint32_t j) -> uint64_t {
+        const uint64_t oj = readlane64(offs, (int)j);
+        const uint32_t fpj = readlane32(fp, (int)j);
+        uint32_t w0 = 0, w1 = 0;
+        uint2 rv[HMAX < 4 ? HMAX : 4];
 #pragma unroll
-          for (int i = 0; i < 8; i++)
-            if (i < (int)H) {
-              const uint32_t row = (uint32_t)(i * L + ((oj >> (8 * i)) & 0xFF));
-              const uint2 rv = *(const uint2 *)(win + row * SB + lane_off);
-              w0 ^= rv.x;
-              w1 ^= rv.y;
-            }
-          m_seed = slice_hits<B>(w0, w1, fpj) & fmask;
+        for (int i = 0; i < (HMAX < 4 ? HMAX : 4); i++) {
+          const uint32_t o64 = (uint32_t)(oj >> (16 * i)) & 0xFFFFu;
+          rv[i] = (HT || i < (int)H) ? *(const uint2 *)(lane_win + o64 * 64) : make_uint2(0, 0);
         }
+#pragma unroll
+        for (int i = 0; i < (HMAX < 4 ? HMAX : 4); i++) {
+          w0 ^= rv[i].x;
+          w1 ^= rv[i].y;
+        }
+        if (!HT && H > 4) {
+          const uint64_t ohj = readlane64(offs_hi, (int)j);
+          for (uint32_t i = 4; i < H; i++) {
+            const uint32_t o64 = (uint32_t)(ohj >> (16 * (i - 4))) & 0xFFFFu;
+            const uint2 r = *(const uint2 *)(lane_win + o64 * 64);
+            w0 ^= r.x;
+            w1 ^= r.y;
+          }
+        }
+        uint64_t mm = slice_hits<B>(w0, w1, fpj);
+        if (!all_fields) mm &= fmask;
+        return mm;
+      };
+      // Hot loop: entries without recovery postings, lowest lane first (a
+      // scalar find-first-set); it reads only LDS and writes only staged
+      // rows. Entries whose merged groups hit are fixed up below.
+      uint64_t fix_mask = 0;
+      for (uint64_t todo = own_mask & ~rec_mask; todo; todo &= todo - 1) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+        if (!((same_mask >> j) & 1)) m_seed = seed_hits(j);
         uint64_t m = m_seed;
-        uint32_t *row_out = a.hits + (uint64_t)pj * a.row_words;
-        // recovery: a recovered colour counts as a hit (storage.rs:2771-2787,
-        // 2916-2924); postings spread over the lanes, 64 per pass
-        const uint32_t rl = readlane32(rec_len, (int)j);
-        if (rl) {
-          const uint32_t rs = readlane32(rec_start, (int)j);
-          rec_bits[lane] = 0;
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          for (uint32_t q0 = 0; q0 < rl; q0 += 64) {
-            const uint32_t qq = q0 + lane;
-            const uint32_t pe = qq < rl ? a.rec.post_col[rs + qq] : kNone;
-            const uint32_t col = pe & 0xFFFFFFu;
-            const bool mine = pe != kNone && (pe >> 24) == bk.colmap && col >= slice_c0 &&
-                              col < slice_c1 && col < bk.groups;
-            if (mine && col < bk.first_merged)
-              atomicOr(&rec_bits[(col - slice_c0) / F], 1ULL << ((col - slice_c0) % F));
-            if (mine && col >= bk.first_merged) {
-              const uint32_t o = a.rec.post[rs + qq];
-              const uint32_t bit =
-                  a.member_bit0 + bk.mg_off[col - bk.first_merged] + bk.color_bit[o];
-              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
-            }
-          }
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          m |= rec_bits[lane] & fmask;
+        if (any_merged) {
+          if (__ballot((m & mmask) != 0)) fix_mask |= 1ULL << j;
+          m &= ~mmask;  // a merged group's own bit is no colour
         }
-        // merged groups: decode the XORed membership mask (get_bits on the
-        // H rows) and set each member's bit
-        const uint64_t kj = mmask_any ? readlane64(key, (int)j) : 0;
-        uint64_t mh = m_seed & mmask;
-        if (mh) {
-          while (mh) {
-            const int f = __builtin_ctzll(mh);
-            mh &= mh - 1;
-            const uint32_t g = (uint32_t)(c0 + f) - bk.first_merged;
-            const uint32_t lo = bk.mg_off[g], width = bk.mg_off[g + 1] - lo;
-            uint64_t mask = xor_rows_bits(bk, kj, a.seed, bk.mg_mask_bit[g], width);
-            while (mask) {
-              const uint32_t bit = a.member_bit0 + lo + __builtin_ctzll(mask);
-              mask &= mask - 1;
-              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
-            }
-          }
-        }
-        m &= ~mmask;  // a merged group's column bit is not a colour
-        uint8_t *out = (uint8_t *)(row_out + a.col_word0) + (uint64_t)q * OUT;
+        // this entry's OUT hit bytes go to slot n_staged of the wave's buffer
+        uint8_t *out = staged + n_staged * OUT;
         if constexpr (F == 64) ((uint64_t *)out)[lane] = m;
         else if constexpr (F == 32) ((uint32_t *)out)[lane] = (uint32_t)m;
         else if constexpr (F == 16) ((uint16_t *)out)[lane] = (uint16_t)m;
@@ -440,6 +488,87 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
           bits2 |= (__shfl_down(bits2, 1) & 3u) << 2;
           bits2 |= (__shfl_down(bits2, 2) & 15u) << 4;
           if (!(lane & 3)) out[lane >> 2] = (uint8_t)bits2;
+        }
+        staged_js |= (uint64_t)j << (8 * n_staged);
+        if (++n_staged == G) {
+          flush_staged();
+          n_staged = 0;
+        }
+      }
+      if (n_staged) {
+        flush_staged();
+        n_staged = 0;
+      }
+      // Rare pass: entries with recovery postings (written here whole) and
+      // entries whose merged groups hit (member bits ORed into their rows).
+      for (uint64_t todo = (own_mask & rec_mask) | fix_mask; todo; todo &= todo - 1) {
+        const uint32_t j = (uint32_t)__builtin_ctzll(todo);
+        const uint64_t ms = seed_hits(j);
+        uint64_t m = ms;
+        uint32_t *row_out = a.hits + readlane64(prow, (int)j);
+        const bool rec = (rec_mask >> j) & 1;
+        // recovery: a recovered colour counts as a hit (storage.rs:2771-2787,
+        // 2916-2924); postings spread over the lanes, 64 per pass
+        if (rec) {
+          const uint32_t rl = readlane32(rec_len, (int)j);
+          const uint32_t rs = readlane32(rec_start, (int)j);
+          rec_bits[lane] = 0;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          for (uint32_t q0 = 0; q0 < rl; q0 += 64) {
+            const uint32_t qq = q0 + lane;
+            const uint32_t pe = qq < rl ? a.rec.post_col[rs + qq] : kNone;
+            const uint32_t col = pe & 0xFFFFFFu;
+            const bool mine = pe != kNone && (pe >> 24) == bk.colmap && col >= slice_c0 &&
+                              col < slice_c1 && col < bk.groups;
+            if (mine && col < bk.first_merged)
+              atomicOr(&rec_bits[(col - slice_c0) / F],
+                       1ULL << field_bit<B>((col - slice_c0) % F));
+            if (mine && col >= bk.first_merged) {
+              const uint32_t o = a.rec.post[rs + qq];
+              const uint32_t bit =
+                  a.member_bit0 + bk.mg_off[col - bk.first_merged] + bk.color_bit[o];
+              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          m |= rec_bits[lane] & fmask;
+        }
+        if (any_merged) {
+          // merged groups: decode the XORed membership mask (get_bits on the
+          // H rows) and set each member's bit
+          const uint64_t kj = readlane64(key, (int)j);
+          uint64_t mh = ms & mmask;
+          while (mh) {
+            const uint32_t f = bit_field<B>((uint32_t)__builtin_ctzll(mh));
+            mh &= mh - 1;
+            const uint32_t g = (uint32_t)(c0 + f) - bk.first_merged;
+            const uint32_t lo = bk.mg_off[g], width = bk.mg_off[g + 1] - lo;
+            uint64_t mask = xor_rows_bits(bk, kj, seed, bk.mg_mask_bit[g], width);
+            while (mask) {
+              const uint32_t bit = a.member_bit0 + lo + __builtin_ctzll(mask);
+              mask &= mask - 1;
+              atomicOr(&row_out[bit >> 5], 1u << (bit & 31));
+            }
+          }
+          m &= ~mmask;
+        }
+        if (rec) {  // the hot loop skipped this entry: its column bytes
+          uint8_t *out = (uint8_t *)(row_out + a.col_word0) + (uint64_t)q * OUT;
+          if constexpr (F == 64) ((uint64_t *)out)[lane] = m;
+          else if constexpr (F == 32) ((uint32_t *)out)[lane] = (uint32_t)m;
+          else if constexpr (F == 16) ((uint16_t *)out)[lane] = (uint16_t)m;
+          else if constexpr (F == 8) out[lane] = (uint8_t)m;
+          else if constexpr (F == 4) {
+            const uint32_t up = __shfl_down((uint32_t)m, 1);
+            if (!(lane & 1)) out[lane >> 1] = (uint8_t)((m & 0xFu) | ((up & 0xFu) << 4));
+          } else {
+            uint32_t bits2 = (uint32_t)m & 3u;
+            bits2 |= (__shfl_down(bits2, 1) & 3u) << 2;
+            bits2 |= (__shfl_down(bits2, 2) & 15u) << 4;
+            if (!(lane & 3)) out[lane >> 2] = (uint8_t)bits2;
+          }
         }
       }
     }
@@ -493,7 +622,9 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
     const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t n_pos,
     uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  // wave-uniform (scalar) so that the span bounds and bitmap loads are too
+  const uint64_t wave =
+      (uint64_t)blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t groups = (row_words + 63) / 64;
   const uint64_t spans = (n_pos + kSumSpan - 1) / kSumSpan;
   if (wave >= groups * spans) return;
@@ -513,27 +644,56 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
 #pragma clang loop unroll(full)
   for (int b = 0; b < 32; b++) tot[b] = 0;
   uint32_t fill = 0;
-  for (uint64_t pp = q0; pp < q1; pp += PPI) {
-    const uint64_t p = pp + slot;
-    uint32_t xw = 0;
-    if (p < q1) {
-      if (records && p >= next) {
-        sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
-        while (pos_off[r + 1] - p0 <= p) r++;
-        next = pos_off[r + 1] - p0;
-      }
-      if (live && (!listed || ((listed[p >> 6] >> (p & 63)) & 1))) xw = hits[p * row_words + w];
-    }
-    uint32_t c = xw;
+  constexpr int U = 16;  // steps whose loads are in flight together
+  auto add = [&](uint32_t c) {
 #pragma clang loop unroll(full)
     for (int j = 0; j < 8; j++) {
       const uint32_t t = pl[j] & c;
       pl[j] ^= c;
       c = t;
     }
-    if (++fill == 255) {
+  };
+  for (uint64_t pp = q0; pp < q1; pp += (uint64_t)PPI * U) {
+    if (fill > 255 - U) {
       sum_spill(pl, tot);
       fill = 0;
+    }
+    fill += U;
+    const uint64_t last = pp + slot + (uint64_t)(U - 1) * PPI;
+    const bool cross = records && (last < q1 ? last : q1 - 1) >= next;
+    if (!__ballot(cross)) {
+      // no lane reaches the end of its read in these U steps
+      uint32_t xw[U];
+      uint64_t lw[U];
+#pragma clang loop unroll(full)
+      for (int u = 0; u < U; u++) {
+        const uint64_t p = pp + slot + (uint64_t)u * PPI;
+        xw[u] = live && p < q1 ? hits[p * row_words + w] : 0u;
+        // PPI divides 64 and pp is a multiple of PPI, so a step's positions
+        // share one word of the bitmap: a wave-uniform load
+        const uint64_t pw = pp + (uint64_t)u * PPI;
+        lw[u] = listed && pw < q1 ? listed[pw >> 6] : ~0ULL;
+      }
+#pragma clang loop unroll(full)
+      for (int u = 0; u < U; u++) {
+        const uint64_t p = pp + slot + (uint64_t)u * PPI;
+        add(((lw[u] >> (p & 63)) & 1) ? xw[u] : 0u);
+      }
+    } else {
+#pragma clang loop unroll(disable)
+      for (int u = 0; u < U; u++) {
+        const uint64_t p = pp + slot + (uint64_t)u * PPI;
+        uint32_t x = 0;
+        if (p < q1) {
+          if (p >= next) {
+            sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
+            while (pos_off[r + 1] - p0 <= p) r++;
+            next = pos_off[r + 1] - p0;
+          }
+          if (live && (!listed || ((listed[p >> 6] >> (p & 63)) & 1))) x = hits[p * row_words + w];
+        }
+        add(x);
+      }
     }
   }
   sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
@@ -566,7 +726,12 @@ bool sweep_layout_ok(const zg_index *idx) {
 
 template <int B>
 static void launch_stage(const StageArgs &a, uint32_t blocks, hipStream_t s) {
-  stage_lookup_kernel<B><<<blocks, kStageThreads, 0, s>>>(a);
+  switch (a.bank.hashes) {
+    case 2: stage_lookup_kernel<B, 2><<<blocks, kStageThreads, 0, s>>>(a); break;
+    case 3: stage_lookup_kernel<B, 3><<<blocks, kStageThreads, 0, s>>>(a); break;
+    case 4: stage_lookup_kernel<B, 4><<<blocks, kStageThreads, 0, s>>>(a); break;
+    default: stage_lookup_kernel<B, 0><<<blocks, kStageThreads, 0, s>>>(a); break;
+  }
 }
 
 static std::vector<uint32_t> fetch_u32(const uint32_t *p, uint64_t n) {
@@ -621,8 +786,13 @@ static void ensure_sweep_tables(zg_index *idx) {
     const BankDev &bk = idx->dbanks[c.banks[0]].view;
     c.member_bit0 += col_words * 32;
     const std::vector<uint32_t> col_color = fetch_u32(bk.col_color, bk.groups);
-    for (uint32_t col = 0; col < bk.first_merged; col++)
-      color[c.col_word0 * 32ULL + col] = col_color[col];
+    // column col = lane fields: bit (col / F) * F + field_bit(col % F) of the region
+    const uint32_t F = 64 / B;
+    for (uint32_t col = 0; col < bk.first_merged; col++) {
+      const uint32_t f = col % F;
+      const uint32_t bit = col - f + (B == 8 ? field_bit<8>(f) : f);
+      color[c.col_word0 * 32ULL + bit] = col_color[col];
+    }
     const std::vector<uint32_t> mem = fetch_u32(bk.mg_members, c.members);
     for (uint32_t i = 0; i < c.members; i++) color[c.member_bit0 + (uint64_t)i] = mem[i];
   }
