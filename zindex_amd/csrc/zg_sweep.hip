@@ -31,12 +31,11 @@ namespace zg {
 namespace {
 
 constexpr uint32_t kSortThreads = 256;               // 4 waves
-constexpr uint32_t kSortRounds = 64;                 // rounds of 64 entries per wave
+constexpr uint32_t kSortRounds = 16;                 // rounds of 64 entries per wave
 constexpr uint64_t kSortWaveItems = 64 * kSortRounds;
-constexpr uint64_t kSortTile = 4 * kSortWaveItems;   // 16384 entries per tile
-// Digit bits per radix pass: 64 buckets per tile keep each bucket's run of
-// writes in few cache lines, which L2 combines into whole-line writes (512
-// buckets left most writes as partial lines: 0.5 TB/s per scatter)
+constexpr uint64_t kSortTile = 4 * kSortWaveItems;   // 4096 entries per tile (48 KiB in LDS)
+// Digit bits per radix pass (64 buckets: a tile's digit run averages 64
+// entries, whole cache lines of the output)
 constexpr uint32_t kMaxDigitBits = 6;
 constexpr uint32_t kMaxBins = 1u << kMaxDigitBits;
 constexpr uint32_t kStageThreads = 1024;             // 16 waves, one block per CU
@@ -102,7 +101,7 @@ __device__ __forceinline__ void resolve(PosSource &) {}
 __device__ __forceinline__ void resolve(KeySource &) {}
 __device__ __forceinline__ void resolve(EntSource &s) { s.n = *s.n_ptr; }
 
-// Tile t, wave w, round r, lane l holds item t*16384 + w*4096 + r*64 + l:
+// Tile t, wave w, round r, lane l holds item t*4096 + w*1024 + r*64 + l:
 // the tile order is the input order, which keeps the sort stable.
 __device__ __forceinline__ uint64_t sort_item(uint64_t tile, uint32_t wave, uint32_t r,
                                               uint32_t lane) {
@@ -135,41 +134,63 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(Src src, uint64_t count
   if (tile == 0 && threadIdx.x == 0) hist[(uint64_t)nb * n_tiles] = 0;  // closes the scan input
 }
 
-// Stable scatter: each wave's items go, in order, after the items of the
-// earlier waves of the tile and of the earlier tiles with the same digit.
+// Stable scatter of one tile. Its items are ranked in LDS first: each
+// wave's items follow the earlier waves' items of the same digit, in input
+// order. The tile is then written out digit run by digit run, so that
+// neighbouring threads write neighbouring entries (whole cache lines) after
+// the same digit's entries of the earlier tiles.
 template <class Src>
 __global__ __launch_bounds__(256) void radix_scatter_kernel(Src src, uint64_t count, uint32_t shift,
                                                             uint32_t dbits, uint64_t n_tiles,
                                                             const uint64_t *off, uint64_t *out_h,
                                                             uint32_t *out_p) {
-  __shared__ uint32_t base[4][kMaxBins];
+  __shared__ uint32_t wbase[4][kMaxBins];  // per wave: digit counts, then next LDS slot
+  __shared__ uint32_t dstart[kMaxBins + 1];  // first LDS slot of each digit
+  __shared__ uint64_t gbase[kMaxBins];       // global index of LDS slot 0 of each digit
+  __shared__ uint64_t th[kSortTile];
+  __shared__ uint32_t tp[kSortTile];
   resolve(src);
   const uint32_t nb = 1u << dbits, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t i = lane; i < nb; i += 64) base[wave][i] = 0;
+  for (uint32_t i = lane; i < nb; i += 64) wbase[wave][i] = 0;
   __syncthreads();
   const uint64_t tile = blockIdx.x;
-  for (uint32_t r = 0; r < kSortRounds; r++) {
-    uint64_t h = 0;
-    uint32_t p = 0;
-    if (src.get(sort_item(tile, wave, r, lane), h, p))
-      atomicAdd(&base[wave][(mulhi64(h, count) >> shift) & (nb - 1)], 1u);
-  }
-  __syncthreads();
-  for (uint32_t d = threadIdx.x; d < nb; d += kSortThreads) {
-    uint32_t b = (uint32_t)off[(uint64_t)d * n_tiles + tile];
-    for (int w = 0; w < 4; w++) {
-      const uint32_t c = base[w][d];
-      base[w][d] = b;
-      b += c;
-    }
-  }
-  __syncthreads();
-  const uint64_t below = lanemask_lt(lane);
+  uint64_t hv[kSortRounds];
+  uint32_t pv[kSortRounds], dv[kSortRounds];
+#pragma unroll
   for (uint32_t r = 0; r < kSortRounds; r++) {
     uint64_t h = 0;
     uint32_t p = 0;
     const bool v = src.get(sort_item(tile, wave, r, lane), h, p);
-    const uint32_t d = v ? (uint32_t)((mulhi64(h, count) >> shift) & (nb - 1)) : 0u;
+    hv[r] = h;
+    pv[r] = p;
+    dv[r] = v ? (uint32_t)((mulhi64(h, count) >> shift) & (nb - 1)) : kNone;
+    if (v) atomicAdd(&wbase[wave][dv[r]], 1u);
+  }
+  __syncthreads();
+  if (wave == 0) {  // nb <= 64: one lane per digit
+    const uint32_t d = lane;
+    uint32_t tot = 0;
+    if (d < nb)
+      for (int w = 0; w < 4; w++) tot += wbase[w][d];
+    const uint64_t excl = wave_inclusive_scan(tot, lane) - tot;
+    if (d < nb) {
+      dstart[d] = (uint32_t)excl;
+      gbase[d] = off[(uint64_t)d * n_tiles + tile] - excl;
+      uint32_t run = (uint32_t)excl;
+      for (int w = 0; w < 4; w++) {
+        const uint32_t c = wbase[w][d];
+        wbase[w][d] = run;
+        run += c;
+      }
+      if (d == nb - 1) dstart[nb] = run;
+    }
+  }
+  __syncthreads();
+  const uint64_t below = lanemask_lt(lane);
+#pragma unroll
+  for (uint32_t r = 0; r < kSortRounds; r++) {
+    const uint32_t d = dv[r];
+    const bool v = d != kNone;
     // lanes holding the same digit: one ballot per digit bit
     uint64_t same = __ballot(v);
     for (uint32_t bit = 0; bit < dbits; bit++) {
@@ -177,11 +198,20 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(Src src, uint64_t co
       same &= ((d >> bit) & 1) ? ones : ~ones;
     }
     if (v) {
-      const uint32_t at = base[wave][d] + (uint32_t)__popcll(same & below);
-      out_h[at] = h;
-      out_p[at] = p;
-      if (!(same & below)) base[wave][d] += (uint32_t)__popcll(same);  // the group's first lane
+      const uint32_t at = wbase[wave][d] + (uint32_t)__popcll(same & below);
+      th[at] = hv[r];
+      tp[at] = pv[r];
+      if (!(same & below)) wbase[wave][d] += (uint32_t)__popcll(same);  // the group's first lane
     }
+  }
+  __syncthreads();
+  const uint32_t n_tile = dstart[nb];
+  for (uint32_t i = threadIdx.x; i < n_tile; i += kSortThreads) {
+    const uint64_t h = th[i];
+    const uint32_t d = (uint32_t)((mulhi64(h, count) >> shift) & (nb - 1));
+    const uint64_t g = gbase[d] + i;
+    out_h[g] = h;
+    out_p[g] = tp[i];
   }
 }
 
