@@ -253,19 +253,23 @@ __device__ __forceinline__ uint32_t bit_field(uint32_t bit) {
   else return bit;
 }
 
+// b = 8: bit field_bit<8>(f) set where byte f of (v0, v1) is zero.
+__device__ __forceinline__ uint64_t zero_bytes_to_bits(uint32_t v0, uint32_t v1) {
+  // 0x80 in each zero byte
+  const uint32_t z0 = ~(((v0 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v0) & 0x80808080u;
+  const uint32_t z1 = ~(((v1 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v1) & 0x80808080u;
+  // byte i's bits at 8i+6 (word 1, field 4+i) and 8i+7 (word 0, field i);
+  // y * (2^18 + 2^12 + 2^6 + 1) moves them to bits 24+2i and 25+2i, and
+  // no two partial products overlap, so nothing carries into bits 24..31
+  const uint32_t y = z0 | (z1 >> 1);
+  return __umulhi(y, 0x04104100u) & 0xFFu;  // (y * M) >> 24, bits 24..31
+}
+
 template <int B>
 __device__ __forceinline__ uint64_t slice_hits(uint32_t w0, uint32_t w1, uint32_t fp) {
   if constexpr (B == 8) {
     const uint32_t fpb = fp * 0x01010101u;
-    const uint32_t v0 = w0 ^ fpb, v1 = w1 ^ fpb;
-    // 0x80 in each zero byte
-    const uint32_t z0 = ~(((v0 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v0) & 0x80808080u;
-    const uint32_t z1 = ~(((v1 & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v1) & 0x80808080u;
-    // byte i's bits at 8i+6 (word 1, field 4+i) and 8i+7 (word 0, field i);
-    // y * (2^18 + 2^12 + 2^6 + 1) moves them to bits 24+2i and 25+2i, and
-    // no two partial products overlap, so nothing carries into bits 24..31
-    const uint32_t y = z0 | (z1 >> 1);
-    return __umulhi(y, 0x04104100u) & 0xFFu;  // (y * M) >> 24, bits 24..31
+    return zero_bytes_to_bits(w0 ^ fpb, w1 ^ fpb);
   } else if constexpr (B == 16) {
     auto two = [&](uint32_t w) {
       return (uint32_t)((w & 0xFFFFu) == fp) | ((uint32_t)((w >> 16) == fp) << 1);
@@ -474,6 +478,24 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
           const uint32_t o64 = (uint32_t)(oj >> (16 * i)) & 0xFFFFu;
           rv[i] = (HT || i < (int)H) ? *(const uint2 *)(lane_win + o64 * 64) : make_uint2(0, 0);
         }
+        if constexpr (B == 8 && (HT == 3 || HT == 4)) {
+          // rows and the fingerprint in 3-input XORs; fp is folded in here
+          const uint32_t fpb = fpj * 0x01010101u;
+          if constexpr (HT == 4) {
+            w0 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(rv[0].x, rv[1].x, rv[2].x, 0x96),
+                                             rv[3].x, fpb, 0x96);
+            w1 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(rv[0].y, rv[1].y, rv[2].y, 0x96),
+                                             rv[3].y, fpb, 0x96);
+          } else {
+            w0 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(rv[0].x, rv[1].x, rv[2].x, 0x96),
+                                             fpb, 0u, 0x96);
+            w1 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(rv[0].y, rv[1].y, rv[2].y, 0x96),
+                                             fpb, 0u, 0x96);
+          }
+          uint64_t mm = zero_bytes_to_bits(w0, w1);
+          if (!all_fields) mm &= fmask;
+          return mm;
+        }
 #pragma unroll
         for (int i = 0; i < (HMAX < 4 ? HMAX : 4); i++) {
           w0 ^= rv[i].x;
@@ -606,30 +628,36 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
   if (folded && lane == 0) atomicAdd(a.folded, (unsigned long long)folded);
 }
 
-// Bit-sliced counters: plane j holds bit j of each of the 32 counts.
-__device__ __forceinline__ void sum_spill(uint32_t (&pl)[8], uint32_t (&tot)[32]) {
+// Bit-sliced counters: plane j holds bit j of each of the 32 counts of a
+// lane's word (kPlanes planes: counts up to 1023 between spills).
+constexpr int kPlanes = 10;
+
+__device__ __forceinline__ void sum_spill(uint32_t (&pl)[kPlanes], uint32_t (&tot)[32]) {
 #pragma clang loop unroll(full)
   for (int b = 0; b < 32; b++) {
     uint32_t c = 0;
 #pragma clang loop unroll(full)
-    for (int j = 0; j < 8; j++) c |= ((pl[j] >> b) & 1u) << j;
+    for (int j = 0; j < kPlanes; j++) c |= ((pl[j] >> b) & 1u) << j;
     tot[b] += c;
   }
 #pragma clang loop unroll(full)
-  for (int j = 0; j < 8; j++) pl[j] = 0;
+  for (int j = 0; j < kPlanes; j++) pl[j] = 0;
 }
 
 // Adds the 32 counts of a lane's word to its read (row: counts + r * n_colors)
 // or to the colour sums; color[b] is the colour of bit b of the word.
-__device__ __forceinline__ void sum_flush(uint32_t (&pl)[8], uint32_t (&tot)[32], bool live,
+__device__ __forceinline__ void sum_flush(uint32_t (&pl)[kPlanes], uint32_t (&tot)[32], bool live,
                                           const uint32_t *color, uint32_t records, uint32_t *row,
                                           uint64_t *partial) {
   sum_spill(pl, tot);
   if (live) {
+    uint32_t cs[32];
+#pragma clang loop unroll(full)
+    for (int b = 0; b < 32; b++) cs[b] = color[b];
 #pragma clang loop unroll(full)
     for (int b = 0; b < 32; b++) {
       if (tot[b]) {
-        const uint32_t c = color[b];
+        const uint32_t c = cs[b];
         if (c != kNone) {
           if (records) atomicAdd(&row[c], tot[b]);
           else atomicAdd((unsigned long long *)&partial[c], (unsigned long long)tot[b]);
@@ -641,50 +669,104 @@ __device__ __forceinline__ void sum_flush(uint32_t (&pl)[8], uint32_t (&tot)[32]
   for (int b = 0; b < 32; b++) tot[b] = 0;
 }
 
+// Full and half adders on 32 independent bit positions.
+__device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uint32_t &s,
+                                         uint32_t &carry) {
+  s = a ^ b ^ c;
+  carry = (a & b) | (c & (a ^ b));
+}
+__device__ __forceinline__ void half_add(uint32_t a, uint32_t b, uint32_t &s, uint32_t &carry) {
+  s = a ^ b;
+  carry = a & b;
+}
+
+// The per-bit count (0..16) of 16 words as 5 bit planes, by a carry-save
+// adder tree: 11 full and 4 half adders.
+__device__ __forceinline__ void count16(const uint32_t (&w)[16], uint32_t (&o)[5]) {
+  uint32_t s1, c1, s2, c2, s3, c3, s4, c4, s5, c5;
+  full_add(w[0], w[1], w[2], s1, c1);
+  full_add(w[3], w[4], w[5], s2, c2);
+  full_add(w[6], w[7], w[8], s3, c3);
+  full_add(w[9], w[10], w[11], s4, c4);
+  full_add(w[12], w[13], w[14], s5, c5);
+  uint32_t S1, C1, S2, C2, C3;  // ones: s1..s5, w15
+  full_add(s1, s2, s3, S1, C1);
+  full_add(s4, s5, w[15], S2, C2);
+  half_add(S1, S2, o[0], C3);
+  uint32_t T1, D1, T2, D2, T3, D3, D4;  // twos: c1..c5, C1, C2, C3
+  full_add(c1, c2, c3, T1, D1);
+  full_add(c4, c5, C1, T2, D2);
+  full_add(C2, C3, T1, T3, D3);
+  half_add(T2, T3, o[1], D4);
+  uint32_t F1, E1, E2;  // fours: D1..D4
+  full_add(D1, D2, D3, F1, E1);
+  half_add(F1, D4, o[2], E2);
+  half_add(E1, E2, o[3], o[4]);  // eights: E1, E2
+}
+
+// pl += x (x: kb planes), rippling through every plane.
+template <int KB>
+__device__ __forceinline__ void add_planes(uint32_t (&pl)[kPlanes], const uint32_t (&x)[KB]) {
+  uint32_t carry = 0;
+#pragma clang loop unroll(full)
+  for (int j = 0; j < kPlanes; j++) {
+    if (j < KB) {
+      uint32_t s;
+      full_add(pl[j], x[j], carry, s, carry);
+      pl[j] = s;
+    } else {
+      uint32_t s;
+      half_add(pl[j], carry, s, carry);
+      pl[j] = s;
+    }
+  }
+}
+
 // Per-read sums of the hit rows, in position order. A wave takes a span of
 // positions and a group of 64 row words; lane l reads word (l % LP) of
 // position (l / LP) + PPI*i (LP lanes per position, PPI = 64 / LP positions
-// per step; listed null: every index holds an entry) and adds its 32 bits to bit-sliced counters (plane j holds bit j
-// of each bit's count), spilled to per-bit counters every 255 steps and
-// added to the read's counts at its end (records) or to partial[colour].
+// per step; listed null: every index holds an entry). Blocks of 16 steps in
+// which no lane reaches the end of its read are counted by a carry-save
+// adder tree and added to bit-sliced counters; other blocks go step by step.
+// Counters spill to per-bit counts before they can overflow and are added to
+// the read's counts at its end (records) or to partial[colour].
 __global__ __launch_bounds__(256) void sweep_sum_kernel(
     const uint32_t *hits, uint32_t row_words, uint32_t lp_log2, const uint32_t *word_color,
     const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t n_pos,
     uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
   const uint32_t lane = threadIdx.x & 63;
-  // wave-uniform (scalar) so that the span bounds and bitmap loads are too
-  const uint64_t wave =
-      (uint64_t)blockIdx.x * 4 + (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t groups = (row_words + 63) / 64;
   const uint64_t spans = (n_pos + kSumSpan - 1) / kSumSpan;
-  if (wave >= groups * spans) return;
-  const uint32_t g = (uint32_t)(wave % groups);
-  const uint64_t sp = wave / groups;
+  // a block's four waves share one group of row words (its colour table is
+  // staged in LDS) and take four consecutive spans; wave-uniform (scalar)
+  const uint32_t g = blockIdx.x % groups;
+  const uint64_t sp = (uint64_t)(blockIdx.x / groups) * 4 +
+                      (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  __shared__ uint32_t color_lds[64 * 32];
+  for (uint32_t i = threadIdx.x; i < 64 * 32; i += 256) {
+    const uint32_t wi = g * 64 + i / 32;
+    color_lds[i] = wi < row_words ? word_color[(uint64_t)g * 64 * 32 + i] : kNone;
+  }
+  __syncthreads();
+  if (sp >= spans) return;
   const uint32_t LP = 1u << lp_log2, PPI = 64 / LP;
   const uint32_t slot = lane >> lp_log2;
   const uint32_t w = g * 64 + (lane & (LP - 1));
-  const bool live = (lane & (LP - 1)) < 64 && w < row_words;
+  const bool live = w < row_words;
   const uint64_t q0 = sp * kSumSpan, q1 = q0 + kSumSpan < n_pos ? q0 + kSumSpan : n_pos;
   uint64_t first = q0 + slot < q1 ? q0 + slot : q1 - 1;
   uint64_t r = records ? find_segment(pos_off, n_reads, p0 + first) : 0;
   uint64_t next = records ? pos_off[r + 1] - p0 : ~0ULL;  // relative end of read r
-  uint32_t pl[8], tot[32];
+  uint32_t pl[kPlanes], tot[32];
 #pragma clang loop unroll(full)
-  for (int j = 0; j < 8; j++) pl[j] = 0;
+  for (int j = 0; j < kPlanes; j++) pl[j] = 0;
 #pragma clang loop unroll(full)
   for (int b = 0; b < 32; b++) tot[b] = 0;
   uint32_t fill = 0;
   constexpr int U = 16;  // steps whose loads are in flight together
-  auto add = [&](uint32_t c) {
-#pragma clang loop unroll(full)
-    for (int j = 0; j < 8; j++) {
-      const uint32_t t = pl[j] & c;
-      pl[j] ^= c;
-      c = t;
-    }
-  };
+  const uint64_t stride = (uint64_t)PPI * row_words;  // words between a lane's steps
   for (uint64_t pp = q0; pp < q1; pp += (uint64_t)PPI * U) {
-    if (fill > 255 - U) {
+    if (fill > (1u << kPlanes) - 1 - U) {
       sum_spill(pl, tot);
       fill = 0;
     }
@@ -695,10 +777,11 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
       // no lane reaches the end of its read in these U steps
       uint32_t xw[U];
       uint64_t lw[U];
+      const uint32_t *at = hits + (pp + slot) * row_words + w;
 #pragma clang loop unroll(full)
       for (int u = 0; u < U; u++) {
         const uint64_t p = pp + slot + (uint64_t)u * PPI;
-        xw[u] = live && p < q1 ? hits[p * row_words + w] : 0u;
+        xw[u] = live && p < q1 ? at[u * stride] : 0u;
         // PPI divides 64 and pp is a multiple of PPI, so a step's positions
         // share one word of the bitmap: a wave-uniform load
         const uint64_t pw = pp + (uint64_t)u * PPI;
@@ -707,8 +790,11 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
 #pragma clang loop unroll(full)
       for (int u = 0; u < U; u++) {
         const uint64_t p = pp + slot + (uint64_t)u * PPI;
-        add(((lw[u] >> (p & 63)) & 1) ? xw[u] : 0u);
+        if (!((lw[u] >> (p & 63)) & 1)) xw[u] = 0;
       }
+      uint32_t c5[5];
+      count16(xw, c5);
+      add_planes<5>(pl, c5);
     } else {
 #pragma clang loop unroll(disable)
       for (int u = 0; u < U; u++) {
@@ -716,17 +802,20 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
         uint32_t x = 0;
         if (p < q1) {
           if (p >= next) {
-            sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
+            sum_flush(pl, tot, live, color_lds + (lane & (LP - 1)) * 32, records,
+                      counts + r * n_colors, partial);
             while (pos_off[r + 1] - p0 <= p) r++;
             next = pos_off[r + 1] - p0;
           }
           if (live && (!listed || ((listed[p >> 6] >> (p & 63)) & 1))) x = hits[p * row_words + w];
         }
-        add(x);
+        const uint32_t x1[1] = {x};
+        add_planes<1>(pl, x1);
       }
     }
   }
-  sum_flush(pl, tot, live, word_color + w * 32, records, counts + r * n_colors, partial);
+  sum_flush(pl, tot, live, color_lds + (lane & (LP - 1)) * 32, records, counts + r * n_colors,
+            partial);
 }
 
 uint32_t bits_for(uint64_t v) {  // bits to hold values 0..v
@@ -1004,9 +1093,9 @@ void sweep_run(zg_index *idx, zg_batch *b) {
       check_hip(hipEventRecord(ev[2 + 2 * k], s), "hipEventRecord");
     }
     const uint64_t groups = (row_words + 63) / 64;
-    const uint64_t waves = groups * ((n_items + kSumSpan - 1) / kSumSpan);
-    if (waves)
-      sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+    const uint64_t blocks = groups * (((n_items + kSumSpan - 1) / kSumSpan + 3) / 4);
+    if (n_items)
+      sweep_sum_kernel<<<(uint32_t)blocks, 256, 0, s>>>(
           b->fd_hits.as<uint32_t>(), row_words, lp_bits, idx->sw_bit_color.as<uint32_t>(),
           keys ? nullptr : b->sw_listed.as<uint64_t>(), keys ? b->key_off.as<uint64_t>() : b->pos_off.as<uint64_t>(), n_rows, I0,
           n_items,
@@ -1086,9 +1175,10 @@ void sweep_sum_rows(const uint32_t *hits, uint32_t row_words, const uint32_t *wo
                     uint32_t n_colors, hipStream_t s) {
   uint32_t lp_bits = 0;
   while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
-  const uint64_t waves = (uint64_t)((row_words + 63) / 64) * ((n_pos + kSumSpan - 1) / kSumSpan);
-  if (waves)
-    sweep_sum_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(
+  const uint64_t blocks =
+      (uint64_t)((row_words + 63) / 64) * (((n_pos + kSumSpan - 1) / kSumSpan + 3) / 4);
+  if (n_pos)
+    sweep_sum_kernel<<<(uint32_t)blocks, 256, 0, s>>>(
         hits, row_words, lp_bits, word_color, listed, pos_off, n_reads, p0, n_pos, records, counts,
         partial, n_colors);
 }
