@@ -131,6 +131,24 @@ int zg_query_reads(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
                    uint64_t n_reads, uint32_t flags, uint64_t *totals,
                    uint32_t *counts, uint64_t *aggregate);
 
+/* No counterpart in the reference: zg_query_reads for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc, pipelines that capture their own graphs around it). Same
+ * arguments, results and errors as zg_query_reads (bit-identical outputs);
+ * every device buffer comes from one static 64 MiB array per GPU, allocated
+ * when the library's code is loaded and held until the process ends. A call
+ * whose working set exceeds it (about 1.4 bytes per base plus 24 bytes per
+ * read, plus 4 bytes per read and colour with ZG_QUERY_RECORDS; minimizers add
+ * about 2 bytes per base and 16 per key) fails with ZG_ERR_INVALID before any
+ * GPU work. Covers k-mers (k <= 32), minimizers and modimizers without
+ * homopolymer compression, every subindex resident; other indexes get
+ * ZG_ERR_UNSUPPORTED. Calls that overlap on one GPU share the array and race:
+ * serialize them (one host thread per GPU). */
+int zg_query_reads_noalloc(zg_index *idx, const uint8_t *seq,
+                           const uint64_t *read_off, uint64_t n_reads,
+                           uint32_t flags, uint64_t *totals, uint32_t *counts,
+                           uint64_t *aggregate);
+
 /* Seeds the GPU selects for each read (scan_index_features, src/lib.rs:574;
  * open_closed_minimizers::scan_positions, utils/ocm-minimizers/src/lib.rs:4):
  * key_off[n_reads+1]; keys and positions (offset within the read) hold up to

@@ -172,6 +172,22 @@ class Index:
                                    _u64p(totals), _u32p(counts) if records else None, _u64p(agg)))
         return totals[:n], (counts[:n] if records else None), agg
 
+    def query_reads_noalloc(self, reads, offsets=None, records: bool = False,
+                            recover: bool = True):
+        """query_reads through the allocation-free entry point (a 64 MiB static
+        device array per GPU; larger queries raise)."""
+        buf, off = _as_reads(reads, offsets)
+        n = len(off) - 1
+        colors = self.meta["n_colors"]
+        flags = (ZG_QUERY_RECORDS if records else 0) | (0 if recover else ZG_QUERY_NO_RECOVERY)
+        totals = np.zeros(max(n, 1), np.uint64)
+        counts = np.zeros((max(n, 1), colors), np.uint32) if records else None
+        agg = np.zeros(colors, np.uint64)
+        check(lib().zg_query_reads_noalloc(self._h, buf.ctypes.data_as(C.c_void_p), _u64p(off), n,
+                                           flags, _u64p(totals),
+                                           _u32p(counts) if records else None, _u64p(agg)))
+        return totals[:n], (counts[:n] if records else None), agg
+
     def extract_seeds(self, reads, offsets=None):
         """GPU-selected seeds: (key_off[n+1], keys, read-relative positions)."""
         buf, off = _as_reads(reads, offsets)

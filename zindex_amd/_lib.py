@@ -114,6 +114,8 @@ SIGNATURES = {
                                     C.POINTER(u32p), u64p]),
     "zg_query_reads": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, C.c_uint32,
                                  u64p, u32p, u64p]),
+    "zg_query_reads_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, C.c_uint32,
+                                         u64p, u32p, u64p]),
     "zg_extract_seeds": (C.c_int, [C.c_void_p, C.c_void_p, u64p, C.c_uint64, u64p, u64p,
                                    u64p, C.c_uint64]),
     "zg_query_command": (C.c_int, [C.c_char_p, C.c_char_p, C.c_double, C.c_int, C.c_int,

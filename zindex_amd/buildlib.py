@@ -22,7 +22,7 @@ LIB = PKG / "libzindex_amd.so"
 CLI = PKG / "zorindex"
 
 HIP_SOURCES = ["zg_kernels.hip", "zg_scan.hip", "zg_sweep.hip", "zg_api.hip", "zg_build.hip",
-               "zg_serve.hip", "zg_peel.hip"]
+               "zg_serve.hip", "zg_peel.hip", "zg_noalloc.hip"]
 CPP_SOURCES = ["zg_format.cpp", "zg_fastx.cpp"]
 
 # Packed FP32 instructions are disabled for all device code (none of these

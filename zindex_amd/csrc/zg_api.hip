@@ -853,7 +853,7 @@ static void ensure_partials(zg_batch *b) {
   }
 }
 
-static LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
+LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank) {
   LookupArgs a{};
   a.bank = idx->dbanks[bank].view;
   a.rec = idx->rec_view;

@@ -13,23 +13,45 @@
 
 namespace zg {
 
-// Owning device allocation.
+// Device allocation: owned (freed on reset) or borrowed from memory the
+// caller manages (never freed here).
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), owned(o.owned) {
+    o.p = nullptr;
+    o.bytes = 0;
+    o.owned = true;
+  }
   DevBuf &operator=(DevBuf &&o) noexcept {
-    if (this != &o) { reset(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      reset();
+      p = o.p;
+      bytes = o.bytes;
+      owned = o.owned;
+      o.p = nullptr;
+      o.bytes = 0;
+      o.owned = true;
+    }
     return *this;
   }
   ~DevBuf() { reset(); }
   void reset() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    owned = true;
+  }
+  // Views n bytes at q (not freed by this object).
+  void borrow(void *q, size_t n) {
+    reset();
+    p = q;
+    bytes = n;
+    owned = false;
   }
   void alloc(size_t n);
   // Grows to at least n bytes (contents are not preserved when it grows).
@@ -195,6 +217,8 @@ void configure(zg_index *idx, zg_batch *b);
 void seed_batch(zg_index *idx, zg_batch *b);
 void enqueue_zero_outputs(zg_index *idx, zg_batch *b);
 void enqueue_lookups(zg_index *idx, zg_batch *b);
+// The lookup arguments of one bank for a seeded batch.
+LookupArgs lookup_args(zg_index *idx, zg_batch *b, size_t bank);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
 int translate_exception();
 // Host-side helpers shared with the builder.
