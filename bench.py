@@ -608,62 +608,84 @@ def main() -> int:
             f"{st['sweep_launches']} chunks, {st['sweep_folded']} folded; seeding "
             f"{st['last_seed_ms']:.1f} ms")
 
-    # roofline of the lookup kernel: algorithmic bytes per launch / launch time
+    # roofline of the dominant kernel: algorithmic bytes per launch / its
+    # average launch time (HIP events on the stream the kernel runs on)
     positions = int(off[-1]) + args.reads
     lk_ms = float(np.mean(lookup_ms))
     sweep = bool(st["tuning"]["sweep"])
     if sweep:
-        # sorted sweep: one lookup launch per chunk of reads over the chunk's
-        # distinct seeds (filter rows + the 8-byte first hash of each seed)
-        launches = max(1, st["sweep_launches"])
-        per_lookup = st["algorithmic_bytes"] // max(1, seeds)
-        algo = seeds * (per_lookup + 8) // launches
+        # sorted sweep: the staged lookup, one launch per chunk of reads and
+        # bank, reads every bank row once, each seed's 8-byte first hash and
+        # 4-byte position once, and writes each seed's hit row once
+        launches = max(1, st["sweep_stage_launches"])
+        algo = st["sweep_algorithmic_bytes"] // launches
         kernel_ms = st["sweep_lookup_ms"] / launches
+        kernel = "stage_lookup_kernel"
     elif args.mode == "kmers":  # packed bases and invalid bits read by the lookup
         algo = st["algorithmic_bytes"] + positions * 3 // 8
         kernel_ms = lk_ms
+        kernel = "lookup_kernel"
     else:  # explicit minimizer keys
         algo = st["algorithmic_bytes"] + seeds * 8
         kernel_ms = lk_ms
+        kernel = "lookup_kernel"
     achieved = algo / (kernel_ms / 1000) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "lookup_kernel", "launch_ms": round(kernel_ms, 3),
+                "kernel": kernel, "launch_ms": round(kernel_ms, 3),
                 "algorithmic_bytes_per_launch": algo}
     if sweep:
-        roofline.update({"sorted_sweep": True, "launches_per_step": launches,
-                         "seeds_per_step": seeds, "folded_per_step": st["sweep_folded"],
-                         "sweep_stage_ms": round(lk_ms, 3)})
-    # the ceiling of the row gathers alone: random whole rows of the bank's row
-    # size, the same bytes per launch, no other work
-    ceiling = gather_ceiling(idx, algo)
-    if ceiling:
-        roofline.update(ceiling)
-        roofline["frac_of_gather_ceiling"] = round(achieved / ceiling["gather_ceiling_gbs"], 4)
-        if "key_rows_ceiling_gbs" in ceiling:
-            roofline["frac_of_key_rows_ceiling"] = round(achieved / ceiling["key_rows_ceiling_gbs"], 4)
-        log(f"gather ceiling at {ceiling['gather_row_bytes']} B rows: "
-            f"{ceiling['gather_ceiling_gbs']:.0f} GB/s (key-row pattern "
-            f"{ceiling.get('key_rows_ceiling_gbs')}); lookup at "
-            f"{roofline['frac_of_gather_ceiling']:.1%} of it")
+        sum_bytes = seeds * st["sweep_hit_row_bytes"]  # hit rows read back once
+        roofline.update({
+            "launches_per_step": launches, "chunks_per_step": st["sweep_launches"],
+            "seeds_per_step": seeds, "hit_row_bytes": st["sweep_hit_row_bytes"],
+            "folded_per_step": st["sweep_folded"],
+            "phases_ms": {"seeding": round(st["last_seed_ms"], 2),
+                          "radix_sort": round(st["sweep_sort_ms"], 2),
+                          "staged_lookup": round(st["sweep_lookup_ms"], 2),
+                          "sum": round(st["sweep_sum_ms"], 2)},
+            "sum_kernel_gbs": round(sum_bytes / (st["sweep_sum_ms"] / 1000) / 1e9, 1)
+            if st["sweep_sum_ms"] > 0 else None})
+    else:
+        # the ceiling of the row gathers alone: random whole rows of the
+        # bank's row size, the same bytes per launch, no other work
+        ceiling = gather_ceiling(idx, algo)
+        if ceiling:
+            roofline.update(ceiling)
+            roofline["frac_of_gather_ceiling"] = round(achieved / ceiling["gather_ceiling_gbs"], 4)
+            if "key_rows_ceiling_gbs" in ceiling:
+                roofline["frac_of_key_rows_ceiling"] = round(
+                    achieved / ceiling["key_rows_ceiling_gbs"], 4)
+            log(f"gather ceiling at {ceiling['gather_row_bytes']} B rows: "
+                f"{ceiling['gather_ceiling_gbs']:.0f} GB/s (key-row pattern "
+                f"{ceiling.get('key_rows_ceiling_gbs')}); lookup at "
+                f"{roofline['frac_of_gather_ceiling']:.1%} of it")
     if args.gather_sweep:
         roofline["gather_sweep_gbs"] = {rb: round(idx.gather_probe(rb, (1 << 40) // rb), 1)
                                         for rb in (512, 1024, 2048, 4096, 8192)}
         log(f"gather sweep: {roofline['gather_sweep_gbs']}")
+    log(f"roofline: {kernel} {achieved:.1f} GB/s algorithmic over {kernel_ms:.2f} ms per launch "
+        f"= {achieved / HBM_PEAK_GBS:.1%} of HBM peak")
     workload = workload_name(args, records)
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes of this
-    # workload (scaled by a copy kernel of known size), when recorded for it
-    for tf in sorted((ROOT / "profiles").glob("*_traffic.json")):
+    # workload (scaled by a copy kernel of known size), and the kernel's
+    # instruction mix from SQ counters, when recorded for it
+    tuned = ",".join(str(st["tuning"][k]) for k in ("span", "tiles_per_wave", "prefetch",
+                                                      "waves_per_cu", "nontemporal", "sweep"))
+    for tf in sorted((ROOT / "profiles").glob("*.json")):
         rec = json.loads(tf.read_text())
-        tuned = ",".join(str(st["tuning"][k]) for k in ("span", "tiles_per_wave", "prefetch",
-                                                          "waves_per_cu", "nontemporal", "sweep"))
-        if (rec.get("workload") == workload and rec.get("algorithmic_bytes") == algo
+        if not (rec.get("workload") == workload and rec.get("kernel_name") == kernel
+                and rec.get("algorithmic_bytes") == algo
                 and tuned.startswith(rec.get("tuning", ""))):
+            continue
+        if "traffic_bytes" in rec:
             roofline["traffic"] = rec["traffic_bytes"]
             roofline["traffic_source"] = str(tf.relative_to(ROOT))
             # the measured HBM bytes at this run's launch time
             roofline["traffic_gbs"] = round(rec["traffic_bytes"] / (kernel_ms / 1000) / 1e9, 1)
             roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
+        if "issue" in rec:
+            roofline["issue"] = dict(rec["issue"], source=str(tf.relative_to(ROOT)))
 
     plain = None
     if args.plain_reads:
@@ -708,7 +730,11 @@ def main() -> int:
             "bank_bytes": meta["payload_bytes"], "tuning": st["tuning"],
             "step": ("pack ASCII->2-bit + k-mer counts" if args.mode == "kmers" else
                      "pack ASCII->2-bit + open-closed minimizer selection and compaction")
-                    + " + persistent lookup (HIP graph)",
+                    + (f" + per chunk of whole reads ({st['sweep_launches']} chunks): stable LSD "
+                       "radix passes of the seeds by PositionHash segment (6-bit digits), the "
+                       "staged lookup from LDS segment windows into hit rows, per-read sums "
+                       "of the hit rows; eager launches with no host synchronization inside "
+                       "the step" if sweep else " + persistent lookup (HIP graph)"),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

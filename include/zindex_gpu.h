@@ -133,7 +133,8 @@ int zg_query_reads(zg_index *idx, const uint8_t *seq, const uint64_t *read_off,
 
 /* No counterpart in the reference: zg_query_reads for callers that cannot
  * allocate device memory (tight serving loops, harnesses that forbid
- * hipMalloc, pipelines that capture their own graphs around it). Same
+ * hipMalloc). It copies host buffers and synchronizes, so it is not for
+ * stream capture. Same
  * arguments, results and errors as zg_query_reads (bit-identical outputs);
  * every device buffer comes from one static 64 MiB array per GPU, allocated
  * when the library's code is loaded and held until the process ends. A call
@@ -269,12 +270,13 @@ typedef struct zg_tuning {
                               4; also 8, 16 for fingerprints of <= 4 bits */
   uint32_t waves_per_cu;   /* persistent waves per compute unit (0 = max) */
   uint32_t nontemporal;    /* 1: row gathers use non-temporal loads */
-  uint32_t sweep;          /* 1: seeds of a single-bank k-mer index are
-                              looked up in PositionHash segment order (a
-                              radix sort per chunk of reads) from segment
-                              windows staged in LDS, each seed's hits into a
-                              hit row at its position, the rows summed per
-                              read; same results as 0 */
+  uint32_t sweep;          /* 1: seeds are looked up in PositionHash segment
+                              order (a radix sort per chunk of reads and size
+                              class) from segment windows staged in LDS, each
+                              seed's hits into a hit row at its position, the
+                              rows summed per read; same results as 0.
+                              Ignored (read order) for findere indexes,
+                              streamed subindexes and rows too wide for LDS */
 } zg_tuning;
 
 typedef struct zg_batch_stats {
@@ -290,10 +292,17 @@ typedef struct zg_batch_stats {
   uint64_t sweep_folded;     /* last sorted-sweep run: seed occurrences answered
                                 from the occurrence before them in bank order
                                 (equal keys, the per-read fold); 0 otherwise */
-  uint64_t sweep_launches;   /* its chunks (one staged lookup launch each) */
+  uint64_t sweep_launches;   /* its chunks of whole reads */
   double sweep_lookup_ms;    /* HIP-event time of its staged lookups */
   double sweep_sort_ms;      /* ... of its radix passes and segment bounds */
   double sweep_sum_ms;       /* ... of its per-read sums */
+  uint64_t sweep_stage_launches;    /* staged lookup launches (chunks x banks) */
+  uint64_t sweep_algorithmic_bytes; /* bytes its staged lookups must move per
+                                       run: every bank row read once per
+                                       chunk, each seed's 8-byte first hash and
+                                       4-byte position read once and its hit
+                                       row written once */
+  uint64_t sweep_hit_row_bytes;     /* hit-row bytes per position */
 } zg_batch_stats;
 
 /* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base

@@ -71,6 +71,8 @@ class BatchStats(C.Structure):
         ("last_total_ms", C.c_double), ("tuning", Tuning), ("sweep_folded", C.c_uint64),
         ("sweep_launches", C.c_uint64), ("sweep_lookup_ms", C.c_double),
         ("sweep_sort_ms", C.c_double), ("sweep_sum_ms", C.c_double),
+        ("sweep_stage_launches", C.c_uint64), ("sweep_algorithmic_bytes", C.c_uint64),
+        ("sweep_hit_row_bytes", C.c_uint64),
     ]
 
 

@@ -1445,6 +1445,14 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->sweep_lookup_ms = o->tuning.sweep ? b->sweep_lookup_ms : 0;
     o->sweep_sort_ms = o->tuning.sweep ? b->sweep_sort_ms : 0;
     o->sweep_sum_ms = o->tuning.sweep ? b->sweep_sum_ms : 0;
+    if (o->tuning.sweep) {
+      uint64_t bank_bytes = 0;
+      for (auto &bk : idx->m.banks) bank_bytes += device_bytes(bk);
+      o->sweep_stage_launches = b->sweep_chunks * idx->m.banks.size();
+      o->sweep_hit_row_bytes = idx->sw_row_words * 4ULL;
+      o->sweep_algorithmic_bytes =
+          b->sweep_chunks * bank_bytes + o->seeds * (12 + o->sweep_hit_row_bytes);
+    }
     return 0;
   } catch (...) {
     return translate_exception();

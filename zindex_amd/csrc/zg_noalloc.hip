@@ -1,8 +1,7 @@
 // zg_noalloc.hip -- zg_query_reads without device allocation.
 //
-// Callers that may not allocate device memory (code under HIP graph capture
-// in the caller's own pipeline, tight serving loops, harnesses that forbid
-// hipMalloc) run the read-ordered lookup out of one static device array per
+// Callers that may not allocate device memory (tight serving loops,
+// harnesses that forbid hipMalloc) run the read-ordered lookup out of one static device array per
 // GPU. Every working buffer of the query -- the reads and their offsets, the
 // packed bases and invalid bits, minimizer flags, keys and scan levels, the
 // per-read totals and counts, per-wave partial sums, the aggregate and the
