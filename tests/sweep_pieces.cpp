@@ -23,7 +23,7 @@ int sweep_sort_entries(const uint64_t *h, const uint32_t *p, const uint64_t *n_d
                        hipStream_t s);
 void sweep_bounds(const uint64_t *h, const uint64_t *n_dev, uint64_t n_max, uint64_t count,
                   uint32_t *start, hipStream_t s);
-void sweep_sum_rows(const uint32_t *hits, uint32_t row_words, const uint32_t *word_color,
+void sweep_sum_rows(uint32_t *hits, uint32_t row_words, const uint32_t *word_color,
                     const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0,
                     uint64_t n_pos, uint32_t records, uint32_t *counts, uint64_t *partial,
                     uint32_t n_colors, hipStream_t s);

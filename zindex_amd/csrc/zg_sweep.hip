@@ -237,6 +237,27 @@ __global__ void zero_members_kernel(uint32_t *hits, uint64_t n_pos, uint32_t row
     hits[(i / words) * row_words + first + i % words] = 0;
 }
 
+// Hit rows of the chunk's positions that hold no entry (read separators,
+// windows with invalid bases) to zero, so the sums read every row: one wave
+// per 64 positions of the bitmap of listed positions.
+__global__ __launch_bounds__(256) void zero_unlisted_kernel(uint32_t *hits, uint32_t row_words,
+                                                            const uint64_t *listed,
+                                                            uint64_t n_pos) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t words = (n_pos + 63) / 64;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t wi = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); wi < words;
+       wi += waves) {
+    uint64_t m = ~listed[wi];
+    if (wi == words - 1 && (n_pos & 63)) m &= (1ULL << (n_pos & 63)) - 1;
+    while (m) {
+      const uint64_t p = wi * 64 + (uint64_t)__builtin_ctzll(m);
+      m &= m - 1;
+      for (uint32_t w = lane; w < row_words; w += 64) hits[p * row_words + w] = 0;
+    }
+  }
+}
+
 // Hit bits of the lane's 8 plane bytes (w0 low, w1 high), fields of b bits
 // little endian (the plane layout, storage.rs:1721-1754): field f equal to
 // fp sets bit field_bit<B>(f). At b = 8 the zero bytes of the two words are
@@ -644,27 +665,36 @@ __device__ __forceinline__ void sum_spill(uint32_t (&pl)[kPlanes], uint32_t (&to
   for (int j = 0; j < kPlanes; j++) pl[j] = 0;
 }
 
-// Adds the 32 counts of a lane's word to its read (row: counts + r * n_colors)
-// or to the colour sums; color[b] is the colour of bit b of the word.
-__device__ __forceinline__ void sum_flush(uint32_t (&pl)[kPlanes], uint32_t (&tot)[32], bool live,
-                                          const uint32_t *color, uint32_t records, uint32_t *row,
+// Adds the counts of a slot (the LP lanes that cover one position's row
+// words; lane j of the slot holds word j) to its read (row: counts +
+// r * n_colors) or to the colour sums. The slot's LP x 32 per-bit counts go
+// through LDS (word-major, `stage`) so that each atomic instruction covers
+// LP consecutive bits of a word -- neighbouring colours -- instead of one bit
+// of every word; color[i] is the colour of bit i % 32 of word i / 32.
+__device__ __forceinline__ void sum_flush(uint32_t (&pl)[kPlanes], uint32_t (&tot)[32],
+                                          uint32_t *stage, const uint32_t *color, uint32_t j,
+                                          uint32_t lp_log2, uint32_t records, uint32_t *row,
                                           uint64_t *partial) {
   sum_spill(pl, tot);
-  if (live) {
-    uint32_t cs[32];
+  uint4 *mine = (uint4 *)(stage + j * 32);
 #pragma clang loop unroll(full)
-    for (int b = 0; b < 32; b++) cs[b] = color[b];
-#pragma clang loop unroll(full)
-    for (int b = 0; b < 32; b++) {
-      if (tot[b]) {
-        const uint32_t c = cs[b];
-        if (c != kNone) {
-          if (records) atomicAdd(&row[c], tot[b]);
-          else atomicAdd((unsigned long long *)&partial[c], (unsigned long long)tot[b]);
-        }
-      }
+  for (int q = 0; q < 8; q++)
+    mine[q] = make_uint4(tot[4 * q], tot[4 * q + 1], tot[4 * q + 2], tot[4 * q + 3]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma clang loop unroll(disable)
+  for (uint32_t t = 0; t < 32; t++) {
+    const uint32_t i = (t << lp_log2) + j;
+    const uint32_t v = stage[i], c = color[i];
+    if (v && c != kNone) {
+      if (records) atomicAdd(&row[c], v);
+      else atomicAdd((unsigned long long *)&partial[c], (unsigned long long)v);
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma clang loop unroll(full)
   for (int b = 0; b < 32; b++) tot[b] = 0;
 }
@@ -725,24 +755,26 @@ __device__ __forceinline__ void add_planes(uint32_t (&pl)[kPlanes], const uint32
 // Per-read sums of the hit rows, in position order. A wave takes a span of
 // positions and a group of 64 row words; lane l reads word (l % LP) of
 // position (l / LP) + PPI*i (LP lanes per position, PPI = 64 / LP positions
-// per step; listed null: every index holds an entry). Blocks of 16 steps in
-// which no lane reaches the end of its read are counted by a carry-save
-// adder tree and added to bit-sliced counters; other blocks go step by step.
-// Counters spill to per-bit counts before they can overflow and are added to
-// the read's counts at its end (records) or to partial[colour].
+// per step). Every row of the span is read: rows of positions without an
+// entry are zero. Blocks of 16 steps in which no lane reaches the end of its
+// read are counted by a carry-save adder tree and added to bit-sliced
+// counters; other blocks go step by step. Counters spill to per-bit counts
+// before they can overflow and are added to the read's counts at its end
+// (records) or to partial[colour].
 __global__ __launch_bounds__(256) void sweep_sum_kernel(
     const uint32_t *hits, uint32_t row_words, uint32_t lp_log2, const uint32_t *word_color,
-    const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t n_pos,
-    uint32_t records, uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
+    const uint64_t *pos_off, uint64_t n_reads, uint64_t p0, uint64_t n_pos, uint32_t records,
+    uint32_t *counts, uint64_t *partial, uint32_t n_colors) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t groups = (row_words + 63) / 64;
   const uint64_t spans = (n_pos + kSumSpan - 1) / kSumSpan;
   // a block's four waves share one group of row words (its colour table is
   // staged in LDS) and take four consecutive spans; wave-uniform (scalar)
   const uint32_t g = blockIdx.x % groups;
-  const uint64_t sp = (uint64_t)(blockIdx.x / groups) * 4 +
-                      (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t sp = (uint64_t)(blockIdx.x / groups) * 4 + wave;
   __shared__ uint32_t color_lds[64 * 32];
+  __shared__ uint32_t flush_lds[4][64 * 32];  // per wave: its slots' per-bit counts
   for (uint32_t i = threadIdx.x; i < 64 * 32; i += 256) {
     const uint32_t wi = g * 64 + i / 32;
     color_lds[i] = wi < row_words ? word_color[(uint64_t)g * 64 * 32 + i] : kNone;
@@ -750,16 +782,17 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
   __syncthreads();
   if (sp >= spans) return;
   const uint32_t LP = 1u << lp_log2, PPI = 64 / LP;
-  const uint32_t slot = lane >> lp_log2;
-  const uint32_t w = g * 64 + (lane & (LP - 1));
+  const uint32_t slot = lane >> lp_log2, j = lane & (LP - 1);
+  const uint32_t w = g * 64 + j;
   const bool live = w < row_words;
+  uint32_t *stage = flush_lds[wave] + slot * LP * 32;
   const uint64_t q0 = sp * kSumSpan, q1 = q0 + kSumSpan < n_pos ? q0 + kSumSpan : n_pos;
   uint64_t first = q0 + slot < q1 ? q0 + slot : q1 - 1;
   uint64_t r = records ? find_segment(pos_off, n_reads, p0 + first) : 0;
   uint64_t next = records ? pos_off[r + 1] - p0 : ~0ULL;  // relative end of read r
   uint32_t pl[kPlanes], tot[32];
 #pragma clang loop unroll(full)
-  for (int j = 0; j < kPlanes; j++) pl[j] = 0;
+  for (int k = 0; k < kPlanes; k++) pl[k] = 0;
 #pragma clang loop unroll(full)
   for (int b = 0; b < 32; b++) tot[b] = 0;
   uint32_t fill = 0;
@@ -776,21 +809,11 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
     if (!__ballot(cross)) {
       // no lane reaches the end of its read in these U steps
       uint32_t xw[U];
-      uint64_t lw[U];
       const uint32_t *at = hits + (pp + slot) * row_words + w;
 #pragma clang loop unroll(full)
       for (int u = 0; u < U; u++) {
         const uint64_t p = pp + slot + (uint64_t)u * PPI;
         xw[u] = live && p < q1 ? at[u * stride] : 0u;
-        // PPI divides 64 and pp is a multiple of PPI, so a step's positions
-        // share one word of the bitmap: a wave-uniform load
-        const uint64_t pw = pp + (uint64_t)u * PPI;
-        lw[u] = listed && pw < q1 ? listed[pw >> 6] : ~0ULL;
-      }
-#pragma clang loop unroll(full)
-      for (int u = 0; u < U; u++) {
-        const uint64_t p = pp + slot + (uint64_t)u * PPI;
-        if (!((lw[u] >> (p & 63)) & 1)) xw[u] = 0;
       }
       uint32_t c5[5];
       count16(xw, c5);
@@ -802,20 +825,19 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
         uint32_t x = 0;
         if (p < q1) {
           if (p >= next) {
-            sum_flush(pl, tot, live, color_lds + (lane & (LP - 1)) * 32, records,
+            sum_flush(pl, tot, stage, color_lds, j, lp_log2, records,
                       counts + r * n_colors, partial);
             while (pos_off[r + 1] - p0 <= p) r++;
             next = pos_off[r + 1] - p0;
           }
-          if (live && (!listed || ((listed[p >> 6] >> (p & 63)) & 1))) x = hits[p * row_words + w];
+          if (live) x = hits[p * row_words + w];
         }
         const uint32_t x1[1] = {x};
         add_planes<1>(pl, x1);
       }
     }
   }
-  sum_flush(pl, tot, live, color_lds + (lane & (LP - 1)) * 32, records, counts + r * n_colors,
-            partial);
+  sum_flush(pl, tot, stage, color_lds, j, lp_log2, records, counts + r * n_colors, partial);
 }
 
 uint32_t bits_for(uint64_t v) {  // bits to hold values 0..v
@@ -1094,11 +1116,14 @@ void sweep_run(zg_index *idx, zg_batch *b) {
     }
     const uint64_t groups = (row_words + 63) / 64;
     const uint64_t blocks = groups * (((n_items + kSumSpan - 1) / kSumSpan + 3) / 4);
+    if (n_items && !keys)
+      zero_unlisted_kernel<<<(uint32_t)std::min<uint64_t>(16384, ((n_items + 63) / 64 + 3) / 4), 256,
+                             0, s>>>(b->fd_hits.as<uint32_t>(), row_words,
+                                     b->sw_listed.as<uint64_t>(), n_items);
     if (n_items)
       sweep_sum_kernel<<<(uint32_t)blocks, 256, 0, s>>>(
           b->fd_hits.as<uint32_t>(), row_words, lp_bits, idx->sw_bit_color.as<uint32_t>(),
-          keys ? nullptr : b->sw_listed.as<uint64_t>(), keys ? b->key_off.as<uint64_t>() : b->pos_off.as<uint64_t>(), n_rows, I0,
-          n_items,
+          keys ? b->key_off.as<uint64_t>() : b->pos_off.as<uint64_t>(), n_rows, I0, n_items,
           (b->records || strobes) ? 1 : 0, b->counts.as<uint32_t>(), b->partial.as<uint64_t>(),
           (uint32_t)m.colors.size());
     check_hip(hipEventRecord(ev[ev_per_chunk - 1], s), "hipEventRecord");
@@ -1169,7 +1194,10 @@ void sweep_bounds(const uint64_t *h, const uint64_t *n_dev, uint64_t n_max, uint
       h, n_dev, count, n_max, start);
 }
 
-void sweep_sum_rows(const uint32_t *hits, uint32_t row_words, const uint32_t *word_color,
+// The per-read sums of hit rows [p0, p0 + n_pos) of a chunk; with a bitmap
+// of listed positions the rows of the others are zeroed first, as the sweep
+// does.
+void sweep_sum_rows(uint32_t *hits, uint32_t row_words, const uint32_t *word_color,
                     const uint64_t *listed, const uint64_t *pos_off, uint64_t n_reads, uint64_t p0,
                     uint64_t n_pos, uint32_t records, uint32_t *counts, uint64_t *partial,
                     uint32_t n_colors, hipStream_t s) {
@@ -1177,10 +1205,13 @@ void sweep_sum_rows(const uint32_t *hits, uint32_t row_words, const uint32_t *wo
   while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
   const uint64_t blocks =
       (uint64_t)((row_words + 63) / 64) * (((n_pos + kSumSpan - 1) / kSumSpan + 3) / 4);
+  if (n_pos && listed)
+    zero_unlisted_kernel<<<(uint32_t)std::min<uint64_t>(16384, ((n_pos + 63) / 64 + 3) / 4), 256,
+                           0, s>>>(hits, row_words, listed, n_pos);
   if (n_pos)
-    sweep_sum_kernel<<<(uint32_t)blocks, 256, 0, s>>>(
-        hits, row_words, lp_bits, word_color, listed, pos_off, n_reads, p0, n_pos, records, counts,
-        partial, n_colors);
+    sweep_sum_kernel<<<(uint32_t)blocks, 256, 0, s>>>(hits, row_words, lp_bits, word_color, pos_off,
+                                                      n_reads, p0, n_pos, records, counts, partial,
+                                                      n_colors);
 }
 
 }  // namespace zg
