@@ -103,3 +103,36 @@ def test_plain_method_small_at_stated_size():
         idx.tuning = tuning
         counts = check_parity(idx, orc, reads, label=f"10 x 2 Mbp, 64 x 1 kbp, {name}")
     assert counts.sum() > 0
+
+
+@pytest.mark.parametrize("case", ["h4 recovery", "h3 partitioned", "h2"])
+def test_lane_lookup_wide_rows(case, monkeypatch):
+    # rows of >= 512 colours at b = 8 (whole 512-byte slices, one entry per
+    # lane in the staged lookup), mixed with the wave-per-entry pass for
+    # entries with recovery postings, routed to partitions, and a last slice
+    # holding fewer than 512 fields
+    rng = np.random.default_rng(500 + len(case))
+    if case == "h4 recovery":
+        colors = make_colors(501, 600, 3000, rates=(0.0, 0.003))
+        cfg = dict(hashes=4, slot_scale=0.5)
+    elif case == "h3 partitioned":
+        colors = make_colors(502, 560, 3000)
+        cfg = dict(hashes=3, slot_scale=0.8, max_subindex_size=1 << 20)
+    else:
+        colors = make_colors(503, 530, 2500)
+        cfg = dict(hashes=2)
+    reads = make_reads(int(rng.integers(1 << 30)), colors, 200, lo=30, hi=2500)
+    idx = build(colors, fingerprint_bits=8, **cfg)
+    if case == "h3 partitioned":
+        assert idx.meta["n_banks"] > 1
+    if case == "h4 recovery":
+        most = postings_of_query_keys(idx, reads)
+        say(f"{case}: largest posting list hit by the reads: {most}")
+        assert most > 64
+    idx.load(recover=True)
+    orc = oracle_for(idx, True)
+    idx.tuning = SWEEP_BENCH
+    assert_sweep_runs(idx, reads)
+    check_parity(idx, orc, reads, True, label=f"lane lookup {case}")
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "9000")
+    check_parity(idx, orc, reads, True, label=f"lane lookup {case}, chunks")

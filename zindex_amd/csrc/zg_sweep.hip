@@ -338,6 +338,67 @@ __device__ __forceinline__ uint64_t fields_below(int64_t n) {
   }
 }
 
+// Hit bits of one entry per lane over a 512-byte slice at b = 8: lane l's
+// entry reads its H window rows (byte offsets rb[i], multiples of 512) in
+// 16-byte pieces, piece (l ^ c) & 31 at step c, so the 16 lanes of each LDS
+// access group read 16 different bank groups. Each piece holds two 8-byte
+// chunks (8 fields each); the entry's 64 hit bytes are built in that order
+// in o[] (half-word c = piece (l ^ c) & 31) and put back in field order.
+template <int HT>
+__device__ __forceinline__ void lane_slice_hits_b8(const uint8_t *win, uint32_t lane,
+                                                   const uint32_t (&rb)[8], uint32_t H,
+                                                   uint32_t fp, uint32_t (&o)[16]) {
+  constexpr int HMAX = HT ? HT : 8;
+  const uint32_t fpb = fp * 0x01010101u;
+  const uint32_t t = lane & 31;
+  uint32_t rl[HMAX];
+#pragma unroll
+  for (int i = 0; i < HMAX; i++) rl[i] = rb[i] ^ (t * 16);
+#pragma unroll
+  for (int c = 0; c < 32; c++) {
+    uint4 r[HMAX];
+#pragma unroll
+    for (int i = 0; i < HMAX; i++)
+      if (HT || i < (int)H) r[i] = *(const uint4 *)(win + (rl[i] ^ (uint32_t)(c * 16)));
+    uint32_t a0, a1, b0, b1;
+    if constexpr (HT == 4) {
+      a0 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(r[0].x, r[1].x, r[2].x, 0x96), r[3].x, fpb, 0x96);
+      a1 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(r[0].y, r[1].y, r[2].y, 0x96), r[3].y, fpb, 0x96);
+      b0 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(r[0].z, r[1].z, r[2].z, 0x96), r[3].z, fpb, 0x96);
+      b1 = __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(r[0].w, r[1].w, r[2].w, 0x96), r[3].w, fpb, 0x96);
+    } else {
+      a0 = a1 = b0 = b1 = fpb;
+#pragma unroll
+      for (int i = 0; i < HMAX; i++)
+        if (HT || i < (int)H) {
+          a0 ^= r[i].x;
+          a1 ^= r[i].y;
+          b0 ^= r[i].z;
+          b1 ^= r[i].w;
+        }
+    }
+    const uint32_t hw = (zero_bytes_to_bits(a0, a1) & 0xFFu) | ((zero_bytes_to_bits(b0, b1) & 0xFFu) << 8);
+    if (c & 1) o[c >> 1] |= hw << 16;
+    else o[c >> 1] = hw;
+  }
+  // field order: half-word k of the result is half-word k ^ t of o
+  if (t & 1) {
+#pragma unroll
+    for (int d = 0; d < 16; d++) o[d] = (o[d] >> 16) | (o[d] << 16);
+  }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const bool sw = (t >> (b + 1)) & 1;
+#pragma unroll
+    for (int d = 0; d < 16; d++)
+      if (!(d & (1 << b))) {
+        const uint32_t x = o[d], y = o[d | (1 << b)];
+        o[d] = sw ? y : x;
+        o[d | (1 << b)] = sw ? x : y;
+      }
+  }
+}
+
 }  // namespace
 
 struct StageArgs {
@@ -462,9 +523,42 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       // a partitioned bank answers the keys routed to it (storage.rs:99-101)
       const uint64_t own_mask =
           __ballot(v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition));
+      // b = 8 over whole 512-byte slices with no merged group: one entry per
+      // lane for the entries routed here without recovery postings
+      bool by_lane = false;
+      if constexpr (B == 8 && HT != 0) {
+        if (SB == 512 && !any_merged) {
+          by_lane = true;
+          if (((own_mask & ~rec_mask) >> lane) & 1) {
+            uint32_t rb[8];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+              rb[i] = (uint32_t)((offs >> (16 * i)) & 0xFFFFu) * 64;
+              rb[i + 4] = (uint32_t)((offs_hi >> (16 * i)) & 0xFFFFu) * 64;
+            }
+            uint32_t o[16];
+            lane_slice_hits_b8<HT>(win, lane, rb, H, fp, o);
+            if (!all_fields) {
+              // fields past the bank's groups: dword d holds fields 32d .. 32d + 31
+              const int64_t lim = (int64_t)bk.groups - (int64_t)slice_c0;
+#pragma unroll
+              for (int d = 0; d < 16; d++) {
+                uint32_t mk = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) mk |= (uint32_t)fields_below<8>(lim - 32 * d - 8 * k) << (8 * k);
+                o[d] &= mk;
+              }
+            }
+            uint4 *dst = (uint4 *)((uint8_t *)(a.hits + prow + a.col_word0) + (uint64_t)q * 64);
+#pragma unroll
+            for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+          }
+        }
+      }
       // an entry whose seed equals the entry before it reuses its hits
       const uint64_t prev_h = shfl_up64(h, 1);
-      const uint64_t same_mask = __ballot(v && lane > 0 && prev_h == h) & own_mask & (own_mask << 1);
+      const uint64_t same_mask =
+          by_lane ? 0 : __ballot(v && lane > 0 && prev_h == h) & own_mask & (own_mask << 1);
       folded += __popcll(same_mask);
       uint64_t m_seed = 0;
       // staged entries (their lanes j) and the write-out: lane l stores 16
@@ -539,7 +633,7 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       // scalar find-first-set); it reads only LDS and writes only staged
       // rows. Entries whose merged groups hit are fixed up below.
       uint64_t fix_mask = 0;
-      for (uint64_t todo = own_mask & ~rec_mask; todo; todo &= todo - 1) {
+      for (uint64_t todo = by_lane ? 0 : own_mask & ~rec_mask; todo; todo &= todo - 1) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         if (!((same_mask >> j) & 1)) m_seed = seed_hits(j);
         uint64_t m = m_seed;
