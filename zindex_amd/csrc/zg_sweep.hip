@@ -456,16 +456,30 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
     const uint32_t q = (uint32_t)(u % a.n_slices);
     const uint32_t e0 = a.start[s], e1 = a.start[s + 1];
     if (e0 == e1) continue;
-    // the window: rows s*L .. (s+H)*L - 1, bytes [q*SB, (q+1)*SB) of each
+    // the window: rows s*L .. (s+H)*L - 1, bytes [q*SB, (q+1)*SB) of each,
+    // 16-byte pieces straight from memory into LDS: a wave's instruction
+    // fills 64 consecutive pieces, piece c at LDS offset 16c (the window's
+    // row stride SB is per_row pieces)
     {
       const uint8_t *src = bk.data + s * L * bk.row_bytes + (uint64_t)q * SB;
       const uint32_t per_row = SB / 16, chunks = H * (uint32_t)L * per_row;
-      for (uint32_t c = threadIdx.x; c < chunks; c += kStageThreads) {
-        const uint32_t row = c / per_row, col = c % per_row;
-        *(uint4 *)(win + row * SB + col * 16) =
-            *(const uint4 *)(src + (uint64_t)row * bk.row_bytes + col * 16);
+      for (uint32_t cb = wave * 64; cb < chunks; cb += kStageThreads) {
+        const uint32_t c = cb + lane;
+        if (c < chunks)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(src + (uint64_t)(c / per_row) * bk.row_bytes + (c % per_row) * 16),
+              (__attribute__((address_space(3))) void *)(win + cb * 16), 16, 0, 0);
       }
     }
+    // this wave's first entries load while the window fills; each batch then
+    // loads the next one's before its own lookups
+    uint64_t h_next = 0;
+    uint32_t p_next = 0;
+    if (e0 + wave * 64 + lane < e1) {
+      h_next = a.ent_h[e0 + wave * 64 + lane];
+      p_next = a.ent_p[e0 + wave * 64 + lane];
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     // this lane's fields: columns c0 .. c0 + F - 1
     const int64_t c0 = (int64_t)q * SB * 8 / B + (int64_t)lane * F;
@@ -477,8 +491,12 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
     for (uint32_t b0 = e0 + wave * 64; b0 < e1; b0 += kStageWaves * 64) {
       const uint32_t e = b0 + lane;
       const bool v = e < e1;
-      const uint64_t h = v ? a.ent_h[e] : 0;
-      const uint32_t p = v ? a.ent_p[e] : 0;
+      const uint64_t h = h_next;
+      const uint32_t p = p_next;
+      if (e + kStageWaves * 64 < e1) {
+        h_next = a.ent_h[e + kStageWaves * 64];
+        p_next = a.ent_p[e + kStageWaves * 64];
+      }
       const uint64_t key = unsplitmix64(h) - seed;  // mixsplit(key, seed) == h
       const uint32_t fp = (uint32_t)fingerprint(key, seed, B);
       const uint64_t step = splitmix64(h);
@@ -502,41 +520,31 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       // row offsets and the fingerprint are computed here, before the entry
       // loop, so no load is outstanding when the loop reads them
       const uint64_t prow = (uint64_t)p * a.row_words;
-      uint32_t rec_start = 0, rec_len = 0;
-      if (a.rec.enabled && v) {
-        const uint32_t ah = abandoned_hash(key, seed);
-        const uint64_t cap_mask = (1ULL << a.rec.cap_log2) - 1;
-        uint64_t slot = a.rec.cap_log2 ? (uint64_t)(ah >> (32 - a.rec.cap_log2)) : 0;
-        while (true) {
-          const uint64_t de = a.rec.dir[slot];
-          const uint32_t pi = (uint32_t)de;
-          if (pi == kNone) break;
-          if ((uint32_t)(de >> 32) == ah) {
-            rec_len = a.rec.post[pi];
-            rec_start = pi + 1;
-            break;
-          }
-          slot = (slot + 1) & cap_mask;
-        }
-      }
-      const uint64_t rec_mask = __ballot(rec_len != 0);
+      // recovery directory probe (open addressing on the high hash bits); its
+      // first load is in flight during the lane lookup below
+      const bool probe = a.rec.enabled && v;
+      const uint32_t ah = probe ? abandoned_hash(key, seed) : 0;
+      const uint64_t cap_mask = (1ULL << a.rec.cap_log2) - 1;
+      uint64_t slot = a.rec.cap_log2 ? (uint64_t)(ah >> (32 - a.rec.cap_log2)) : 0;
+      uint64_t de = probe ? a.rec.dir[slot] : (uint64_t)kNone;
       // a partitioned bank answers the keys routed to it (storage.rs:99-101)
       const uint64_t own_mask =
           __ballot(v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition));
       // b = 8 over whole 512-byte slices with no merged group: one entry per
-      // lane for the entries routed here without recovery postings
+      // lane; the rows of entries without recovery postings are written once
+      // the probe has ended (the others are written whole by the rare pass)
       bool by_lane = false;
+      uint32_t o[16];
       if constexpr (B == 8 && HT != 0) {
         if (SB == 512 && !any_merged) {
           by_lane = true;
-          if (((own_mask & ~rec_mask) >> lane) & 1) {
+          if ((own_mask >> lane) & 1) {
             uint32_t rb[8];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
               rb[i] = (uint32_t)((offs >> (16 * i)) & 0xFFFFu) * 64;
               rb[i + 4] = (uint32_t)((offs_hi >> (16 * i)) & 0xFFFFu) * 64;
             }
-            uint32_t o[16];
             lane_slice_hits_b8<HT>(win, lane, rb, H, fp, o);
             if (!all_fields) {
               // fields past the bank's groups: dword d holds fields 32d .. 32d + 31
@@ -549,10 +557,29 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
                 o[d] &= mk;
               }
             }
-            uint4 *dst = (uint4 *)((uint8_t *)(a.hits + prow + a.col_word0) + (uint64_t)q * 64);
-#pragma unroll
-            for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
           }
+        }
+      }
+      uint32_t rec_start = 0, rec_len = 0;
+      if (probe) {
+        while (true) {
+          const uint32_t pi = (uint32_t)de;
+          if (pi == kNone) break;
+          if ((uint32_t)(de >> 32) == ah) {
+            rec_len = a.rec.post[pi];
+            rec_start = pi + 1;
+            break;
+          }
+          slot = (slot + 1) & cap_mask;
+          de = a.rec.dir[slot];
+        }
+      }
+      const uint64_t rec_mask = __ballot(rec_len != 0);
+      if constexpr (B == 8 && HT != 0) {
+        if (by_lane && (((own_mask & ~rec_mask) >> lane) & 1)) {
+          uint4 *dst = (uint4 *)((uint8_t *)(a.hits + prow + a.col_word0) + (uint64_t)q * 64);
+#pragma unroll
+          for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
         }
       }
       // an entry whose seed equals the entry before it reuses its hits
