@@ -488,6 +488,24 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
     const bool all_fields = __ballot(fmask != fields_below<B>(F)) == 0;
     const bool any_merged = __ballot(mmask != 0) != 0;  // merged groups in this slice
     const uint32_t slice_c0 = q * SB * 8 / B, slice_c1 = slice_c0 + 64 * F;
+    // b = 8, one entry per lane: the valid fields of a slice's 16 hit words
+    // (the slice holding the last groups has fewer than 512)
+    uint32_t tail_mask[16];
+    if constexpr (B == 8 && HT != 0) {
+      const int32_t nv = (int32_t)std::min<int64_t>(std::max<int64_t>((int64_t)bk.groups - slice_c0, 0), 512);
+      const int32_t full = nv >> 3;  // whole valid 8-field chunks
+      const uint32_t part = (uint32_t)fields_below<8>(nv & 7);
+#pragma unroll
+      for (int d = 0; d < 16; d++) {
+        uint32_t mk = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int32_t c = 4 * d + k;
+          mk |= (c < full ? 0xFFu : c == full ? part : 0u) << (8 * k);
+        }
+        tail_mask[d] = mk;
+      }
+    }
     for (uint32_t b0 = e0 + wave * 64; b0 < e1; b0 += kStageWaves * 64) {
       const uint32_t e = b0 + lane;
       const bool v = e < e1;
@@ -531,8 +549,7 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       const uint64_t own_mask =
           __ballot(v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition));
       // b = 8 over whole 512-byte slices with no merged group: one entry per
-      // lane; the rows of entries without recovery postings are written once
-      // the probe has ended (the others are written whole by the rare pass)
+      // lane, its row written once the probe has ended
       bool by_lane = false;
       uint32_t o[16];
       if constexpr (B == 8 && HT != 0) {
@@ -547,15 +564,8 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
             }
             lane_slice_hits_b8<HT>(win, lane, rb, H, fp, o);
             if (!all_fields) {
-              // fields past the bank's groups: dword d holds fields 32d .. 32d + 31
-              const int64_t lim = (int64_t)bk.groups - (int64_t)slice_c0;
 #pragma unroll
-              for (int d = 0; d < 16; d++) {
-                uint32_t mk = 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) mk |= (uint32_t)fields_below<8>(lim - 32 * d - 8 * k) << (8 * k);
-                o[d] &= mk;
-              }
+              for (int d = 0; d < 16; d++) o[d] &= tail_mask[d];
             }
           }
         }
@@ -576,10 +586,52 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       }
       const uint64_t rec_mask = __ballot(rec_len != 0);
       if constexpr (B == 8 && HT != 0) {
-        if (by_lane && (((own_mask & ~rec_mask) >> lane) & 1)) {
-          uint4 *dst = (uint4 *)((uint8_t *)(a.hits + prow + a.col_word0) + (uint64_t)q * 64);
+        if (by_lane) {
+          if ((own_mask >> lane) & 1) {
+            uint4 *dst = (uint4 *)((uint8_t *)(a.hits + prow + a.col_word0) + (uint64_t)q * 64);
 #pragma unroll
-          for (int k = 0; k < 4; k++) dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+            for (int k = 0; k < 4; k++)
+              dst[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+          }
+          const uint64_t fix = own_mask & rec_mask;
+          if (fix) {
+            // recovery (storage.rs:2771-2787, 2916-2924): a posting's colour
+            // counts as a hit. The rows are in memory before the bits of
+            // the recovered colours in this slice are ORed in; the postings
+            // of all recovered entries of the batch go 64 per pass.
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const uint32_t len = ((fix >> lane) & 1) ? rec_len : 0u;
+            const uint32_t end = (uint32_t)wave_inclusive_scan(len, lane);
+            const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
+            uint32_t *ends = (uint32_t *)rec_bits;  // inclusive ends of the lanes' postings
+            ends[lane] = end;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+              const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
+              uint32_t j = 0;  // the first lane whose postings end after g
+#pragma unroll
+              for (uint32_t st = 32; st; st >>= 1)
+                if (ends[j + st - 1] <= g) j += st;
+              const uint32_t first = j ? ends[j - 1] : 0u;
+              const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)j, 64);
+              const uint64_t pj = shfl64(prow, j);
+              if (g0 + lane < total) {
+                const uint32_t pe = a.rec.post_col[rs + (g - first)];
+                const uint32_t col = pe & 0xFFFFFFu;
+                if (pe != kNone && (pe >> 24) == bk.colmap && col >= slice_c0 && col < slice_c1 &&
+                    col < bk.groups) {
+                  const uint32_t rel = col - slice_c0, ch = rel / 8;
+                  atomicOr(a.hits + pj + a.col_word0 + q * 16 + ch / 4,
+                           1u << (8 * (ch % 4) + field_bit<8>(rel % 8)));
+                }
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          }
         }
       }
       // an entry whose seed equals the entry before it reuses its hits
@@ -695,7 +747,7 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
       }
       // Rare pass: entries with recovery postings (written here whole) and
       // entries whose merged groups hit (member bits ORed into their rows).
-      for (uint64_t todo = (own_mask & rec_mask) | fix_mask; todo; todo &= todo - 1) {
+      for (uint64_t todo = by_lane ? 0 : (own_mask & rec_mask) | fix_mask; todo; todo &= todo - 1) {
         const uint32_t j = (uint32_t)__builtin_ctzll(todo);
         const uint64_t ms = seed_hits(j);
         uint64_t m = ms;
