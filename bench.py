@@ -8,12 +8,17 @@ balanced tree, 1% indels of length 1-10 per leaf), indexed as dense canonical
 1024 bytes). Query: 1,000,000 reads with lognormal lengths (median 2 kbp,
 sigma 1.0, clipped to 100 bp-100 kbp) at 2% substitution error, random strand.
 All reads sit in HBM as ASCII; one step packs them to 2 bits, extracts
-k-mers and runs the persistent lookup (HIP graph) over every read, producing
-per-read per-colour counts.
+k-mers and runs the lookup over every read (the autotuned order: the sorted
+sweep -- per chunk of reads, radix passes by PositionHash segment, the staged
+lookup from LDS segment windows, per-read sums of hit rows -- or the
+persistent read-ordered lookup under a HIP graph), producing per-read
+per-colour counts.
 
-Printed: one JSON line with the throughput (queried k-mers/s), the lookup
-kernel's roofline against HBM bandwidth, the one-launch-per-read path on a
-sample of reads, and the CPU oracle timed on the host on a bounded sample.
+Printed: one JSON line with the throughput (queried k-mers/s), the dominant
+kernel's roofline against HBM bandwidth, a bounded serving leg (Poisson
+requests through the continuous-batching server: p50/p99 latency), the
+one-launch-per-read path on a sample of reads, and the reference's optimised
+CPU lookup restated in C, timed on the host on a bounded sample.
 """
 from __future__ import annotations
 
@@ -83,6 +88,12 @@ def parse_args():
                    help="serving: Poisson arrival rates (requests/s), comma-separated; the "
                         "first is the headline")
     p.add_argument("--slots", type=int, default=2, help="serving: batches in flight")
+    p.add_argument("--serve-requests", type=int, default=240,
+                   help="batched workload: requests of the bounded serving leg (0: none)")
+    p.add_argument("--serve-qps", type=float, default=40.0,
+                   help="batched workload: Poisson arrival rate of the serving leg")
+    p.add_argument("--serve-max-reads", type=int, default=64,
+                   help="batched workload: reads per serving-leg request, U{1..this}")
     p.add_argument("--gpu-peel", type=int, default=0, help="build the index with the GPU peel")
     p.add_argument("--disk-load", default="",
                    help="directory to write the index to and time loading it back from")
@@ -303,25 +314,48 @@ def cpu_baseline(args, idx, buf, off, gpu_counts):
         keys, koff = keys_for(n)
         counts = np.zeros((n, orc.n_colors), np.uint64)
         t = time.perf_counter()
-        zr.oracle().zo_lookup_keys(C.byref(orc.z), keys.ctypes.data_as(zr.u64p),
-                                   koff.ctypes.data_as(zr.u64p), n,
-                                   counts.ctypes.data_as(zr.u64p), threads)
-        return time.perf_counter() - t, int(koff[-1])
+        zr.oracle().zo_lookup_keys_fast(C.byref(orc.z), keys.ctypes.data_as(zr.u64p),
+                                        koff.ctypes.data_as(zr.u64p), n,
+                                        counts.ctypes.data_as(zr.u64p), threads)
+        return time.perf_counter() - t, int(koff[-1]), counts
     n = min(200, len(off) - 1)
-    dt, nk = timed(n)
+    dt, nk, cpu_counts = timed(n)
     while dt < args.cpu_seconds / 8 and n < len(off) - 1:
         n = min(len(off) - 1, n * 4)
-        dt, nk = timed(n)
+        dt, nk, cpu_counts = timed(n)
     if dt < args.cpu_seconds / 2 and n < len(off) - 1:
         n = min(len(off) - 1, int(n * args.cpu_seconds / max(dt, 1e-3)))
-        dt, nk = timed(n)
+        dt, nk, cpu_counts = timed(n)
+    same = None if gpu_counts is None else \
+        bool(np.array_equal(gpu_counts[:n].astype(np.uint64), cpu_counts))
     return {
         "value": nk / dt, "unit": "k-mers/s" if args.mode == "kmers" else "minimizers/s",
         "cores": threads, "kind": "port",
-        "sample": f"lookup phase of the CPU oracle (oracle/oracle.c, sorted keys with "
-                  f"duplicate folding, 64-bit XOR screens) over the first {n} reads "
+        "sample": f"lookup phase of the reference's optimised CPU query restated in C "
+                  f"(oracle/oracle.c zo_lookup_keys_fast: per-read sort with duplicate "
+                  f"folding, blocks of 16 keys, rows prefetched 4 keys ahead, AVX2 screen of "
+                  f"the fingerprint plane) on {threads} threads over the first {n} reads "
                   f"({nk} k-mers) of the same batch and index, {dt:.1f}s, keys pre-extracted",
+        "host": host_cpu(), "counts_equal_gpu": same,
     }, parity, sel
+
+
+def host_cpu() -> dict:
+    """The host the CPU leg ran on: its CPU model, logical CPUs and the ones
+    this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
 
 
 def parity_stride(off, n_reads):
@@ -388,6 +422,44 @@ def serve_pass(zindex_amd, idx, seq, off, starts, sizes, arrivals, slots, max_re
     sst = srv.stats()
     srv.close()
     return stats, wall, aggs, sst
+
+
+def serving_leg(args, zindex_amd, idx, buf, off, counts):
+    """A bounded serving pass over the batch's index: requests of U{1..
+    serve_max_reads} consecutive reads from the start of the batch, Poisson
+    arrivals at serve_qps, through the continuous-batching server (per-request
+    aggregates). Latency percentiles, and each request's aggregate checked
+    against the batch's per-read counts of the same reads."""
+    rng = np.random.default_rng(args.seed + 3)
+    sizes = rng.integers(1, args.serve_max_reads + 1, args.serve_requests)
+    n = int(sizes.sum())
+    if n > len(off) - 1:
+        return None
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    pinned = zindex_amd.HostBuffer((int(off[n]),), np.uint8)
+    pinned.array[:] = buf[: int(off[n])]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_qps, args.serve_requests))
+    serve_pass(zindex_amd, idx, pinned.array, off, starts[:4], sizes[:4], np.zeros(4), args.slots,
+               256)  # warm-up
+    st, wall, aggs, sst = serve_pass(zindex_amd, idx, pinned.array, off, starts, sizes, arrivals,
+                                     args.slots, 256)
+    pinned.free()
+    lat = [x["latency_ms"] for x in st]
+    seeds = sum(x["seeds"] for x in st)
+    same = None
+    if counts is not None:
+        same = all(np.array_equal(agg, counts[a: a + k].astype(np.uint64).sum(axis=0))
+                   for a, k, agg in zip(starts, sizes, aggs))
+    out = {"requests": int(args.serve_requests), "qps": args.serve_qps,
+           "reads_per_request": f"U{{1..{args.serve_max_reads}}}", "slots": args.slots,
+           "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+           "mean_ms": float(np.mean(lat)), "max_ms": float(np.max(lat)),
+           "kmers_per_s": seeds / wall, "batches": sst["batches"],
+           "max_requests_per_batch": sst["max_batch_requests"], "equals_batch_counts": same}
+    log(f"serving leg: {args.serve_requests} requests at {args.serve_qps}/s: p50 "
+        f"{out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms, {sst['batches']} batches, "
+        f"aggregates equal the batch's counts: {same}")
+    return out
 
 
 def serve_main(args, zindex_amd, synth) -> int:
@@ -710,6 +782,10 @@ def main() -> int:
         log(f"cpu baseline: {cpu['value']:.3e} k-mers/s on {cpu['cores']} threads; "
             f"parity on {len(sel)} reads across the batch: {parity}")
 
+    latency = None
+    if args.serve_requests > 0:
+        latency = serving_leg(args, zindex_amd, idx, buf, off, counts if records else None)
+
     out = {
         "metric": "queried k-mers/s at 1024 colors, % of HBM peak; p50/p99 request latency",
         "value": value,
@@ -737,6 +813,7 @@ def main() -> int:
                        "the step" if sweep else " + persistent lookup (HIP graph)"),
         },
         "roofline": roofline,
+        "latency": latency,
         "cpu_baseline": cpu,
         "plain_path": plain,
         "speedup_vs_plain": (plain["batched_kmers_per_s_same_reads"] / plain["plain_kmers_per_s"])

@@ -98,7 +98,7 @@ int zg_index_load(zg_index *idx, int recover, uint64_t hbm_budget_bytes);
  * ZG_MEMORY_SUBINDEX and an index larger than the budget, the banks that do
  * not fit are decoded and streamed in per query (the reference's default
  * subindex-at-a-time mode); the budget must hold the largest bank beside the
- * resident ones. Servers need every bank resident. */
+ * resident ones. */
 int zg_index_load_ex(zg_index *idx, int recover, uint64_t hbm_budget_bytes,
                      uint32_t memory);
 /* Host copy of one decoded bank payload (valid after zg_index_load, or after
@@ -376,9 +376,13 @@ typedef struct zg_server_stats {
 } zg_server_stats;
 
 void zg_server_config_default(zg_server_config *cfg);
-/* The index must be resident (zg_index_load) and stay open and loaded until
- * zg_server_destroy returns. Fails with ZG_ERR_BUDGET when the index and the
- * slots do not fit hbm_budget. */
+/* The index must be loaded (zg_index_load / zg_index_load_ex) and stay open
+ * and loaded until zg_server_destroy returns. Fails with ZG_ERR_BUDGET when
+ * the resident part of the index and the slots do not fit hbm_budget. An
+ * index loaded with ZG_MEMORY_SUBINDEX under a budget keeps the banks that do
+ * not fit on the host: each batch streams them through HBM one at a time
+ * (query_stream.rs:228-265), one batch at a time, while the other slots copy
+ * and seed their next batches. */
 int zg_server_create(zg_index *idx, const zg_server_config *cfg, zg_server **out);
 /* Queues one request: reads seq[read_off[i] .. read_off[i+1]) (ASCII). The
  * input and the outputs stay owned by the caller and must remain valid until

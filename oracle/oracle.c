@@ -1017,3 +1017,235 @@ int zo_lookup_keys(const zo_index *idx, const uint64_t *keys,
   return run_jobs(idx, NULL, NULL, keys, key_off, n_reads, counts, NULL, NULL,
                   NULL, threads);
 }
+
+/* ---------------------------------------------------------------------- */
+/* The reference's optimised CPU lookup, restated as the timed CPU baseline */
+/* ---------------------------------------------------------------------- */
+/* record_query.rs:337-404 (query_feature_range): each read's keys sorted and
+ * equal keys folded into weights; blocks of 16 distinct keys whose positions
+ * are computed first, rows of the key 4 ahead prefetched (storage.rs:
+ * 2980-3015, the start of each row stream only) while the current key is
+ * looked up; byte-wide fingerprint planes screened 32 bytes at a time with
+ * AVX2 (storage.rs:3309-3350). Other layouts and widths take lookup_bank.
+ * Counts equal zo_lookup_keys' exactly. */
+#include <immintrin.h>
+#include <stdatomic.h>
+
+#define FAST_BLOCK 16
+#define FAST_AHEAD 4
+#define FAST_MAX_H 16
+
+/* LSD radix sort of 64-bit keys, 8-bit digits; a digit every key shares is
+ * skipped. */
+static void radix_sort_u64(uint64_t *a, uint64_t *tmp, size_t n) {
+  if (n < 64) {
+    for (size_t i = 1; i < n; i++) {
+      uint64_t v = a[i];
+      size_t j = i;
+      while (j > 0 && a[j - 1] > v) {
+        a[j] = a[j - 1];
+        j--;
+      }
+      a[j] = v;
+    }
+    return;
+  }
+  uint64_t *src = a, *dst = tmp;
+  for (int shift = 0; shift < 64; shift += 8) {
+    size_t count[256] = {0};
+    for (size_t i = 0; i < n; i++) count[(src[i] >> shift) & 255]++;
+    if (count[(src[0] >> shift) & 255] == n) continue;
+    size_t run = 0;
+    for (int d = 0; d < 256; d++) {
+      size_t c = count[d];
+      count[d] = run;
+      run += c;
+    }
+    for (size_t i = 0; i < n; i++) dst[count[(src[i] >> shift) & 255]++] = src[i];
+    uint64_t *t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != a) memcpy(a, src, n * sizeof(uint64_t));
+}
+
+/* The banks a key probes (selected_banks, storage.rs:2717-2745). */
+static uint32_t fast_selected(const zo_index *idx, uint64_t key, const zo_bank **out) {
+  uint32_t n = 0, next = 0;
+  while (next < idx->n_banks) {
+    uint32_t start = next;
+    const zo_bank *bank = &idx->banks[start];
+    if (bank->partitions > 1) {
+      uint64_t target = zo_partition_id(key, idx->seed, bank->partitions);
+      if (bank->partition == 0 && bank->partitions <= idx->n_banks - start) {
+        next += bank->partitions;
+        out[n++] = &idx->banks[start + target];
+        continue;
+      }
+      next += 1;
+      if (target == bank->partition) out[n++] = bank;
+    } else {
+      next += 1;
+      out[n++] = bank;
+    }
+  }
+  return n;
+}
+
+static inline void fast_hit(const zo_index *idx, const zo_bank *bank, const uint64_t *slots,
+                            uint64_t row_bytes, uint64_t i, uint64_t weight, uint64_t *scores) {
+  uint32_t m0 = bank->member_off[i], m1 = bank->member_off[i + 1];
+  if (m1 - m0 == 1) {
+    scores[bank->members[m0]] += weight;
+    return;
+  }
+  size_t off = bank->n_groups * (size_t)idx->bits + bank->offset[i] - i * (size_t)idx->bits;
+  uint64_t mask = row_xor_bits(bank->data, slots, idx->hashes, row_bytes, off,
+                               bank->width[i] - (size_t)idx->bits);
+  while (mask) {
+    scores[bank->members[m0 + __builtin_ctzll(mask)]] += weight;
+    mask &= mask - 1;
+  }
+}
+
+__attribute__((target("avx2"))) static void fast_lookup_bank(const zo_index *idx,
+                                                             const zo_bank *bank,
+                                                             const uint64_t *slots, uint64_t fp,
+                                                             uint64_t key, uint64_t weight,
+                                                             uint64_t *scores) {
+  const int b = idx->bits, H = idx->hashes;
+  if (!bank->fingerprint_first || (b != 8 && b != 16 && b != 32)) {
+    lookup_bank(idx, bank, key, fp, weight, scores);
+    return;
+  }
+  const uint64_t row_bytes = (bank->row_bits + 7) / 8;
+  const uint64_t G = bank->n_groups, plane = G * (uint64_t)b / 8, width = (uint64_t)b / 8;
+  const uint8_t *rows[FAST_MAX_H];
+  for (int h = 0; h < H; h++) rows[h] = bank->data + slots[h] * row_bytes;
+  const __m256i target = b == 8 ? _mm256_set1_epi8((char)fp)
+                         : b == 16 ? _mm256_set1_epi16((short)fp)
+                                   : _mm256_set1_epi32((int)fp);
+  const uint32_t lanes = b == 8 ? 0xFFFFFFFFu : b == 16 ? 0x55555555u : 0x11111111u;
+  uint64_t byte = 0;
+  for (; byte + 32 <= plane; byte += 32) {
+    __m256i v = _mm256_loadu_si256((const __m256i *)(rows[0] + byte));
+    for (int h = 1; h < H; h++)
+      v = _mm256_xor_si256(v, _mm256_loadu_si256((const __m256i *)(rows[h] + byte)));
+    __m256i eq = b == 8 ? _mm256_cmpeq_epi8(v, target)
+                 : b == 16 ? _mm256_cmpeq_epi16(v, target)
+                           : _mm256_cmpeq_epi32(v, target);
+    uint32_t mask = (uint32_t)_mm256_movemask_epi8(eq) & lanes;
+    while (mask) {
+      fast_hit(idx, bank, slots, row_bytes, (byte + __builtin_ctz(mask)) / width, weight, scores);
+      mask &= mask - 1;
+    }
+  }
+  for (uint64_t i = byte / width; i < G; i++) {
+    uint64_t v = 0;
+    for (int h = 0; h < H; h++) {
+      uint64_t x = 0;
+      memcpy(&x, rows[h] + i * width, width);
+      v ^= x;
+    }
+    if (v == fp) fast_hit(idx, bank, slots, row_bytes, i, weight, scores);
+  }
+}
+
+typedef struct {
+  const zo_index *idx;
+  const uint64_t *keys, *key_off;
+  uint64_t n_reads;
+  uint64_t *counts;
+  atomic_uint_fast64_t *next;
+} fast_job;
+
+static void *fast_worker(void *arg) {
+  fast_job *job = arg;
+  const zo_index *idx = job->idx;
+  const int H = idx->hashes;
+  size_t cap = 0;
+  uint64_t *buf = NULL, *tmp = NULL;
+  const zo_bank *sel[FAST_BLOCK][64];
+  uint32_t nsel[FAST_BLOCK];
+  uint64_t slots[FAST_BLOCK][64 * FAST_MAX_H / 4];
+  uint64_t bkeys[FAST_BLOCK], weights[FAST_BLOCK];
+  uint64_t snap_small[64];
+  for (;;) {
+    uint64_t r0 = atomic_fetch_add(job->next, 16);
+    if (r0 >= job->n_reads) break;
+    uint64_t r1 = r0 + 16 < job->n_reads ? r0 + 16 : job->n_reads;
+    for (uint64_t r = r0; r < r1; r++) {
+      size_t n = job->key_off[r + 1] - job->key_off[r];
+      uint64_t *scores = job->counts + r * (uint64_t)idx->n_colors;
+      memset(scores, 0, idx->n_colors * sizeof(uint64_t));
+      if (n > cap) {
+        cap = n;
+        buf = realloc(buf, cap * sizeof(uint64_t));
+        tmp = realloc(tmp, cap * sizeof(uint64_t));
+      }
+      memcpy(buf, job->keys + job->key_off[r], n * sizeof(uint64_t));
+      radix_sort_u64(buf, tmp, n);
+      size_t i = 0;
+      while (i < n) {
+        /* a block of up to 16 distinct keys with their weights and rows */
+        int count = 0;
+        while (count < FAST_BLOCK && i < n) {
+          size_t j = i;
+          while (j < n && buf[j] == buf[i]) j++;
+          bkeys[count] = buf[i];
+          weights[count] = j - i;
+          nsel[count] = fast_selected(idx, buf[i], sel[count]);
+          for (uint32_t s = 0; s < nsel[count]; s++)
+            zo_positions(buf[i], idx->seed, sel[count][s]->slots, H, idx->segment_length,
+                         &slots[count][s * H]);
+          count++;
+          i = j;
+        }
+        for (int k = 0; k < count + FAST_AHEAD; k++) {
+          if (k < count && k + FAST_AHEAD < count + FAST_AHEAD) {
+            /* rows of the key FAST_AHEAD after the one looked up below */
+            for (uint32_t s = 0; s < nsel[k]; s++) {
+              const zo_bank *bank = sel[k][s];
+              uint64_t row_bytes = (bank->row_bits + 7) / 8;
+              for (int h = 0; h < H; h++)
+                _mm_prefetch((const char *)(bank->data + slots[k][s * H + h] * row_bytes),
+                             _MM_HINT_T0);
+            }
+          }
+          int n_look = k - FAST_AHEAD;
+          if (n_look < 0) continue;
+          uint64_t key = bkeys[n_look], w = weights[n_look];
+          uint64_t n_rec = 0;
+          const uint32_t *rec = recovered_origins(idx->recovery,
+                                                  zo_abandoned_hash(key, idx->seed), &n_rec);
+          uint64_t *snap = n_rec <= 64 ? snap_small : malloc(n_rec * sizeof(uint64_t));
+          for (uint64_t q = 0; q < n_rec; q++) snap[q] = scores[rec[q]];
+          uint64_t fp = zo_fingerprint(key, idx->seed, idx->bits);
+          for (uint32_t s = 0; s < nsel[n_look]; s++)
+            fast_lookup_bank(idx, sel[n_look][s], &slots[n_look][s * H], fp, key, w, scores);
+          for (uint64_t q = 0; q < n_rec; q++)
+            if (scores[rec[q]] == snap[q]) scores[rec[q]] += w;
+          if (snap != snap_small) free(snap);
+        }
+      }
+    }
+  }
+  free(buf);
+  free(tmp);
+  return NULL;
+}
+
+int zo_lookup_keys_fast(const zo_index *idx, const uint64_t *keys, const uint64_t *key_off,
+                        uint64_t n_reads, uint64_t *counts, int threads) {
+  if (idx->hashes > FAST_MAX_H || idx->n_banks * (uint64_t)idx->hashes > 64 * FAST_MAX_H / 4 ||
+      idx->n_banks > 64)
+    return zo_lookup_keys(idx, keys, key_off, n_reads, counts, threads);
+  if (threads <= 0) threads = 1;
+  atomic_uint_fast64_t next = 0;
+  fast_job job = {idx, keys, key_off, n_reads, counts, &next};
+  pthread_t tids[256];
+  if (threads > 256) threads = 256;
+  for (int t = 0; t < threads; t++) pthread_create(&tids[t], NULL, fast_worker, &job);
+  for (int t = 0; t < threads; t++) pthread_join(tids[t], NULL);
+  return 0;
+}

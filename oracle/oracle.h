@@ -161,6 +161,15 @@ int zo_lookup_keys(const zo_index *idx, const uint64_t *keys,
                    const uint64_t *key_off, uint64_t n_reads,
                    uint64_t *counts, int threads);
 
+/* The same counts by the reference's optimised lookup (record_query.rs:
+ * 337-404: sorted keys folded into weights, blocks of 16 distinct keys, rows
+ * prefetched 4 keys ahead; storage.rs:3309-3350: AVX2 screen of byte-wide
+ * fingerprint planes), reads spread over the threads: the timed CPU
+ * baseline. */
+int zo_lookup_keys_fast(const zo_index *idx, const uint64_t *keys,
+                        const uint64_t *key_off, uint64_t n_reads,
+                        uint64_t *counts, int threads);
+
 /* Fingerprint screen (storage.rs:3241-3350, packed_fingerprints.rs:7-126):
  * writes hit group indices of XOR(rows) == fp, returns their number. */
 size_t zo_screen_rows(const uint8_t *const *rows, int n_rows, int bits,

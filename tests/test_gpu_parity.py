@@ -654,8 +654,21 @@ def test_subindex_mode_streams_banks_beyond_budget(tmp_path):
             assert np.array_equal(b.results()[1], expect[1])
             b.run_plain()
             assert np.array_equal(b.results()[1], expect[1])
+        # a server over the same index: each batch streams the banks beyond
+        # the budget (one batch at a time); every request equals its rows of
+        # the all-resident answer
+        cuts = [0, 1, 37, 38, 120, 200]
+        with zindex_amd.Server(z, slots=2, max_requests=2) as srv:
+            tickets = [srv.submit(reads[a:b]) for a, b in zip(cuts[:-1], cuts[1:])]
+            for (a, b), t in zip(zip(cuts[:-1], cuts[1:]), tickets):
+                tot, cnt, agg, st = srv.wait(t)
+                assert st["status"] == 0
+                assert np.array_equal(tot, expect[0][a:b])
+                assert np.array_equal(cnt, expect[1][a:b])
+                assert np.array_equal(agg, expect[1][a:b].astype(np.uint64).sum(axis=0))
+            assert srv.stats()["batches"] >= 3
         with pytest.raises(zindex_amd.ZindexError) as e:
-            zindex_amd.Server(z)
+            zindex_amd.Server(z, hbm_budget=budget)  # no room left for a slot
         assert e.value.code == -7
     # the CLI's -M subindex under the same budget prints what -M all prints
     q = tmp_path / "q.fa"

@@ -548,3 +548,63 @@ def test_ratio_formatting_ties_round_to_even():
     # 1/512 = 0.001953125 exactly: the 9th decimal is a tie
     assert zr.fmt_ratio(1, 512) == "0.00195312"
     assert zr.fmt_ratio(3, 512) == "0.00585938"
+
+
+def _random_plane_index(rng, bits, hashes, n_groups, merged_every, partitions, recovered_keys,
+                        seed=77, segment_length=64):
+    """A plane-layout (fingerprint_first) index with random rows: singleton
+    groups and every `merged_every`-th group a 3-member group, split into
+    `partitions` routed banks; the recovery map lists `recovered_keys`."""
+    groups, nxt, offset = [], 0, 0
+    for i in range(n_groups):
+        count = 3 if merged_every and i % merged_every == merged_every - 1 else 1
+        width = bits + (0 if count == 1 else count)
+        groups.append({"members": list(range(nxt, nxt + count)), "width": width, "offset": offset})
+        nxt += count
+        offset += width
+    row_bytes = (offset + 7) // 8
+    slots = 64 * (16 + hashes - 1)
+    banks, payloads = [], []
+    for p in range(partitions):
+        meta = {"slots": slots, "row_bits": offset, "fingerprint_first": True, "groups": groups}
+        if partitions > 1:
+            meta["routing"] = {"partition": p, "partitions": partitions}
+        banks.append(meta)
+        payloads.append(rng.integers(0, 256, slots * row_bytes, dtype=np.uint8))
+    manifest = {"k": 31, "hashes": hashes, "fingerprint_bits": bits, "index_mode": 0,
+                "minimizer_size": 31, "seed": seed, "segment_length": segment_length,
+                "colors": [{"name": str(i)} for i in range(nxt)], "banks": banks}
+    hs = sorted({oracle().zo_abandoned_hash(int(k), seed) for k in recovered_keys})
+    origins = [sorted(rng.choice(nxt, int(rng.integers(1, 5)), replace=False).tolist()) for _ in hs]
+    off = np.zeros(len(hs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(o) for o in origins])
+    rec = (np.array(hs, np.uint32), off,
+           np.array([c for o in origins for c in o] or [0], np.uint32)) if hs else None
+    return zr.OracleIndex(manifest, payloads, rec)
+
+
+@pytest.mark.parametrize("bits,hashes,merged_every,partitions",
+                         [(8, 4, 0, 1), (8, 3, 5, 1), (16, 4, 0, 2), (32, 2, 7, 1), (4, 4, 0, 1),
+                          (8, 4, 0, 3)])
+def test_fast_cpu_lookup_equals_oracle(bits, hashes, merged_every, partitions):
+    """The timed CPU baseline (blocks of 16 keys, rows prefetched 4 ahead,
+    AVX2 plane screen) counts exactly what the scalar oracle counts, with
+    duplicate keys, recovered keys, merged groups, partitions and a plane
+    length that is not a multiple of 32 bytes."""
+    rng = np.random.default_rng(bits * 100 + hashes * 10 + partitions)
+    n_reads = 40
+    lens = rng.integers(0, 300, n_reads)
+    pool = rng.integers(0, 1 << 62, 500, dtype=np.uint64)  # repeats within reads
+    keys = pool[rng.integers(0, len(pool), int(lens.sum()))]
+    key_off = np.zeros(n_reads + 1, np.uint64)
+    key_off[1:] = np.cumsum(lens)
+    idx = _random_plane_index(rng, bits, hashes, 613 if bits == 8 else 301, merged_every,
+                              partitions, pool[::7])
+    want = np.zeros((n_reads, idx.n_colors), np.uint64)
+    got = np.zeros((n_reads, idx.n_colors), np.uint64)
+    oracle().zo_lookup_keys(C.byref(idx.z), keys.ctypes.data_as(u64p),
+                            key_off.ctypes.data_as(u64p), n_reads, want.ctypes.data_as(u64p), 2)
+    oracle().zo_lookup_keys_fast(C.byref(idx.z), keys.ctypes.data_as(u64p),
+                                 key_off.ctypes.data_as(u64p), n_reads, got.ctypes.data_as(u64p), 3)
+    assert want.sum() > 0
+    assert np.array_equal(want, got)

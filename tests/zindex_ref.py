@@ -128,6 +128,9 @@ def oracle() -> C.CDLL:
                                          C.c_int]
         o.zo_lookup_keys.restype = C.c_int
         o.zo_lookup_keys.argtypes = [C.POINTER(ZoIndex), u64p, u64p, C.c_uint64, u64p, C.c_int]
+        o.zo_lookup_keys_fast.restype = C.c_int
+        o.zo_lookup_keys_fast.argtypes = [C.POINTER(ZoIndex), u64p, u64p, C.c_uint64, u64p,
+                                          C.c_int]
         o.zo_screen_rows.restype = C.c_size_t
         o.zo_screen_rows.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_uint64,
                                      C.c_size_t, u32p]

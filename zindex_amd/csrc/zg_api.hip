@@ -565,7 +565,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
 
 uint64_t resident_bytes(const zg_index *idx) {
   uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->rec_post_col.bytes +
-               idx->inner_table.bytes;
+               idx->inner_table.bytes + idx->stream_data.bytes;
   for (auto &d : idx->dbanks)
     n += d.data.bytes + d.col_color.bytes + d.mg_off.bytes + d.mg_members.bytes +
          d.mg_mask_bit.bytes + d.color_col.bytes + d.color_bit.bytes;

@@ -76,6 +76,10 @@ struct zg_server {
   bool stopping = false;
   std::vector<std::unique_ptr<zg::ServeSlot>> slots;
   uint64_t device_bytes = 0;
+  // Subindex mode (banks beyond the HBM budget streamed in per query): the
+  // index has one streaming buffer, so one batch at a time streams its banks
+  // and runs their lookups (QueryMemory::Subindex, mod.rs:52-59).
+  std::mutex stream_mu;
   // statistics (under mu)
   uint64_t done_requests = 0, batches = 0, reads = 0, seeds = 0;
   uint32_t max_batch_requests = 0;
@@ -236,8 +240,17 @@ void zg_server::run_batch(ServeSlot *slot, const std::vector<RequestPtr> &batch)
   check_hip(hipEventRecord(b.ev[0], s), "hipEventRecord");
   enqueue_zero_outputs(idx, &b);
   check_hip(hipEventRecord(b.ev[1], s), "hipEventRecord");
-  enqueue_lookups(idx, &b);
-  check_hip(hipEventRecord(b.ev[2], s), "hipEventRecord");
+  if (any_streamed(idx)) {
+    // the streamed banks pass through the shared buffer: this batch's
+    // lookups end before another batch streams a bank into it
+    std::lock_guard<std::mutex> lk(stream_mu);
+    enqueue_lookups(idx, &b);
+    check_hip(hipEventRecord(b.ev[2], s), "hipEventRecord");
+    check_hip(hipEventSynchronize(b.ev[2]), "serve lookups");
+  } else {
+    enqueue_lookups(idx, &b);
+    check_hip(hipEventRecord(b.ev[2], s), "hipEventRecord");
+  }
   // per-request results straight into the callers' buffers
   launch_fill_u64(slot->req_agg.as<uint64_t>(), (uint64_t)batch.size() * C, 0, s);
   slot->h_totals.assign(n_reads + 1, 0);
@@ -343,9 +356,6 @@ extern "C" int zg_server_create(zg_index *idx, const zg_server_config *cfg, zg_s
     ensure(idx && out, "null argument", ZG_ERR_INVALID);
     ensure(idx->on_device, "index is not resident on the device (call zg_index_load)",
            ZG_ERR_INVALID);
-    ensure(!any_streamed(idx),
-           "a server needs every subindex resident (load with ZG_MEMORY_ALL or a larger budget)",
-           ZG_ERR_BUDGET);
     zg_server_config c;
     zg_server_config_default(&c);
     if (cfg) c = *cfg;
