@@ -68,6 +68,8 @@ def parse_args():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-graph", action="store_true",
+                   help="skip the steps with the sweep captured as one HIP graph")
     p.add_argument("--parity-reads", type=int, default=2000)
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="first copy this many MiB device-to-device (a plain copy kernel of "
@@ -680,6 +682,30 @@ def main() -> int:
             f"{st['sweep_launches']} chunks, {st['sweep_folded']} folded; seeding "
             f"{st['last_seed_ms']:.1f} ms")
 
+    # the same steps with the sweep captured as one HIP graph per batch (its
+    # phases are then not timed, so the line's value and roofline come from
+    # the eager steps above)
+    graph = None
+    if st["tuning"]["sweep"] and not args.no_graph:
+        tuned = dict(idx.tuning)
+        idx.tuning = dict(tuned, graph=1)
+        batch.reseed()
+        batch.run()  # eager, then captured
+        g0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.reseed()
+            batch.run()
+        g_elapsed = time.perf_counter() - g0
+        gst = batch.stats()
+        _, g_counts, g_agg = batch.results()
+        graph = {"captured": bool(gst["tuning"]["graph"]),
+                 "ms_per_step": g_elapsed * 1000 / args.steps,
+                 "kmers_per_s": seeds * args.steps / g_elapsed,
+                 "equals_eager": bool(np.array_equal(g_agg, agg) and
+                                      (g_counts is None or np.array_equal(g_counts, counts)))}
+        idx.tuning = tuned
+        log(f"graph steps: {graph}")
+
     # roofline of the dominant kernel: algorithmic bytes per launch / its
     # average launch time (HIP events on the stream the kernel runs on)
     positions = int(off[-1]) + args.reads
@@ -813,6 +839,7 @@ def main() -> int:
                        "the step" if sweep else " + persistent lookup (HIP graph)"),
         },
         "roofline": roofline,
+        "graph": graph,
         "latency": latency,
         "cpu_baseline": cpu,
         "plain_path": plain,

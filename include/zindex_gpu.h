@@ -277,6 +277,11 @@ typedef struct zg_tuning {
                               rows summed per read; same results as 0.
                               Ignored (read order) for findere indexes,
                               streamed subindexes and rows too wide for LDS */
+  uint32_t graph;          /* with sweep: a batch's sweep (every chunk's radix
+                              passes, staged lookups and sums) is captured
+                              as one HIP graph after its first run and
+                              replayed as one launch; its phase times are
+                              then not measured (sweep_*_ms = -1) */
 } zg_tuning;
 
 typedef struct zg_batch_stats {

@@ -136,3 +136,38 @@ def test_lane_lookup_wide_rows(case, monkeypatch):
     check_parity(idx, orc, reads, True, label=f"lane lookup {case}")
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "9000")
     check_parity(idx, orc, reads, True, label=f"lane lookup {case}, chunks")
+
+
+@pytest.mark.parametrize("records", [True, False])
+def test_sweep_as_one_graph_replays(records, monkeypatch):
+    # the sweep of a batch captured as one HIP graph after its first (eager)
+    # run and replayed: every replay, with reseeding and host work between,
+    # equals the eager run and the oracle; several chunks per replay
+    colors = make_colors(520, 560, 2500)
+    reads = make_reads(521, colors, 150, lo=30, hi=2500)
+    idx = build(colors, fingerprint_bits=8, hashes=4, slot_scale=0.6)
+    idx.load(recover=True)
+    orc = oracle_for(idx, True)
+    buf, off = zindex_amd.pack_reads(reads)
+    exp_totals, exp_counts = orc.query_reads(buf, off, threads=zr.cpu_count())
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "40000")
+    idx.tuning = dict(SWEEP_BENCH, graph=1)
+    with idx.batch(buf, off, records=records) as b:
+        b.run()  # eager, then captured
+        first = b.results()
+        st = b.stats()
+        assert st["sweep_launches"] > 1 and st["tuning"]["graph"] == 1, st
+        for rep in range(4):
+            if rep == 2:
+                b.reseed()
+            b.run()
+            st = b.stats()
+            assert st["tuning"]["graph"] == 1 and st["sweep_lookup_ms"] == -1, st
+            got = b.results()
+            say(f"graph replay {rep}: records={records}, {st['last_total_ms']:.2f} ms")
+            assert np.array_equal(got[0], exp_totals)
+            assert np.array_equal(got[2], exp_counts.sum(axis=0))
+            assert np.array_equal(got[0], first[0]) and np.array_equal(got[2], first[2])
+            if records:
+                assert np.array_equal(got[1].astype(np.uint64), exp_counts)
+    idx.tuning = dict(SWEEP_BENCH, graph=0)

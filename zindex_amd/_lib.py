@@ -60,7 +60,7 @@ class BuildConfig(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("span", C.c_uint32), ("tiles_per_wave", C.c_uint32),
                 ("prefetch", C.c_uint32), ("waves_per_cu", C.c_uint32),
-                ("nontemporal", C.c_uint32), ("sweep", C.c_uint32)]
+                ("nontemporal", C.c_uint32), ("sweep", C.c_uint32), ("graph", C.c_uint32)]
 
 
 class BatchStats(C.Structure):

@@ -594,6 +594,8 @@ zg_index::~zg_index() {
 zg_batch::~zg_batch() {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
+  if (sw_exec) (void)hipGraphExecDestroy(sw_exec);
+  if (sw_graph) (void)hipGraphDestroy(sw_graph);
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
   for (auto &e : sw_ev) (void)hipEventDestroy(e);
@@ -788,6 +790,7 @@ extern "C" int zg_index_set_tuning(zg_index *idx, const zg_tuning *t) {
     idx->tuning.waves_per_cu = t->waves_per_cu;
     idx->tuning.nontemporal = t->nontemporal ? 1 : 0;
     idx->tuning.sweep = t->sweep ? 1 : 0;
+    idx->tuning.graph = t->graph ? 1 : 0;
     return 0;
   } catch (...) {
     return translate_exception();
@@ -805,6 +808,7 @@ extern "C" int zg_index_get_tuning(const zg_index *idx, zg_tuning *t) {
   t->waves_per_cu = idx->tuning.waves_per_cu;
   t->nontemporal = idx->tuning.nontemporal;
   t->sweep = idx->tuning.sweep;
+  t->graph = idx->tuning.graph;
   return 0;
 }
 
@@ -983,6 +987,47 @@ static void drop_graph(zg_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->graph_exec = nullptr;
   b->graph = nullptr;
+}
+
+static void drop_sweep_graph(zg_batch *b) {
+  if (b->sw_exec) (void)hipGraphExecDestroy(b->sw_exec);
+  if (b->sw_graph) (void)hipGraphDestroy(b->sw_graph);
+  b->sw_exec = nullptr;
+  b->sw_graph = nullptr;
+}
+
+// The sweep of a batch as one graph: zeroed outputs, every chunk's sorts,
+// staged lookups and sums, the per-colour sums. Captured on the stream after
+// an eager run has sized every buffer (nothing allocates during the capture);
+// a refused capture leaves the batch on eager launches.
+static void capture_sweep(zg_index *idx, zg_batch *b) {
+  hipStream_t s = b->stream;
+  bool ok = false;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    bool enqueued = true;
+    b->sw_capturing = true;
+    try {
+      enqueue_zero_outputs(idx, b);
+      sweep_run(idx, b);
+      enqueue_reduce(idx, b);
+    } catch (...) {
+      enqueued = false;
+    }
+    b->sw_capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &g);
+    if (enqueued && end == hipSuccess && g) {
+      b->sw_graph = g;
+      ok = hipGraphInstantiate(&b->sw_exec, g, nullptr, nullptr, 0) == hipSuccess;
+    } else if (g) {
+      (void)hipGraphDestroy(g);
+    }
+    (void)hipGetLastError();
+  }
+  if (!ok) {
+    drop_sweep_graph(b);
+    b->sw_graph_failed = true;
+  }
 }
 
 // Milliseconds between two completed events; -1 when the runtime cannot time
@@ -1264,6 +1309,18 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
     hipStream_t s = b->stream;
     if (zg::sweep_on(idx, b)) {
       zg::drop_graph(b);
+      if (!idx->tuning.graph) zg::drop_sweep_graph(b);
+      if (b->sw_exec) {
+        // one launch; only the whole run is timed
+        check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
+        check_hip(hipGraphLaunch(b->sw_exec, s), "hipGraphLaunch");
+        check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
+        check_hip(hipStreamSynchronize(s), "lookup");
+        b->last_total_ms = b->last_lookup_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
+        b->sweep_sort_ms = b->sweep_lookup_ms = b->sweep_sum_ms = -1;
+        b->sweep_folded = zg::sweep_folded(b);
+        return 0;
+      }
       check_hip(hipEventRecord(b->ev[0], s), "hipEventRecord");
       zg::enqueue_zero_outputs(idx, b);
       check_hip(hipEventRecord(b->ev[1], s), "hipEventRecord");
@@ -1276,6 +1333,7 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
       b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
       zg::sweep_times(b, b->sweep_sort_ms, b->sweep_lookup_ms, b->sweep_sum_ms);
       b->sweep_folded = zg::sweep_folded(b);
+      if (idx->tuning.graph && !b->sw_graph_failed) zg::capture_sweep(idx, b);
       return 0;
     }
     zg::release_sweep(idx, b);
@@ -1356,6 +1414,7 @@ extern "C" int zg_batch_run_plain(zg_index *idx, zg_batch *b, uint64_t first, ui
         zg::LookupArgs a = zg::lookup_args(idx, b, i);
         a.begin = lo;
         a.end = hi;
+        a.span = 64;  // one read per launch: the smallest unit spreads it widest
         uint64_t spans = (hi - lo + a.span - 1) / a.span;
         a.n_units = spans * a.n_tg;
         a.queue = queues.as<uint32_t>() + q * nb + i;
@@ -1440,6 +1499,7 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->tuning.waves_per_cu = b->kcfg.blocks * 4 / std::max(1u, idx->cu_count);
     o->tuning.nontemporal = b->nontemporal;
     o->tuning.sweep = zg::sweep_on(idx, b) ? 1 : 0;
+    o->tuning.graph = o->tuning.sweep && b->sw_exec ? 1 : 0;
     o->sweep_folded = o->tuning.sweep ? b->sweep_folded : 0;
     o->sweep_launches = o->tuning.sweep ? b->sweep_chunks : 0;
     o->sweep_lookup_ms = o->tuning.sweep ? b->sweep_lookup_ms : 0;

@@ -89,6 +89,7 @@ struct Tuning {
   uint32_t waves_per_cu = 0;  // 0 = occupancy limit
   uint32_t nontemporal = 0;   // non-temporal row loads
   uint32_t sweep = 0;         // seeds in segment order (sorted sweep)
+  uint32_t graph = 0;         // the sweep of a batch as one HIP graph
 };
 
 }  // namespace zg
@@ -165,6 +166,12 @@ struct zg_batch {
   uint32_t sweep_classes = 0;     // its size classes (a sort and lookups each per chunk)
   double sweep_lookup_ms = 0, sweep_sort_ms = 0, sweep_sum_ms = 0;  // their phase times
   uint64_t sweep_folded = 0;      // entries answered from the entry before them
+  // the sweep as one graph (tuning.graph): captured after an eager run, with
+  // that run's chunk size
+  hipGraph_t sw_graph = nullptr;
+  hipGraphExec_t sw_exec = nullptr;
+  bool sw_graph_failed = false, sw_capturing = false;
+  uint64_t sw_cap = 0;            // chunk capacity (positions) of the last eager sweep
   // randstrobemers: candidate bitmap, per-word counts and offsets, run-id
   // offsets, candidate hashes and run ids (starts go to key_win); keys hold
   // the forward keys then the reverse-complement keys (rs_forward each)

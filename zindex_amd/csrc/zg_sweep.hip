@@ -1179,7 +1179,9 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   // device memory free or already held by them
   uint64_t cap = std::min<uint64_t>((1ULL << 32) - 64, kSweepHitBytes / (row_words * 4ULL));
   size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+  if (b->sw_capturing) {
+    cap = b->sw_cap;  // the eager run's chunks: every buffer already holds them
+  } else if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
     uint64_t held = 0;
     for (const DevBuf *d : {&b->fd_hits, &b->sw_h[0], &b->sw_h[1], &b->sw_p[0], &b->sw_p[1],
                             &b->sw_listed, &b->sw_hist, &b->sw_off, &b->scan_temp})
@@ -1191,6 +1193,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   }
   if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // smaller chunks
     cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, strtoull(e, nullptr, 10)));
+  b->sw_cap = cap;
   const std::vector<uint64_t> &off = b->h_seg_off;  // row starts in the item space
   std::vector<uint64_t> cuts{0};
   uint64_t most = 0;
