@@ -517,6 +517,50 @@ def test_bacteria_readme_and_demo_known_answers(bacteria_index):
     assert out == zr.records_text(names, totals, counts, colors, 0.0, "compact")
 
 
+@pytest.mark.parametrize("codec", ["plain", "gzip", "zstd", "xz", "bzip2", "fastq"])
+def test_streamed_query_file_in_record_batches(bacteria_index, tmp_path, codec):
+    """The query file is read and decompressed chunk by chunk and parsed in
+    batches of -B records on a thread of its own (records.rs:58-148): every
+    batch size and codec (concatenated members, FASTQ) prints exactly the
+    oracle's text; a truncated stream fails."""
+    import bz2
+    import gzip
+    import lzma
+    qpath = GOLDEN / "bacteria" / "queries.fa"
+    recs = zr.read_fasta_records(qpath)
+    raw = qpath.read_bytes()
+    if codec == "fastq":
+        raw = b"".join(b"@%s extra words\n%s\n+\n%s\n" % (n.encode(), s, b"I" * len(s))
+                       for n, s in recs)
+    half = len(raw) // 2
+    enc = {"plain": lambda b: b, "fastq": lambda b: b,
+           "gzip": lambda b: gzip.compress(b[:half]) + gzip.compress(b[half:]),
+           "zstd": lambda b: zr.zstd_compress(b),
+           "xz": lambda b: lzma.compress(b[:half]) + lzma.compress(b[half:]),
+           "bzip2": lambda b: bz2.compress(b[:half]) + bz2.compress(b[half:])}[codec]
+    path = tmp_path / "q.x"
+    path.write_bytes(enc(raw))
+    orc = zr.OracleIndex.from_dir(bacteria_index)
+    colors = [(c["name"], c["unique_kmers"]) for c in orc.manifest["colors"]]
+    buf, off = zindex_amd.pack_reads([s for _, s in recs])
+    totals, counts = orc.query_reads(buf, off)
+    names = [n for n, _ in recs]
+    want = zr.records_text(names, totals, counts, colors, 0.0, "compact")
+    for batch in (1, 7, 4096):
+        out = _run_cli(["query", "-i", str(bacteria_index), "-q", str(path), "--records",
+                        "-F", "compact", "-B", str(batch)])
+        assert out == want, (codec, batch)
+    out = _run_cli(["query", "-i", str(bacteria_index), "-q", str(path)])
+    assert out == zr.aggregate_text(colors, counts.sum(axis=0), int(totals.sum()), 0.0)
+    if codec in ("gzip", "xz", "bzip2", "zstd"):
+        bad = tmp_path / "bad.x"
+        bad.write_bytes(path.read_bytes()[:-12])
+        cli = ROOT / "zindex_amd" / "zorindex"
+        r = subprocess.run([str(cli), "query", "-i", str(bacteria_index), "-q", str(bad),
+                            "--records", "-B", "3"], capture_output=True, text=True, timeout=300)
+        assert r.returncode != 0 and codec in r.stderr, r.stderr
+
+
 def test_bacteria_minimizer_demo(tmp_path):
     out = tmp_path / "min_idx"
     zindex_amd.build(str(GOLDEN / "bacteria" / "samples.fof"), str(out), index_mode=1, k=49,

@@ -10,6 +10,9 @@
 #include <cstdio>
 #include <cstring>
 #include <dirent.h>
+#include <condition_variable>
+#include <deque>
+#include <exception>
 #include <mutex>
 #include <set>
 #include <thread>
@@ -1860,6 +1863,79 @@ static double now_s() {
 
 }  // namespace zg
 
+namespace zg {
+// The query file parsed on a thread of its own into batches of at most
+// `per_batch` records, at most two batches ahead of the consumer
+// (records.rs:58-148: the parser thread and its bounded channel).
+class BatchStream {
+ public:
+  BatchStream(const std::string &path, bool strict, size_t per_batch)
+      : lines_(open_lines(path)), reader_(*lines_, strict), per_batch_(per_batch) {
+    thread_ = std::thread([this] { run(); });
+  }
+  ~BatchStream() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    if (thread_.joinable()) thread_.join();
+  }
+  // The next batch; false after the last one. Rethrows a parse error.
+  bool next(Records &out) {
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&] { return !queue_.empty() || done_; });
+    if (queue_.empty()) {
+      if (error_) std::rethrow_exception(error_);
+      return false;
+    }
+    out = std::move(queue_.front());
+    queue_.pop_front();
+    cv_.notify_all();
+    return true;
+  }
+  uint64_t records() const { return reader_.records(); }
+  double parse_s() const { return parse_s_; }
+
+ private:
+  void run() {
+    try {
+      for (;;) {
+        const double t = now_s();
+        Records r;
+        const bool more = reader_.next(r, per_batch_);
+        parse_s_ += now_s() - t;
+        if (!more) break;
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return queue_.size() < 2 || stop_; });
+        if (stop_) break;
+        queue_.push_back(std::move(r));
+        cv_.notify_all();
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mu_);
+      error_ = std::current_exception();
+    }
+    std::lock_guard<std::mutex> lk(mu_);
+    done_ = true;
+    cv_.notify_all();
+  }
+  static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  }
+  std::unique_ptr<LineSource> lines_;
+  RecordReader reader_;
+  size_t per_batch_;
+  std::thread thread_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<Records> queue_;
+  bool done_ = false, stop_ = false;
+  std::exception_ptr error_;
+  double parse_s_ = 0;
+};
+}  // namespace zg
+
 extern "C" int zg_query_command(const char *dir, const char *query, double min_ratio,
                                 int recover, int out_fd, int err_fd) {
   return zg_query_command_ex(dir, query, min_ratio, recover, ZG_MEMORY_ALL, 0, out_fd, err_fd);
@@ -1877,22 +1953,29 @@ extern "C" int zg_query_command_ex(const char *dir, const char *query, double mi
     if (rc) return rc;
     std::unique_ptr<zg_index> hold(idx);
     check_query_support(idx->m);
-    double tp = now_s();
-    Records r = parse_sequences(read_maybe_compressed(query));
-    double prepare = now_s() - tp;
+    // sequences parsed in bounded batches on their own thread while the
+    // index loads and earlier batches are looked up
+    BatchStream batches(query, false, 1 << 16);
     double tl = now_s();
     rc = zg_index_load_ex(idx, recover, hbm_budget, memory);
     if (rc) return rc;
-    double load = now_s() - tl + (tp - t0);
+    double load = now_s() - tl + (tl - t0);
     const uint32_t C = (uint32_t)idx->m.colors.size();
-    std::vector<uint64_t> totals(r.size()), agg(C);
-    double tq = now_s();
-    rc = zg_query_reads(idx, (const uint8_t *)r.seq.data(), r.off.data(), r.size(), 0,
-                        totals.data(), nullptr, agg.data());
-    if (rc) return rc;
-    double lookup = now_s() - tq;
+    std::vector<uint64_t> totals, agg(C), part(C);
     uint64_t total = 0;
-    for (uint64_t t : totals) total += t;
+    double lookup = 0;
+    Records r;
+    while (batches.next(r)) {
+      totals.assign(r.size(), 0);
+      double tq = now_s();
+      rc = zg_query_reads(idx, (const uint8_t *)r.seq.data(), r.off.data(), r.size(), 0,
+                          totals.data(), nullptr, part.data());
+      if (rc) return rc;
+      lookup += now_s() - tq;
+      for (uint64_t t : totals) total += t;
+      for (uint32_t c = 0; c < C; c++) agg[c] += part[c];
+    }
+    const double prepare = batches.parse_s();
     const Manifest &m = idx->m;
     uint64_t classes = m.classes();
     double to = now_s();
@@ -1956,19 +2039,27 @@ extern "C" int zg_query_records_command_ex(const char *dir, const char *query, d
     if (rc) return rc;
     std::unique_ptr<zg_index> hold(idx);
     check_query_support(idx->m);
-    Records r = parse_records(read_maybe_compressed(query));
+    // records parsed in batches of batch_size on their own thread, while the
+    // index loads and the previous batch is looked up and written
+    BatchStream batches(query, true, (size_t)std::min<uint64_t>(batch_size, SIZE_MAX));
     rc = zg_index_load_ex(idx, recover, hbm_budget, memory);
     if (rc) return rc;
     double load = now_s() - t0;
     const Manifest &m = idx->m;
     const uint32_t C = (uint32_t)m.colors.size();
     FdWriter out(out_fd);
-    out.put(format == 0 ? "query_id\tdataset_id\tdataset_name\tmatches\tqueried_seeds\tratio\n"
-                        : "query_id\tvalid_kmers\thits\n");
     // bounded chunks of reads (per-read counters are n_reads x colours)
     const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(1ULL << 20, (1ULL << 28) / std::max(1u, C)));
     std::vector<uint64_t> totals;
     std::vector<uint32_t> counts;
+    bool header = false;
+    Records r;
+    while (batches.next(r)) {
+    if (!header) {
+      out.put(format == 0 ? "query_id\tdataset_id\tdataset_name\tmatches\tqueried_seeds\tratio\n"
+                          : "query_id\tvalid_kmers\thits\n");
+      header = true;
+    }
     for (uint64_t r0 = 0; r0 < r.size(); r0 += chunk) {
       uint64_t n = std::min<uint64_t>(chunk, r.size() - r0);
       totals.assign(n, 0);
@@ -2021,6 +2112,8 @@ extern "C" int zg_query_records_command_ex(const char *dir, const char *query, d
       }
       out.put(s);
     }
+    }
+    ensure(batches.records() > 0, "no FASTA/FASTQ records", -1);
     out.flush();
     FdWriter err(err_fd);
     err.put("query_memory\tmode=gpu-resident\tmax_loaded_subindexes=" +
