@@ -718,7 +718,7 @@ def main() -> int:
         launches = max(1, st["sweep_stage_launches"])
         algo = st["sweep_algorithmic_bytes"] // launches
         kernel_ms = st["sweep_lookup_ms"] / launches
-        kernel = "stage_lookup_kernel"
+        kernel = "plane_lookup_kernel" if st["sweep_plane_banks"] else "stage_lookup_kernel"
     elif args.mode == "kmers":  # packed bases and invalid bits read by the lookup
         algo = st["algorithmic_bytes"] + positions * 3 // 8
         kernel_ms = lk_ms
@@ -834,7 +834,8 @@ def main() -> int:
                      "pack ASCII->2-bit + open-closed minimizer selection and compaction")
                     + (f" + per chunk of whole reads ({st['sweep_launches']} chunks): stable LSD "
                        "radix passes of the seeds by PositionHash segment (6-bit digits), the "
-                       "staged lookup from LDS segment windows into hit rows, per-read sums "
+                       + ("bit-plane lookup" if st["sweep_plane_banks"] else "staged lookup")
+                       + " from LDS segment windows into hit rows, per-read sums "
                        "of the hit rows; eager launches with no host synchronization inside "
                        "the step" if sweep else " + persistent lookup (HIP graph)"),
         },

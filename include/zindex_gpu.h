@@ -308,6 +308,9 @@ typedef struct zg_batch_stats {
                                        4-byte position read once and its hit
                                        row written once */
   uint64_t sweep_hit_row_bytes;     /* hit-row bytes per position */
+  uint64_t sweep_plane_banks;       /* banks its lookups read as bit planes (a
+                                       copy of the bank made on first use: per
+                                       32 fields, b words of one bit each) */
 } zg_batch_stats;
 
 /* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base

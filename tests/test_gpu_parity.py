@@ -90,7 +90,8 @@ def check_parity(idx, orc, reads, recover=True, label="", sweep=True):
 
 
 def assert_sweep_runs(idx, reads):
-    """The batch runs as the sorted sweep (not the read-ordered lookup)."""
+    """The batch runs as the sorted sweep (not the read-ordered lookup);
+    returns its stats."""
     buf, off = zindex_amd.pack_reads(reads)
     saved = idx.tuning
     idx.tuning = dict(saved, sweep=1)
@@ -99,6 +100,7 @@ def assert_sweep_runs(idx, reads):
         st = b.stats()
     idx.tuning = saved
     assert st["tuning"]["sweep"] == 1 and st["sweep_launches"] >= 1, st
+    return st
 
 
 @pytest.fixture(scope="module")

@@ -37,8 +37,14 @@ def postings_of_query_keys(idx, reads, k=31):
     return best
 
 
+# the sweep's lookup kernels: over bit-plane copies of the banks (banks
+# without merged groups), or the staged lookup over the byte planes
+KERNELS = {"planes": "1", "staged": "0"}
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("group", [1, 4])
-def test_recovery_keys_with_many_postings(group, monkeypatch):
+def test_recovery_keys_with_many_postings(group, kernel, monkeypatch):
     # 160 near-identical colours at load 2 (slot_scale 0.5): k-mers shared by
     # every colour stay in the 2-core of every column, so their recovery
     # entries list well over 64 colours and the lookup walks their postings
@@ -49,18 +55,21 @@ def test_recovery_keys_with_many_postings(group, monkeypatch):
     most = postings_of_query_keys(idx, reads)
     say(f"group {group}: largest posting list hit by the reads: {most}")
     assert most > 128
+    monkeypatch.setenv("ZG_SWEEP_PLANES", KERNELS[kernel])
     idx.load(recover=True)
     orc = oracle_for(idx, True)
     for name, tuning in (("read order", READ_ORDER), ("sweep", SWEEP_BENCH)):
         idx.tuning = tuning
         check_parity(idx, orc, reads, True, label=f"postings>128 g{group} {name}")
-    assert_sweep_runs(idx, reads)
+    st = assert_sweep_runs(idx, reads)
+    assert st["sweep_plane_banks"] == (1 if kernel == "planes" and group == 1 else 0), st
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "20000")
-    check_parity(idx, orc, reads, True, label=f"postings>128 g{group} sweep chunks")
+    check_parity(idx, orc, reads, True, label=f"postings>128 g{group} {kernel} sweep chunks")
 
 
+@pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("n_colors", [1024, 1100])
-def test_sorted_sweep_at_bench_shape(n_colors, monkeypatch):
+def test_sorted_sweep_at_bench_shape(n_colors, kernel, monkeypatch):
     # 1024 singleton colours (one 1 KiB tile per row) and 1100 (two tiles)
     rng = np.random.default_rng(400 + n_colors)
     root = synth.random_genome(rng, 4000)
@@ -68,14 +77,16 @@ def test_sorted_sweep_at_bench_shape(n_colors, monkeypatch):
               for _ in range(n_colors)]
     reads = make_reads(401, colors, 300, lo=30, hi=3000)
     idx = build(colors, fingerprint_bits=8, hashes=4)
+    monkeypatch.setenv("ZG_SWEEP_PLANES", KERNELS[kernel])
     idx.load()
     orc = oracle_for(idx)
     idx.tuning = SWEEP_BENCH
-    assert_sweep_runs(idx, reads)
-    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours")
+    st = assert_sweep_runs(idx, reads)
+    assert st["sweep_plane_banks"] == (1 if kernel == "planes" else 0), st
+    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours {kernel}")
     # many chunks: each a few reads, chunk starts far from position 0
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "7000")
-    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours, chunks")
+    check_parity(idx, orc, reads, label=f"sweep bench shape {n_colors} colours {kernel}, chunks")
 
 
 def test_plain_method_small_at_stated_size():
@@ -105,8 +116,9 @@ def test_plain_method_small_at_stated_size():
     assert counts.sum() > 0
 
 
+@pytest.mark.parametrize("kernel", list(KERNELS))
 @pytest.mark.parametrize("case", ["h4 recovery", "h3 partitioned", "h2"])
-def test_lane_lookup_wide_rows(case, monkeypatch):
+def test_lane_lookup_wide_rows(case, kernel, monkeypatch):
     # rows of >= 512 colours at b = 8 (whole 512-byte slices, one entry per
     # lane in the staged lookup), mixed with the wave-per-entry pass for
     # entries with recovery postings, routed to partitions, and a last slice
@@ -129,13 +141,15 @@ def test_lane_lookup_wide_rows(case, monkeypatch):
         most = postings_of_query_keys(idx, reads)
         say(f"{case}: largest posting list hit by the reads: {most}")
         assert most > 64
+    monkeypatch.setenv("ZG_SWEEP_PLANES", KERNELS[kernel])
     idx.load(recover=True)
     orc = oracle_for(idx, True)
     idx.tuning = SWEEP_BENCH
-    assert_sweep_runs(idx, reads)
-    check_parity(idx, orc, reads, True, label=f"lane lookup {case}")
+    st = assert_sweep_runs(idx, reads)
+    assert (st["sweep_plane_banks"] > 0) == (kernel == "planes"), st
+    check_parity(idx, orc, reads, True, label=f"lane lookup {case} {kernel}")
     monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "9000")
-    check_parity(idx, orc, reads, True, label=f"lane lookup {case}, chunks")
+    check_parity(idx, orc, reads, True, label=f"lane lookup {case} {kernel}, chunks")
 
 
 @pytest.mark.parametrize("records", [True, False])

@@ -374,6 +374,9 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
     }
   }
   hipStream_t s = idx->stream;
+  // the sweep's tables (and bit-plane copies) describe the banks uploaded here
+  idx->sw_row_words = 0;
+  idx->sw_classes.clear();
   idx->dbanks.clear();
   idx->dbanks.resize(nb);
   idx->perm.assign(nb, {});
@@ -571,7 +574,7 @@ uint64_t resident_bytes(const zg_index *idx) {
                idx->inner_table.bytes + idx->stream_data.bytes;
   for (auto &d : idx->dbanks)
     n += d.data.bytes + d.col_color.bytes + d.mg_off.bytes + d.mg_members.bytes +
-         d.mg_mask_bit.bytes + d.color_col.bytes + d.color_bit.bytes;
+         d.mg_mask_bit.bytes + d.color_col.bytes + d.color_bit.bytes + d.planes.bytes;
   return n;
 }
 
@@ -1515,6 +1518,9 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
       o->sweep_hit_row_bytes = idx->sw_row_words * 4ULL;
       o->sweep_algorithmic_bytes =
           b->sweep_chunks * bank_bytes + o->seeds * (12 + o->sweep_hit_row_bytes);
+      o->sweep_plane_banks = 0;
+      for (const SweepClass &c : idx->sw_classes)
+        if (c.planes) o->sweep_plane_banks += c.banks.size();
     }
     return 0;
   } catch (...) {

@@ -74,11 +74,15 @@ struct SweepClass {
   uint32_t col_word0 = 0;        // first word of its column bits in a row
   uint32_t member_bit0 = 0;      // first bit of its member bits in a row
   uint32_t members = 0;
+  bool planes = false;           // looked up over bit-plane copies of its banks
 };
 
 struct DeviceBank {
   DevBuf data, col_color, mg_off, mg_members, mg_mask_bit, color_col, color_bit;
   BankDev view{};
+  // the payload as bit planes for the sorted sweep (built on first use)
+  DevBuf planes;
+  uint64_t planes_pitch = 0;
 };
 
 // Index options: the persistent lookup's launch shape.
@@ -136,6 +140,7 @@ struct zg_index {
   std::vector<zg::SweepClass> sw_classes;
   uint32_t sw_row_words = 0, sw_col_words = 0, sw_members = 0;
   zg::DevBuf sw_bit_color;
+  int sw_planes = -1;  // whether the tables were built with bit-plane copies allowed
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();

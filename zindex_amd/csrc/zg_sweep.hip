@@ -46,6 +46,12 @@ constexpr uint32_t kStageOut = 512;                  // staged hit bytes per wav
 constexpr uint64_t kSweepHitBytes = 64ULL << 30;     // hit rows of one chunk, at most
 constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
 constexpr uint32_t kXcds = 8;
+// The bit-plane lookup: 512-thread blocks, two per CU, each with a 64 KiB
+// window (H x L rows of one slice of at most 256 bytes), so one block's
+// window fill overlaps the other's lookups.
+constexpr uint32_t kPlaneThreads = 512;
+constexpr uint32_t kPlaneWaves = kPlaneThreads / 64;
+constexpr uint32_t kPlaneWindow = 64 * 1024;
 
 // The seeds of positions [p0, p1): canonical k-mers (k <= 32), optionally
 // through a spaced seed, keyed by their first PositionHash word; modimizers
@@ -417,6 +423,8 @@ struct StageArgs {
   uint32_t *queue;        // per-XCD unit counters at stride 32
   uint32_t xcds;
   unsigned long long *folded;  // entries answered from the entry before them
+  const uint8_t *planes;  // the bank as bit planes (plane_lookup_kernel), or null
+  uint64_t planes_pitch;  // its row pitch
 };
 
 namespace {
@@ -822,6 +830,312 @@ __global__ __launch_bounds__(1024) void stage_lookup_kernel(StageArgs a) {
   if (folded && lane == 0) atomicAdd(a.folded, (unsigned long long)folded);
 }
 
+// Field c (B bits from bit c * B, little endian) of a plane-layout row.
+template <int B>
+__device__ __forceinline__ uint32_t plane_field(const uint8_t *row, uint32_t c) {
+  if constexpr (B >= 8) {
+    const uint8_t *p = row + (uint64_t)c * (B / 8);
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < B / 8; i++) v |= (uint32_t)p[i] << (8 * i);
+    return v;
+  } else {
+    const uint64_t bit = (uint64_t)c * B;
+    return (row[bit >> 3] >> (bit & 7)) & ((1u << B) - 1);
+  }
+}
+
+// The bank as bit planes: row r of `out` (pitch bytes) holds the row's
+// fingerprint fields in groups of 32; group g, at byte 4 * B * g, is B words,
+// bit f of word k being bit k of field 32 g + f. Fields past the last column
+// stay zero (the rows are cleared before).
+template <int B>
+__global__ void bit_planes_kernel(const uint8_t *data, uint64_t row_bytes, uint64_t rows,
+                                  uint32_t groups, uint32_t g32, uint64_t pitch, uint8_t *out) {
+  const uint64_t total = rows * g32;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / g32;
+    const uint32_t g = (uint32_t)(i % g32);
+    const uint8_t *row = data + r * row_bytes;
+    uint32_t pl[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) pl[k] = 0;
+    for (uint32_t f = 0; f < 32; f++) {
+      const uint32_t c = g * 32 + f;
+      if (c >= groups) break;
+      const uint32_t v = plane_field<B>(row, c);
+#pragma unroll
+      for (int k = 0; k < B; k++) pl[k] |= ((v >> k) & 1u) << f;
+    }
+    uint32_t *dst = (uint32_t *)(out + r * pitch) + (uint64_t)g * B;
+#pragma unroll
+    for (int k = 0; k < B; k++) dst[k] = pl[k];
+  }
+}
+
+// XOR of the H row words of one plane word and the fingerprint bit mask fm.
+template <int HT, int HMAX>
+__device__ __forceinline__ uint32_t rows_xor(const uint32_t (&w)[HMAX], uint32_t fm, uint32_t H) {
+  if constexpr (HT == 4) {
+    return __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(w[0], w[1], w[2], 0x96), w[3], fm,
+                                       0x96);
+  } else if constexpr (HT == 3) {
+    return __builtin_amdgcn_bitop3_b32(__builtin_amdgcn_bitop3_b32(w[0], w[1], w[2], 0x96), fm, 0u,
+                                       0x96);
+  } else if constexpr (HT == 2) {
+    return __builtin_amdgcn_bitop3_b32(w[0], w[1], fm, 0x96);
+  } else {
+    uint32_t d = fm;
+#pragma unroll
+    for (int i = 0; i < HMAX; i++)
+      if (i < (int)H) d ^= w[i];
+    return d;
+  }
+}
+
+// The staged lookup over the bank as bit planes (bit_planes_kernel), for
+// banks without merged groups: per 32 fields, the H rows' B plane words are
+// XORed with the fingerprint's bits (all-ones or all-zero words) and ORed, so
+// a field hits where no plane differs -- about 5 instructions per 8 fields
+// at b = 8 and H = 4. A block (8 waves, two blocks per CU) takes units
+// (segment, slice of NP 16-byte pieces per row) from its XCD's counter and
+// copies the slice of the segment's H x L window rows into LDS. Each lane
+// takes one entry and reads its rows' pieces in the order (l ^ c) mod NP, so
+// the 16 lanes of an LDS access read 16 different bank groups. Hit words
+// are written in field order: bit f of word w of slice q is column
+// 32 (q SG + w) + f. Recovery postings are ORed in afterwards, once per
+// entry (plane_recovery_kernel).
+template <int B, int HT, int NP>
+__global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a) {
+  constexpr uint32_t SB = NP * 16;              // bytes of a row slice
+  constexpr uint32_t SG = SB / (4 * B);         // 32-field groups (hit words) per slice
+  constexpr uint32_t PPG = B >= 4 ? B / 4 : 1;  // pieces per group
+  constexpr uint32_t GPP = B >= 4 ? 1 : 4 / B;  // groups per piece
+  constexpr uint32_t BS = GPP;                  // hit words per block of the piece order
+  constexpr uint32_t NB = SG / BS;
+  constexpr int HMAX = HT ? HT : 8;
+  static_assert(SG >= 1 && NB >= 1 && (NP & (NP - 1)) == 0, "slice geometry");
+  __shared__ __attribute__((aligned(16))) uint8_t win[kPlaneWindow];
+  __shared__ uint32_t unit_sh;
+  const BankDev &bk = a.bank;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t H = HT ? (uint32_t)HT : bk.hashes;
+  const uint64_t L = bk.geo.length, count = bk.geo.count;
+  const uint32_t x = blockIdx.x % a.xcds;
+  const uint64_t s_lo = count * x / a.xcds, s_hi = count * (x + 1) / a.xcds;
+  const uint64_t units = (s_hi - s_lo) * a.n_slices;
+  const uint64_t seed = a.seed;
+  const uint32_t lp = (lane & (NP - 1)) * 16;  // the lane's first piece
+  // hit words come out in blocks of BS words, block j of the slice at
+  // block j ^ tq of o[]
+  const uint32_t tq = (B >= 4 ? lane / PPG : lane) & (NB - 1);
+  // the planes each piece of a group holds for this lane: 4 ((l ^ t) mod PPG)
+  // .. + 3 at run position t (b >= 4); planes 0, 1, 0, 1 (b = 2); plane 0 (b = 1)
+  uint32_t psh[PPG];
+#pragma unroll
+  for (uint32_t t = 0; t < PPG; t++) psh[t] = B >= 4 ? 4 * ((lane ^ t) & (PPG - 1)) : 0;
+  while (true) {
+    __syncthreads();  // the previous unit's window is no longer read
+    if (threadIdx.x == 0) unit_sh = atomicAdd(a.queue + 32 * x, 1u);
+    __syncthreads();
+    const uint64_t u = unit_sh;
+    if (u >= units) break;
+    const uint64_t s = s_lo + u / a.n_slices;
+    const uint32_t q = (uint32_t)(u % a.n_slices);
+    const uint32_t e0 = a.start[s], e1 = a.start[s + 1];
+    if (e0 == e1) continue;
+    // the window: slice q of rows s*L .. (s+H)*L - 1, straight into LDS
+    {
+      const uint8_t *src = a.planes + s * L * a.planes_pitch + (uint64_t)q * SB;
+      const uint32_t chunks = H * (uint32_t)L * NP;
+      for (uint32_t cb = wave * 64; cb < chunks; cb += kPlaneThreads) {
+        const uint32_t c = cb + lane;
+        if (c < chunks)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(src + (uint64_t)(c / NP) * a.planes_pitch + (c % NP) * 16),
+              (__attribute__((address_space(3))) void *)(win + cb * 16), 16, 0, 0);
+      }
+    }
+    uint64_t h_next = 0;
+    uint32_t p_next = 0;
+    if (e0 + wave * 64 + lane < e1) {
+      h_next = a.ent_h[e0 + wave * 64 + lane];
+      p_next = a.ent_p[e0 + wave * 64 + lane];
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (uint32_t b0 = e0 + wave * 64; b0 < e1; b0 += kPlaneWaves * 64) {
+      const uint32_t e = b0 + lane;
+      const bool v = e < e1;
+      const uint64_t h = h_next;
+      const uint32_t p = p_next;
+      if (e + kPlaneWaves * 64 < e1) {
+        h_next = a.ent_h[e + kPlaneWaves * 64];
+        p_next = a.ent_p[e + kPlaneWaves * 64];
+      }
+      const uint64_t key = unsplitmix64(h) - seed;  // mixsplit(key, seed) == h
+      const uint32_t fp = (uint32_t)fingerprint(key, seed, B);
+      const uint64_t step = splitmix64(h);
+      // window byte offsets of the H rows (i*L + ((h_i ^ h_i >> 33) & (L - 1)))
+      // plus this lane's first piece
+      uint32_t rb[HMAX];
+      uint64_t hv = h;
+#pragma unroll
+      for (int i = 0; i < HMAX; i++) {
+        rb[i] = 0;
+        if (HT || i < (int)H) {
+          rb[i] = (uint32_t)(i * L + ((hv ^ (hv >> 33)) & (L - 1))) * SB + lp;
+          hv += step;
+        }
+      }
+      const uint64_t prow = (uint64_t)p * a.row_words;
+      // a partitioned bank answers the keys routed to it (storage.rs:99-101)
+      const bool own =
+          v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition);
+      if (own) {
+        uint32_t fm[PPG][4];
+#pragma unroll
+        for (uint32_t t = 0; t < PPG; t++) {
+          const uint32_t fpt = fp >> psh[t];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            fm[t][k] = 0u - ((fpt >> (B >= 4 ? k : (B == 2 ? (k & 1) : 0))) & 1u);
+            asm volatile("" : "+v"(fm[t][k]));  // kept in a register across the pieces
+          }
+        }
+        uint32_t o[SG];
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < NP; c++) {
+          uint4 r[HMAX];
+#pragma unroll
+          for (int i = 0; i < HMAX; i++)
+            if (HT || i < (int)H) r[i] = *(const uint4 *)(win + (rb[i] ^ (c * 16)));
+          const uint32_t t = c & (PPG - 1);
+          uint32_t wx[HMAX], wy[HMAX], wz[HMAX], ww[HMAX];
+#pragma unroll
+          for (int i = 0; i < HMAX; i++) {
+            wx[i] = r[i].x;
+            wy[i] = r[i].y;
+            wz[i] = r[i].z;
+            ww[i] = r[i].w;
+          }
+          const uint32_t d0 = rows_xor<HT, HMAX>(wx, fm[t][0], H);
+          const uint32_t d1 = rows_xor<HT, HMAX>(wy, fm[t][1], H);
+          const uint32_t d2 = rows_xor<HT, HMAX>(wz, fm[t][2], H);
+          const uint32_t d3 = rows_xor<HT, HMAX>(ww, fm[t][3], H);
+          if constexpr (B >= 4) {
+            // a field differs from the fingerprint where any plane does
+            const uint32_t any = (t == 0 ? 0u : acc) | d0 | d1 | d2 | d3;
+            if (t == PPG - 1) o[c / PPG] = ~any;
+            else acc = any;
+          } else if constexpr (B == 2) {
+            o[2 * c] = ~(d0 | d1);
+            o[2 * c + 1] = ~(d2 | d3);
+          } else {
+            o[4 * c] = ~d0;
+            o[4 * c + 1] = ~d1;
+            o[4 * c + 2] = ~d2;
+            o[4 * c + 3] = ~d3;
+          }
+        }
+        // into field order: block j <- block j ^ tq
+#pragma unroll
+        for (uint32_t sbit = 1; sbit < NB; sbit <<= 1) {
+          const bool sw = (tq & sbit) != 0;
+#pragma unroll
+          for (uint32_t j = 0; j < NB; j++)
+            if (!(j & sbit)) {
+#pragma unroll
+              for (uint32_t w = 0; w < BS; w++) {
+                const uint32_t x0 = o[j * BS + w], x1 = o[(j | sbit) * BS + w];
+                o[j * BS + w] = sw ? x1 : x0;
+                o[(j | sbit) * BS + w] = sw ? x0 : x1;
+              }
+            }
+        }
+        uint32_t *dst = a.hits + prow + a.col_word0 + (uint64_t)q * SG;
+        if constexpr (SG % 4 == 0) {
+#pragma unroll
+          for (uint32_t k = 0; k < SG / 4; k++)
+            ((uint4 *)dst)[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+        } else if constexpr (SG == 2) {
+          *(uint2 *)dst = make_uint2(o[0], o[1]);
+        } else {
+          dst[0] = o[0];
+        }
+      }
+    }
+  }
+}
+
+// Recovery for the classes looked up as bit planes, once per entry after all
+// of the class's lookups (so it follows every row store): an entry whose key
+// has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
+// bits of its postings' columns of this class ORed into its hit row. The
+// postings of a wave's recovered entries go 64 per pass, lane g finding its
+// entry by binary search over the inclusive ends. Partitions of a class share
+// its columns, so every entry is handled here whichever bank it routes to.
+__global__ __launch_bounds__(256) void plane_recovery_kernel(
+    const uint64_t *ent_h, const uint32_t *ent_p, const uint64_t *n_ptr, uint64_t seed, RecDev rec,
+    uint32_t colmap, uint32_t groups, uint32_t *hits, uint32_t row_words, uint32_t col_word0) {
+  __shared__ uint32_t ends_all[4 * 64];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t *ends = ends_all + wave * 64;
+  const uint64_t n = *n_ptr;
+  const uint64_t stride = (uint64_t)gridDim.x * 4 * 64;
+  const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * 4 + wave) * 64; w0 < n; w0 += stride) {
+    const uint64_t e = w0 + lane;
+    uint32_t rec_start = 0, rec_len = 0;
+    uint64_t prow = 0;
+    if (e < n) {
+      const uint64_t key = unsplitmix64(ent_h[e]) - seed;  // mixsplit(key, seed) == h
+      prow = (uint64_t)ent_p[e] * row_words;
+      const uint32_t ah = abandoned_hash(key, seed);
+      uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
+      while (true) {
+        const uint64_t de = rec.dir[slot];
+        const uint32_t pi = (uint32_t)de;
+        if (pi == kNone) break;
+        if ((uint32_t)(de >> 32) == ah) {
+          rec_len = rec.post[pi];
+          rec_start = pi + 1;
+          break;
+        }
+        slot = (slot + 1) & cap_mask;
+      }
+    }
+    if (!__ballot(rec_len != 0)) continue;
+    const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
+    const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
+    ends[lane] = end;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+      const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
+      uint32_t j = 0;  // the first lane whose postings end after g
+#pragma unroll
+      for (uint32_t st = 32; st; st >>= 1)
+        if (ends[j + st - 1] <= g) j += st;
+      const uint32_t first = j ? ends[j - 1] : 0u;
+      const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)j, 64);
+      const uint64_t pj = shfl64(prow, j);
+      if (g0 + lane < total) {
+        const uint32_t pe = rec.post_col[rs + (g - first)];
+        const uint32_t col = pe & 0xFFFFFFu;
+        if (pe != kNone && (pe >> 24) == colmap && col < groups)
+          atomicOr(hits + pj + col_word0 + col / 32, 1u << (col % 32));
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // Bit-sliced counters: plane j holds bit j of each of the 32 counts of a
 // lane's word (kPlanes planes: counts up to 1023 between spills).
 constexpr int kPlanes = 10;
@@ -1038,6 +1352,82 @@ bool sweep_layout_ok(const zg_index *idx) {
   return true;
 }
 
+// Row-slice bytes of the bit-plane lookup for a bank of b-bit fields, 0 when
+// it does not apply: banks without merged groups, H in {2, 3, 4, 8}, the window of
+// H x L slices within 64 KiB. Slices are 256 bytes (16 pieces) for H <= 4,
+// 128 bytes for larger H and for b <= 2 (a 1024-column row at b = 1).
+static uint32_t plane_slice_bytes(const BankDev &bk, uint32_t B) {
+  if (bk.first_merged != bk.groups || bk.groups == 0) return 0;
+  if (bk.hashes != 2 && bk.hashes != 3 && bk.hashes != 4 && bk.hashes != 8) return 0;
+  if (B != 1 && B != 2 && B != 4 && B != 8 && B != 16 && B != 32) return 0;
+  const uint32_t sb = (bk.hashes <= 4 && B >= 4) ? 256 : 128;
+  if ((uint64_t)bk.hashes * bk.geo.length * sb > kPlaneWindow) return 0;
+  return sb;
+}
+
+static bool planes_enabled() {
+  const char *e = getenv("ZG_SWEEP_PLANES");  // 0: always the staged lookup
+  return !e || strcmp(e, "0") != 0;
+}
+
+// Bank rows any lookup reads: (count + H - 1) segments of L rows.
+static uint64_t lookup_rows(const BankDev &bk) {
+  return (bk.geo.count + bk.hashes - 1) * bk.geo.length;
+}
+
+template <int B>
+static void launch_bit_planes(const BankDev &bk, uint64_t rows, uint64_t pitch, uint8_t *out) {
+  const uint32_t g32 = (bk.groups + 31) / 32;
+  const uint64_t n = rows * g32;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(65536, (n + 255) / 256);
+  bit_planes_kernel<B><<<blocks, 256, 0, 0>>>(bk.data, bk.row_bytes, rows, bk.groups, g32, pitch, out);
+}
+
+// The bit-plane copy of a bank: slices of sb bytes, n_slices per row.
+static void build_planes(DeviceBank &db, uint32_t B, uint32_t sb, uint32_t n_slices) {
+  const BankDev &bk = db.view;
+  const uint64_t rows = lookup_rows(bk), pitch = (uint64_t)n_slices * sb;
+  db.planes.alloc(rows * pitch);
+  check_hip(hipMemsetAsync(db.planes.p, 0, rows * pitch, 0), "bit planes");
+  uint8_t *out = db.planes.as<uint8_t>();
+  switch (B) {
+    case 1: launch_bit_planes<1>(bk, rows, pitch, out); break;
+    case 2: launch_bit_planes<2>(bk, rows, pitch, out); break;
+    case 4: launch_bit_planes<4>(bk, rows, pitch, out); break;
+    case 8: launch_bit_planes<8>(bk, rows, pitch, out); break;
+    case 16: launch_bit_planes<16>(bk, rows, pitch, out); break;
+    default: launch_bit_planes<32>(bk, rows, pitch, out); break;
+  }
+  check_hip(hipGetLastError(), "bit planes");
+  check_hip(hipStreamSynchronize(0), "bit planes");
+  db.planes_pitch = pitch;
+}
+
+template <int B>
+static void launch_planes(const StageArgs &a, uint32_t blocks, hipStream_t s) {
+  // 256-byte slices at b >= 4 and H <= 4; 128-byte slices otherwise
+  if constexpr (B >= 4) {
+    ensure(a.slice_bytes == (a.bank.hashes <= 4 ? 256u : 128u), "bit-plane lookup: slice size",
+           ZG_ERR_INVALID);
+    switch (a.bank.hashes) {
+      case 2: plane_lookup_kernel<B, 2, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 3: plane_lookup_kernel<B, 3, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 4: plane_lookup_kernel<B, 4, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 8: plane_lookup_kernel<B, 8, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      default: fail(ZG_ERR_INVALID, "bit-plane lookup: hash count");
+    }
+  } else {
+    ensure(a.slice_bytes == 128, "bit-plane lookup: slice size", ZG_ERR_INVALID);
+    switch (a.bank.hashes) {
+      case 2: plane_lookup_kernel<B, 2, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 3: plane_lookup_kernel<B, 3, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 4: plane_lookup_kernel<B, 4, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      case 8: plane_lookup_kernel<B, 8, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
+      default: fail(ZG_ERR_INVALID, "bit-plane lookup: hash count");
+    }
+  }
+}
+
 template <int B>
 static void launch_stage(const StageArgs &a, uint32_t blocks, hipStream_t s) {
   switch (a.bank.hashes) {
@@ -1056,10 +1446,17 @@ static std::vector<uint32_t> fetch_u32(const uint32_t *p, uint64_t n) {
 
 // The per-index tables of the sweep (built on first use, outside any
 // capture): size classes, the hit-row layout and the colour of every row bit.
-// Row layout: each class's column region (n_slices x 512/b bytes, column c
-// of the class at bit c of its region), then every class's member bits.
+// Row layout: each class's column region (bit-plane classes: n_slices x SG
+// words, column c at bit c; others: n_slices x 512/b bytes, column c at the
+// bit of its lane field), then every class's member bits.
 static void ensure_sweep_tables(zg_index *idx) {
-  if (idx->sw_row_words) return;
+  const int allow = planes_enabled() ? 1 : 0;
+  if (idx->sw_row_words && idx->sw_planes == allow) return;
+  for (DeviceBank &db : idx->dbanks) {  // tables of the other setting
+    db.planes.reset();
+    db.planes_pitch = 0;
+  }
+  idx->sw_planes = allow;
   const uint32_t B = (uint32_t)idx->m.fingerprint_bits;
   std::vector<SweepClass> classes;
   for (uint32_t bi = 0; bi < idx->dbanks.size(); bi++) {
@@ -1083,11 +1480,35 @@ static void ensure_sweep_tables(zg_index *idx) {
       classes.push_back(c);
     }
   }
+  // bit-plane copies for the classes the plane lookup serves, while device
+  // memory allows (a copy costs about one bank; a quarter of the free memory
+  // stays for the chunk buffers)
+  for (SweepClass &c : classes) {
+    const BankDev &bk = idx->dbanks[c.banks[0]].view;
+    const uint32_t sb = allow ? plane_slice_bytes(bk, B) : 0;
+    if (!sb) continue;
+    const uint32_t sg = sb / (4 * B), g32 = (bk.groups + 31) / 32;
+    const uint32_t n_slices = (g32 + sg - 1) / sg;
+    const uint64_t need = lookup_rows(bk) * n_slices * (uint64_t)sb * c.banks.size();
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (need > free_b / 4 * 3) continue;
+    for (uint32_t bi : c.banks) build_planes(idx->dbanks[bi], B, sb, n_slices);
+    c.planes = true;
+    c.slice_bytes = sb;
+    c.n_slices = n_slices;
+  }
   uint32_t col_words = 0, members = 0;
   for (SweepClass &c : classes) {
     const BankDev &bk = idx->dbanks[c.banks[0]].view;
     c.col_word0 = col_words;
-    col_words += c.n_slices * (kSliceBytes / B) / 4;  // 64 lanes x 64/b bits per slice
+    if (c.planes)  // 32 columns per word, regions in whole 16-byte pieces
+      col_words += (c.n_slices * (c.slice_bytes / (4 * B)) + 3) / 4 * 4;
+    else
+      col_words += c.n_slices * (kSliceBytes / B) / 4;  // 64 lanes x 64/b bits per slice
     c.members = bk.groups > bk.first_merged
                     ? fetch_u32(bk.mg_off, bk.groups - bk.first_merged + 1)[bk.groups - bk.first_merged]
                     : 0;
@@ -1100,11 +1521,12 @@ static void ensure_sweep_tables(zg_index *idx) {
     const BankDev &bk = idx->dbanks[c.banks[0]].view;
     c.member_bit0 += col_words * 32;
     const std::vector<uint32_t> col_color = fetch_u32(bk.col_color, bk.groups);
-    // column col = lane fields: bit (col / F) * F + field_bit(col % F) of the region
+    // column col = bit col of the region (bit planes), or lane fields: bit
+    // (col / F) * F + field_bit(col % F)
     const uint32_t F = 64 / B;
     for (uint32_t col = 0; col < bk.first_merged; col++) {
       const uint32_t f = col % F;
-      const uint32_t bit = col - f + (B == 8 ? field_bit<8>(f) : f);
+      const uint32_t bit = c.planes ? col : col - f + (B == 8 ? field_bit<8>(f) : f);
       color[c.col_word0 * 32ULL + bit] = col_color[col];
     }
     const std::vector<uint32_t> mem = fetch_u32(bk.mg_members, c.members);
@@ -1120,10 +1542,11 @@ static void ensure_sweep_tables(zg_index *idx) {
 }
 
 // The sort of one class: LSD passes on the first segment, then the segment
-// starts. Returns the buffer pair holding the sorted entries.
+// starts. Returns the buffer pair holding the sorted entries; *n_dev is the
+// entry count in device memory.
 template <class Src>
 static int sort_class(zg_batch *b, Src first_pass, uint64_t count, uint64_t n_items,
-                      uint64_t *listed, hipStream_t s) {
+                      uint64_t *listed, const uint64_t **n_dev, hipStream_t s) {
   const uint32_t seg_bits = bits_for(count - 1);
   const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
   const uint32_t dbits = (seg_bits + passes - 1) / passes, nb = 1u << dbits;
@@ -1131,6 +1554,7 @@ static int sort_class(zg_batch *b, Src first_pass, uint64_t count, uint64_t n_it
   const uint64_t hn = (uint64_t)nb * tiles + 1;
   const size_t tb = scan_temp_bytes_u32_to_u64(hn);
   const uint64_t *n_ptr = b->sw_off.as<uint64_t>() + (hn - 1);  // the scan's total
+  *n_dev = n_ptr;
   int cur = 0;
   for (uint32_t pass = 0; pass < passes; pass++) {
     const uint32_t shift = pass * dbits;
@@ -1246,6 +1670,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   sa.xcds = idx->cu_count >= kXcds ? kXcds : 1;
   sa.folded = folded;
   const uint32_t stage_blocks = std::max<uint32_t>(sa.xcds, idx->cu_count);
+  const uint32_t plane_blocks = std::max<uint32_t>(sa.xcds, 2 * idx->cu_count);
   // sum: lanes per item, the row's words rounded up to a power of two (at most 64)
   uint32_t lp_bits = 0;
   while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
@@ -1261,13 +1686,15 @@ void sweep_run(zg_index *idx, zg_batch *b) {
     for (size_t k = 0; k < idx->sw_classes.size(); k++) {
       const SweepClass &c = idx->sw_classes[k];
       int cur;
+      const uint64_t *n_dev = nullptr;  // the class's entry count
       if (keys) {
         KeySource ks{b->keys.as<uint64_t>(), I0, n_items, m.seed};
-        cur = sort_class(b, ks, c.count, n_items, nullptr, s);
+        cur = sort_class(b, ks, c.count, n_items, nullptr, &n_dev, s);
       } else {
         PosSource ps{b->codes.as<uint64_t>(), b->invalid.as<uint64_t>(), I0, n_items, (uint32_t)m.k,
                      m.seed, m.index_mode == 2 ? m.modimizer_sampling : 0, idx->spaced};
-        cur = sort_class(b, ps, c.count, n_items, k == 0 ? b->sw_listed.as<uint64_t>() : nullptr, s);
+        cur = sort_class(b, ps, c.count, n_items, k == 0 ? b->sw_listed.as<uint64_t>() : nullptr,
+                         &n_dev, s);
       }
       check_hip(hipEventRecord(ev[1 + 2 * k], s), "hipEventRecord");
       sa.ent_h = b->sw_h[cur].as<uint64_t>();
@@ -1279,6 +1706,20 @@ void sweep_run(zg_index *idx, zg_batch *b) {
       for (uint32_t bi : c.banks) {
         sa.bank = idx->dbanks[bi].view;
         launch_fill_u32(sa.queue, 32 * kXcds, 0, s);
+        if (c.planes) {
+          sa.planes = idx->dbanks[bi].planes.as<uint8_t>();
+          sa.planes_pitch = idx->dbanks[bi].planes_pitch;
+          switch (B) {
+            case 1: launch_planes<1>(sa, plane_blocks, s); break;
+            case 2: launch_planes<2>(sa, plane_blocks, s); break;
+            case 4: launch_planes<4>(sa, plane_blocks, s); break;
+            case 8: launch_planes<8>(sa, plane_blocks, s); break;
+            case 16: launch_planes<16>(sa, plane_blocks, s); break;
+            default: launch_planes<32>(sa, plane_blocks, s); break;
+          }
+          continue;
+        }
+        sa.planes = nullptr;
         switch (B) {
           case 1: launch_stage<1>(sa, stage_blocks, s); break;
           case 2: launch_stage<2>(sa, stage_blocks, s); break;
@@ -1287,6 +1728,12 @@ void sweep_run(zg_index *idx, zg_batch *b) {
           case 16: launch_stage<16>(sa, stage_blocks, s); break;
           default: launch_stage<32>(sa, stage_blocks, s); break;
         }
+      }
+      if (c.planes && sa.rec.enabled) {
+        const BankDev &bk0 = idx->dbanks[c.banks[0]].view;
+        plane_recovery_kernel<<<(uint32_t)std::min<uint64_t>(16384, n_items / 256 + 1), 256, 0, s>>>(
+            sa.ent_h, sa.ent_p, n_dev, sa.seed, sa.rec, bk0.colmap, bk0.groups, sa.hits, row_words,
+            c.col_word0);
       }
       check_hip(hipEventRecord(ev[2 + 2 * k], s), "hipEventRecord");
     }
