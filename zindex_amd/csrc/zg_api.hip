@@ -537,7 +537,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
           color_tag[c] = ((uint32_t)id << 24) | maps[id][c];
         }
     std::vector<uint64_t> dir(cap, (uint64_t)kNone);
-    std::vector<uint32_t> post, post_col;
+    std::vector<uint32_t> post, post_col, filter(kRecFilterWords, 0);
     post.reserve(n + r.origins.size());
     post_col.reserve(n + r.origins.size());
     for (uint64_t i = 0; i < n; i++) {
@@ -551,8 +551,12 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       uint64_t slot = r.hashes[i] >> (32 - bits);
       while ((uint32_t)dir[slot] != kNone) slot = (slot + 1) & (cap - 1);
       dir[slot] = ((uint64_t)r.hashes[i] << 32) | pi;
+      const uint32_t fb = r.hashes[i] >> (32 - kRecFilterLog2);
+      filter[fb / 32] |= 1u << (fb % 32);
     }
     upload(idx->rec_dir, dir, s);
+    upload(idx->rec_filter, filter, s);
+    idx->rec_view.filter = idx->rec_filter.as<uint32_t>();
     upload(idx->rec_post, post, s);
     upload(idx->rec_post_col, post_col, s);
     idx->rec_view.dir = idx->rec_dir.as<uint64_t>();
@@ -570,7 +574,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
 }
 
 uint64_t resident_bytes(const zg_index *idx) {
-  uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->rec_post_col.bytes +
+  uint64_t n = idx->rec_dir.bytes + idx->rec_post.bytes + idx->rec_post_col.bytes + idx->rec_filter.bytes +
                idx->inner_table.bytes + idx->stream_data.bytes;
   for (auto &d : idx->dbanks)
     n += d.data.bytes + d.col_color.bytes + d.mg_off.bytes + d.mg_members.bytes +
@@ -593,6 +597,7 @@ zg_index::~zg_index() {
   rec_dir.reset();
   rec_post.reset();
   rec_post_col.reset();
+  rec_filter.reset();
   inner_table.reset();
   if (stream) (void)hipStreamDestroy(stream);
 }

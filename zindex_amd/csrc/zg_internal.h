@@ -123,7 +123,7 @@ struct zg_index {
   int stream_holds = -1;                        // bank in stream_data, -1 none
   void *staging[2] = {nullptr, nullptr};        // page-locked upload chunks
   hipEvent_t staging_ev[2] = {nullptr, nullptr};
-  zg::DevBuf rec_dir, rec_post, rec_post_col, inner_table;
+  zg::DevBuf rec_dir, rec_post, rec_post_col, rec_filter, inner_table;
   zg::SpacedRuns spaced;  // the manifest's seed pattern (n = 0: none)
   // findere: per-seed hit rows. Banks with the same colour -> bit map (the
   // partitions of one size class) share a region of fd_row_words words.

@@ -46,12 +46,14 @@ constexpr uint32_t kStageOut = 512;                  // staged hit bytes per wav
 constexpr uint64_t kSweepHitBytes = 64ULL << 30;     // hit rows of one chunk, at most
 constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
 constexpr uint32_t kXcds = 8;
-// The bit-plane lookup: 512-thread blocks, two per CU, each with a 64 KiB
-// window (H x L rows of one slice of at most 256 bytes), so one block's
-// window fill overlaps the other's lookups.
-constexpr uint32_t kPlaneThreads = 512;
-constexpr uint32_t kPlaneWaves = kPlaneThreads / 64;
-constexpr uint32_t kPlaneWindow = 64 * 1024;
+// The bit-plane lookup at b >= 4 and b < 32: one 1024-thread block per CU
+// with a 128 KiB window (H x L rows of one slice: 512 bytes for H <= 4,
+// 256 for H = 8), so each hit row is written in halves; at b <= 2 and
+// b = 32 (more registers per lane): two 512-thread blocks per CU with 64 KiB
+// windows, one block's window fill overlapping the other's lookups.
+constexpr bool plane_wide(int b) { return b >= 4 && b < 32; }
+constexpr uint32_t plane_threads(int b) { return plane_wide(b) ? 1024 : 512; }
+constexpr uint32_t plane_window(int b) { return plane_wide(b) ? 128 * 1024 : 64 * 1024; }
 
 // The seeds of positions [p0, p1): canonical k-mers (k <= 32), optionally
 // through a spaced seed, keyed by their first PositionHash word; modimizers
@@ -425,6 +427,7 @@ struct StageArgs {
   unsigned long long *folded;  // entries answered from the entry before them
   const uint8_t *planes;  // the bank as bit planes (plane_lookup_kernel), or null
   uint64_t planes_pitch;  // its row pitch
+  uint32_t parts;         // plane_lookup_kernel: entry ranges per (segment, slice)
 };
 
 namespace {
@@ -907,7 +910,8 @@ __device__ __forceinline__ uint32_t rows_xor(const uint32_t (&w)[HMAX], uint32_t
 // 32 (q SG + w) + f. Recovery postings are ORed in afterwards, once per
 // entry (plane_recovery_kernel).
 template <int B, int HT, int NP>
-__global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a) {
+__global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArgs a) {
+  constexpr uint32_t kPlaneWaves = plane_threads(B) / 64;
   constexpr uint32_t SB = NP * 16;              // bytes of a row slice
   constexpr uint32_t SG = SB / (4 * B);         // 32-field groups (hit words) per slice
   constexpr uint32_t PPG = B >= 4 ? B / 4 : 1;  // pieces per group
@@ -916,7 +920,7 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a
   constexpr uint32_t NB = SG / BS;
   constexpr int HMAX = HT ? HT : 8;
   static_assert(SG >= 1 && NB >= 1 && (NP & (NP - 1)) == 0, "slice geometry");
-  __shared__ __attribute__((aligned(16))) uint8_t win[kPlaneWindow];
+  __shared__ __attribute__((aligned(16))) uint8_t win[plane_window(B)];
   __shared__ uint32_t unit_sh;
   const BankDev &bk = a.bank;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -924,7 +928,7 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a
   const uint64_t L = bk.geo.length, count = bk.geo.count;
   const uint32_t x = blockIdx.x % a.xcds;
   const uint64_t s_lo = count * x / a.xcds, s_hi = count * (x + 1) / a.xcds;
-  const uint64_t units = (s_hi - s_lo) * a.n_slices;
+  const uint64_t units = (s_hi - s_lo) * a.parts * a.n_slices;
   const uint64_t seed = a.seed;
   const uint32_t lp = (lane & (NP - 1)) * 16;  // the lane's first piece
   // hit words come out in blocks of BS words, block j of the slice at
@@ -941,15 +945,22 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a
     __syncthreads();
     const uint64_t u = unit_sh;
     if (u >= units) break;
-    const uint64_t s = s_lo + u / a.n_slices;
+    // unit u: slice q of entry range `part` of segment s, the slices and
+    // ranges of one segment on neighbouring units (so the pieces of each
+    // hit row are written close together in time)
     const uint32_t q = (uint32_t)(u % a.n_slices);
-    const uint32_t e0 = a.start[s], e1 = a.start[s + 1];
+    const uint64_t t = u / a.n_slices;
+    const uint32_t part = (uint32_t)(t % a.parts);
+    const uint64_t s = s_lo + t / a.parts;
+    const uint32_t s0 = a.start[s], len = a.start[s + 1] - s0;
+    const uint32_t e0 = s0 + (uint32_t)((uint64_t)len * part / a.parts);
+    const uint32_t e1 = s0 + (uint32_t)((uint64_t)len * (part + 1) / a.parts);
     if (e0 == e1) continue;
     // the window: slice q of rows s*L .. (s+H)*L - 1, straight into LDS
     {
       const uint8_t *src = a.planes + s * L * a.planes_pitch + (uint64_t)q * SB;
       const uint32_t chunks = H * (uint32_t)L * NP;
-      for (uint32_t cb = wave * 64; cb < chunks; cb += kPlaneThreads) {
+      for (uint32_t cb = wave * 64; cb < chunks; cb += plane_threads(B)) {
         const uint32_t c = cb + lane;
         if (c < chunks)
           __builtin_amdgcn_global_load_lds(
@@ -1073,10 +1084,12 @@ __global__ __launch_bounds__(kPlaneThreads) void plane_lookup_kernel(StageArgs a
 // Recovery for the classes looked up as bit planes, once per entry after all
 // of the class's lookups (so it follows every row store): an entry whose key
 // has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
-// bits of its postings' columns of this class ORed into its hit row. The
-// postings of a wave's recovered entries go 64 per pass, lane g finding its
-// entry by binary search over the inclusive ends. Partitions of a class share
-// its columns, so every entry is handled here whichever bank it routes to.
+// bits of its postings' columns of this class ORed into its hit row. Keys
+// whose bit in the directory's hash filter (32 KiB, cache resident) is clear
+// have no postings and skip the probe. The postings of a wave's recovered
+// entries go 64 per pass, lane g finding its entry by binary search over the
+// inclusive ends. Partitions of a class share its columns, so every entry is
+// handled here whichever bank it routes to.
 __global__ __launch_bounds__(256) void plane_recovery_kernel(
     const uint64_t *ent_h, const uint32_t *ent_p, const uint64_t *n_ptr, uint64_t seed, RecDev rec,
     uint32_t colmap, uint32_t groups, uint32_t *hits, uint32_t row_words, uint32_t col_word0) {
@@ -1088,23 +1101,26 @@ __global__ __launch_bounds__(256) void plane_recovery_kernel(
   const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
   for (uint64_t w0 = ((uint64_t)blockIdx.x * 4 + wave) * 64; w0 < n; w0 += stride) {
     const uint64_t e = w0 + lane;
-    uint32_t rec_start = 0, rec_len = 0;
     uint64_t prow = 0;
+    uint32_t rec_start = 0, rec_len = 0;
     if (e < n) {
       const uint64_t key = unsplitmix64(ent_h[e]) - seed;  // mixsplit(key, seed) == h
       prow = (uint64_t)ent_p[e] * row_words;
       const uint32_t ah = abandoned_hash(key, seed);
-      uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
-      while (true) {
-        const uint64_t de = rec.dir[slot];
-        const uint32_t pi = (uint32_t)de;
-        if (pi == kNone) break;
-        if ((uint32_t)(de >> 32) == ah) {
-          rec_len = rec.post[pi];
-          rec_start = pi + 1;
-          break;
+      const uint32_t fb = ah >> (32 - kRecFilterLog2);
+      if ((rec.filter[fb / 32] >> (fb % 32)) & 1u) {
+        uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
+        while (true) {
+          const uint64_t de = rec.dir[slot];
+          const uint32_t pi = (uint32_t)de;
+          if (pi == kNone) break;
+          if ((uint32_t)(de >> 32) == ah) {
+            rec_len = rec.post[pi];
+            rec_start = pi + 1;
+            break;
+          }
+          slot = (slot + 1) & cap_mask;
         }
-        slot = (slot + 1) & cap_mask;
       }
     }
     if (!__ballot(rec_len != 0)) continue;
@@ -1353,15 +1369,22 @@ bool sweep_layout_ok(const zg_index *idx) {
 }
 
 // Row-slice bytes of the bit-plane lookup for a bank of b-bit fields, 0 when
-// it does not apply: banks without merged groups, H in {2, 3, 4, 8}, the window of
-// H x L slices within 64 KiB. Slices are 256 bytes (16 pieces) for H <= 4,
-// 128 bytes for larger H and for b <= 2 (a 1024-column row at b = 1).
+// it does not apply: banks without merged groups, H in {2, 3, 4, 8}, the
+// window of H x L slices within the kernel's LDS window. The largest slice
+// that fits: at b in {4, 8, 16} 512 bytes (32 pieces) for H <= 4 and 256 for
+// H = 8; at b = 32, 256 and 128; at b <= 2, 128 bytes (a 1024-column row at
+// b = 1).
 static uint32_t plane_slice_bytes(const BankDev &bk, uint32_t B) {
   if (bk.first_merged != bk.groups || bk.groups == 0) return 0;
   if (bk.hashes != 2 && bk.hashes != 3 && bk.hashes != 4 && bk.hashes != 8) return 0;
-  if (B != 1 && B != 2 && B != 4 && B != 8 && B != 16 && B != 32) return 0;
-  const uint32_t sb = (bk.hashes <= 4 && B >= 4) ? 256 : 128;
-  if ((uint64_t)bk.hashes * bk.geo.length * sb > kPlaneWindow) return 0;
+  uint32_t window = 0, sb = 0;
+  switch (B) {
+    case 1: case 2: window = plane_window(1); sb = 128; break;
+    case 4: case 8: case 16: window = plane_window(8); sb = bk.hashes <= 4 ? 512 : 256; break;
+    case 32: window = plane_window(32); sb = bk.hashes <= 4 ? 256 : 128; break;
+    default: return 0;
+  }
+  if ((uint64_t)bk.hashes * bk.geo.length * sb > window) return 0;
   return sb;
 }
 
@@ -1404,27 +1427,20 @@ static void build_planes(DeviceBank &db, uint32_t B, uint32_t sb, uint32_t n_sli
 }
 
 template <int B>
-static void launch_planes(const StageArgs &a, uint32_t blocks, hipStream_t s) {
-  // 256-byte slices at b >= 4 and H <= 4; 128-byte slices otherwise
-  if constexpr (B >= 4) {
-    ensure(a.slice_bytes == (a.bank.hashes <= 4 ? 256u : 128u), "bit-plane lookup: slice size",
-           ZG_ERR_INVALID);
-    switch (a.bank.hashes) {
-      case 2: plane_lookup_kernel<B, 2, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 3: plane_lookup_kernel<B, 3, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 4: plane_lookup_kernel<B, 4, 16><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 8: plane_lookup_kernel<B, 8, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      default: fail(ZG_ERR_INVALID, "bit-plane lookup: hash count");
-    }
-  } else {
-    ensure(a.slice_bytes == 128, "bit-plane lookup: slice size", ZG_ERR_INVALID);
-    switch (a.bank.hashes) {
-      case 2: plane_lookup_kernel<B, 2, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 3: plane_lookup_kernel<B, 3, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 4: plane_lookup_kernel<B, 4, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      case 8: plane_lookup_kernel<B, 8, 8><<<blocks, kPlaneThreads, 0, s>>>(a); break;
-      default: fail(ZG_ERR_INVALID, "bit-plane lookup: hash count");
-    }
+static void launch_planes(const StageArgs &a, uint32_t cus, hipStream_t s) {
+  // one 1024-thread block per CU (wide kernels) or two 512-thread blocks
+  const uint32_t blocks = std::max<uint32_t>(a.xcds, plane_wide(B) ? cus : 2 * cus);
+  constexpr uint32_t T = plane_threads(B);
+  constexpr int NP4 = plane_wide(B) ? 32 : (B == 32 ? 16 : 8);  // pieces at H <= 4
+  constexpr int NP8 = B <= 2 ? 8 : NP4 / 2;                      // pieces at H = 8
+  ensure(a.slice_bytes == (a.bank.hashes <= 4 ? NP4 : NP8) * 16u, "bit-plane lookup: slice size",
+         ZG_ERR_INVALID);
+  switch (a.bank.hashes) {
+    case 2: plane_lookup_kernel<B, 2, NP4><<<blocks, T, 0, s>>>(a); break;
+    case 3: plane_lookup_kernel<B, 3, NP4><<<blocks, T, 0, s>>>(a); break;
+    case 4: plane_lookup_kernel<B, 4, NP4><<<blocks, T, 0, s>>>(a); break;
+    case 8: plane_lookup_kernel<B, 8, NP8><<<blocks, T, 0, s>>>(a); break;
+    default: fail(ZG_ERR_INVALID, "bit-plane lookup: hash count");
   }
 }
 
@@ -1670,7 +1686,11 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   sa.xcds = idx->cu_count >= kXcds ? kXcds : 1;
   sa.folded = folded;
   const uint32_t stage_blocks = std::max<uint32_t>(sa.xcds, idx->cu_count);
-  const uint32_t plane_blocks = std::max<uint32_t>(sa.xcds, 2 * idx->cu_count);
+  // entry ranges per unit of the bit-plane lookup: fewer segments in flight
+  // per XCD, so more hit rows are whole in L2 before they are written back
+  uint32_t plane_parts = 2;
+  if (const char *e = getenv("ZG_SWEEP_PARTS"))
+    plane_parts = (uint32_t)std::min<unsigned long>(64, std::max<unsigned long>(1, strtoul(e, nullptr, 10)));
   // sum: lanes per item, the row's words rounded up to a power of two (at most 64)
   uint32_t lp_bits = 0;
   while ((1u << lp_bits) < std::min<uint32_t>(64, row_words)) lp_bits++;
@@ -1709,13 +1729,14 @@ void sweep_run(zg_index *idx, zg_batch *b) {
         if (c.planes) {
           sa.planes = idx->dbanks[bi].planes.as<uint8_t>();
           sa.planes_pitch = idx->dbanks[bi].planes_pitch;
+          sa.parts = plane_parts;
           switch (B) {
-            case 1: launch_planes<1>(sa, plane_blocks, s); break;
-            case 2: launch_planes<2>(sa, plane_blocks, s); break;
-            case 4: launch_planes<4>(sa, plane_blocks, s); break;
-            case 8: launch_planes<8>(sa, plane_blocks, s); break;
-            case 16: launch_planes<16>(sa, plane_blocks, s); break;
-            default: launch_planes<32>(sa, plane_blocks, s); break;
+            case 1: launch_planes<1>(sa, idx->cu_count, s); break;
+            case 2: launch_planes<2>(sa, idx->cu_count, s); break;
+            case 4: launch_planes<4>(sa, idx->cu_count, s); break;
+            case 8: launch_planes<8>(sa, idx->cu_count, s); break;
+            case 16: launch_planes<16>(sa, idx->cu_count, s); break;
+            default: launch_planes<32>(sa, idx->cu_count, s); break;
           }
           continue;
         }
