@@ -677,8 +677,9 @@ def main() -> int:
     totals, counts, agg = batch.results()
     log(f"{seeds} k-mers per step, {ms_per_step:.1f} ms/step, lookup {np.mean(lookup_ms):.1f} ms")
     if st["tuning"]["sweep"]:
-        log(f"sweep phases (last step): sort {st['sweep_sort_ms']:.1f} ms, staged lookup "
-            f"{st['sweep_lookup_ms']:.1f} ms, sum {st['sweep_sum_ms']:.1f} ms, "
+        log(f"sweep phases (last step): sort {st['sweep_sort_ms']:.1f} ms, lookup "
+            f"{st['sweep_lookup_ms']:.1f} ms, recovery {st['sweep_recovery_ms']:.1f} ms, "
+            f"sum {st['sweep_sum_ms']:.1f} ms, "
             f"{st['sweep_launches']} chunks, {st['sweep_folded']} folded; seeding "
             f"{st['last_seed_ms']:.1f} ms")
 
@@ -740,7 +741,8 @@ def main() -> int:
             "folded_per_step": st["sweep_folded"],
             "phases_ms": {"seeding": round(st["last_seed_ms"], 2),
                           "radix_sort": round(st["sweep_sort_ms"], 2),
-                          "staged_lookup": round(st["sweep_lookup_ms"], 2),
+                          "lookup": round(st["sweep_lookup_ms"], 2),
+                          "recovery": round(st["sweep_recovery_ms"], 2),
                           "sum": round(st["sweep_sum_ms"], 2)},
             "sum_kernel_gbs": round(sum_bytes / (st["sweep_sum_ms"] / 1000) / 1e9, 1)
             if st["sweep_sum_ms"] > 0 else None})

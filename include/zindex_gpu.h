@@ -311,6 +311,10 @@ typedef struct zg_batch_stats {
   uint64_t sweep_plane_banks;       /* banks its lookups read as bit planes (a
                                        copy of the bank made on first use: per
                                        32 fields, b words of one bit each) */
+  double sweep_recovery_ms;         /* HIP-event time of its recovery passes
+                                       (bit-plane classes; their postings
+                                       ORed into the hit rows after the
+                                       lookups); not in sweep_lookup_ms */
 } zg_batch_stats;
 
 /* Uploads reads (ASCII) to HBM, packs them to 2 bits plus an invalid-base

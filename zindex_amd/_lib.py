@@ -73,6 +73,7 @@ class BatchStats(C.Structure):
         ("sweep_sort_ms", C.c_double), ("sweep_sum_ms", C.c_double),
         ("sweep_stage_launches", C.c_uint64), ("sweep_algorithmic_bytes", C.c_uint64),
         ("sweep_hit_row_bytes", C.c_uint64), ("sweep_plane_banks", C.c_uint64),
+        ("sweep_recovery_ms", C.c_double),
     ]
 
 

@@ -1328,7 +1328,7 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
         check_hip(hipEventRecord(b->ev[3], s), "hipEventRecord");
         check_hip(hipStreamSynchronize(s), "lookup");
         b->last_total_ms = b->last_lookup_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
-        b->sweep_sort_ms = b->sweep_lookup_ms = b->sweep_sum_ms = -1;
+        b->sweep_sort_ms = b->sweep_lookup_ms = b->sweep_sum_ms = b->sweep_recovery_ms = -1;
         b->sweep_folded = zg::sweep_folded(b);
         return 0;
       }
@@ -1342,7 +1342,8 @@ extern "C" int zg_batch_run(zg_index *idx, zg_batch *b) {
       check_hip(hipStreamSynchronize(s), "lookup");
       b->last_lookup_ms = zg::elapsed_ms(b->ev[1], b->ev[2]);
       b->last_total_ms = zg::elapsed_ms(b->ev[0], b->ev[3]);
-      zg::sweep_times(b, b->sweep_sort_ms, b->sweep_lookup_ms, b->sweep_sum_ms);
+      zg::sweep_times(b, b->sweep_sort_ms, b->sweep_lookup_ms, b->sweep_recovery_ms,
+                      b->sweep_sum_ms);
       b->sweep_folded = zg::sweep_folded(b);
       if (idx->tuning.graph && !b->sw_graph_failed) zg::capture_sweep(idx, b);
       return 0;
@@ -1516,6 +1517,7 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
     o->sweep_lookup_ms = o->tuning.sweep ? b->sweep_lookup_ms : 0;
     o->sweep_sort_ms = o->tuning.sweep ? b->sweep_sort_ms : 0;
     o->sweep_sum_ms = o->tuning.sweep ? b->sweep_sum_ms : 0;
+    o->sweep_recovery_ms = o->tuning.sweep ? b->sweep_recovery_ms : 0;
     if (o->tuning.sweep) {
       uint64_t bank_bytes = 0;
       for (auto &bk : idx->m.banks) bank_bytes += device_bytes(bk);

@@ -170,6 +170,7 @@ struct zg_batch {
   uint64_t sweep_chunks = 0;      // chunks of the last sweep
   uint32_t sweep_classes = 0;     // its size classes (a sort and lookups each per chunk)
   double sweep_lookup_ms = 0, sweep_sort_ms = 0, sweep_sum_ms = 0;  // their phase times
+  double sweep_recovery_ms = 0;
   uint64_t sweep_folded = 0;      // entries answered from the entry before them
   // the sweep as one graph (tuning.graph): captured after an eager run, with
   // that run's chunk size
@@ -222,7 +223,8 @@ const std::vector<uint8_t> &host_payload(zg_index *idx, size_t bank);
 // times and fold count of the last run (after the stream has drained).
 bool sweep_layout_ok(const zg_index *idx);
 void sweep_run(zg_index *idx, zg_batch *b);
-void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &sum_ms);
+void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &recovery_ms,
+                 double &sum_ms);
 uint64_t sweep_folded(zg_batch *b);
 // Batch pipeline pieces, all enqueued on b->stream (zg_api.hip).
 void configure(zg_index *idx, zg_batch *b);

@@ -45,11 +45,11 @@ struct RecDev {
   const uint32_t *post_col;
   uint32_t cap_log2;          // directory capacity 2^cap_log2
   uint32_t enabled;
-  const uint32_t *filter;     // kRecFilterBits bits: bit (hash >> (32 - 18)) set for
+  const uint32_t *filter;     // 2^kRecFilterLog2 bits: bit (hash >> (32 - 20)) set for
                               // every directory hash (a key whose bit is clear has
                               // no postings)
 };
-constexpr uint32_t kRecFilterLog2 = 18;
+constexpr uint32_t kRecFilterLog2 = 20;
 constexpr uint32_t kRecFilterWords = (1u << kRecFilterLog2) / 32;
 
 enum SeedSource : uint32_t { kSeedKmers = 0, kSeedKeys = 1 };

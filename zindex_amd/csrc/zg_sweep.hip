@@ -43,7 +43,7 @@ constexpr uint32_t kStageWaves = kStageThreads / 64;
 constexpr uint32_t kStageWindow = 128 * 1024;        // LDS rows of one segment window
 constexpr uint32_t kSliceBytes = 512;                // plane bytes per slice (8 per lane)
 constexpr uint32_t kStageOut = 512;                  // staged hit bytes per wave
-constexpr uint64_t kSweepHitBytes = 64ULL << 30;     // hit rows of one chunk, at most
+constexpr uint64_t kSweepHitBytes = 128ULL << 30;    // hit rows of one chunk, at most
 constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
 constexpr uint32_t kXcds = 8;
 // The bit-plane lookup at b >= 4 and b < 32: one 1024-thread block per CU
@@ -1085,30 +1085,45 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
 // of the class's lookups (so it follows every row store): an entry whose key
 // has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
 // bits of its postings' columns of this class ORed into its hit row. Keys
-// whose bit in the directory's hash filter (32 KiB, cache resident) is clear
-// have no postings and skip the probe. The postings of a wave's recovered
-// entries go 64 per pass, lane g finding its entry by binary search over the
-// inclusive ends. Partitions of a class share its columns, so every entry is
-// handled here whichever bank it routes to.
-__global__ __launch_bounds__(256) void plane_recovery_kernel(
+// whose bit in the directory's hash filter (128 KiB, staged in LDS by each
+// persistent block) is clear have no postings and skip the probe. Each lane
+// takes J entries per batch, their loads in flight together. The postings of
+// a wave's recovered entries go 64 per pass, lane g finding its entry by
+// binary search over the inclusive ends. Partitions of a class share its
+// columns, so every entry is handled here whichever bank it routes to.
+constexpr uint32_t kRecThreads = 1024;
+__global__ __launch_bounds__(kRecThreads) void plane_recovery_kernel(
     const uint64_t *ent_h, const uint32_t *ent_p, const uint64_t *n_ptr, uint64_t seed, RecDev rec,
     uint32_t colmap, uint32_t groups, uint32_t *hits, uint32_t row_words, uint32_t col_word0) {
-  __shared__ uint32_t ends_all[4 * 64];
+  constexpr int J = 4;
+  constexpr uint32_t W = kRecThreads / 64;
+  __shared__ uint32_t filter[kRecFilterWords];
+  __shared__ uint32_t ends_all[W * 64];
+  for (uint32_t i = threadIdx.x; i < kRecFilterWords; i += kRecThreads) filter[i] = rec.filter[i];
+  __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t *ends = ends_all + wave * 64;
   const uint64_t n = *n_ptr;
-  const uint64_t stride = (uint64_t)gridDim.x * 4 * 64;
+  const uint64_t stride = (uint64_t)gridDim.x * W * 64 * J;
   const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
-  for (uint64_t w0 = ((uint64_t)blockIdx.x * 4 + wave) * 64; w0 < n; w0 += stride) {
-    const uint64_t e = w0 + lane;
-    uint64_t prow = 0;
-    uint32_t rec_start = 0, rec_len = 0;
-    if (e < n) {
-      const uint64_t key = unsplitmix64(ent_h[e]) - seed;  // mixsplit(key, seed) == h
-      prow = (uint64_t)ent_p[e] * row_words;
-      const uint32_t ah = abandoned_hash(key, seed);
+  for (uint64_t w0 = ((uint64_t)blockIdx.x * W + wave) * 64 * J; w0 < n; w0 += stride) {
+    uint64_t hj[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const uint64_t e = w0 + (uint64_t)j * 64 + lane;
+      hj[j] = e < n ? ent_h[e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+      const uint64_t e = w0 + (uint64_t)j * 64 + lane;
+      const uint32_t ah = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
       const uint32_t fb = ah >> (32 - kRecFilterLog2);
-      if ((rec.filter[fb / 32] >> (fb % 32)) & 1u) {
+      const bool cand = e < n && ((filter[fb / 32] >> (fb % 32)) & 1u);
+      if (!__ballot(cand)) continue;
+      uint32_t rec_start = 0, rec_len = 0;
+      uint64_t prow = 0;
+      if (cand) {
+        prow = (uint64_t)ent_p[e] * row_words;
         uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
         while (true) {
           const uint64_t de = rec.dir[slot];
@@ -1122,33 +1137,33 @@ __global__ __launch_bounds__(256) void plane_recovery_kernel(
           slot = (slot + 1) & cap_mask;
         }
       }
-    }
-    if (!__ballot(rec_len != 0)) continue;
-    const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
-    const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
-    ends[lane] = end;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t g0 = 0; g0 < total; g0 += 64) {
-      const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
-      uint32_t j = 0;  // the first lane whose postings end after g
+      if (!__ballot(rec_len != 0)) continue;
+      const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
+      const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
+      ends[lane] = end;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+        const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
+        uint32_t k = 0;  // the first lane whose postings end after g
 #pragma unroll
-      for (uint32_t st = 32; st; st >>= 1)
-        if (ends[j + st - 1] <= g) j += st;
-      const uint32_t first = j ? ends[j - 1] : 0u;
-      const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)j, 64);
-      const uint64_t pj = shfl64(prow, j);
-      if (g0 + lane < total) {
-        const uint32_t pe = rec.post_col[rs + (g - first)];
-        const uint32_t col = pe & 0xFFFFFFu;
-        if (pe != kNone && (pe >> 24) == colmap && col < groups)
-          atomicOr(hits + pj + col_word0 + col / 32, 1u << (col % 32));
+        for (uint32_t st = 32; st; st >>= 1)
+          if (ends[k + st - 1] <= g) k += st;
+        const uint32_t first = k ? ends[k - 1] : 0u;
+        const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)k, 64);
+        const uint64_t pk = shfl64(prow, k);
+        if (g0 + lane < total) {
+          const uint32_t pe = rec.post_col[rs + (g - first)];
+          const uint32_t col = pe & 0xFFFFFFu;
+          if (pe != kNone && (pe >> 24) == colmap && col < groups)
+            atomicOr(hits + pk + col_word0 + col / 32, 1u << (col % 32));
+        }
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -1617,7 +1632,10 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   // chunk size: hit rows of at most 64 GiB, entries below 2^32, and the
   // chunk's buffers (hit row + 24 B of entries per item) within 3/4 of the
   // device memory free or already held by them
-  uint64_t cap = std::min<uint64_t>((1ULL << 32) - 64, kSweepHitBytes / (row_words * 4ULL));
+  uint64_t hit_budget = kSweepHitBytes;
+  if (const char *e = getenv("ZG_SWEEP_HIT_GIB"))  // hit rows of one chunk, GiB
+    hit_budget = std::max<uint64_t>(1, strtoull(e, nullptr, 10)) << 30;
+  uint64_t cap = std::min<uint64_t>((1ULL << 32) - 64, hit_budget / (row_words * 4ULL));
   size_t free_b = 0, total_b = 0;
   if (b->sw_capturing) {
     cap = b->sw_cap;  // the eager run's chunks: every buffer already holds them
@@ -1667,7 +1685,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   b->sw_queue.reserve(32 * kXcds * 4 + 64);
   b->fd_hits.reserve(std::max<uint64_t>(1, most * row_words) * 4);
   const uint32_t n_chunks = (uint32_t)cuts.size() - 1;
-  const size_t ev_per_chunk = 2 + 2 * idx->sw_classes.size();
+  const size_t ev_per_chunk = 2 + 3 * idx->sw_classes.size();
   while (b->sw_ev.size() < ev_per_chunk * n_chunks) {
     hipEvent_t e;
     check_hip(hipEventCreate(&e), "hipEventCreate");
@@ -1716,7 +1734,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
         cur = sort_class(b, ps, c.count, n_items, k == 0 ? b->sw_listed.as<uint64_t>() : nullptr,
                          &n_dev, s);
       }
-      check_hip(hipEventRecord(ev[1 + 2 * k], s), "hipEventRecord");
+      check_hip(hipEventRecord(ev[1 + 3 * k], s), "hipEventRecord");
       sa.ent_h = b->sw_h[cur].as<uint64_t>();
       sa.ent_p = b->sw_p[cur].as<uint32_t>();
       sa.slice_bytes = c.slice_bytes;
@@ -1750,13 +1768,15 @@ void sweep_run(zg_index *idx, zg_batch *b) {
           default: launch_stage<32>(sa, stage_blocks, s); break;
         }
       }
+      check_hip(hipEventRecord(ev[2 + 3 * k], s), "hipEventRecord");
       if (c.planes && sa.rec.enabled) {
         const BankDev &bk0 = idx->dbanks[c.banks[0]].view;
-        plane_recovery_kernel<<<(uint32_t)std::min<uint64_t>(16384, n_items / 256 + 1), 256, 0, s>>>(
+        plane_recovery_kernel<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(idx->cu_count, n_items / 4096 + 1)),
+                                kRecThreads, 0, s>>>(
             sa.ent_h, sa.ent_p, n_dev, sa.seed, sa.rec, bk0.colmap, bk0.groups, sa.hits, row_words,
             c.col_word0);
       }
-      check_hip(hipEventRecord(ev[2 + 2 * k], s), "hipEventRecord");
+      check_hip(hipEventRecord(ev[3 + 3 * k], s), "hipEventRecord");
     }
     const uint64_t groups = (row_words + 63) / 64;
     const uint64_t blocks = groups * (((n_items + kSumSpan - 1) / kSumSpan + 3) / 4);
@@ -1775,15 +1795,19 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   if (strobes) launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * m.colors.size(), s);
 }
 
-// Phase times of the last sweep (its events have completed).
-void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &sum_ms) {
-  sort_ms = lookup_ms = sum_ms = 0;
-  const size_t per = 2 + 2 * (size_t)b->sweep_classes;
+// Phase times of the last sweep (its events have completed). Per chunk:
+// start, then per class the ends of its sort, its lookups and its recovery
+// pass, then the end of the sums.
+void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &recovery_ms,
+                 double &sum_ms) {
+  sort_ms = lookup_ms = recovery_ms = sum_ms = 0;
+  const size_t per = 2 + 3 * (size_t)b->sweep_classes;
   for (uint64_t ci = 0; ci < b->sweep_chunks; ci++) {
     hipEvent_t *ev = &b->sw_ev[per * ci];
     for (uint32_t k = 0; k < b->sweep_classes; k++) {
-      sort_ms += elapsed_ms(ev[2 * k], ev[1 + 2 * k]);
-      lookup_ms += elapsed_ms(ev[1 + 2 * k], ev[2 + 2 * k]);
+      sort_ms += elapsed_ms(ev[3 * k], ev[1 + 3 * k]);
+      lookup_ms += elapsed_ms(ev[1 + 3 * k], ev[2 + 3 * k]);
+      recovery_ms += elapsed_ms(ev[2 + 3 * k], ev[3 + 3 * k]);
     }
     sum_ms += elapsed_ms(ev[per - 2], ev[per - 1]);
   }
