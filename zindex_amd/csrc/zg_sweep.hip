@@ -30,10 +30,11 @@ namespace zg {
 
 namespace {
 
-constexpr uint32_t kSortThreads = 256;               // 4 waves
-constexpr uint32_t kSortRounds = 16;                 // rounds of 64 entries per wave
+constexpr uint32_t kSortThreads = 512;               // 8 waves
+constexpr uint32_t kSortWaves = kSortThreads / 64;
+constexpr uint32_t kSortRounds = 8;                  // rounds of 64 entries per wave
 constexpr uint64_t kSortWaveItems = 64 * kSortRounds;
-constexpr uint64_t kSortTile = 4 * kSortWaveItems;   // 4096 entries per tile (48 KiB in LDS)
+constexpr uint64_t kSortTile = kSortWaves * kSortWaveItems;  // 4096 entries per tile (48 KiB in LDS)
 // Digit bits per radix pass (64 buckets: a tile's digit run averages 64
 // entries, whole cache lines of the output)
 constexpr uint32_t kMaxDigitBits = 6;
@@ -109,7 +110,7 @@ __device__ __forceinline__ void resolve(PosSource &) {}
 __device__ __forceinline__ void resolve(KeySource &) {}
 __device__ __forceinline__ void resolve(EntSource &s) { s.n = *s.n_ptr; }
 
-// Tile t, wave w, round r, lane l holds item t*4096 + w*1024 + r*64 + l:
+// Tile t, wave w, round r, lane l holds item t*4096 + w*512 + r*64 + l:
 // the tile order is the input order, which keeps the sort stable.
 __device__ __forceinline__ uint64_t sort_item(uint64_t tile, uint32_t wave, uint32_t r,
                                               uint32_t lane) {
@@ -117,7 +118,7 @@ __device__ __forceinline__ uint64_t sort_item(uint64_t tile, uint32_t wave, uint
 }
 
 template <class Src>
-__global__ __launch_bounds__(256) void radix_hist_kernel(Src src, uint64_t count, uint32_t shift,
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(Src src, uint64_t count, uint32_t shift,
                                                          uint32_t dbits, uint64_t n_tiles,
                                                          uint32_t *hist, uint64_t *listed) {
   __shared__ uint32_t bins[kMaxBins];
@@ -148,11 +149,11 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(Src src, uint64_t count
 // neighbouring threads write neighbouring entries (whole cache lines) after
 // the same digit's entries of the earlier tiles.
 template <class Src>
-__global__ __launch_bounds__(256) void radix_scatter_kernel(Src src, uint64_t count, uint32_t shift,
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(Src src, uint64_t count, uint32_t shift,
                                                             uint32_t dbits, uint64_t n_tiles,
                                                             const uint64_t *off, uint64_t *out_h,
                                                             uint32_t *out_p) {
-  __shared__ uint32_t wbase[4][kMaxBins];  // per wave: digit counts, then next LDS slot
+  __shared__ uint32_t wbase[kSortWaves][kMaxBins];  // per wave: digit counts, then next LDS slot
   __shared__ uint32_t dstart[kMaxBins + 1];  // first LDS slot of each digit
   __shared__ uint64_t gbase[kMaxBins];       // global index of LDS slot 0 of each digit
   __shared__ uint64_t th[kSortTile];
@@ -179,13 +180,13 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(Src src, uint64_t co
     const uint32_t d = lane;
     uint32_t tot = 0;
     if (d < nb)
-      for (int w = 0; w < 4; w++) tot += wbase[w][d];
+      for (uint32_t w = 0; w < kSortWaves; w++) tot += wbase[w][d];
     const uint64_t excl = wave_inclusive_scan(tot, lane) - tot;
     if (d < nb) {
       dstart[d] = (uint32_t)excl;
       gbase[d] = off[(uint64_t)d * n_tiles + tile] - excl;
       uint32_t run = (uint32_t)excl;
-      for (int w = 0; w < 4; w++) {
+      for (uint32_t w = 0; w < kSortWaves; w++) {
         const uint32_t c = wbase[w][d];
         wbase[w][d] = run;
         run += c;
@@ -1107,37 +1108,43 @@ __global__ __launch_bounds__(kRecThreads) void plane_recovery_kernel(
   const uint64_t stride = (uint64_t)gridDim.x * W * 64 * J;
   const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
   for (uint64_t w0 = ((uint64_t)blockIdx.x * W + wave) * 64 * J; w0 < n; w0 += stride) {
+    // the J entries' first hashes, then their keys' filter bits, then the
+    // first directory slot of every candidate: each step's loads in flight
+    // together
     uint64_t hj[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
       const uint64_t e = w0 + (uint64_t)j * 64 + lane;
       hj[j] = e < n ? ent_h[e] : 0;
     }
+    uint32_t ah[J];
+    uint64_t slot[J], de[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
       const uint64_t e = w0 + (uint64_t)j * 64 + lane;
-      const uint32_t ah = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
-      const uint32_t fb = ah >> (32 - kRecFilterLog2);
+      ah[j] = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
+      const uint32_t fb = ah[j] >> (32 - kRecFilterLog2);
       const bool cand = e < n && ((filter[fb / 32] >> (fb % 32)) & 1u);
-      if (!__ballot(cand)) continue;
+      slot[j] = rec.cap_log2 ? (uint64_t)(ah[j] >> (32 - rec.cap_log2)) : 0;
+      de[j] = cand ? rec.dir[slot[j]] : (uint64_t)kNone;
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
       uint32_t rec_start = 0, rec_len = 0;
-      uint64_t prow = 0;
-      if (cand) {
-        prow = (uint64_t)ent_p[e] * row_words;
-        uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
-        while (true) {
-          const uint64_t de = rec.dir[slot];
-          const uint32_t pi = (uint32_t)de;
-          if (pi == kNone) break;
-          if ((uint32_t)(de >> 32) == ah) {
-            rec_len = rec.post[pi];
-            rec_start = pi + 1;
-            break;
-          }
-          slot = (slot + 1) & cap_mask;
+      while (true) {  // open addressing: on to the next slot until the hash or an empty one
+        const uint32_t pi = (uint32_t)de[j];
+        if (pi == kNone) break;
+        if ((uint32_t)(de[j] >> 32) == ah[j]) {
+          rec_len = rec.post[pi];
+          rec_start = pi + 1;
+          break;
         }
+        slot[j] = (slot[j] + 1) & cap_mask;
+        de[j] = rec.dir[slot[j]];
       }
       if (!__ballot(rec_len != 0)) continue;
+      const uint64_t e = w0 + (uint64_t)j * 64 + lane;
+      const uint64_t prow = rec_len ? (uint64_t)ent_p[e] * row_words : 0;
       const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
       const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
       ends[lane] = end;
