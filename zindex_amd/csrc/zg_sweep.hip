@@ -1087,90 +1087,129 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
 // has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
 // bits of its postings' columns of this class ORed into its hit row. Keys
 // whose bit in the directory's hash filter (128 KiB, staged in LDS by each
-// persistent block) is clear have no postings and skip the probe. Each lane
-// takes J entries per batch, their loads in flight together. The postings of
-// a wave's recovered entries go 64 per pass, lane g finding its entry by
-// binary search over the inclusive ends. Partitions of a class share its
+// persistent block) is clear have no postings. The others queue up per wave
+// in LDS and are probed 64 at a time, one per lane, so a wave waits on the
+// directory once per 64 candidates rather than once per 64 entries. The
+// postings of the recovered entries go 64 per pass, lane g finding its entry
+// by binary search over the inclusive ends. Partitions of a class share its
 // columns, so every entry is handled here whichever bank it routes to.
 constexpr uint32_t kRecThreads = 1024;
+constexpr uint32_t kRecQueue = 128;  // candidates queued per wave, at most
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Probes the directory for the first cnt (<= 64) queued candidates, one per
+// lane, and ORs the postings of those that have some into their hit rows.
+__device__ void recover_queued(const uint32_t *q_e, const uint32_t *q_ah, uint32_t cnt,
+                               uint32_t *ends, const uint32_t *ent_p, const RecDev &rec,
+                               uint32_t colmap, uint32_t groups, uint32_t *hits, uint32_t row_words,
+                               uint32_t col_word0, uint32_t lane) {
+  const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
+  uint32_t rec_start = 0, rec_len = 0, e = 0;
+  if (lane < cnt) {
+    e = q_e[lane];
+    const uint32_t ah = q_ah[lane];
+    uint64_t slot = rec.cap_log2 ? (uint64_t)(ah >> (32 - rec.cap_log2)) : 0;
+    while (true) {  // open addressing: on to the next slot until the hash or an empty one
+      const uint64_t de = rec.dir[slot];
+      const uint32_t pi = (uint32_t)de;
+      if (pi == kNone) break;
+      if ((uint32_t)(de >> 32) == ah) {
+        rec_len = rec.post[pi];
+        rec_start = pi + 1;
+        break;
+      }
+      slot = (slot + 1) & cap_mask;
+    }
+  }
+  if (!__ballot(rec_len != 0)) return;
+  const uint64_t prow = rec_len ? (uint64_t)ent_p[e] * row_words : 0;
+  const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
+  const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
+  ends[lane] = end;
+  wave_lds_sync();
+  for (uint32_t g0 = 0; g0 < total; g0 += 64) {
+    const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
+    uint32_t k = 0;  // the first lane whose postings end after g
+#pragma unroll
+    for (uint32_t st = 32; st; st >>= 1)
+      if (ends[k + st - 1] <= g) k += st;
+    const uint32_t first = k ? ends[k - 1] : 0u;
+    const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)k, 64);
+    const uint64_t pk = shfl64(prow, k);
+    if (g0 + lane < total) {
+      const uint32_t pe = rec.post_col[rs + (g - first)];
+      const uint32_t col = pe & 0xFFFFFFu;
+      if (pe != kNone && (pe >> 24) == colmap && col < groups)
+        atomicOr(hits + pk + col_word0 + col / 32, 1u << (col % 32));
+    }
+  }
+  wave_lds_sync();
+}
+
 __global__ __launch_bounds__(kRecThreads) void plane_recovery_kernel(
     const uint64_t *ent_h, const uint32_t *ent_p, const uint64_t *n_ptr, uint64_t seed, RecDev rec,
     uint32_t colmap, uint32_t groups, uint32_t *hits, uint32_t row_words, uint32_t col_word0) {
-  constexpr int J = 4;
+  constexpr int J = 4;  // batches of 64 entries per wave and step, their loads in flight together
   constexpr uint32_t W = kRecThreads / 64;
   __shared__ uint32_t filter[kRecFilterWords];
   __shared__ uint32_t ends_all[W * 64];
+  __shared__ uint32_t q_e_all[W * kRecQueue], q_ah_all[W * kRecQueue];
   for (uint32_t i = threadIdx.x; i < kRecFilterWords; i += kRecThreads) filter[i] = rec.filter[i];
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t *ends = ends_all + wave * 64;
-  const uint64_t n = *n_ptr;
+  uint32_t *q_e = q_e_all + wave * kRecQueue, *q_ah = q_ah_all + wave * kRecQueue;
+  const uint64_t n = *n_ptr;  // below 2^32: a chunk's entries
   const uint64_t stride = (uint64_t)gridDim.x * W * 64 * J;
-  const uint64_t cap_mask = (1ULL << rec.cap_log2) - 1;
+  const uint64_t below = lanemask_lt(lane);
+  uint32_t qn = 0;  // queued candidates (wave-uniform)
   for (uint64_t w0 = ((uint64_t)blockIdx.x * W + wave) * 64 * J; w0 < n; w0 += stride) {
-    // the J entries' first hashes, then their keys' filter bits, then the
-    // first directory slot of every candidate: each step's loads in flight
-    // together
     uint64_t hj[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
       const uint64_t e = w0 + (uint64_t)j * 64 + lane;
       hj[j] = e < n ? ent_h[e] : 0;
     }
-    uint32_t ah[J];
-    uint64_t slot[J], de[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
       const uint64_t e = w0 + (uint64_t)j * 64 + lane;
-      ah[j] = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
-      const uint32_t fb = ah[j] >> (32 - kRecFilterLog2);
+      const uint32_t ah = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
+      const uint32_t fb = ah >> (32 - kRecFilterLog2);
       const bool cand = e < n && ((filter[fb / 32] >> (fb % 32)) & 1u);
-      slot[j] = rec.cap_log2 ? (uint64_t)(ah[j] >> (32 - rec.cap_log2)) : 0;
-      de[j] = cand ? rec.dir[slot[j]] : (uint64_t)kNone;
-    }
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-      uint32_t rec_start = 0, rec_len = 0;
-      while (true) {  // open addressing: on to the next slot until the hash or an empty one
-        const uint32_t pi = (uint32_t)de[j];
-        if (pi == kNone) break;
-        if ((uint32_t)(de[j] >> 32) == ah[j]) {
-          rec_len = rec.post[pi];
-          rec_start = pi + 1;
-          break;
-        }
-        slot[j] = (slot[j] + 1) & cap_mask;
-        de[j] = rec.dir[slot[j]];
+      const uint64_t m = __ballot(cand);
+      if (cand) {
+        const uint32_t at = qn + (uint32_t)__popcll(m & below);
+        q_e[at] = (uint32_t)e;
+        q_ah[at] = ah;
       }
-      if (!__ballot(rec_len != 0)) continue;
-      const uint64_t e = w0 + (uint64_t)j * 64 + lane;
-      const uint64_t prow = rec_len ? (uint64_t)ent_p[e] * row_words : 0;
-      const uint32_t end = (uint32_t)wave_inclusive_scan(rec_len, lane);
-      const uint32_t total = (uint32_t)__shfl((int)end, 63, 64);
-      ends[lane] = end;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (uint32_t g0 = 0; g0 < total; g0 += 64) {
-        const uint32_t g = g0 + lane < total ? g0 + lane : total - 1;
-        uint32_t k = 0;  // the first lane whose postings end after g
-#pragma unroll
-        for (uint32_t st = 32; st; st >>= 1)
-          if (ends[k + st - 1] <= g) k += st;
-        const uint32_t first = k ? ends[k - 1] : 0u;
-        const uint32_t rs = (uint32_t)__shfl((int)rec_start, (int)k, 64);
-        const uint64_t pk = shfl64(prow, k);
-        if (g0 + lane < total) {
-          const uint32_t pe = rec.post_col[rs + (g - first)];
-          const uint32_t col = pe & 0xFFFFFFu;
-          if (pe != kNone && (pe >> 24) == colmap && col < groups)
-            atomicOr(hits + pk + col_word0 + col / 32, 1u << (col % 32));
+      qn += (uint32_t)__popcll(m);  // < kRecQueue: at most 63 were queued before
+      if (qn >= 64) {
+        wave_lds_sync();
+        recover_queued(q_e, q_ah, 64, ends, ent_p, rec, colmap, groups, hits, row_words, col_word0,
+                       lane);
+        // the rest moves to the front of the queue
+        uint32_t e2 = 0, ah2 = 0;
+        if (64 + lane < qn) {
+          e2 = q_e[64 + lane];
+          ah2 = q_ah[64 + lane];
         }
+        wave_lds_sync();
+        if (64 + lane < qn) {
+          q_e[lane] = e2;
+          q_ah[lane] = ah2;
+        }
+        qn -= 64;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+  }
+  if (qn) {
+    wave_lds_sync();
+    recover_queued(q_e, q_ah, qn, ends, ent_p, rec, colmap, groups, hits, row_words, col_word0, lane);
   }
 }
 
