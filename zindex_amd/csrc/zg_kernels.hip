@@ -64,36 +64,64 @@ void launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Packing: one thread per 64 positions.
+// Packing: one wave per 64 output words. In round i the wave's lanes take the
+// 64 consecutive positions of word w0 + i, so each byte load instruction
+// reads 64 consecutive bytes of a read; ballots give the word's invalid bits
+// and the two bit planes of its codes, and lane i keeps word w0 + i for one
+// coalesced store at the end.
 // ---------------------------------------------------------------------------
-__global__ void pack_kernel(const uint8_t *seq, const uint64_t *read_off,
-                            const uint64_t *pos_off, uint64_t n_reads, uint64_t positions,
-                            uint64_t *codes, uint64_t *invalid) {
-  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  uint64_t n_words = (positions + 63) / 64;
-  if (w >= n_words) return;
-  uint64_t p0 = w * 64;
-  uint64_t r = find_segment(pos_off, n_reads, p0);
-  uint64_t inv = 0, c0 = 0, c1 = 0;
-  for (int i = 0; i < 64; i++) {
-    uint64_t p = p0 + i;
+// Bits 0..31 of x to the even bits of the result.
+__device__ __forceinline__ uint64_t spread_even(uint64_t x) {
+  x &= 0xFFFFFFFFULL;
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFULL;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFULL;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0FULL;
+  x = (x | (x << 2)) & 0x3333333333333333ULL;
+  x = (x | (x << 1)) & 0x5555555555555555ULL;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void pack_kernel(const uint8_t *seq, const uint64_t *read_off,
+                                                   const uint64_t *pos_off, uint64_t n_reads,
+                                                   uint64_t positions, uint64_t *codes,
+                                                   uint64_t *invalid) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t n_words = (positions + 63) / 64;
+  const uint64_t w0 = ((blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6) * 64;
+  if (w0 >= n_words) return;  // wave-uniform
+  const uint64_t w1 = w0 + 64 < n_words ? w0 + 64 : n_words;
+  uint64_t p = w0 * 64 + lane;
+  // the lane's read: its positions end at next_pos, byte = p + delta while
+  // below byte_end (the read's last position is the separator)
+  uint64_t r = find_segment(pos_off, n_reads, p < positions ? p : positions - 1);
+  uint64_t next_pos = pos_off[r + 1], delta = read_off[r] - pos_off[r], byte_end = read_off[r + 1];
+  uint64_t my_inv = 0, my_c0 = 0, my_c1 = 0;
+  for (uint64_t w = w0; w < w1; w++, p += 64) {
     int code = -1;
     if (p < positions) {
-      while (pos_off[r + 1] <= p) r++;
-      uint64_t off = p - pos_off[r];
-      uint64_t start = read_off[r], len = read_off[r + 1] - start;
-      if (off < len) code = encode_ascii(seq[start + off]);
+      while (p >= next_pos) {
+        r++;
+        next_pos = pos_off[r + 1];
+        delta = read_off[r] - pos_off[r];
+        byte_end = read_off[r + 1];
+      }
+      const uint64_t byte = p + delta;
+      if (byte < byte_end) code = encode_ascii(seq[byte]);
     }
-    if (code < 0) {
-      inv |= 1ULL << i;
-      code = 0;
+    const uint64_t inv = __ballot(code < 0);
+    const uint64_t b0 = __ballot(code > 0 && (code & 1));
+    const uint64_t b1 = __ballot(code > 0 && (code & 2));
+    if (lane == (uint32_t)(w - w0)) {
+      my_inv = inv;  // invalid positions hold code 0
+      my_c0 = spread_even(b0) | (spread_even(b1) << 1);
+      my_c1 = spread_even(b0 >> 32) | (spread_even(b1 >> 32) << 1);
     }
-    if (i < 32) c0 |= (uint64_t)code << (2 * i);
-    else c1 |= (uint64_t)code << (2 * (i - 32));
   }
-  invalid[w] = inv;
-  codes[2 * w] = c0;
-  codes[2 * w + 1] = c1;
+  if (w0 + lane < w1) {
+    invalid[w0 + lane] = my_inv;
+    codes[2 * (w0 + lane)] = my_c0;
+    codes[2 * (w0 + lane) + 1] = my_c1;
+  }
 }
 
 void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *pos_off,
@@ -101,8 +129,9 @@ void launch_pack(const uint8_t *seq, const uint64_t *read_off, const uint64_t *p
                  hipStream_t s) {
   uint64_t words = (positions + 63) / 64;
   if (!words || !n_reads) return;
-  pack_kernel<<<(uint32_t)((words + 255) / 256), 256, 0, s>>>(seq, read_off, pos_off, n_reads,
-                                                              positions, codes, invalid);
+  const uint64_t waves = (words + 63) / 64;
+  pack_kernel<<<(uint32_t)((waves + 3) / 4), 256, 0, s>>>(seq, read_off, pos_off, n_reads, positions,
+                                                          codes, invalid);
 }
 
 // ---------------------------------------------------------------------------

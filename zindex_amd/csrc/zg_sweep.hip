@@ -117,6 +117,14 @@ __device__ __forceinline__ uint64_t sort_item(uint64_t tile, uint32_t wave, uint
   return tile * kSortTile + wave * kSortWaveItems + (uint64_t)r * 64 + lane;
 }
 
+// The tile of a block: blocks go to the XCDs in turn, and each XCD takes a
+// contiguous range of tiles, so the digit runs of neighbouring tiles (which
+// are neighbours in the output) are written through the same L2.
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t n) {
+  const uint64_t x = b % kXcds, per = n / kXcds, rem = n % kXcds;
+  return x * per + (x < rem ? x : rem) + b / kXcds;
+}
+
 template <class Src>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(Src src, uint64_t count, uint32_t shift,
                                                          uint32_t dbits, uint64_t n_tiles,
@@ -126,7 +134,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(Src src, uint6
   const uint32_t nb = 1u << dbits, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t i = threadIdx.x; i < nb; i += kSortThreads) bins[i] = 0;
   __syncthreads();
-  const uint64_t tile = blockIdx.x;
+  const uint64_t tile = xcd_tile(blockIdx.x, n_tiles);
   for (uint32_t r = 0; r < kSortRounds; r++) {
     const uint64_t i = sort_item(tile, wave, r, lane);
     uint64_t h = 0;
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(Src src, ui
   const uint32_t nb = 1u << dbits, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t i = lane; i < nb; i += 64) wbase[wave][i] = 0;
   __syncthreads();
-  const uint64_t tile = blockIdx.x;
+  const uint64_t tile = xcd_tile(blockIdx.x, n_tiles);
   uint64_t hv[kSortRounds];
   uint32_t pv[kSortRounds], dv[kSortRounds];
 #pragma unroll
