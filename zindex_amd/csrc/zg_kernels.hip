@@ -22,14 +22,14 @@ namespace zg {
 
 namespace {
 
+// A/a -> 0, C/c -> 1, G/g -> 2, T/t -> 3, anything else -1, without
+// branches: u = c with bit 5 cleared is one of 0x41, 0x43, 0x47, 0x54 for a
+// base (offsets 0, 2, 6, 19 from 'A': mask 0x80045), and ((u >> 1) ^ (u >> 2))
+// & 3 is its code.
 __device__ __forceinline__ int encode_ascii(uint8_t c) {
-  switch (c) {
-    case 'A': case 'a': return 0;
-    case 'C': case 'c': return 1;
-    case 'G': case 'g': return 2;
-    case 'T': case 't': return 3;
-    default: return -1;
-  }
+  const uint32_t u = c & 0xDFu, d = u - 0x41u;
+  const bool base = d < 20u && ((0x80045u >> d) & 1u);
+  return base ? (int)(((u >> 1) ^ (u >> 2)) & 3u) : -1;
 }
 
 }  // namespace
@@ -95,32 +95,46 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint8_t *seq, const uin
   // below byte_end (the read's last position is the separator)
   uint64_t r = find_segment(pos_off, n_reads, p < positions ? p : positions - 1);
   uint64_t next_pos = pos_off[r + 1], delta = read_off[r] - pos_off[r], byte_end = read_off[r + 1];
-  uint64_t my_inv = 0, my_c0 = 0, my_c1 = 0;
-  for (uint64_t w = w0; w < w1; w++, p += 64) {
-    int code = -1;
-    if (p < positions) {
-      while (p >= next_pos) {
-        r++;
-        next_pos = pos_off[r + 1];
-        delta = read_off[r] - pos_off[r];
-        byte_end = read_off[r + 1];
+  uint64_t my_inv = 0, my_b0 = 0, my_b1 = 0;  // lane i: word w0 + i's ballots
+  constexpr int U = 16;  // rounds whose byte loads are in flight together
+  for (uint64_t w = w0; w < w1; w += U, p += 64 * U) {
+    uint64_t at[U];
+    bool have[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t pu = p + 64 * (uint64_t)u;
+      have[u] = false;
+      at[u] = 0;
+      if (w + u < w1 && pu < positions) {
+        while (pu >= next_pos) {
+          r++;
+          next_pos = pos_off[r + 1];
+          delta = read_off[r] - pos_off[r];
+          byte_end = read_off[r + 1];
+        }
+        at[u] = pu + delta;
+        have[u] = at[u] < byte_end;
       }
-      const uint64_t byte = p + delta;
-      if (byte < byte_end) code = encode_ascii(seq[byte]);
     }
-    const uint64_t inv = __ballot(code < 0);
-    const uint64_t b0 = __ballot(code > 0 && (code & 1));
-    const uint64_t b1 = __ballot(code > 0 && (code & 2));
-    if (lane == (uint32_t)(w - w0)) {
-      my_inv = inv;  // invalid positions hold code 0
-      my_c0 = spread_even(b0) | (spread_even(b1) << 1);
-      my_c1 = spread_even(b0 >> 32) | (spread_even(b1 >> 32) << 1);
+    uint8_t ch[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) ch[u] = have[u] ? seq[at[u]] : (uint8_t)'N';
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int code = encode_ascii(ch[u]);
+      const uint64_t inv = __ballot(code < 0);
+      const uint64_t b0 = __ballot(code > 0 && (code & 1));
+      const uint64_t b1 = __ballot(code > 0 && (code & 2));
+      const bool mine = lane == (uint32_t)(w + u - w0);
+      my_inv = mine ? inv : my_inv;  // invalid positions hold code 0
+      my_b0 = mine ? b0 : my_b0;
+      my_b1 = mine ? b1 : my_b1;
     }
   }
   if (w0 + lane < w1) {
     invalid[w0 + lane] = my_inv;
-    codes[2 * (w0 + lane)] = my_c0;
-    codes[2 * (w0 + lane) + 1] = my_c1;
+    codes[2 * (w0 + lane)] = spread_even(my_b0) | (spread_even(my_b1) << 1);
+    codes[2 * (w0 + lane) + 1] = spread_even(my_b0 >> 32) | (spread_even(my_b1 >> 32) << 1);
   }
 }
 
