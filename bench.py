@@ -746,6 +746,21 @@ def main() -> int:
                           "sum": round(st["sweep_sum_ms"], 2)},
             "sum_kernel_gbs": round(sum_bytes / (st["sweep_sum_ms"] / 1000) / 1e9, 1)
             if st["sweep_sum_ms"] > 0 else None})
+        # the lookups' hit-row stores against the same stores replayed alone
+        # (one whole row per seed of the last chunk, in the sweep's order, one
+        # lane per row): the lookup is store-bound when it writes as fast
+        try:
+            ceiling = batch.store_probe()
+        except zindex_amd._lib.ZindexError as e:
+            log(f"store probe: {e}")
+            ceiling = None
+        if ceiling:
+            write_gbs = sum_bytes / launches / (kernel_ms / 1000) / 1e9
+            roofline.update({"hit_row_write_gbs": round(write_gbs, 1),
+                             "store_replay_gbs": round(ceiling, 1),
+                             "frac_of_store_replay": round(write_gbs / ceiling, 4)})
+            log(f"hit-row stores {write_gbs:.0f} GB/s; replayed alone {ceiling:.0f} GB/s "
+                f"({write_gbs / ceiling:.1%})")
     else:
         # the ceiling of the row gathers alone: random whole rows of the
         # bank's row size, the same bytes per launch, no other work

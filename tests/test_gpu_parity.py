@@ -219,6 +219,29 @@ def test_sorted_sweep_parity(bits, group, scale, mode, hpc, pattern, monkeypatch
     check_parity(idx, orc, reads, label=f"sweep chunks b{bits} g{group} m{mode}")
 
 
+def test_store_probe_needs_a_sweep_and_keeps_results(base_case):
+    # zg_batch_store_probe replays the last chunk's hit-row stores: refused
+    # before any sweep, a positive rate after one, and the batch's results
+    # (per-read counts) unchanged by it
+    colors, reads, idx, orc = base_case
+    buf, off = zindex_amd.pack_reads(reads)
+    saved = idx.tuning
+    idx.tuning = dict(SWEEP)
+    try:
+        with idx.batch(buf, off, records=True) as b:
+            with pytest.raises(zindex_amd._lib.ZindexError):
+                b.store_probe()
+            b.run()
+            before = b.results()
+            gbs = b.store_probe()
+            after = b.results()
+    finally:
+        idx.tuning = saved
+    say(f"store probe: {gbs:.1f} GB/s")
+    assert gbs > 0
+    assert all(np.array_equal(x, y) for x, y in zip(before, after))
+
+
 def test_sweep_pieces_against_cpu():
     # the sweep's scans, stable radix passes by first segment, segment starts
     # and per-read row sums, each against its CPU restatement

@@ -284,6 +284,13 @@ class Batch:
                                      _u32p(counts) if self.records else None, _u64p(agg)))
         return totals[:n], (counts[:n] if self.records else None), agg
 
+    def store_probe(self) -> float:
+        """GB/s of the last sweep chunk's hit-row stores replayed alone
+        (zg_batch_store_probe); overwrites the hit rows, not the results."""
+        gbs = C.c_double()
+        check(lib().zg_batch_store_probe(self._h, C.byref(gbs)))
+        return gbs.value
+
     def stats(self) -> dict:
         s = BatchStats()
         check(lib().zg_batch_get_stats(self._h, C.byref(s)))

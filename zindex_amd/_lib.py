@@ -158,6 +158,7 @@ SIGNATURES = {
     "zg_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(RequestStats)]),
     "zg_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "zg_server_destroy": (C.c_int, [C.c_void_p]),
+    "zg_batch_store_probe": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "zg_gather_probe": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64,
                                   C.POINTER(C.c_double)]),
     "zg_gather_probe_keys": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64,

@@ -990,6 +990,8 @@ static void release_sweep(const zg_index *idx, zg_batch *b) {
   for (DevBuf *d : {&b->sw_h[0], &b->sw_h[1], &b->sw_p[0], &b->sw_p[1], &b->sw_listed,
                     &b->sw_hist, &b->sw_off, &b->sw_start, &b->sw_queue})
     d->reset();
+  b->sw_last_p = nullptr;
+  b->sw_last_n = nullptr;
   if (idx->m.index_mode != 3) b->fd_hits.reset();
 }
 
@@ -1529,6 +1531,16 @@ extern "C" int zg_batch_get_stats(const zg_batch *b, zg_batch_stats *o) {
       for (const SweepClass &c : idx->sw_classes)
         if (c.planes) o->sweep_plane_banks += c.banks.size();
     }
+    return 0;
+  } catch (...) {
+    return translate_exception();
+  }
+}
+
+extern "C" int zg_batch_store_probe(zg_batch *b, double *gbs) {
+  try {
+    ensure(b && gbs, "null argument", ZG_ERR_INVALID);
+    *gbs = zg::sweep_store_probe(b);
     return 0;
   } catch (...) {
     return translate_exception();

@@ -178,6 +178,10 @@ struct zg_batch {
   hipGraphExec_t sw_exec = nullptr;
   bool sw_graph_failed = false, sw_capturing = false;
   uint64_t sw_cap = 0;            // chunk capacity (positions) of the last eager sweep
+  // the last chunk's sorted seed positions (last class) and their device count
+  const uint32_t *sw_last_p = nullptr;
+  const uint64_t *sw_last_n = nullptr;
+  uint64_t sw_last_items = 0;
   // randstrobemers: candidate bitmap, per-word counts and offsets, run-id
   // offsets, candidate hashes and run ids (starts go to key_win); keys hold
   // the forward keys then the reverse-complement keys (rs_forward each)
@@ -226,6 +230,9 @@ void sweep_run(zg_index *idx, zg_batch *b);
 void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &recovery_ms,
                  double &sum_ms);
 uint64_t sweep_folded(zg_batch *b);
+// GB/s of writing one hit row per listed seed of the last sweep chunk, at its
+// position, in the sweep's order, with nothing else (the lookup's stores alone).
+double sweep_store_probe(zg_batch *b);
 // Batch pipeline pieces, all enqueued on b->stream (zg_api.hip).
 void configure(zg_index *idx, zg_batch *b);
 void seed_batch(zg_index *idx, zg_batch *b);

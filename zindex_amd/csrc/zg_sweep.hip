@@ -1791,6 +1791,9 @@ void sweep_run(zg_index *idx, zg_batch *b) {
       check_hip(hipEventRecord(ev[1 + 3 * k], s), "hipEventRecord");
       sa.ent_h = b->sw_h[cur].as<uint64_t>();
       sa.ent_p = b->sw_p[cur].as<uint32_t>();
+      b->sw_last_p = sa.ent_p;
+      b->sw_last_n = n_dev;
+      b->sw_last_items = n_items;
       sa.slice_bytes = c.slice_bytes;
       sa.n_slices = c.n_slices;
       sa.col_word0 = c.col_word0;
@@ -1847,6 +1850,43 @@ void sweep_run(zg_index *idx, zg_batch *b) {
     check_hip(hipEventRecord(ev[ev_per_chunk - 1], s), "hipEventRecord");
   }
   if (strobes) launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * m.colors.size(), s);
+}
+
+// The lookups' hit-row stores alone: lane i writes the whole row (row_words
+// words, 16 bytes at a time) of position p[i], for the n entries in the
+// sweep's order (segment, then position).
+__global__ __launch_bounds__(256) void store_probe_kernel(const uint32_t *p, const uint64_t *n_ptr,
+                                                          uint32_t *hits, uint32_t row_words) {
+  const uint64_t n = *n_ptr;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint4 *row = (uint4 *)(hits + (uint64_t)p[i] * row_words);
+    for (uint32_t k = 0; k < row_words / 4; k++) row[k] = make_uint4((uint32_t)i, k, 0, 0);
+  }
+}
+
+double sweep_store_probe(zg_batch *b) {
+  zg_index *idx = b->idx;
+  ensure(b->sw_last_p && b->sw_last_n && idx->sw_row_words % 4 == 0,
+         "store probe: no sorted sweep has run on this batch", ZG_ERR_INVALID);
+  hipStream_t s = b->stream;
+  hipEvent_t e0, e1;
+  check_hip(hipEventCreate(&e0), "hipEventCreate");
+  check_hip(hipEventCreate(&e1), "hipEventCreate");
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(idx->cu_count * 16, b->sw_last_items / 256 + 1);
+  store_probe_kernel<<<blocks, 256, 0, s>>>(b->sw_last_p, b->sw_last_n, b->fd_hits.as<uint32_t>(),
+                                            idx->sw_row_words);  // warm-up
+  check_hip(hipEventRecord(e0, s), "hipEventRecord");
+  store_probe_kernel<<<blocks, 256, 0, s>>>(b->sw_last_p, b->sw_last_n, b->fd_hits.as<uint32_t>(),
+                                            idx->sw_row_words);
+  check_hip(hipEventRecord(e1, s), "hipEventRecord");
+  check_hip(hipStreamSynchronize(s), "store probe");
+  uint64_t n = 0;
+  check_hip(hipMemcpy(&n, b->sw_last_n, 8, hipMemcpyDeviceToHost), "store probe");
+  const double ms = elapsed_ms(e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return (double)n * idx->sw_row_words * 4 / (ms / 1000) / 1e9;
 }
 
 // Phase times of the last sweep (its events have completed). Per chunk:
