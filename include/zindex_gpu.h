@@ -392,6 +392,10 @@ typedef struct zg_server_stats {
   uint32_t slots;
   double lookup_ms;         /* summed HIP-event time of the batches' lookups */
   uint64_t device_bytes;    /* resident index + slot buffers */
+  uint64_t sweep_batches;   /* batches run as the sorted sweep (the index tuned
+                               to sweep, k-mer seeds, banks resident, and the
+                               batch large enough that the sweep is faster by
+                               the server's model of both orders) */
 } zg_server_stats;
 
 void zg_server_config_default(zg_server_config *cfg);

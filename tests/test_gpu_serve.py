@@ -130,6 +130,41 @@ def test_server_seed_modes_mixed_equals_alone(mode):
             assert np.array_equal(m[0], et) and np.array_equal(m[1].astype(np.uint64), ec), i
 
 
+@pytest.mark.parametrize("colors_n,recover", [(600, True), (12, False)])
+def test_server_sweep_batches_equal_alone_and_oracle(colors_n, recover, monkeypatch):
+    # an index tuned to the sorted sweep: every batch runs as the sweep
+    # (ZG_SERVE_SWEEP=1) from the slot's reserved chunk buffers, several
+    # chunks per batch; each request's results equal the read-ordered query
+    # alone and the oracle, mixed and alone, with recovery postings at 600
+    # colours (the bit-plane lookup and its recovery pass)
+    colors = make_colors(80 + colors_n, colors_n, 3000 if colors_n > 100 else 30000,
+                         rates=(0.0, 0.01))
+    reads = make_reads(81, colors, 240, lo=30, hi=3000)
+    idx = build(colors, fingerprint_bits=8, hashes=4, slot_scale=0.6)
+    idx.load(recover=recover)
+    orc = oracle_for(idx, recover)
+    requests = split_requests(82, reads, 16)
+    expect = [idx.query_reads(r, records=True) for r in requests]  # read order
+    idx.tuning = {"span": 1024, "tiles_per_wave": 1, "prefetch": 1, "waves_per_cu": 0,
+                  "nontemporal": 0, "sweep": 1}
+    monkeypatch.setenv("ZG_SERVE_SWEEP", "1")
+    monkeypatch.setenv("ZG_SWEEP_CHUNK_POSITIONS", "20000")
+    mixed, st = serve_all(idx, requests, slots=2, max_requests=5)
+    alone, st1 = serve_all(idx, requests, slots=1, max_requests=1)
+    say(f"sweep server: {st}")
+    assert st["sweep_batches"] > 0 and st1["sweep_batches"] == len([r for r in requests if r])
+    assert st["max_batch_requests"] > 1
+    for i, (req, m, a, e) in enumerate(zip(requests, mixed, alone, expect)):
+        for got in (m, a):
+            assert got[3]["status"] == 0
+            assert np.array_equal(got[0], e[0]) and np.array_equal(got[1], e[1]), i
+            assert np.array_equal(got[2], e[2]), i
+        if req:
+            buf, off = zindex_amd.pack_reads(req)
+            et, ec = orc.query_reads(buf, off, threads=zr.cpu_count())
+            assert np.array_equal(m[0], et) and np.array_equal(m[1].astype(np.uint64), ec), i
+
+
 def test_server_pinned_outputs_and_budget(serve_case):
     colors, reads, idx = serve_case
     C = idx.meta["n_colors"]

@@ -98,6 +98,7 @@ class ServerStats(C.Structure):
         ("requests", C.c_uint64), ("batches", C.c_uint64), ("reads", C.c_uint64),
         ("seeds", C.c_uint64), ("max_batch_requests", C.c_uint32), ("slots", C.c_uint32),
         ("lookup_ms", C.c_double), ("device_bytes", C.c_uint64),
+        ("sweep_batches", C.c_uint64),
     ]
 
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -141,6 +142,7 @@ struct zg_index {
   uint32_t sw_row_words = 0, sw_col_words = 0, sw_members = 0;
   zg::DevBuf sw_bit_color;
   int sw_planes = -1;  // whether the tables were built with bit-plane copies allowed
+  std::mutex sw_mu;    // guards building the tables
   zg::RecDev rec_view{};
   zg::Tuning tuning;
   ~zg_index();
@@ -178,6 +180,7 @@ struct zg_batch {
   hipGraphExec_t sw_exec = nullptr;
   bool sw_graph_failed = false, sw_capturing = false;
   uint64_t sw_cap = 0;            // chunk capacity (positions) of the last eager sweep
+  uint64_t sw_hit_budget = 0;     // hit-row bytes per chunk at most (0: the library default)
   // the last chunk's sorted seed positions (last class) and their device count
   const uint32_t *sw_last_p = nullptr;
   const uint64_t *sw_last_n = nullptr;
@@ -230,6 +233,13 @@ void sweep_run(zg_index *idx, zg_batch *b);
 void sweep_times(zg_batch *b, double &sort_ms, double &lookup_ms, double &recovery_ms,
                  double &sum_ms);
 uint64_t sweep_folded(zg_batch *b);
+// The sweep's tables and bit-plane copies built ahead (thread-safe); device
+// bytes of the chunk buffers for `positions` positions of k-mer seeds, and
+// reserving them in a batch; the hit-row words per position.
+void sweep_prepare(zg_index *idx);
+uint64_t sweep_chunk_bytes(zg_index *idx, uint64_t positions);
+void sweep_reserve(zg_index *idx, zg_batch *b, uint64_t positions);
+uint32_t sweep_row_words(zg_index *idx);
 // GB/s of writing one hit row per listed seed of the last sweep chunk, at its
 // position, in the sweep's order, with nothing else (the lookup's stores alone).
 double sweep_store_probe(zg_batch *b);

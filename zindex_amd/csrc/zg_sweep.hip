@@ -1536,6 +1536,7 @@ static std::vector<uint32_t> fetch_u32(const uint32_t *p, uint64_t n) {
 // bit of its lane field), then every class's member bits.
 static void ensure_sweep_tables(zg_index *idx) {
   const int allow = planes_enabled() ? 1 : 0;
+  std::lock_guard<std::mutex> lock(idx->sw_mu);  // server slots share the index
   if (idx->sw_row_words && idx->sw_planes == allow) return;
   for (DeviceBank &db : idx->dbanks) {  // tables of the other setting
     db.planes.reset();
@@ -1669,6 +1670,53 @@ static int sort_class(zg_batch *b, Src first_pass, uint64_t count, uint64_t n_it
   return cur;
 }
 
+// Device bytes of each buffer of a chunk of `most` items (entries twice, the
+// listed-position bitmap, tile histograms and offsets, scan space, segment
+// starts, work counters, hit rows), reserved in that order.
+static std::vector<uint64_t> chunk_buffer_bytes(const zg_index *idx, uint64_t most, bool keys) {
+  uint64_t max_count = 1, max_hist = 1;
+  for (const SweepClass &c : idx->sw_classes) {
+    max_count = std::max(max_count, c.count);
+    const uint32_t seg_bits = bits_for(c.count - 1);
+    const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const uint32_t dbits = (seg_bits + passes - 1) / passes;
+    max_hist = std::max<uint64_t>(
+        max_hist, ((uint64_t)1 << dbits) * std::max<uint64_t>(1, (most + kSortTile - 1) / kSortTile) + 1);
+  }
+  return {most * 8 + 64, most * 8 + 64, most * 4 + 64, most * 4 + 64,
+          keys ? 0 : ((most + 63) / 64 + 1) * 8, max_hist * 4, max_hist * 8,
+          scan_temp_bytes_u32_to_u64(max_hist), (max_count + 1) * 4, 32 * kXcds * 4 + 64,
+          std::max<uint64_t>(1, most * idx->sw_row_words) * 4};
+}
+
+static void reserve_chunk(zg_index *idx, zg_batch *b, uint64_t most, bool keys) {
+  const std::vector<uint64_t> n = chunk_buffer_bytes(idx, most, keys);
+  DevBuf *bufs[] = {&b->sw_h[0], &b->sw_h[1], &b->sw_p[0], &b->sw_p[1], &b->sw_listed,
+                    &b->sw_hist, &b->sw_off, &b->scan_temp, &b->sw_start, &b->sw_queue,
+                    &b->fd_hits};
+  for (size_t i = 0; i < n.size(); i++)
+    if (n[i]) bufs[i]->reserve(n[i]);
+}
+
+void sweep_prepare(zg_index *idx) { ensure_sweep_tables(idx); }
+
+uint64_t sweep_chunk_bytes(zg_index *idx, uint64_t positions) {
+  ensure_sweep_tables(idx);
+  uint64_t n = 0;
+  for (uint64_t v : chunk_buffer_bytes(idx, positions, false)) n += v;
+  return n;
+}
+
+void sweep_reserve(zg_index *idx, zg_batch *b, uint64_t positions) {
+  ensure_sweep_tables(idx);
+  reserve_chunk(idx, b, positions, false);
+}
+
+uint32_t sweep_row_words(zg_index *idx) {
+  ensure_sweep_tables(idx);
+  return idx->sw_row_words;
+}
+
 // The lookups of a batch as the sweep, every chunk enqueued on b->stream with
 // no host synchronization (sizes come from the host's read offsets). Entries
 // are the valid windows (positions) or the explicit keys of a chunk of whole
@@ -1689,6 +1737,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   uint64_t hit_budget = kSweepHitBytes;
   if (const char *e = getenv("ZG_SWEEP_HIT_GIB"))  // hit rows of one chunk, GiB
     hit_budget = std::max<uint64_t>(1, strtoull(e, nullptr, 10)) << 30;
+  if (b->sw_hit_budget) hit_budget = std::min(hit_budget, b->sw_hit_budget);
   uint64_t cap = std::min<uint64_t>((1ULL << 32) - 64, hit_budget / (row_words * 4ULL));
   size_t free_b = 0, total_b = 0;
   if (b->sw_capturing) {
@@ -1718,26 +1767,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
     most = std::max(most, off[r1] - off[r0]);
     cuts.push_back(r1);
   }
-  uint64_t max_count = 1, max_hist = 1;
-  for (const SweepClass &c : idx->sw_classes) {
-    max_count = std::max(max_count, c.count);
-    const uint32_t seg_bits = bits_for(c.count - 1);
-    const uint32_t passes = (seg_bits + kMaxDigitBits - 1) / kMaxDigitBits;
-    const uint32_t dbits = (seg_bits + passes - 1) / passes;
-    max_hist = std::max<uint64_t>(
-        max_hist, ((uint64_t)1 << dbits) * std::max<uint64_t>(1, (most + kSortTile - 1) / kSortTile) + 1);
-  }
-  for (int i = 0; i < 2; i++) {
-    b->sw_h[i].reserve(most * 8 + 64);
-    b->sw_p[i].reserve(most * 4 + 64);
-  }
-  if (!keys) b->sw_listed.reserve(((most + 63) / 64 + 1) * 8);
-  b->sw_hist.reserve(max_hist * 4);
-  b->sw_off.reserve(max_hist * 8);
-  b->scan_temp.reserve(scan_temp_bytes_u32_to_u64(max_hist));
-  b->sw_start.reserve((max_count + 1) * 4);
-  b->sw_queue.reserve(32 * kXcds * 4 + 64);
-  b->fd_hits.reserve(std::max<uint64_t>(1, most * row_words) * 4);
+  reserve_chunk(idx, b, most, keys);
   const uint32_t n_chunks = (uint32_t)cuts.size() - 1;
   const size_t ev_per_chunk = 2 + 3 * idx->sw_classes.size();
   while (b->sw_ev.size() < ev_per_chunk * n_chunks) {
