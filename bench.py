@@ -504,10 +504,11 @@ def serve_main(args, zindex_amd, synth) -> int:
                          "seeds": seeds, "batches": sst["batches"],
                          "mean_requests_per_batch": args.requests / max(1, sst["batches"]),
                          "lookup_ms_sum": sst["lookup_ms"],
+                         "sweep_batches": sst["sweep_batches"],
                          "device_bytes": sst["device_bytes"]}
         results[name + "_aggs"] = aggs
-        log(f"{name} closed loop: {seeds / wall:.3e} k-mers/s, {sst['batches']} batches, "
-            f"{wall:.2f}s")
+        log(f"{name} closed loop: {seeds / wall:.3e} k-mers/s, {sst['batches']} batches "
+            f"({sst['sweep_batches']} as the sorted sweep), {wall:.2f}s")
     batched_aggs, plain_aggs = results.pop("batched_aggs"), results.pop("plain_aggs")
     same = all(np.array_equal(a, b) for a, b in zip(batched_aggs, plain_aggs))
     # requests alone through the batch query, on a sample, against the mixed pass
@@ -529,14 +530,18 @@ def serve_main(args, zindex_amd, synth) -> int:
             row[name] = {"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
                          "mean_ms": float(np.mean(lat)), "kmers_per_s": seeds / wall,
                          "mean_requests_per_batch": args.requests / max(1, sst["batches"]),
-                         "max_requests_per_batch": sst["max_batch_requests"]}
+                         "max_requests_per_batch": sst["max_batch_requests"],
+                         "sweep_batches": sst["sweep_batches"]}
             log(f"qps {q} {name}: p50 {row[name]['p50_ms']:.1f} ms, p99 {row[name]['p99_ms']:.1f} ms, "
                 f"{seeds / wall:.3e} k-mers/s")
         sweep.append(row)
     b = results["batched"]
-    # the lookup kernel alone at this geometry: one persistent launch over the
-    # first requests' reads (aggregate counts), timed with HIP events
+    # the read-ordered lookup kernel alone at this geometry: one persistent
+    # launch over the first requests' reads (aggregate counts), timed with HIP
+    # events
     n_k = int(min(args.reads, 200_000))
+    served_tuning = dict(idx.tuning)
+    idx.tuning = dict(served_tuning, sweep=0)
     with idx.batch(buf[: int(off[n_k])], off[: n_k + 1], records=False) as kb:
         kb.run()
         lk = []
@@ -544,6 +549,7 @@ def serve_main(args, zindex_amd, synth) -> int:
             kb.run()
             lk.append(kb.stats()["last_lookup_ms"])
         kst = kb.stats()
+    idx.tuning = served_tuning
     positions = int(off[n_k]) + n_k
     algo = kst["algorithmic_bytes"] + positions * 3 // 8
     lk_ms = float(np.mean(lk))
@@ -578,8 +584,9 @@ def serve_main(args, zindex_amd, synth) -> int:
                          f"{head['qps']} requests/s; per-request aggregate counts"),
             "colors": args.colors, "reads": args.reads, "bases": int(off[-1]),
             "bank_bytes": meta["payload_bytes"], "slots": args.slots, "tuning": idx.tuning,
-            "step": "one request: H2D from page-locked memory, pack, k-mers, persistent lookup, "
-                    "per-request sums, D2H",
+            "step": "one request: H2D from page-locked memory, pack, k-mers, the lookups (the "
+                    "persistent read-ordered lookup, or the sorted sweep for batches the server's "
+                    "model finds faster that way), per-request sums, D2H",
         },
         "latency": {"qps": head["qps"], "batched_p50_ms": head["batched"]["p50_ms"],
                     "batched_p99_ms": head["batched"]["p99_ms"],
