@@ -218,8 +218,16 @@ def sweep_main(args, zindex_amd, synth) -> int:
                 lk.append(b.stats()["last_lookup_ms"])
             wall = (time.perf_counter() - t1) / args.steps
             st = b.stats()
-        algo = st["algorithmic_bytes"] + (int(off[-1]) + args.reads) * 3 // 8
-        gbs = algo / (np.mean(lk) / 1000) / 1e9
+        if st["tuning"]["sweep"]:
+            # the sweep's lookups: the bank once per chunk, 12 B of entry and
+            # the hit row per seed, over the lookup launches' HIP-event time
+            algo = st["sweep_algorithmic_bytes"]
+            gbs = algo / (st["sweep_lookup_ms"] / 1000) / 1e9
+            kernel = "plane_lookup_kernel" if st["sweep_plane_banks"] else "stage_lookup_kernel"
+        else:
+            algo = st["algorithmic_bytes"] + (int(off[-1]) + args.reads) * 3 // 8
+            gbs = algo / (np.mean(lk) / 1000) / 1e9
+            kernel = "lookup_kernel"
         # false negatives: every stored k-mer hits its own colour
         totals, counts, _ = idx.query_reads(stored, records=True)
         fn = int(sum(int(totals[i]) - int(counts[i, c]) for i, c in enumerate(picks)))
@@ -236,6 +244,10 @@ def sweep_main(args, zindex_amd, synth) -> int:
                "kmers_per_s": st["seeds"] / wall, "lookup_ms": round(float(np.mean(lk)), 3),
                "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "achieved_gbs": round(gbs, 1),
                "bytes_per_kmer": st["algorithmic_bytes"] // max(1, st["seeds"]),
+               "kernel": kernel, "sweep_phases_ms": {
+                   "radix_sort": round(st["sweep_sort_ms"], 2), "lookup": round(st["sweep_lookup_ms"], 2),
+                   "recovery": round(st["sweep_recovery_ms"], 2), "sum": round(st["sweep_sum_ms"], 2)}
+               if st["tuning"]["sweep"] else None,
                "tuning": tuning, "false_negatives": fn,
                "stored_kmers": int(totals.sum()), "absent_kmers": n_abs,
                "fpr_mean": float(rates.mean()), "fpr_max": float(rates.max()),
@@ -261,7 +273,7 @@ def sweep_main(args, zindex_amd, synth) -> int:
         "sweep": rows,
         "roofline": {"bound": "hbm", "achieved": rows[0]["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": rows[0]["roofline_frac"], "traffic": None,
-                     "kernel": "lookup_kernel"},
+                     "kernel": rows[0]["kernel"]},
         "cpu_baseline": None,
     }
     print(json.dumps(out))
