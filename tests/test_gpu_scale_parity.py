@@ -185,3 +185,18 @@ def test_sweep_as_one_graph_replays(records, monkeypatch):
             if records:
                 assert np.array_equal(got[1].astype(np.uint64), exp_counts)
     idx.tuning = dict(SWEEP_BENCH, graph=0)
+
+
+def test_sweep_where_only_the_bit_planes_fit():
+    # b = 16 at H = 8 over 300 colours: the staged lookup's window (8 x 64 rows
+    # of a 512-byte slice) exceeds LDS, the bit-plane lookup's (256-byte
+    # slices) fits, so the sweep runs over the bit planes only
+    colors = make_colors(610, 300, 3000)
+    reads = make_reads(611, colors, 150, lo=30, hi=2500)
+    idx = build(colors, fingerprint_bits=16, hashes=8)
+    idx.load()
+    orc = oracle_for(idx)
+    idx.tuning = SWEEP_BENCH
+    st = assert_sweep_runs(idx, reads)
+    assert st["sweep_plane_banks"] == idx.meta["n_banks"], st
+    check_parity(idx, orc, reads, label="b16 H8, bit planes only")

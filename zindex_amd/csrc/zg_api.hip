@@ -1593,6 +1593,10 @@ extern "C" int zg_autotune(zg_index *idx, zg_batch *b, zg_tuning *chosen) {
             double ms = 1e30;
             for (int rep = 0; rep < 2; rep++) {
               int rc = zg_batch_run(idx, b);
+              if (rc && sw) {  // the sweep cannot run here (device memory): not a candidate
+                ms = 1e30;
+                break;
+              }
               if (rc) return rc;
               ms = std::min(ms, b->last_lookup_ms);
             }

@@ -1424,19 +1424,6 @@ static uint32_t slice_bytes_for(const BankDev &bk) {
   return (uint32_t)std::min<uint64_t>(kSliceBytes, (bk.plane_bytes + 63) / 64 * 64);
 }
 
-// Indexes the sweep serves: every seed mode but findere, every bank's
-// segment window of one 512-byte slice within the LDS window
-// (H * L * slice <= 128 KiB, L <= 256, H <= 8), and banks resident.
-bool sweep_layout_ok(const zg_index *idx) {
-  if (idx->m.index_mode == 3 || idx->dbanks.empty()) return false;
-  for (const DeviceBank &db : idx->dbanks) {
-    const BankDev &bk = db.view;
-    if (bk.geo.length > 256 || bk.hashes > 8 || bk.geo.count < 1 || bk.groups == 0) return false;
-    if ((uint64_t)bk.hashes * bk.geo.length * slice_bytes_for(bk) > kStageWindow) return false;
-  }
-  return true;
-}
-
 // Row-slice bytes of the bit-plane lookup for a bank of b-bit fields, 0 when
 // it does not apply: banks without merged groups, H in {2, 3, 4, 8}, the
 // window of H x L slices within the kernel's LDS window. The largest slice
@@ -1460,6 +1447,26 @@ static uint32_t plane_slice_bytes(const BankDev &bk, uint32_t B) {
 static bool planes_enabled() {
   const char *e = getenv("ZG_SWEEP_PLANES");  // 0: always the staged lookup
   return !e || strcmp(e, "0") != 0;
+}
+
+// Whether the staged lookup's window (H x L rows of a 512-byte slice) fits LDS.
+static bool stage_fits(const BankDev &bk) {
+  return (uint64_t)bk.hashes * bk.geo.length * slice_bytes_for(bk) <= kStageWindow;
+}
+
+// Indexes the sweep serves: every seed mode but findere, banks resident, and
+// every bank's segment window within LDS for the staged lookup
+// (H * L * slice <= 128 KiB, L <= 256, H <= 8) or for the bit-plane lookup
+// (then the bank must get its bit-plane copy).
+bool sweep_layout_ok(const zg_index *idx) {
+  if (idx->m.index_mode == 3 || idx->dbanks.empty()) return false;
+  const uint32_t B = (uint32_t)idx->m.fingerprint_bits;
+  for (const DeviceBank &db : idx->dbanks) {
+    const BankDev &bk = db.view;
+    if (bk.geo.length > 256 || bk.hashes > 8 || bk.geo.count < 1 || bk.groups == 0) return false;
+    if (!stage_fits(bk) && !(planes_enabled() && plane_slice_bytes(bk, B))) return false;
+  }
+  return true;
 }
 
 // Bank rows any lookup reads: (count + H - 1) segments of L rows.
@@ -1579,9 +1586,16 @@ static void ensure_sweep_tables(zg_index *idx) {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
       (void)hipGetLastError();
+      free_b = 0;
+    }
+    // a quarter of the free memory stays for the chunks, unless only the bit
+    // planes can serve the class (the staged window does not fit LDS)
+    const bool only = !stage_fits(bk);
+    if (need > (only ? free_b : free_b / 4 * 3)) {
+      ensure(!only, "the sorted sweep needs a bit-plane copy of the banks that does not fit the "
+                    "free device memory", ZG_ERR_NOMEM);
       continue;
     }
-    if (need > free_b / 4 * 3) continue;
     for (uint32_t bi : c.banks) build_planes(idx->dbanks[bi], B, sb, n_slices);
     c.planes = true;
     c.slice_bytes = sb;
