@@ -437,6 +437,7 @@ struct StageArgs {
   const uint8_t *planes;  // the bank as bit planes (plane_lookup_kernel), or null
   uint64_t planes_pitch;  // its row pitch
   uint32_t parts;         // plane_lookup_kernel: entry ranges per (segment, slice)
+  uint32_t row_stage;     // plane_lookup_kernel: hit rows through LDS, whole per store
 };
 
 namespace {
@@ -906,6 +907,16 @@ __device__ __forceinline__ uint32_t rows_xor(const uint32_t (&w)[HMAX], uint32_t
   }
 }
 
+// LDS written by a wave visible to its own later LDS reads (and the reverse)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bytes of LDS per wave through which plane_lookup_kernel passes its hit rows
+constexpr uint32_t kRowStageBytes = 2048;
+
 // The staged lookup over the bank as bit planes (bit_planes_kernel), for
 // banks without merged groups: per 32 fields, the H rows' B plane words are
 // XORed with the fingerprint's bits (all-ones or all-zero words) and ORed, so
@@ -916,7 +927,11 @@ __device__ __forceinline__ uint32_t rows_xor(const uint32_t (&w)[HMAX], uint32_t
 // takes one entry and reads its rows' pieces in the order (l ^ c) mod NP, so
 // the 16 lanes of an LDS access read 16 different bank groups. Hit words
 // are written in field order: bit f of word w of slice q is column
-// 32 (q SG + w) + f. Recovery postings are ORed in afterwards, once per
+// 32 (q SG + w) + f. A lane's hit words for the slice (SG words) go to
+// LDS, R entries at a time, and are read back 16 bytes per lane so that each
+// store instruction writes 64 / NQ whole row slices: random rows written
+// whole by one instruction are stored several times faster than 16-byte
+// pieces of 64 rows. Recovery postings are ORed in afterwards, once per
 // entry (plane_recovery_kernel).
 template <int B, int HT, int NP>
 __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArgs a) {
@@ -929,8 +944,16 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
   constexpr uint32_t NB = SG / BS;
   constexpr int HMAX = HT ? HT : 8;
   static_assert(SG >= 1 && NB >= 1 && (NP & (NP - 1)) == 0, "slice geometry");
+  // hit rows through LDS: NQ 16-byte pieces per entry, R entries per round
+  constexpr uint32_t NQ = SG % 4 == 0 ? SG / 4 : 1;
+  constexpr bool kRows = NQ >= 2;
+  constexpr uint32_t R = kRowStageBytes / (16 * NQ) < 64 ? kRowStageBytes / (16 * NQ) : 64;
+  static_assert(!kRows || (R % NQ == 0 && 64 % R == 0 && (R * NQ) % 64 == 0), "row staging");
   __shared__ __attribute__((aligned(16))) uint8_t win[plane_window(B)];
-  __shared__ uint32_t unit_sh;
+  __shared__ __attribute__((aligned(16))) uint8_t rowst[kRows ? kPlaneWaves * kRowStageBytes : 16];
+  // the unit number shares wave 0's staging: it is read between the two
+  // barriers that open a unit, staging is written only after the third
+  uint32_t &unit_sh = *(uint32_t *)rowst;
   const BankDev &bk = a.bank;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t H = HT ? (uint32_t)HT : bk.hashes;
@@ -1013,7 +1036,12 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
       // a partitioned bank answers the keys routed to it (storage.rs:99-101)
       const bool own =
           v && (bk.partitions <= 1 || partition_id(key, seed, bk.partitions) == bk.partition);
-      if (own) {
+      const bool staged = kRows && a.row_stage;  // uniform
+      uint32_t o[SG];
+      if (!own) {
+#pragma unroll
+        for (uint32_t w = 0; w < SG; w++) o[w] = 0;
+      } else {
         uint32_t fm[PPG][4];
 #pragma unroll
         for (uint32_t t = 0; t < PPG; t++) {
@@ -1024,7 +1052,6 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
             asm volatile("" : "+v"(fm[t][k]));  // kept in a register across the pieces
           }
         }
-        uint32_t o[SG];
         uint32_t acc = 0;
 #pragma unroll
         for (uint32_t c = 0; c < NP; c++) {
@@ -1076,14 +1103,48 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
             }
         }
         uint32_t *dst = a.hits + prow + a.col_word0 + (uint64_t)q * SG;
-        if constexpr (SG % 4 == 0) {
+        if (!staged) {
+          if constexpr (SG % 4 == 0) {
 #pragma unroll
-          for (uint32_t k = 0; k < SG / 4; k++)
-            ((uint4 *)dst)[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
-        } else if constexpr (SG == 2) {
-          *(uint2 *)dst = make_uint2(o[0], o[1]);
-        } else {
-          dst[0] = o[0];
+            for (uint32_t k = 0; k < SG / 4; k++)
+              ((uint4 *)dst)[k] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+          } else if constexpr (SG == 2) {
+            *(uint2 *)dst = make_uint2(o[0], o[1]);
+          } else {
+            dst[0] = o[0];
+          }
+        }
+      }
+      if constexpr (kRows) {
+        if (staged) {
+          // round r: entries (lanes) rR .. rR + R - 1 put piece k of their
+          // slice at 16-byte slot k ^ (lane mod NQ) of their NQ slots; lane l
+          // then reads slot 64i + l and stores it as piece (slot mod NQ) ^
+          // (entry mod NQ) of that entry's slice
+          uint8_t *st = rowst + wave * kRowStageBytes;
+          const uint32_t pv = own ? p : ~0u;
+          const uint32_t wb = (lane % R) * NQ * 16 + (lane % NQ) * 16;
+#pragma unroll
+          for (uint32_t r = 0; r < 64 / R; r++) {
+            if (lane / R == r) {
+#pragma unroll
+              for (uint32_t k = 0; k < NQ; k++)
+                *(uint4 *)(st + (wb ^ (k * 16))) =
+                    make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (uint32_t i = 0; i < R * NQ / 64; i++) {
+              const uint32_t slot = i * 64 + lane, ent = slot / NQ;
+              const uint4 val = *(const uint4 *)(st + slot * 16);
+              const uint32_t pe = (uint32_t)__shfl((int)pv, (int)(r * R + ent), 64);
+              const uint32_t k = (slot % NQ) ^ (ent % NQ);
+              if (pe != ~0u)
+                ((uint4 *)(a.hits + (uint64_t)pe * a.row_words + a.col_word0 +
+                           (uint64_t)q * SG))[k] = val;
+            }
+            wave_lds_sync();
+          }
         }
       }
     }
@@ -1103,12 +1164,6 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
 // columns, so every entry is handled here whichever bank it routes to.
 constexpr uint32_t kRecThreads = 1024;
 constexpr uint32_t kRecQueue = 128;  // candidates queued per wave, at most
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Probes the directory for the first cnt (<= 64) queued candidates, one per
 // lane, and ORs the postings of those that have some into their hit rows.
@@ -1804,7 +1859,11 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   const uint32_t stage_blocks = std::max<uint32_t>(sa.xcds, idx->cu_count);
   // entry ranges per unit of the bit-plane lookup: fewer segments in flight
   // per XCD, so more hit rows are whole in L2 before they are written back
-  uint32_t plane_parts = 2;
+  // hit rows of the bit-plane lookup through LDS, whole per store (1, the
+  // default) or stored 16 bytes per lane (0)
+  sa.row_stage = 1;
+  if (const char *e = getenv("ZG_SWEEP_ROW_STAGE")) sa.row_stage = atoi(e) ? 1 : 0;
+  uint32_t plane_parts = 1;
   if (const char *e = getenv("ZG_SWEEP_PARTS"))
     plane_parts = (uint32_t)std::min<unsigned long>(64, std::max<unsigned long>(1, strtoul(e, nullptr, 10)));
   // sum: lanes per item, the row's words rounded up to a power of two (at most 64)
