@@ -200,3 +200,21 @@ def test_sweep_where_only_the_bit_planes_fit():
     st = assert_sweep_runs(idx, reads)
     assert st["sweep_plane_banks"] == idx.meta["n_banks"], st
     check_parity(idx, orc, reads, label="b16 H8, bit planes only")
+
+
+@pytest.mark.parametrize("bits,hashes,stage", [(1, 4, "1"), (2, 3, "1"), (4, 4, "1"), (4, 8, "0"),
+                                               (16, 2, "1"), (32, 3, "1"), (8, 8, "1")])
+def test_bit_plane_lookup_across_widths(bits, hashes, stage, monkeypatch):
+    # 600 colours: one slice per row at b = 1 and 4, two at b = 2 and 8,
+    # three at b = 16 and more at b = 32; hit rows through LDS (stage 1,
+    # whole row slices per store) or stored by each lane (stage 0)
+    colors = make_colors(620 + bits + hashes, 600, 2000)
+    reads = make_reads(621 + bits, colors, 120, lo=30, hi=1800)
+    idx = build(colors, fingerprint_bits=bits, hashes=hashes)
+    idx.load()
+    orc = oracle_for(idx)
+    idx.tuning = SWEEP_BENCH
+    monkeypatch.setenv("ZG_SWEEP_ROW_STAGE", stage)
+    st = assert_sweep_runs(idx, reads)
+    assert st["sweep_plane_banks"] == idx.meta["n_banks"], st
+    check_parity(idx, orc, reads, label=f"planes b{bits} H{hashes} stage{stage}")

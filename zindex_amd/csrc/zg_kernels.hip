@@ -487,31 +487,56 @@ __global__ __launch_bounds__(256) void ocm_select_kernel(
     block_counts[blockIdx.x] = wave_counts[0] + wave_counts[1] + wave_counts[2] + wave_counts[3];
 }
 
+// Compaction of the selected windows: one wave per tile of kOcmTile window
+// starts (the unit the selection counts in), 64 starts per step; an emitted
+// window's key goes to the tile's offset plus the emitted windows before it.
+// Waves stride over the tiles.
 __global__ __launch_bounds__(256) void ocm_compact_kernel(const uint64_t *codes,
                                                           const uint8_t *flag,
                                                           const uint8_t *delta,
-                                                          const uint64_t *block_off,
+                                                          const uint64_t *tile_off,
                                                           uint64_t positions, uint32_t m,
                                                           SpacedRuns spaced, uint64_t *keys,
                                                           uint64_t *key_win) {
-  __shared__ uint32_t wave_counts[4];
-  const uint32_t tid = threadIdx.x;
-  uint64_t S = (uint64_t)blockIdx.x * kOcmTile + tid;
-  bool emit = S < positions && flag[S];
-  uint64_t b = __ballot(emit);
-  uint32_t lane = tid & 63;
-  uint32_t below = (uint32_t)__popcll(b & ((1ULL << lane) - 1));
-  if (lane == 0) wave_counts[tid >> 6] = (uint32_t)__popcll(b);
-  __syncthreads();
-  uint32_t before = 0;
-  for (uint32_t w = 0; w < (tid >> 6); w++) before += wave_counts[w];
-  if (emit) {
-    uint64_t at = block_off[blockIdx.x] + before + below;
-    uint64_t pos = S + delta[S];
-    uint64_t key = canonical_from_window(stream_bits(codes, 2 * pos), (int)m);
-    if (spaced.n) key = spaced_canonical(key, spaced);  // masked after selection
-    keys[at] = key;
-    key_win[at] = S;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t n_tiles = (positions + kOcmTile - 1) / kOcmTile;
+  const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64);
+  for (uint64_t tile = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+       tile < n_tiles; tile += waves) {
+    constexpr uint32_t Q = kOcmTile / 64;
+    const uint64_t S0 = tile * kOcmTile;
+    // the tile's flags and deltas, then the code words of its selected
+    // windows, each group of loads issued together
+    uint32_t f[Q], d[Q];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint64_t S = S0 + 64 * q + lane;
+      f[q] = S < positions ? flag[S] : 0;
+      d[q] = S < positions ? delta[S] : 0;
+    }
+    uint64_t lo[Q], hi[Q];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint64_t pos = S0 + 64 * q + lane + d[q];
+      const uint32_t sh = 2 * (uint32_t)(pos & 31);
+      lo[q] = f[q] ? codes[pos >> 5] : 0;
+      hi[q] = f[q] && sh ? codes[(pos >> 5) + 1] : 0;
+    }
+    uint64_t at = tile_off[tile];
+#pragma unroll
+    for (uint32_t q = 0; q < Q; q++) {
+      const uint64_t b = __ballot(f[q] != 0);
+      if (f[q]) {
+        const uint64_t S = S0 + 64 * q + lane, pos = S + d[q];
+        const uint32_t sh = 2 * (uint32_t)(pos & 31);
+        const uint64_t i = at + (uint64_t)__popcll(b & ((1ULL << lane) - 1));
+        uint64_t key = canonical_from_window(sh ? (lo[q] >> sh) | (hi[q] << (64 - sh)) : lo[q], (int)m);
+        if (spaced.n) key = spaced_canonical(key, spaced);  // masked after selection
+        keys[i] = key;
+        key_win[i] = S;
+      }
+      at += (uint64_t)__popcll(b);
+    }
   }
 }
 
@@ -533,8 +558,10 @@ void launch_ocm_compact(const uint64_t *codes, const uint8_t *flag, const uint8_
                         const uint64_t *block_off, uint64_t positions, uint32_t m,
                         const SpacedRuns &spaced, uint64_t *keys, uint64_t *key_pos,
                         hipStream_t s) {
-  uint64_t blocks = (positions + kOcmTile - 1) / kOcmTile;
-  if (!blocks) return;
+  const uint64_t tiles = (positions + kOcmTile - 1) / kOcmTile;
+  if (!tiles) return;
+  // a wave per tile, at most 4096 blocks of four waves
+  const uint64_t blocks = std::min<uint64_t>((tiles + 3) / 4, 4096);
   ocm_compact_kernel<<<(uint32_t)blocks, 256, 0, s>>>(codes, flag, delta, block_off, positions,
                                                       m, spaced, keys, key_pos);
 }

@@ -1087,9 +1087,12 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
             o[4 * c + 3] = ~d3;
           }
         }
-        // into field order: block j <- block j ^ tq
+        // into field order: block j <- block j ^ tq, i.e. word w <- word
+        // w ^ tq BS. Staged rows take the part of the XOR above the low two
+        // bits in the LDS slot of each 16-byte group instead.
 #pragma unroll
         for (uint32_t sbit = 1; sbit < NB; sbit <<= 1) {
+          if (staged && sbit * BS >= 4) continue;
           const bool sw = (tq & sbit) != 0;
 #pragma unroll
           for (uint32_t j = 0; j < NB; j++)
@@ -1117,13 +1120,15 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
       }
       if constexpr (kRows) {
         if (staged) {
-          // round r: entries (lanes) rR .. rR + R - 1 put piece k of their
-          // slice at 16-byte slot k ^ (lane mod NQ) of their NQ slots; lane l
-          // then reads slot 64i + l and stores it as piece (slot mod NQ) ^
-          // (entry mod NQ) of that entry's slice
+          // round r: entries (lanes) rR .. rR + R - 1 put their 16-byte
+          // group k, which is piece k ^ (tq BS / 4) of the slice, at slot
+          // k ^ (tq BS / 4) ^ (lane mod NQ) of their NQ slots; lane l then
+          // reads slot 64i + l and stores it as piece (slot mod NQ) ^ (entry
+          // mod NQ) of that entry's slice
           uint8_t *st = rowst + wave * kRowStageBytes;
           const uint32_t pv = own ? p : ~0u;
-          const uint32_t wb = (lane % R) * NQ * 16 + (lane % NQ) * 16;
+          const uint32_t xk = (((tq * BS) >> 2) ^ lane) % NQ;
+          const uint32_t wb = (lane % R) * NQ * 16 + xk * 16;
 #pragma unroll
           for (uint32_t r = 0; r < 64 / R; r++) {
             if (lane / R == r) {
