@@ -765,9 +765,10 @@ def main() -> int:
                           "sum": round(st["sweep_sum_ms"], 2)},
             "sum_kernel_gbs": round(sum_bytes / (st["sweep_sum_ms"] / 1000) / 1e9, 1)
             if st["sweep_sum_ms"] > 0 else None})
-        # the lookups' hit-row stores against the same stores replayed alone
-        # (one whole row per seed of the last chunk, in the sweep's order, one
-        # lane per row): the lookup is store-bound when it writes as fast
+        # the lookups' hit-row stores against the same rows stored alone (one
+        # whole row per seed of the last chunk, in the sweep's order, each row
+        # written by neighbouring lanes of one store instruction): the store
+        # ceiling of the lookups' position pattern
         try:
             ceiling = batch.store_probe()
         except zindex_amd._lib.ZindexError as e:

@@ -338,9 +338,10 @@ int zg_batch_destroy(zg_batch *batch);
 
 /* Measurement, no reference counterpart: after a sorted-sweep run, writes one
  * whole hit row at the position of every seed of the batch's last chunk, in
- * the sweep's order (segment, then position), with no lookups, and reports
- * the rate in GB/s: the ceiling of the lookups' hit-row stores. Overwrites
- * the batch's hit rows (not its results). ZG_ERR_INVALID if no sweep ran. */
+ * the sweep's order (segment, then position), each row by neighbouring lanes
+ * of one store instruction, with no lookups, and reports the rate in GB/s:
+ * the ceiling of the lookups' hit-row stores. Overwrites the batch's hit rows
+ * (not its results). ZG_ERR_INVALID if no sweep ran. */
 int zg_batch_store_probe(zg_batch *batch, double *gbs);
 
 int zg_index_set_tuning(zg_index *idx, const zg_tuning *tuning);

@@ -551,8 +551,10 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       uint64_t slot = r.hashes[i] >> (32 - bits);
       while ((uint32_t)dir[slot] != kNone) slot = (slot + 1) & (cap - 1);
       dir[slot] = ((uint64_t)r.hashes[i] << 32) | pi;
-      const uint32_t fb = r.hashes[i] >> (32 - kRecFilterLog2);
-      filter[fb / 32] |= 1u << (fb % 32);
+      for (uint32_t j = 0; j < kRecFilterProbes; j++) {
+        const uint32_t fb = rec_filter_bit(r.hashes[i], j);
+        filter[fb / 32] |= 1u << (fb % 32);
+      }
     }
     upload(idx->rec_dir, dir, s);
     upload(idx->rec_filter, filter, s);

@@ -1160,8 +1160,8 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
 // of the class's lookups (so it follows every row store): an entry whose key
 // has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
 // bits of its postings' columns of this class ORed into its hit row. Keys
-// whose bit in the directory's hash filter (128 KiB, staged in LDS by each
-// persistent block) is clear have no postings. The others queue up per wave
+// with any of their three bits clear in the directory's hash filter (128 KiB,
+// staged in LDS by each persistent block) have no postings. The others queue up per wave
 // in LDS and are probed 64 at a time, one per lane, so a wave waits on the
 // directory once per 64 candidates rather than once per 64 entries. The
 // postings of the recovered entries go 64 per pass, lane g finding its entry
@@ -1247,8 +1247,12 @@ __global__ __launch_bounds__(kRecThreads) void plane_recovery_kernel(
     for (int j = 0; j < J; j++) {
       const uint64_t e = w0 + (uint64_t)j * 64 + lane;
       const uint32_t ah = abandoned_hash(unsplitmix64(hj[j]) - seed, seed);  // mixsplit(key, seed) == h
-      const uint32_t fb = ah >> (32 - kRecFilterLog2);
-      const bool cand = e < n && ((filter[fb / 32] >> (fb % 32)) & 1u);
+      bool cand = e < n;
+#pragma unroll
+      for (uint32_t q = 0; q < kRecFilterProbes; q++) {
+        const uint32_t fb = rec_filter_bit(ah, q);
+        cand = cand && ((filter[fb / 32] >> (fb % 32)) & 1u);
+      }
       const uint64_t m = __ballot(cand);
       if (cand) {
         const uint32_t at = qn + (uint32_t)__popcll(m & below);
@@ -1960,16 +1964,21 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   if (strobes) launch_strand_max(b->counts.as<uint32_t>(), b->n_reads * m.colors.size(), s);
 }
 
-// The lookups' hit-row stores alone: lane i writes the whole row (row_words
-// words, 16 bytes at a time) of position p[i], for the n entries in the
-// sweep's order (segment, then position).
+// The lookups' hit-row stores alone, for the n entries in the sweep's order
+// (segment, then position): the row of position p[i] (row_words words)
+// written whole by 2^lpr_log2 neighbouring lanes of one store instruction,
+// 16 bytes per lane -- the store ceiling of the lookups' position pattern.
 __global__ __launch_bounds__(256) void store_probe_kernel(const uint32_t *p, const uint64_t *n_ptr,
-                                                          uint32_t *hits, uint32_t row_words) {
+                                                          uint32_t *hits, uint32_t row_words,
+                                                          uint32_t lpr_log2) {
   const uint64_t n = *n_ptr;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x) {
+  const uint32_t pieces = row_words / 4, lpr = 1u << lpr_log2;
+  const uint32_t j = threadIdx.x & (lpr - 1);
+  const uint64_t groups = ((uint64_t)gridDim.x * blockDim.x) >> lpr_log2;
+  for (uint64_t i = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> lpr_log2; i < n;
+       i += groups) {
     uint4 *row = (uint4 *)(hits + (uint64_t)p[i] * row_words);
-    for (uint32_t k = 0; k < row_words / 4; k++) row[k] = make_uint4((uint32_t)i, k, 0, 0);
+    for (uint32_t k = j; k < pieces; k += lpr) row[k] = make_uint4((uint32_t)i, k, 0, 0);
   }
 }
 
@@ -1981,12 +1990,16 @@ double sweep_store_probe(zg_batch *b) {
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "hipEventCreate");
   check_hip(hipEventCreate(&e1), "hipEventCreate");
-  const uint32_t blocks = (uint32_t)std::min<uint64_t>(idx->cu_count * 16, b->sw_last_items / 256 + 1);
+  // lanes per row: the row's 16-byte pieces rounded up to a power of two, at most 64
+  uint32_t lpr_log2 = 0;
+  while ((1u << lpr_log2) < std::min<uint32_t>(64, idx->sw_row_words / 4)) lpr_log2++;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(
+      idx->cu_count * 16, (b->sw_last_items << lpr_log2) / 256 + 1);
   store_probe_kernel<<<blocks, 256, 0, s>>>(b->sw_last_p, b->sw_last_n, b->fd_hits.as<uint32_t>(),
-                                            idx->sw_row_words);  // warm-up
+                                            idx->sw_row_words, lpr_log2);  // warm-up
   check_hip(hipEventRecord(e0, s), "hipEventRecord");
   store_probe_kernel<<<blocks, 256, 0, s>>>(b->sw_last_p, b->sw_last_n, b->fd_hits.as<uint32_t>(),
-                                            idx->sw_row_words);
+                                            idx->sw_row_words, lpr_log2);
   check_hip(hipEventRecord(e1, s), "hipEventRecord");
   check_hip(hipStreamSynchronize(s), "store probe");
   uint64_t n = 0;
