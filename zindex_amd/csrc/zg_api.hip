@@ -538,6 +538,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
         }
     std::vector<uint64_t> dir(cap, (uint64_t)kNone);
     std::vector<uint32_t> post, post_col, filter(kRecFilterWords, 0);
+    const uint32_t probes = rec_filter_probes(n);
     post.reserve(n + r.origins.size());
     post_col.reserve(n + r.origins.size());
     for (uint64_t i = 0; i < n; i++) {
@@ -551,7 +552,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
       uint64_t slot = r.hashes[i] >> (32 - bits);
       while ((uint32_t)dir[slot] != kNone) slot = (slot + 1) & (cap - 1);
       dir[slot] = ((uint64_t)r.hashes[i] << 32) | pi;
-      for (uint32_t j = 0; j < kRecFilterProbes; j++) {
+      for (uint32_t j = 0; j < probes; j++) {
         const uint32_t fb = rec_filter_bit(r.hashes[i], j);
         filter[fb / 32] |= 1u << (fb % 32);
       }
@@ -559,6 +560,7 @@ void upload_index(zg_index *idx, uint64_t budget, uint32_t memory) {
     upload(idx->rec_dir, dir, s);
     upload(idx->rec_filter, filter, s);
     idx->rec_view.filter = idx->rec_filter.as<uint32_t>();
+    idx->rec_view.filter_probes = probes;
     upload(idx->rec_post, post, s);
     upload(idx->rec_post_col, post_col, s);
     idx->rec_view.dir = idx->rec_dir.as<uint64_t>();

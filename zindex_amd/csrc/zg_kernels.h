@@ -46,16 +46,25 @@ struct RecDev {
   uint32_t cap_log2;          // directory capacity 2^cap_log2
   uint32_t enabled;
   const uint32_t *filter;     // 2^kRecFilterLog2 bits: bits rec_filter_bit(hash, j),
-                              // j < kRecFilterProbes, set for every directory hash
+                              // j < filter_probes, set for every directory hash
                               // (a key with any of its bits clear has no postings)
+  uint32_t filter_probes;     // 1 .. kRecFilterProbes
 };
 constexpr uint32_t kRecFilterLog2 = 20;
 constexpr uint32_t kRecFilterWords = (1u << kRecFilterLog2) / 32;
 constexpr uint32_t kRecFilterProbes = 3;
 
+// Probes of the recovery filter for n directory hashes: the Bloom-filter
+// optimum 2^20 ln 2 / n, between 1 and kRecFilterProbes (three probes pass
+// about 2 % of absent keys at ~110k hashes, one 10 %; at a million hashes
+// more probes pass as many and cost more LDS reads).
+inline uint32_t rec_filter_probes(uint64_t n) {
+  const double k = n ? (double)(1u << kRecFilterLog2) * 0.6931 / (double)n : kRecFilterProbes;
+  return k < 1.5 ? 1 : (k < 2.5 ? 2 : kRecFilterProbes);
+}
+
 // Bit j of the recovery filter for a key's abandoned hash: the top 20 bits of
-// the hash, then of the hash times two odd constants. With ~110k directory
-// hashes in 2^20 bits, three probes pass about 2 % of absent keys, one 10 %.
+// the hash, then of the hash times two odd constants.
 __host__ __device__ inline uint32_t rec_filter_bit(uint32_t ah, uint32_t j) {
   const uint32_t x = j == 0 ? ah : ah * (j == 1 ? 0x9E3779B1u : 0x85EBCA77u);
   return x >> (32 - kRecFilterLog2);

@@ -44,7 +44,7 @@ constexpr uint32_t kStageWaves = kStageThreads / 64;
 constexpr uint32_t kStageWindow = 128 * 1024;        // LDS rows of one segment window
 constexpr uint32_t kSliceBytes = 512;                // plane bytes per slice (8 per lane)
 constexpr uint32_t kStageOut = 512;                  // staged hit bytes per wave
-constexpr uint64_t kSweepHitBytes = 128ULL << 30;    // hit rows of one chunk, at most
+constexpr uint64_t kSweepHitBytes = 160ULL << 30;    // hit rows of one chunk, at most
 constexpr uint32_t kSumSpan = 4096;                  // positions per wave of the sum
 constexpr uint32_t kXcds = 8;
 // The bit-plane lookup at b >= 4 and b < 32: one 1024-thread block per CU
@@ -1160,7 +1160,7 @@ __global__ __launch_bounds__(plane_threads(B)) void plane_lookup_kernel(StageArg
 // of the class's lookups (so it follows every row store): an entry whose key
 // has postings in the directory (storage.rs:2771-2787, 2916-2924) gets the
 // bits of its postings' columns of this class ORed into its hit row. Keys
-// with any of their three bits clear in the directory's hash filter (128 KiB,
+// with any of their bits clear in the directory's hash filter (128 KiB,
 // staged in LDS by each persistent block) have no postings. The others queue up per wave
 // in LDS and are probed 64 at a time, one per lane, so a wave waits on the
 // directory once per 64 candidates rather than once per 64 entries. The
@@ -1251,7 +1251,7 @@ __global__ __launch_bounds__(kRecThreads) void plane_recovery_kernel(
 #pragma unroll
       for (uint32_t q = 0; q < kRecFilterProbes; q++) {
         const uint32_t fb = rec_filter_bit(ah, q);
-        cand = cand && ((filter[fb / 32] >> (fb % 32)) & 1u);
+        if (q < rec.filter_probes) cand = cand && ((filter[fb / 32] >> (fb % 32)) & 1u);
       }
       const uint64_t m = __ballot(cand);
       if (cand) {
@@ -1809,7 +1809,7 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   const bool strobes = m.has_randstrobemers;
   // rows of counts: reads, or both orientations of each read for randstrobes
   const uint64_t n_rows = strobes ? 2 * b->n_reads : b->n_reads;
-  // chunk size: hit rows of at most 64 GiB, entries below 2^32, and the
+  // chunk size: hit rows of at most 160 GiB, entries below 2^32, and the
   // chunk's buffers (hit row + 24 B of entries per item) within 3/4 of the
   // device memory free or already held by them
   uint64_t hit_budget = kSweepHitBytes;
