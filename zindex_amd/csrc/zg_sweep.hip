@@ -1832,6 +1832,12 @@ void sweep_run(zg_index *idx, zg_batch *b) {
   }
   if (const char *e = getenv("ZG_SWEEP_CHUNK_POSITIONS"))  // smaller chunks
     cap = std::min<uint64_t>(cap, std::max<uint64_t>(64, strtoull(e, nullptr, 10)));
+  // hit rows already held cover this many positions: a cap less than 1/8
+  // above it is not worth reallocating the chunk buffers for (free memory
+  // as reported moves by a few hundred MB between runs)
+  const uint64_t held_cap = b->fd_hits.bytes / (row_words * 4ULL);
+  if (!b->sw_capturing && held_cap && cap > held_cap && cap - held_cap < held_cap / 8)
+    cap = held_cap;
   b->sw_cap = cap;
   const std::vector<uint64_t> &off = b->h_seg_off;  // row starts in the item space
   std::vector<uint64_t> cuts{0};
