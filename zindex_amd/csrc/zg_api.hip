@@ -92,7 +92,7 @@ namespace zg {
 
 static unsigned host_threads() { return zg_host_threads(); }
 
-// Query support of this library for an index (the modes BASELINE targets).
+// Query support of this library for an index (the seed modes and schemes the GPU query runs).
 static void check_query_support(const Manifest &m) {
   if (m.index_mode == 0 || m.index_mode == 2) {
     if (m.k > 32 && m.has_randstrobemers)
