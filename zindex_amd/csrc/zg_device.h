@@ -41,6 +41,24 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t *off, uint64_t n
   return lo;
 }
 
+// find_segment by the whole wave: each round its 64 lanes probe 64 evenly
+// spaced entries of the range and a ballot keeps the piece holding the
+// answer (4 dependent loads at a million segments instead of 20). x and n
+// are wave-uniform and every lane is active.
+__device__ __forceinline__ uint64_t wave_find_segment(const uint64_t *off, uint64_t n, uint64_t x,
+                                                      uint32_t lane) {
+  uint64_t lo = 0, hi = n - 1;  // off[lo] <= x; the answer lies in [lo, hi]
+  while (lo < hi) {
+    const uint64_t step = (hi - lo + 64) / 64;  // 64 pieces of at most step entries
+    const uint64_t m = lo + (uint64_t)lane * step;
+    const uint64_t ok = __ballot(m <= hi && off[m] <= x);  // lane 0 always
+    const uint64_t t = 63 - (uint64_t)__builtin_clzll(ok);
+    lo += t * step;
+    hi = hi < lo + step - 1 ? hi : lo + step - 1;
+  }
+  return lo;
+}
+
 // Lane src's value of v, src per lane (through the LDS crossbar).
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
   const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src, 64);

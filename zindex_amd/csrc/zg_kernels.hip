@@ -92,8 +92,11 @@ __global__ __launch_bounds__(256) void pack_kernel(const uint8_t *seq, const uin
   const uint64_t w1 = w0 + 64 < n_words ? w0 + 64 : n_words;
   uint64_t p = w0 * 64 + lane;
   // the lane's read: its positions end at next_pos, byte = p + delta while
-  // below byte_end (the read's last position is the separator)
-  uint64_t r = find_segment(pos_off, n_reads, p < positions ? p : positions - 1);
+  // below byte_end (the read's last position is the separator); found from
+  // the read holding the wave's first position
+  uint64_t r = wave_find_segment(pos_off, n_reads, w0 * 64, lane);
+  const uint64_t px = p < positions ? p : positions - 1;
+  while (pos_off[r + 1] <= px) r++;  // pos_off[n_reads] = positions > px
   uint64_t next_pos = pos_off[r + 1], delta = read_off[r] - pos_off[r], byte_end = read_off[r + 1];
   uint64_t my_inv = 0, my_b0 = 0, my_b1 = 0;  // lane i: word w0 + i's ballots
   constexpr int U = 16;  // rounds whose byte loads are in flight together
@@ -275,18 +278,21 @@ __global__ void count_kmers_kernel(const uint64_t *invalid, const uint64_t *code
                                    uint64_t sampling, const uint64_t *pos_off,
                                    uint64_t n_reads, uint64_t positions, uint32_t k,
                                    uint64_t *totals) {
-  uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-  uint64_t n_words = (positions + 63) / 64;
+  const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  const uint64_t n_words = (positions + 63) / 64;
+  if ((w & ~63ULL) >= n_words) return;  // wave-uniform
+  // the read holding the wave's first position, then this word's first
+  uint64_t r = wave_find_segment(pos_off, n_reads, (w & ~63ULL) * 64, threadIdx.x & 63);
   if (w >= n_words) return;
   uint64_t valid = ~invalid_window_starts(invalid, w, k);
   uint64_t p0 = w * 64;
+  while (pos_off[r + 1] <= p0) r++;  // pos_off[n_reads] = positions > p0
   if (sampling)  // scan_modimizers (src/lib.rs:796-806)
     for (uint64_t m = valid; m; m &= m - 1) {
       const int j = __builtin_ctzll(m);
       const uint64_t key = canonical_from_window(stream_bits(codes, 2 * (p0 + j)), (int)k);
       if (mixsplit(key, seed) % sampling) valid &= ~(1ULL << j);
     }
-  uint64_t r = find_segment(pos_off, n_reads, p0);
   while (r < n_reads && pos_off[r] < p0 + 64) {
     uint64_t a = pos_off[r] > p0 ? pos_off[r] - p0 : 0;
     uint64_t b = pos_off[r + 1] - p0;

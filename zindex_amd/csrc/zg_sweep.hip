@@ -1424,7 +1424,11 @@ __global__ __launch_bounds__(256) void sweep_sum_kernel(
   uint32_t *stage = flush_lds[wave] + slot * LP * 32;
   const uint64_t q0 = sp * kSumSpan, q1 = q0 + kSumSpan < n_pos ? q0 + kSumSpan : n_pos;
   uint64_t first = q0 + slot < q1 ? q0 + slot : q1 - 1;
-  uint64_t r = records ? find_segment(pos_off, n_reads, p0 + first) : 0;
+  uint64_t r = 0;
+  if (records) {  // the read of the span's first position, then of the slot's
+    r = wave_find_segment(pos_off, n_reads, p0 + q0, lane);
+    while (pos_off[r + 1] <= p0 + first) r++;
+  }
   uint64_t next = records ? pos_off[r + 1] - p0 : ~0ULL;  // relative end of read r
   uint32_t pl[kPlanes], tot[32];
 #pragma clang loop unroll(full)
