@@ -156,11 +156,16 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(Src src, uint6
 // order. The tile is then written out digit run by digit run, so that
 // neighbouring threads write neighbouring entries (whole cache lines) after
 // the same digit's entries of the earlier tiles.
+// With `bounds` (the last pass), each entry that starts its segment in the
+// output lowers bounds[segment] to its index: an entry starts it when it
+// opens its tile's run of its digit (its predecessor is in another tile) or
+// its predecessor in the run has another segment. bounds_fill_kernel then
+// gives empty segments the start of the next one.
 template <class Src>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(Src src, uint64_t count, uint32_t shift,
                                                             uint32_t dbits, uint64_t n_tiles,
                                                             const uint64_t *off, uint64_t *out_h,
-                                                            uint32_t *out_p) {
+                                                            uint32_t *out_p, uint32_t *bounds) {
   __shared__ uint32_t wbase[kSortWaves][kMaxBins];  // per wave: digit counts, then next LDS slot
   __shared__ uint32_t dstart[kMaxBins + 1];  // first LDS slot of each digit
   __shared__ uint64_t gbase[kMaxBins];       // global index of LDS slot 0 of each digit
@@ -225,11 +230,45 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(Src src, ui
   const uint32_t n_tile = dstart[nb];
   for (uint32_t i = threadIdx.x; i < n_tile; i += kSortThreads) {
     const uint64_t h = th[i];
-    const uint32_t d = (uint32_t)((mulhi64(h, count) >> shift) & (nb - 1));
+    const uint64_t seg = mulhi64(h, count);
+    const uint32_t d = (uint32_t)((seg >> shift) & (nb - 1));
     const uint64_t g = gbase[d] + i;
     out_h[g] = h;
     out_p[g] = tp[i];
+    if (bounds && (i == dstart[d] || mulhi64(th[i - 1], count) != seg))
+      atomicMin(bounds + seg, (uint32_t)g);
   }
+}
+
+// bounds[s] = *n_ptr for s <= count, before the last pass lowers them.
+__global__ void bounds_init_kernel(uint32_t *bounds, uint64_t count, const uint64_t *n_ptr) {
+  const uint32_t n = (uint32_t)*n_ptr;
+  for (uint64_t s = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; s <= count;
+       s += (uint64_t)gridDim.x * blockDim.x)
+    bounds[s] = n;
+}
+
+// One block: bounds[s] = min(bounds[s], bounds[s + 1], ...), so an empty
+// segment starts where the next non-empty one does. Each thread takes a
+// contiguous piece, then the pieces' minima are combined from the end.
+__global__ __launch_bounds__(1024) void bounds_fill_kernel(uint32_t *bounds, uint64_t count) {
+  __shared__ uint32_t piece_min[1024];
+  const uint64_t n = count + 1, per = (n + 1023) / 1024;
+  const uint64_t a = threadIdx.x * per, e = a + per < n ? a + per : n;
+  uint32_t m = ~0u;
+  for (uint64_t s = e; s-- > a;) {
+    m = bounds[s] < m ? bounds[s] : m;
+    bounds[s] = m;
+  }
+  piece_min[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int t = 1022; t >= 0; t--)
+      piece_min[t] = piece_min[t + 1] < piece_min[t] ? piece_min[t + 1] : piece_min[t];
+  __syncthreads();
+  const uint32_t later = threadIdx.x < 1023 ? piece_min[threadIdx.x + 1] : ~0u;
+  for (uint64_t s = a; s < e; s++)
+    if (later < bounds[s]) bounds[s] = later;
 }
 
 // start[s] = first entry of segment s (start[count] = n).
@@ -1735,20 +1774,26 @@ static int sort_class(zg_batch *b, Src first_pass, uint64_t count, uint64_t n_it
           es, count, shift, dbits, tiles, b->sw_hist.as<uint32_t>(), nullptr);
     launch_exclusive_scan_u32_to_u64(b->sw_hist.as<uint32_t>(), b->sw_off.as<uint64_t>(), hn,
                                      b->scan_temp.p, tb, s);
+    // the last pass also marks each segment's first entry
+    uint32_t *bounds = nullptr;
+    if (pass + 1 == passes) {
+      bounds = b->sw_start.as<uint32_t>();
+      bounds_init_kernel<<<(uint32_t)std::min<uint64_t>(1024, count / 256 + 1), 256, 0, s>>>(
+          bounds, count, n_ptr);
+    }
     if (pass == 0) {
       radix_scatter_kernel<Src><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
           first_pass, count, shift, dbits, tiles, b->sw_off.as<uint64_t>(),
-          b->sw_h[0].as<uint64_t>(), b->sw_p[0].as<uint32_t>());
+          b->sw_h[0].as<uint64_t>(), b->sw_p[0].as<uint32_t>(), bounds);
       cur = 0;
     } else {
       radix_scatter_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
           es, count, shift, dbits, tiles, b->sw_off.as<uint64_t>(), b->sw_h[1 - cur].as<uint64_t>(),
-          b->sw_p[1 - cur].as<uint32_t>());
+          b->sw_p[1 - cur].as<uint32_t>(), bounds);
       cur = 1 - cur;
     }
   }
-  bucket_bounds_kernel<<<(uint32_t)std::min<uint64_t>(65536, n_items / 256 + 2), 256, 0, s>>>(
-      b->sw_h[cur].as<uint64_t>(), n_ptr, count, n_items, b->sw_start.as<uint32_t>());
+  bounds_fill_kernel<<<1, 1024, 0, s>>>(b->sw_start.as<uint32_t>(), count);
   return cur;
 }
 
@@ -2073,7 +2118,7 @@ int sweep_sort_entries(const uint64_t *h, const uint32_t *p, const uint64_t *n_d
     launch_exclusive_scan_u32_to_u64(hist.as<uint32_t>(), off.as<uint64_t>(), hn, temp.p, tb, s);
     const int nxt = cur < 0 ? 0 : 1 - cur;
     radix_scatter_kernel<EntSource><<<(uint32_t)tiles, kSortThreads, 0, s>>>(
-        es, count, pass * dbits, dbits, tiles, off.as<uint64_t>(), hs[nxt], ps[nxt]);
+        es, count, pass * dbits, dbits, tiles, off.as<uint64_t>(), hs[nxt], ps[nxt], nullptr);
     es = EntSource{hs[nxt], ps[nxt], n_dev, 0};
     cur = nxt;
   }
