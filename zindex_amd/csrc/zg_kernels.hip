@@ -266,8 +266,17 @@ __device__ __forceinline__ uint64_t invalid_window_starts(const uint64_t *invali
   // invalid words are padded for the window length, invalid_words)
   uint64_t m = 0;
   if (k <= 64) {
+    // over the 128 bits (lo, hi): a span of s bits ORed at each position,
+    // doubled while 2s <= k, then one more OR at offset k - s (< s) covers k
     uint64_t lo = invalid[w], hi = invalid[w + 1];
-    for (uint32_t d = 0; d < k; d++) m |= d ? ((lo >> d) | (hi << (64 - d))) : lo;
+    uint32_t s = 1;
+    while (2 * s <= k) {
+      lo |= (lo >> s) | (hi << (64 - s));
+      hi |= hi >> s;
+      s *= 2;
+    }
+    if (s < k) lo |= (lo >> (k - s)) | (hi << (64 - (k - s)));
+    m = lo;
   } else {
     for (uint32_t d = 0; d < k; d++) m |= stream_bits(invalid, 64 * w + d);
   }
